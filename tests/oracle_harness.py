@@ -1,0 +1,153 @@
+"""Test-side access to the CPU oracle and, where built, the reference library.
+
+TEST INFRASTRUCTURE ONLY.  Loads ``oracle/liboracle.so`` (the C restatement
+of the reference routines) and ``oracle/_ref/libref_*.so`` (the reference's
+own routines, built from its sources by ``oracle/ref/build_ref.sh`` where the
+reference tree is available) through ctypes, and lays out fields exactly like
+the reference's ``double (*)[Y_m][Z_m]`` arrays.
+"""
+import ctypes as C
+import glob
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+REF_DIR = os.path.join(ORACLE_DIR, "_ref")
+
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
+
+
+class OrcCfg(C.Structure):
+    _fields_ = [(n, C.c_int) for n in (
+        "nx", "ny", "nz", "first", "last", "gstart", "gend0", "nx_global",
+        "nx_inlet", "nx_outlet", "sponge", "xi_mode")] + [(n, C.c_double) for n in (
+        "dxi", "deta", "dzeta", "high", "K", "R", "Cv", "mu_L", "Pr_L", "suth_T",
+        "suth_S", "rho_ref", "grav", "f_const", "f_dplusgs", "E_ref")]
+
+
+def load_oracle():
+    path = os.path.join(ORACLE_DIR, "liboracle.so")
+    if not os.path.exists(path):
+        subprocess.run(["make", "-C", ORACLE_DIR, "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    lib = C.CDLL(path)
+    lib.orc_fill.argtypes = [DP, C.c_long, C.c_uint64, C.c_double, C.c_double]
+    lib.orc_fill_state.argtypes = [C.c_long, C.c_uint64, DP, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_double, C.POINTER(DP)]
+    lib.orc_cfg_reference.argtypes = [C.POINTER(OrcCfg), C.c_int, C.c_int, C.c_int]
+    return lib
+
+
+def ref_libraries():
+    return sorted(glob.glob(os.path.join(REF_DIR, "libref_*.so")))
+
+
+def ref_cfg_from(lib):
+    """(X_out, Y_out, Z_out, Y_m, Z_m, np, nx_inlet, nx_outlet), deltas."""
+    dims = (C.c_int * 8)()
+    lib.ref_dims(dims)
+    d = (C.c_double * 3)()
+    lib.ref_deltas(d)
+    return tuple(dims), tuple(d)
+
+
+def ptrs(arrs):
+    """double** from a list of contiguous float64 arrays."""
+    return (DP * len(arrs))(*[a.ctypes.data_as(DP) for a in arrs])
+
+
+def dp(a):
+    return a.ctypes.data_as(DP)
+
+
+class Slab:
+    """All fields of one slab in the reference layout [X_np][Y_m][Z_m]."""
+
+    def __init__(self, nx, ny, nz):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.shape = (nx + 6, ny + 4, nz + 4)
+        self.n = int(np.prod(self.shape))
+
+    def zeros(self, count=None):
+        if count is None:
+            return np.zeros(self.shape)
+        return [np.zeros(self.shape) for _ in range(count)]
+
+    # reference index ranges (Resolution.h:15-29)
+    @property
+    def gend(self):
+        return self.nx + 2
+
+    @property
+    def NY(self):
+        return self.ny + 1
+
+    @property
+    def NZ(self):
+        return self.nz + 1
+
+
+def fill(orc, shape, seed, lo, hi):
+    a = np.empty(shape)
+    orc.orc_fill(dp(a), a.size, seed, lo, hi)
+    return a
+
+
+def make_inputs(orc, slab, seed, K=1.4, dimensional=True):
+    """Seeded valid state, metrics and time levels for one slab.
+
+    dimensional=True draws air at reference conditions (the reference's SI
+    units); otherwise a nondimensional state (rho~1, p~1/(gamma M^2))."""
+    sh = slab.shape
+    f = {}
+    f["J"] = fill(orc, sh, seed + 1, 0.7, 1.3)
+    # cell metrics: xi_x, eta_y, zeta_z dominant, cross terms small
+    names = ["xidx", "xidy", "xidz", "etdx", "etdy", "etdz", "ztdx", "ztdy", "ztdz"]
+    diag = {"xidx", "etdy", "ztdz"}
+    for tag, base in (("", 10), ("_u", 30), ("_v", 50), ("_w", 70)):
+        for q, nme in enumerate(names):
+            lo, hi = (0.6, 1.4) if nme in diag else (-0.25, 0.25)
+            f[nme + tag] = fill(orc, sh, seed + base + q, lo, hi)
+        if tag:
+            f["J" + tag] = fill(orc, sh, seed + base + 9, 0.7, 1.3)
+    if dimensional:
+        r0, r1, ua, p0, p1 = 1.05, 1.35, 30.0, 95000.0, 107000.0
+    else:
+        r0, r1, ua, p0, p1 = 0.9, 1.1, 0.5, 2.6, 3.1
+    for lvl, base in (("U_", 100), ("U", 200), ("Uq", 300)):
+        arrs = [np.empty(sh) for _ in range(5)]
+        orc.orc_fill_state(slab.n, seed + base, dp(f["J"]), K, r0, r1, ua, p0, p1, ptrs(arrs))
+        f[lvl] = arrs
+    return f
+
+
+def met9(f, tag=""):
+    names = ["xidx", "xidy", "xidz", "etdx", "etdy", "etdz", "ztdx", "ztdy", "ztdz"]
+    return [f[n + tag] for n in names]
+
+
+def ref_cfg(orc, nx, ny, nz, deltas=None):
+    c = OrcCfg()
+    orc.orc_cfg_reference(C.byref(c), nx, ny, nz)
+    if deltas is not None:
+        c.dxi, c.deta, c.dzeta = deltas
+    return c
+
+
+def hyperplane_sizes(slab):
+    rows = slab.gend + slab.NY + slab.NZ + 8
+    cells = slab.nx * slab.ny * slab.nz
+    return rows, cells
+
+
+def rel_maxnorm(a, b):
+    """max|a-b| / max|b| over the arrays (the parity metric)."""
+    den = np.max(np.abs(b))
+    num = np.max(np.abs(a - b))
+    if den == 0:
+        return float(num)
+    return float(num / den)
