@@ -151,3 +151,106 @@ def rel_maxnorm(a, b):
     if den == 0:
         return float(num)
     return float(num / den)
+
+
+# ---------------------------------------------------------------------------
+# one inner iteration of the reference call sequence (main.cpp:213-274),
+# run either through the reference library or through the oracle
+# ---------------------------------------------------------------------------
+class Case:
+    """A slab configuration matching one reference build."""
+
+    def __init__(self, X, Y, Z, NP, nin, nout, myid, deltas):
+        self.X, self.Y, self.Z, self.NP, self.nin, self.nout, self.myid = X, Y, Z, NP, nin, nout, myid
+        self.gc = X // NP
+        self.slab = Slab(self.gc, Y, Z)
+        self.deltas = deltas
+        self.xnp = self.gc + 6
+        self.overrides = {}
+
+    @property
+    def tag(self):
+        return "%dx%dx%d_np%d_in%d_out%d" % (self.X, self.Y, self.Z, self.NP, self.nin, self.nout)
+
+    def cfg(self, orc, xi_mode=0):
+        c = ref_cfg(orc, self.gc, self.Y, self.Z, self.deltas)
+        c.nx_inlet, c.nx_outlet = self.nin, self.nout
+        c.first = int(self.myid == 0)
+        c.last = int(self.myid == self.NP - 1)
+        c.gstart = self.myid * self.gc
+        c.gend0 = c.gstart + self.gc - 1
+        c.nx_global = self.X
+        c.xi_mode = xi_mode
+        for k, v in self.overrides.items():
+            setattr(c, k, v)
+        return c
+
+
+def run_chain(backend, lib, case, f, orc=None, Ep=1.0, e=1e-4, deltaT=2e-4, dts=True, sweeps=2):
+    """Fluxes, viscous terms and (optionally) one Dual_time_stepping call.
+    Returns a dict of output arrays."""
+    slab = case.slab
+    out = {}
+    U_ = [a.copy() for a in f["U_"]]
+    MR, ML = slab.zeros(5), slab.zeros(5)
+    for d, tag in enumerate(("_u", "_v", "_w")):
+        F = slab.zeros(5)
+        if backend == "ref":
+            lib.ref_flux(d, case.xnp, case.myid, C.c_double(Ep), C.c_double(e), ptrs(U_), ptrs(MR), ptrs(ML),
+                         dp(f["J"]), dp(f["J" + tag]), ptrs(met9(f, tag)), ptrs(F), dp(slab.zeros()))
+        else:
+            cfg = case.cfg(lib)
+            fn = (lib.orc_flux_x, lib.orc_flux_y, lib.orc_flux_z)[d]
+            fn(C.byref(cfg), C.c_double(Ep), C.c_double(e), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]),
+               dp(f["J" + tag]), ptrs(met9(f, tag)), ptrs(F))
+        out["inF" + "xyz"[d]] = F
+    vF = slab.zeros(4)
+    S = [slab.zeros(5) for _ in range(6)]
+    if backend == "ref":
+        lib.ref_viscous(case.xnp, case.myid, ptrs(U_), ptrs(vF), dp(f["J"]), ptrs(met9(f)), dp(f["J_u"]),
+                        ptrs(met9(f, "_u")), dp(f["J_v"]), ptrs(met9(f, "_v")), dp(f["J_w"]), ptrs(met9(f, "_w")),
+                        *[ptrs(s) for s in S])
+    else:
+        cfg = case.cfg(lib)
+        lib.orc_viscous(C.byref(cfg), ptrs(U_), ptrs(vF), dp(f["J"]), dp(f["J_u"]), ptrs(met9(f, "_u")),
+                        dp(f["J_v"]), ptrs(met9(f, "_v")), dp(f["J_w"]), ptrs(met9(f, "_w")),
+                        ptrs(S[1]), ptrs(S[3]), ptrs(S[5]))
+    out["vF"] = vF
+    if not dts:
+        return out
+    rows, cells = hyperplane_sizes(slab)
+    N = np.zeros((rows, 2), np.int32)
+    I = np.zeros((3 * cells + 1, 3), np.int32)
+    NR, NL, Res = slab.zeros(5), slab.zeros(5), slab.zeros(5)
+    MR = slab.zeros(5)
+    er = (C.c_double * 5)()
+    if backend == "ref":
+        lib.ref_hyperplane(case.xnp, case.myid, N.ctypes.data_as(IP), I.ctypes.data_as(IP))
+        lib.ref_dts(case.xnp, case.myid, C.c_double(deltaT), C.c_double(e), er, N.ctypes.data_as(IP),
+                    I.ctypes.data_as(IP), ptrs(U_), ptrs(f["U"]), ptrs(f["Uq"]), dp(f["J"]), ptrs(met9(f)),
+                    ptrs(out["inFx"]), ptrs(out["inFy"]), ptrs(out["inFz"]), ptrs(vF), ptrs(MR), ptrs(NR),
+                    ptrs(NL), ptrs(Res), dp(f["etdy_v"]), dp(f["J_v"]))
+    else:
+        cfg = case.cfg(lib)
+        lib.orc_hyperplane(C.byref(cfg), N.ctypes.data_as(IP), I.ctypes.data_as(IP))
+        lib.orc_dual_time_stepping(C.byref(cfg), C.c_double(deltaT), C.c_double(e), er, N.ctypes.data_as(IP),
+                                   I.ctypes.data_as(IP), ptrs(U_), ptrs(f["U"]), ptrs(f["Uq"]), dp(f["J"]),
+                                   ptrs(met9(f)), ptrs(out["inFx"]), ptrs(out["inFy"]), ptrs(out["inFz"]),
+                                   ptrs(vF), ptrs(MR), ptrs(NR), ptrs(NL), ptrs(Res), dp(f["etdy_v"]),
+                                   dp(f["J_v"]), sweeps)
+    out["Nijk"], out["ijk"] = N, I
+    out["U_"], out["NR"], out["MR"] = U_, NR, MR
+    out["er"] = np.array(list(er))
+    return out
+
+
+def regions(slab):
+    """Index windows each output is defined on (reference loop ranges)."""
+    g, NY, NZ = slab.gend, slab.NY, slab.NZ
+    return {
+        "inFx": (slice(2, g + 1), slice(1, NY), slice(1, NZ)),        # Flux_X.cpp:229-245
+        "inFy": (slice(2, g), slice(1, NY + 1), slice(1, NZ)),        # Flux_Y.cpp:272-289
+        "inFz": (slice(2, g), slice(1, NY), slice(1, NZ + 1)),        # Flux_Z.cpp:230-248
+        "vF": (slice(3, g + 1), slice(2, NY + 1), slice(2, NZ + 1)),  # Viscous_terms.cpp:875-878
+        "cell": (slice(3, g + 1), slice(2, NY + 1), slice(2, NZ + 1)),
+    }

@@ -1,0 +1,228 @@
+"""The CPU oracle against the reference's own outputs and against invariants.
+
+The golden fixtures under tests/golden/ were written by make_golden.py from
+the reference routines compiled from their own sources.  These tests rebuild
+the same seeded inputs, run the oracle and require bit-identical results on
+every output (SHA-256 of the full index window) and on the stored values.
+"""
+import ctypes as C
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle_harness import (Case, REF_DIR, Slab, dp, hyperplane_sizes, load_oracle, make_inputs, met9,
+                            ptrs, ref_cfg, regions, run_chain, IP)
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+MANIFEST = json.load(open(os.path.join(GOLD, "manifest.json")))
+SEED = MANIFEST["seed"]
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return load_oracle()
+
+
+def case_from(tag, entry):
+    X, Y, Z, Ym, Zm, NP, nin, nout = entry["dims"]
+    return Case(X, Y, Z, NP, nin, nout, entry["myid"], tuple(entry["deltas"]))
+
+
+def oracle_fixtures(orc, case):
+    f = make_inputs(orc, case.slab, SEED)
+    has_dts = case.NP == 1
+    out = run_chain("orc", orc, case, f, dts=has_dts)
+    reg = regions(case.slab)
+    fx = {k: np.stack([a[reg[k]] for a in out[k]]) for k in ("inFx", "inFy", "inFz")}
+    fx["vF"] = np.stack([a[reg["vF"]] for a in out["vF"]])
+    if has_dts:
+        fx["U_"] = np.stack([a[reg["cell"]] for a in out["U_"]])
+        fx["NR"] = np.stack([a[reg["cell"]] for a in out["NR"]])
+        fx["er"] = out["er"]
+        fx["Nijk"] = out["Nijk"]
+        fx["ijk"] = out["ijk"][: case.slab.nx * case.slab.ny * case.slab.nz + 1]
+    return f, fx
+
+
+@pytest.mark.parametrize("tag", sorted(MANIFEST["cases"]))
+def test_oracle_bit_identical_to_reference_outputs(orc, tag):
+    entry = MANIFEST["cases"][tag]
+    case = case_from(tag, entry)
+    gold = np.load(os.path.join(GOLD, entry["file"]))
+    _, fx = oracle_fixtures(orc, case)
+    for k, v in fx.items():
+        assert sha(v) == entry["sha256"][k], "%s: %s differs from the reference" % (tag, k)
+        if k in gold.files:
+            g = gold[k]
+            w = v if g.shape == v.shape else v[:, : g.shape[1]]
+            assert np.array_equal(w, g), k
+
+
+@pytest.mark.parametrize("tag", [t for t in sorted(MANIFEST["cases"]) if "np1" in t])
+def test_metrics_and_ghosts_match_reference(orc, tag):
+    entry = MANIFEST["cases"][tag]
+    case = case_from(tag, entry)
+    cfg = case.cfg(orc)
+    sl = case.slab
+    J, m = sl.zeros(), sl.zeros(9)
+    orc.orc_jacobian(C.byref(cfg), dp(J), ptrs(m))
+    Ju, mu = sl.zeros(), sl.zeros(9)
+    orc.orc_jacobian_u(C.byref(cfg), dp(Ju), ptrs(mu))
+    Jv, mv = sl.zeros(), sl.zeros(9)
+    orc.orc_jacobian_v(C.byref(cfg), dp(Jv), ptrs(mv), dp(J), ptrs(m))
+    Jw, mw = sl.zeros(), sl.zeros(9)
+    orc.orc_jacobian_w(C.byref(cfg), dp(Jw), ptrs(mw))
+    for name, arr in (("jac_cell", [J] + m), ("jac_u", [Ju] + mu), ("jac_v", [Jv] + mv), ("jac_w", [Jw] + mw)):
+        assert sha(np.stack(arr)) == entry["sha256"][name], name
+    f = make_inputs(orc, sl, SEED)
+    U_ = [a.copy() for a in f["U_"]]
+    lo, hi = case.nin + 2, case.X - case.nout + 3   # heated section, Y_boundary_condition.cpp:119
+    orc.orc_bc_y(C.byref(cfg), ptrs(U_), dp(f["J"]), C.c_double(343.0592), C.c_double(298.0592), lo, hi)
+    orc.orc_bc_z(C.byref(cfg), ptrs(U_))
+    assert sha(np.stack(U_)) == entry["sha256"]["bc_U_"]
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_DIR), reason="reference build not present")
+@pytest.mark.parametrize("xi_mode", [0, 1])
+def test_flux_y_xi_term(orc, xi_mode):
+    """Flux_Y.cpp:383-385 multiplies deltaP by never-assigned XIx/XIy/XIz.  The
+    reference build behaves as XI=0; xi_mode=1 (face unit normal) must differ."""
+    import glob
+    path = sorted(glob.glob(os.path.join(REF_DIR, "libref_8x10x6_np1*.so")))
+    if not path:
+        pytest.skip("reference build not present")
+    ref = C.CDLL(path[0])
+    ref.ref_init()
+    entry = MANIFEST["cases"]["8x10x6_np1_in3_out3"]
+    case = case_from("8x10x6_np1_in3_out3", entry)
+    f = make_inputs(orc, case.slab, SEED + 7)
+    outs = []
+    for backend, lib in (("ref", ref), ("orc", orc)):
+        sl = case.slab
+        U_ = [a.copy() for a in f["U_"]]
+        MR, ML, F = sl.zeros(5), sl.zeros(5), sl.zeros(5)
+        if backend == "ref":
+            ref.ref_flux(1, case.xnp, 0, C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML),
+                         dp(f["J"]), dp(f["J_v"]), ptrs(met9(f, "_v")), ptrs(F), dp(sl.zeros()))
+        else:
+            cfg = case.cfg(orc, xi_mode=xi_mode)
+            orc.orc_flux_y(C.byref(cfg), C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML),
+                           dp(f["J"]), dp(f["J_v"]), ptrs(met9(f, "_v")), ptrs(F))
+        outs.append(F)
+    same = all(np.array_equal(a, b) for a, b in zip(outs[0], outs[1]))
+    assert same == (xi_mode == 0)
+
+
+# ----------------------------------------------------------------- invariants
+
+def uniform_case(orc, nx=6, ny=8, nz=6):
+    case = Case(nx, ny, nz, 1, 2, 2, 0, (0.01, 0.02, 0.015))
+    sl = case.slab
+    f = {"J": np.full(sl.shape, 1.0)}
+    for tag in ("", "_u", "_v", "_w"):
+        for q, n in enumerate(["xidx", "xidy", "xidz", "etdx", "etdy", "etdz", "ztdx", "ztdy", "ztdz"]):
+            f[n + tag] = np.full(sl.shape, 1.0 if q in (0, 4, 8) else 0.0)
+        if tag:
+            f["J" + tag] = np.full(sl.shape, 1.0)
+    return case, f
+
+
+def test_muscl_reproduces_quartics(orc):
+    """5-point weights (2,-13,47,27,-3)/60 are exact for degree <= 4 at the
+    face when the cell values are cell averages (uniform J): the left state
+    of face i+1/2 and the right state of face i-1/2... equal p at the face."""
+    case, f = uniform_case(orc)
+    sl = case.slab
+    cfg = case.cfg(orc)
+    cfg.first = cfg.last = 0
+    i = np.arange(sl.shape[0], dtype=float)[:, None, None] * np.ones(sl.shape)
+    coef = [1.5, 0.01, -0.002, 3e-4, -1e-5]
+    p = lambda x: sum(c * x ** n for n, c in enumerate(coef))
+    P = lambda x: sum(c * x ** (n + 1) / (n + 1) for n, c in enumerate(coef))
+    avg = P(i + 0.5) - P(i - 0.5)
+    U_ = [avg.copy() for _ in range(5)]
+    MR, ML, F = sl.zeros(5), sl.zeros(5), sl.zeros(5)
+    orc.orc_flux_x(C.byref(cfg), C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]),
+                   dp(f["J_u"]), ptrs(met9(f, "_u")), ptrs(F))
+    for ii in range(3, sl.gend):
+        assert abs(ML[0][ii, 3, 3] - p(ii + 0.5)) < 1e-12
+        assert abs(MR[0][ii, 3, 3] - p(ii + 0.5)) < 1e-12
+    # the reconstruction is genuinely 5-point: a quintic is not reproduced
+    U5 = [(i + 0.5) ** 6 / 6 - (i - 0.5) ** 6 / 6 for _ in range(5)]
+    orc.orc_flux_x(C.byref(cfg), C.c_double(1.0), C.c_double(1e-4), ptrs(U5), ptrs(MR), ptrs(ML), dp(f["J"]),
+                   dp(f["J_u"]), ptrs(met9(f, "_u")), ptrs(F))
+    assert abs(ML[0][5, 3, 3] - 5.5 ** 5) > 1e-6
+
+
+def test_roe_flux_of_equal_states_is_physical_flux(orc):
+    case, f = uniform_case(orc)
+    sl = case.slab
+    cfg = case.cfg(orc)
+    K = cfg.K
+    rho, u, v, w, p = 1.1, 0.3, -0.2, 0.1, 2.5
+    xix, xiy, xiz = 1.2, 0.3, -0.1
+    f["xidx_u"][:], f["xidy_u"][:], f["xidz_u"][:] = xix, xiy, xiz
+    Ju = 0.8
+    f["J_u"][:] = Ju
+    E = p / (K - 1) + 0.5 * rho * (u * u + v * v + w * w)
+    U_ = [np.full(sl.shape, x) for x in (rho, rho * u, rho * v, rho * w, E)]
+    MR, ML, F = sl.zeros(5), sl.zeros(5), sl.zeros(5)
+    orc.orc_flux_x(C.byref(cfg), C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]),
+                   dp(f["J_u"]), ptrs(met9(f, "_u")), ptrs(F))
+    Un = xix * u + xiy * v + xiz * w
+    exact = [rho * Un, rho * u * Un + xix * p, rho * v * Un + xiy * p, rho * w * Un + xiz * p, Un * (E + p)]
+    for q in range(5):
+        assert abs(F[q][4, 3, 3] - exact[q] / Ju) < 1e-12 * max(1.0, abs(exact[q]))
+
+
+def test_free_stream_gives_zero_increment(orc):
+    case, f = uniform_case(orc)
+    sl = case.slab
+    K = 1.4
+    rho, u, v, w, p = 1.0, 0.4, 0.0, 0.0, 1.0 / (K * 0.25)
+    E = p / (K - 1) + 0.5 * rho * u * u
+    for lvl in ("U_", "U", "Uq"):
+        f[lvl] = [np.full(sl.shape, x) for x in (rho, rho * u, rho * v, rho * w, E)]
+    case.overrides = {"sponge": 0, "grav": 0.0}   # no absorbing zones, no buoyancy
+    out = run_chain("orc", orc, case, f, dts=True)
+    cfg = case.cfg(orc)
+    assert max(np.max(np.abs(a)) for a in out["vF"]) == 0.0
+    assert np.max(np.abs(out["er"])) < 1e-14
+    for q in range(5):
+        assert np.max(np.abs(out["U_"][q] - f["U_"][q])) <= 1e-14 * max(1.0, np.abs(f["U_"][q]).max())
+
+
+def test_second_sweep_repeats_the_first_on_one_slab(orc):
+    """With zero increment ghosts the second symmetric sweep recomputes the
+    first exactly (Dual_time_stepping.cpp:931-1689 at nproc=1)."""
+    entry = MANIFEST["cases"]["8x10x6_np1_in3_out3"]
+    case = case_from("8x10x6_np1_in3_out3", entry)
+    f = make_inputs(orc, case.slab, SEED)
+    a = run_chain("orc", orc, case, f, sweeps=1)
+    b = run_chain("orc", orc, case, f, sweeps=2)
+    for q in range(5):
+        assert np.array_equal(a["NR"][q], b["NR"][q])
+        assert np.array_equal(a["U_"][q], b["U_"][q])
+
+
+def test_hyperplane_lists_every_cell_once_in_plane_order(orc):
+    sl = Slab(5, 4, 3)
+    cfg = ref_cfg(orc, 5, 4, 3)
+    rows, cells = hyperplane_sizes(sl)
+    N = np.zeros((rows, 2), np.int32)
+    I = np.zeros((cells + 1, 3), np.int32)
+    orc.orc_hyperplane(C.byref(cfg), N.ctypes.data_as(IP), I.ctypes.data_as(IP))
+    listed = [tuple(r) for r in I[1:]]
+    brute = sorted((i, j, k) for i in range(3, sl.gend + 1) for j in range(2, sl.NY + 1) for k in range(2, sl.NZ + 1))
+    assert sorted(listed) == brute
+    sums = [sum(t) for t in listed]
+    assert sums == sorted(sums)
+    for m in range(7, sl.gend + sl.NY + sl.NZ + 1):
+        assert N[m, 1] == sum(1 for s in sums if s == m)
