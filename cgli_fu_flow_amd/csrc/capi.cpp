@@ -1,0 +1,543 @@
+// capi.cpp -- the C ABI of include/cgli.h: contexts, resident fields and
+// the drop-in entry points.  Host-side orchestration only; every number is
+// computed by the kernels in flux.hip, viscous.hip, implicit.hip, bc.hip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "cgli_internal.hpp"
+
+using namespace cgli;
+
+namespace {
+
+enum { T_FLUX_X, T_FLUX_Y, T_FLUX_Z, T_VISCOUS, T_RHS, T_FWD, T_BWD, T_UPDATE, T_NCLASS };
+
+struct EventPair { hipEvent_t a, b; };
+
+}  // namespace
+
+struct cgli_ctx {
+    cgli_config cfg;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Geo g;
+    Phys p;
+    double *f[CGLI_NFIELDS] = {};
+    bool valid[CGLI_NFIELDS] = {};
+    double *prim4 = nullptr;     // 4 fields: u, v, w, T (viscous pass 1)
+    double *faces12 = nullptr;   // 12 fields: viscous face fluxes
+    double *scratch = nullptr;   // reductions
+    long scratch_len = 0;
+    double *er_host = nullptr;   // pinned
+    int *ijk_dev = nullptr;
+    std::vector<int> Nijk;       // (gend+NY+NZ+8) x 2
+    std::vector<int> ijk;        // (cells+1) x 3
+    bool timing = false;
+    std::vector<EventPair> ev[T_NCLASS];
+    size_t ev_used[T_NCLASS] = {};
+};
+
+#define HIPCHK(x)                                      \
+    do {                                               \
+        hipError_t e_ = (x);                           \
+        if (e_ != hipSuccess) return CGLI_E_HIP;       \
+    } while (0)
+
+static int ensure(cgli_ctx *c, int id)
+{
+    if (c->f[id]) return CGLI_OK;
+    void *p = nullptr;
+    if (hipMalloc(&p, sizeof(double) * c->g.N) != hipSuccess) return CGLI_E_NOMEM;
+    if (hipMemsetAsync(p, 0, sizeof(double) * c->g.N, c->stream) != hipSuccess) return CGLI_E_HIP;
+    c->f[id] = static_cast<double *>(p);
+    return CGLI_OK;
+}
+
+static int ensure_range(cgli_ctx *c, int id, int n)
+{
+    for (int q = 0; q < n; q++) {
+        int r = ensure(c, id + q);
+        if (r) return r;
+    }
+    return CGLI_OK;
+}
+
+static bool have(cgli_ctx *c, int id, int n)
+{
+    for (int q = 0; q < n; q++)
+        if (!c->valid[id + q]) return false;
+    return true;
+}
+
+static void mark(cgli_ctx *c, int id, int n)
+{
+    for (int q = 0; q < n; q++) c->valid[id + q] = true;
+}
+
+static F5 f5(cgli_ctx *c, int id) { return F5{{c->f[id], c->f[id + 1], c->f[id + 2], c->f[id + 3], c->f[id + 4]}}; }
+static C5 c5(cgli_ctx *c, int id) { return C5{{c->f[id], c->f[id + 1], c->f[id + 2], c->f[id + 3], c->f[id + 4]}}; }
+static M9 m9(cgli_ctx *c, int id)
+{
+    M9 m;
+    for (int q = 0; q < 9; q++) m.m[q] = c->f[id + q];
+    return m;
+}
+
+static int scratch(cgli_ctx *c, long n)
+{
+    if (c->scratch_len >= n) return CGLI_OK;
+    if (c->scratch) hipFree(c->scratch);
+    c->scratch = nullptr;
+    void *p = nullptr;
+    if (hipMalloc(&p, sizeof(double) * n) != hipSuccess) return CGLI_E_NOMEM;
+    c->scratch = static_cast<double *>(p);
+    c->scratch_len = n;
+    return CGLI_OK;
+}
+
+static void tbegin(cgli_ctx *c, int cls)
+{
+    if (!c->timing) return;
+    auto &v = c->ev[cls];
+    if (c->ev_used[cls] == v.size()) {
+        EventPair e;
+        hipEventCreate(&e.a);
+        hipEventCreate(&e.b);
+        v.push_back(e);
+    }
+    hipEventRecord(v[c->ev_used[cls]].a, c->stream);
+}
+
+static void tend(cgli_ctx *c, int cls)
+{
+    if (!c->timing) return;
+    hipEventRecord(c->ev[cls][c->ev_used[cls]].b, c->stream);
+    c->ev_used[cls]++;
+}
+
+extern "C" {
+
+void cgli_config_reference(cgli_config *c, int nx, int ny, int nz)
+{
+    std::memset(c, 0, sizeof(*c));
+    c->nx = nx; c->ny = ny; c->nz = nz;
+    c->first = 1; c->last = 1;
+    c->gstart = 0; c->gend0 = nx - 1; c->nx_global = nx;
+    c->nx_inlet = 100; c->nx_outlet = 150;
+    c->sponge = 1; c->xi_mode = 0;
+    c->dxi = 0.001; c->deta = 0.005; c->dzeta = 0.002;
+    c->high = 0.05;
+    c->K = 1.4; c->R = 287;
+    c->Cv = 717.5; c->mu_L = 0.0000185; c->Pr_L = 0.71;
+    c->suth_T = 298.0592; c->suth_S = 110.;
+    c->rho_ref = 1.1842; c->grav = 9.81; c->f_const = 0.0; c->f_dplusgs = 0.5993;
+    c->E_ref = 253250.0;
+}
+
+const char *cgli_error_string(int code)
+{
+    switch (code) {
+    case CGLI_OK: return "ok";
+    case CGLI_E_ARG: return "invalid argument";
+    case CGLI_E_HIP: return "HIP runtime error";
+    case CGLI_E_NOMEM: return "device allocation failed";
+    case CGLI_E_NONFINITE: return "non-finite residual norm";
+    case CGLI_E_STATE: return "a required input field was never set";
+    case CGLI_E_NODEV: return "no HIP device";
+    case CGLI_E_TIMEOUT: return "bounded device wait expired";
+    default: return "unknown error";
+    }
+}
+
+int cgli_create(const cgli_config *cfg, int device, cgli_ctx **out)
+{
+    if (!cfg || !out) return CGLI_E_ARG;
+    *out = nullptr;
+    // the reconstruction stencils need 5 cells per direction inside the slab
+    if (cfg->nx < 4 || cfg->ny < 6 || cfg->nz < 4) return CGLI_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CGLI_E_NODEV;
+    if (device < 0 || device >= ndev) return CGLI_E_ARG;
+    cgli_ctx *c = new (std::nothrow) cgli_ctx();
+    if (!c) return CGLI_E_NOMEM;
+    c->cfg = *cfg;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return CGLI_E_HIP;
+    }
+    Geo &g = c->g;
+    g.nx = cfg->nx; g.ny = cfg->ny; g.nz = cfg->nz;
+    g.Ym = cfg->ny + 4; g.Zm = cfg->nz + 4;
+    g.N = (long)(cfg->nx + 6) * g.Ym * g.Zm;
+    g.gend = cfg->nx + 2; g.NY = cfg->ny + 1; g.NZ = cfg->nz + 1;
+    g.NYY = g.NY + 1; g.NZZ = g.NZ + 1;
+    g.first = cfg->first; g.last = cfg->last;
+    Phys &p = c->p;
+    p.K = cfg->K; p.R = cfg->R; p.Cv = cfg->Cv; p.mu_L = cfg->mu_L; p.Pr_L = cfg->Pr_L;
+    p.suth_T = cfg->suth_T; p.suth_S = cfg->suth_S;
+    p.dxi = cfg->dxi; p.deta = cfg->deta; p.dzeta = cfg->dzeta;
+    p.rho_ref = cfg->rho_ref; p.grav = cfg->grav; p.f_const = cfg->f_const; p.E_ref = cfg->E_ref;
+    p.high = cfg->high;
+    void *h = nullptr;
+    if (hipHostMalloc(&h, sizeof(double) * 8) != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return CGLI_E_HIP;
+    }
+    c->er_host = static_cast<double *>(h);
+    *out = c;
+    return CGLI_OK;
+}
+
+int cgli_destroy(cgli_ctx *c)
+{
+    if (!c) return CGLI_E_ARG;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (int q = 0; q < CGLI_NFIELDS; q++)
+        if (c->f[q]) hipFree(c->f[q]);
+    if (c->prim4) hipFree(c->prim4);
+    if (c->faces12) hipFree(c->faces12);
+    if (c->scratch) hipFree(c->scratch);
+    if (c->ijk_dev) hipFree(c->ijk_dev);
+    if (c->er_host) hipHostFree(c->er_host);
+    for (int k = 0; k < T_NCLASS; k++)
+        for (auto &e : c->ev[k]) {
+            hipEventDestroy(e.a);
+            hipEventDestroy(e.b);
+        }
+    hipStreamDestroy(c->stream);
+    delete c;
+    return CGLI_OK;
+}
+
+int cgli_set_field(cgli_ctx *c, int id, const double *host)
+{
+    if (!c || !host || id < 0 || id >= CGLI_NFIELDS) return CGLI_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    int r = ensure(c, id);
+    if (r) return r;
+    HIPCHK(hipMemcpyAsync(c->f[id], host, sizeof(double) * c->g.N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->valid[id] = true;
+    return CGLI_OK;
+}
+
+int cgli_get_field(cgli_ctx *c, int id, double *host)
+{
+    if (!c || !host || id < 0 || id >= CGLI_NFIELDS) return CGLI_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    int r = ensure(c, id);
+    if (r) return r;
+    HIPCHK(hipMemcpyAsync(host, c->f[id], sizeof(double) * c->g.N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CGLI_OK;
+}
+
+int cgli_field_device_ptr(cgli_ctx *c, int id, void **dptr)
+{
+    if (!c || !dptr || id < 0 || id >= CGLI_NFIELDS) return CGLI_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    int r = ensure(c, id);
+    if (r) return r;
+    c->valid[id] = true;   // the caller writes it
+    *dptr = c->f[id];
+    return CGLI_OK;
+}
+
+int cgli_synchronize(cgli_ctx *c)
+{
+    if (!c) return CGLI_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CGLI_OK;
+}
+
+static int flux_common(cgli_ctx *c, int dir, double Ep, double e)
+{
+    if (!c) return CGLI_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    const int jf = dir == 0 ? CGLI_J_U : dir == 1 ? CGLI_J_V : CGLI_J_W;
+    const int mf = jf + 1;
+    if (!have(c, CGLI_U_, 5) || !have(c, CGLI_J, 1) || !have(c, jf, 1) || !have(c, mf + 3 * dir, 3))
+        return CGLI_E_STATE;
+    const int out = dir == 0 ? CGLI_INFX : dir == 1 ? CGLI_INFY : CGLI_INFZ;
+    int r = ensure_range(c, out, 5);
+    if (r) return r;
+    r = ensure_range(c, mf, 9);   // unset metric components are never read
+    if (r) return r;
+    tbegin(c, T_FLUX_X + dir);
+    if (dir == 0)
+        launch_flux_x(c->g, c->p, e, c5(c, CGLI_U_), c->f[CGLI_J], c->f[jf], m9(c, mf), f5(c, out), c->stream);
+    else if (dir == 1)
+        launch_flux_y(c->g, c->p, Ep, e, c->cfg.xi_mode, c5(c, CGLI_U_), c->f[CGLI_J], c->f[jf], m9(c, mf),
+                      f5(c, out), c->stream);
+    else
+        launch_flux_z(c->g, c->p, e, c->cfg.xi_mode, c5(c, CGLI_U_), c->f[CGLI_J], c->f[jf], m9(c, mf),
+                      f5(c, out), c->stream);
+    tend(c, T_FLUX_X + dir);
+    HIPCHK(hipGetLastError());
+    mark(c, out, 5);
+    return CGLI_OK;
+}
+
+int cgli_flux_x(cgli_ctx *c, double Ep, double e) { return flux_common(c, 0, Ep, e); }
+int cgli_flux_y(cgli_ctx *c, double Ep, double e) { return flux_common(c, 1, Ep, e); }
+int cgli_flux_z(cgli_ctx *c, double Ep, double e) { return flux_common(c, 2, Ep, e); }
+
+int cgli_viscous(cgli_ctx *c)
+{
+    if (!c) return CGLI_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    if (!have(c, CGLI_U_, 5) || !have(c, CGLI_J, 1) || !have(c, CGLI_J_U, 10) || !have(c, CGLI_J_V, 10) ||
+        !have(c, CGLI_J_W, 10))
+        return CGLI_E_STATE;
+    int r = ensure_range(c, CGLI_VF, 4);
+    if (r) return r;
+    if (!c->prim4) {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(double) * 4 * c->g.N) != hipSuccess) return CGLI_E_NOMEM;
+        c->prim4 = static_cast<double *>(p);
+    }
+    if (!c->faces12) {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(double) * 12 * c->g.N) != hipSuccess) return CGLI_E_NOMEM;
+        c->faces12 = static_cast<double *>(p);
+    }
+    F4 vF{{c->f[CGLI_VF], c->f[CGLI_VF + 1], c->f[CGLI_VF + 2], c->f[CGLI_VF + 3]}};
+    tbegin(c, T_VISCOUS);
+    launch_viscous(c->g, c->p, c5(c, CGLI_U_), c->f[CGLI_J], c->f[CGLI_J_U], m9(c, CGLI_MET_U), c->f[CGLI_J_V],
+                   m9(c, CGLI_MET_V), c->f[CGLI_J_W], m9(c, CGLI_MET_W), c->prim4, c->faces12, vF, c->stream);
+    tend(c, T_VISCOUS);
+    HIPCHK(hipGetLastError());
+    mark(c, CGLI_VF, 4);
+    return CGLI_OK;
+}
+
+// Hyperplane.cpp:38-78 ordering: planes m ascending; inside a plane i, j, k
+// ascending.  Built in O(cells) and uploaded once.
+static int build_planes(cgli_ctx *c)
+{
+    if (c->ijk_dev) return CGLI_OK;
+    const Geo &g = c->g;
+    const int rows = g.gend + g.NY + g.NZ + 8;
+    const long cells = (long)g.nx * g.ny * g.nz;
+    c->Nijk.assign(2 * rows, 0);
+    c->ijk.assign(3 * (cells + 1), 0);
+    int ii = 0;
+    for (int m = 7; m <= g.gend + g.NY + g.NZ; m++) {
+        int mi = 0;
+        for (int i = 3; i <= g.gend; i++)
+            for (int j = 2; j <= g.NY; j++) {
+                const int k = m - i - j;
+                if (k < 2 || k > g.NZ) continue;
+                mi++;
+                ii++;
+                c->ijk[3 * ii] = i;
+                c->ijk[3 * ii + 1] = j;
+                c->ijk[3 * ii + 2] = k;
+            }
+        c->Nijk[2 * m] = ii;
+        c->Nijk[2 * m + 1] = mi;
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, sizeof(int) * c->ijk.size()) != hipSuccess) return CGLI_E_NOMEM;
+    c->ijk_dev = static_cast<int *>(p);
+    HIPCHK(hipMemcpy(c->ijk_dev, c->ijk.data(), sizeof(int) * c->ijk.size(), hipMemcpyHostToDevice));
+    return CGLI_OK;
+}
+
+int cgli_hyperplane(cgli_ctx *c, int *Nijk, int *ijk)
+{
+    if (!c || !Nijk || !ijk) return CGLI_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    int r = build_planes(c);
+    if (r) return r;
+    std::memcpy(Nijk, c->Nijk.data(), sizeof(int) * c->Nijk.size());
+    std::memcpy(ijk, c->ijk.data(), sizeof(int) * c->ijk.size());
+    return CGLI_OK;
+}
+
+static int implicit_inputs(cgli_ctx *c)
+{
+    if (!have(c, CGLI_U_, 5) || !have(c, CGLI_U, 5) || !have(c, CGLI_UQ, 5) || !have(c, CGLI_J, 1) ||
+        !have(c, CGLI_MET, 9) || !have(c, CGLI_INFX, 15) || !have(c, CGLI_VF, 4))
+        return CGLI_E_STATE;
+    int r = ensure_range(c, CGLI_MR, 15);
+    if (r) return r;
+    return ensure_range(c, CGLI_RES, 5);
+}
+
+static int finish_norm(cgli_ctx *c, int nb, double er[5])
+{
+    const double DN = 1. / (double)((long)c->cfg.nx_global * c->cfg.ny * c->cfg.nz);
+    double *er_dev = c->scratch + c->scratch_len - 8;
+    launch_finish_norm(nb, c->scratch, DN, 0, er_dev, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->er_host, er_dev, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    bool finite = true;
+    for (int v = 0; v < 5; v++) {
+        er[v] = c->er_host[v];
+        if (!std::isfinite(er[v])) finite = false;
+    }
+    return finite ? CGLI_OK : CGLI_E_NONFINITE;
+}
+
+int cgli_dual_time_stepping(cgli_ctx *c, double deltaT, double e, double er[5], int sweeps, int write_residual)
+{
+    if (!c || !er || sweeps < 1) return CGLI_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    int r = implicit_inputs(c);
+    if (r) return r;
+    r = build_planes(c);
+    if (r) return r;
+    const Geo &g = c->g;
+    const long ncell = (long)g.nx * g.ny * g.nz;
+    const long nb = (ncell + 255) / 256;
+    r = scratch(c, 5 * nb + (long)g.NYY * g.NZZ + 64);
+    if (r) return r;
+    const hipStream_t s = c->stream;
+    launch_zero_increment_ghosts(g, f5(c, CGLI_NL), f5(c, CGLI_NR), s);
+    if (c->cfg.sponge || write_residual)
+        for (int v = 0; v < 5; v++) launch_zero(c->f[CGLI_RES + v], g.N, s);
+    if (c->cfg.sponge)
+        launch_sponge(g, c->p, e, c->cfg.gstart, c->cfg.gend0, c->cfg.nx_global, c->cfg.nx_inlet,
+                      c->cfg.nx_outlet, c5(c, CGLI_U_), c->f[CGLI_J], c->f[CGLI_MET + 4], f5(c, CGLI_RES),
+                      c->scratch, s);
+    tbegin(c, T_RHS);
+    launch_rhs_diag(g, c->p, deltaT, e, c->cfg.sponge, c5(c, CGLI_U_), c5(c, CGLI_U), c5(c, CGLI_UQ),
+                    c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_INFX), c5(c, CGLI_INFY), c5(c, CGLI_INFZ),
+                    C4{{c->f[CGLI_VF], c->f[CGLI_VF + 1], c->f[CGLI_VF + 2], c->f[CGLI_VF + 3]}},
+                    c5(c, CGLI_RES), f5(c, CGLI_MR), f5(c, CGLI_NL), f5(c, CGLI_NR), s);
+    tend(c, T_RHS);
+    const int mlast = g.gend + g.NY + g.NZ;
+    for (int sw = 0; sw < sweeps; sw++) {
+        tbegin(c, T_FWD);
+        for (int m = 7; m <= mlast; m++) {
+            const int cnt = c->Nijk[2 * m + 1];
+            if (!cnt) continue;
+            launch_lusgs_plane(g, c->p, deltaT, e, 1, m, c->Nijk[2 * m] - cnt + 1, cnt, c->ijk_dev,
+                               c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_MR), f5(c, CGLI_NL),
+                               f5(c, CGLI_NR), s);
+        }
+        tend(c, T_FWD);
+        tbegin(c, T_BWD);
+        for (int m = mlast; m >= 7; m--) {
+            const int cnt = c->Nijk[2 * m + 1];
+            if (!cnt) continue;
+            launch_lusgs_plane(g, c->p, deltaT, e, 0, m, c->Nijk[2 * m] - cnt + 1, cnt, c->ijk_dev,
+                               c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_MR), f5(c, CGLI_NL),
+                               f5(c, CGLI_NR), s);
+        }
+        tend(c, T_BWD);
+        launch_copy_interior(g, c5(c, CGLI_NR), f5(c, CGLI_NL), s);
+    }
+    tbegin(c, T_UPDATE);
+    const int nblk = launch_update_norm(g, c->p, f5(c, CGLI_U_), c->f[CGLI_J], c5(c, CGLI_NR), f5(c, CGLI_RES),
+                                        write_residual, 0, c->scratch, s);
+    tend(c, T_UPDATE);
+    HIPCHK(hipGetLastError());
+    mark(c, CGLI_MR, 15);
+    return finish_norm(c, nblk, er);
+}
+
+int cgli_dplusgs(cgli_ctx *c, double deltaT, double er[5], double e, int sweeps)
+{
+    if (!c || !er || sweeps < 1) return CGLI_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    int r = implicit_inputs(c);
+    if (r) return r;
+    const Geo &g = c->g;
+    const long ncell = (long)g.nx * g.ny * g.nz;
+    const long nb = (ncell + 255) / 256;
+    r = scratch(c, 5 * nb + 64);
+    if (r) return r;
+    const hipStream_t s = c->stream;
+    Phys pd = c->p;
+    pd.grav = 0.0;            // DPLUSGS.cpp:417,420 use the constant f of prm.h:11
+    pd.f_const = c->cfg.f_dplusgs;
+    launch_zero_increment_ghosts(g, f5(c, CGLI_NL), f5(c, CGLI_NR), s);
+    launch_rhs_diag(g, pd, deltaT, e, 0, c5(c, CGLI_U_), c5(c, CGLI_U), c5(c, CGLI_UQ), c->f[CGLI_J],
+                    m9(c, CGLI_MET), c5(c, CGLI_INFX), c5(c, CGLI_INFY), c5(c, CGLI_INFZ),
+                    C4{{c->f[CGLI_VF], c->f[CGLI_VF + 1], c->f[CGLI_VF + 2], c->f[CGLI_VF + 3]}},
+                    c5(c, CGLI_RES), f5(c, CGLI_MR), f5(c, CGLI_NL), f5(c, CGLI_NR), s);
+    for (int sw = 0; sw < sweeps; sw++) {
+        launch_dplur_sweep(g, c->p, deltaT, e, c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_MR),
+                           c5(c, CGLI_NL), f5(c, CGLI_NR), s);
+        launch_copy_interior(g, c5(c, CGLI_NR), f5(c, CGLI_NL), s);
+    }
+    for (int v = 0; v < 5; v++) launch_zero(c->f[CGLI_RES + v], g.N, s);
+    const int nblk = launch_update_norm(g, c->p, f5(c, CGLI_U_), c->f[CGLI_J], c5(c, CGLI_NR), f5(c, CGLI_RES),
+                                        1, 1, c->scratch, s);
+    HIPCHK(hipGetLastError());
+    mark(c, CGLI_MR, 15);
+    return finish_norm(c, nblk, er);
+}
+
+int cgli_bc_y(cgli_ctx *c, double T_low, double T_high, int heated_lo, int heated_hi)
+{
+    if (!c) return CGLI_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    if (!have(c, CGLI_U_, 5) || !have(c, CGLI_J, 1)) return CGLI_E_STATE;
+    launch_bc_y(c->g, c->p, c->cfg.gstart, f5(c, CGLI_U_), c->f[CGLI_J], T_low, T_high, heated_lo, heated_hi,
+                c->stream);
+    HIPCHK(hipGetLastError());
+    return CGLI_OK;
+}
+
+int cgli_bc_z(cgli_ctx *c)
+{
+    if (!c) return CGLI_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    if (!have(c, CGLI_U_, 5)) return CGLI_E_STATE;
+    launch_bc_z(c->g, f5(c, CGLI_U_), c->stream);
+    HIPCHK(hipGetLastError());
+    return CGLI_OK;
+}
+
+int cgli_bc_x_periodic(cgli_ctx *c)
+{
+    if (!c) return CGLI_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    if (!have(c, CGLI_U_, 5)) return CGLI_E_STATE;
+    launch_bc_x_periodic(c->g, f5(c, CGLI_U_), c->stream);
+    HIPCHK(hipGetLastError());
+    return CGLI_OK;
+}
+
+int cgli_timing_enable(cgli_ctx *c, int on)
+{
+    if (!c) return CGLI_E_ARG;
+    c->timing = on != 0;
+    return CGLI_OK;
+}
+
+int cgli_timing_reset(cgli_ctx *c)
+{
+    if (!c) return CGLI_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < T_NCLASS; k++) c->ev_used[k] = 0;
+    return CGLI_OK;
+}
+
+int cgli_timing_get(cgli_ctx *c, int cls, double *total_ms, long *count)
+{
+    if (!c || cls < 0 || cls >= T_NCLASS || !total_ms || !count) return CGLI_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double t = 0;
+    for (size_t q = 0; q < c->ev_used[cls]; q++) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev[cls][q].a, c->ev[cls][q].b));
+        t += ms;
+    }
+    *total_ms = t;
+    *count = (long)c->ev_used[cls];
+    return CGLI_OK;
+}
+
+}  // extern "C"

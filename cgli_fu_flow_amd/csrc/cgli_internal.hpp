@@ -1,0 +1,75 @@
+// cgli_internal.hpp -- shared between the kernels and the C-ABI layer.
+//
+// Field layout is the reference's (Array.h:7): (i*Ym + j)*Zm + k, k
+// fastest.  Thread x runs along k so a wavefront reads consecutive doubles.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "cgli.h"
+
+namespace cgli {
+
+// Slab geometry in the reference's index names (Resolution.h:15-29):
+// interior i=3..gend, j=2..NY, k=2..NZ; wall/periodic ghosts j=1,NYY and
+// k=1,NZZ; x ghosts i=0..2 and gend+1..gend+3.
+struct Geo {
+    long Ym, Zm, N;
+    int nx, ny, nz;
+    int gend, NY, NZ, NYY, NZZ;
+    int first, last;
+};
+
+struct Phys {
+    double K, R, Cv, mu_L, Pr_L, suth_T, suth_S;
+    double dxi, deta, dzeta;
+    double rho_ref, grav, f_const, E_ref, high;
+};
+
+struct F5 { double *v[5]; };
+struct C5 { const double *v[5]; };
+struct C4 { const double *v[4]; };
+struct F4 { double *v[4]; };
+struct M9 { const double *m[9]; };
+
+__host__ __device__ inline long ix(const Geo &g, int i, int j, int k)
+{
+    return ((long)i * g.Ym + j) * g.Zm + k;
+}
+
+// launchers (kernels.hip)
+void launch_flux_x(const Geo &g, const Phys &p, double e, C5 U, const double *J, const double *Jf, M9 mf,
+                   F5 F, hipStream_t s);
+void launch_flux_y(const Geo &g, const Phys &p, double Ep, double e, int xi_mode, C5 U, const double *J,
+                   const double *Jf, M9 mf, F5 F, hipStream_t s);
+void launch_flux_z(const Geo &g, const Phys &p, double e, int xi_mode, C5 U, const double *J, const double *Jf,
+                   M9 mf, F5 F, hipStream_t s);
+void launch_viscous(const Geo &g, const Phys &p, C5 U, const double *J, const double *Ju, M9 mu,
+                    const double *Jv, M9 mv, const double *Jw, M9 mw, double *prim4, double *faces12,
+                    F4 vF, hipStream_t s);
+void launch_zero_increment_ghosts(const Geo &g, F5 NL, F5 NR, hipStream_t s);
+void launch_sponge(const Geo &g, const Phys &p, double e, int gstart, int gend0, int nx_global, int nx_inlet,
+                   int nx_outlet, C5 U, const double *J, const double *etdy, F5 Res, double *scratch,
+                   hipStream_t s);
+void launch_rhs_diag(const Geo &g, const Phys &p, double deltaT, double e, int with_res, C5 U_, C5 U, C5 Uq,
+                     const double *J, M9 met, C5 Fx, C5 Fy, C5 Fz, C4 vF, C5 Res, F5 MR, F5 NL, F5 NR,
+                     hipStream_t s);
+void launch_lusgs_plane(const Geo &g, const Phys &p, double deltaT, double e, int forward, int m,
+                        int first_cell, int count, const int *ijk, C5 U_, const double *J, M9 met, C5 MR,
+                        F5 NL, F5 NR, hipStream_t s);
+void launch_dplur_sweep(const Geo &g, const Phys &p, double deltaT, double e, C5 U_, const double *J, M9 met,
+                        C5 MR, C5 NLold, F5 NRnew, hipStream_t s);
+void launch_copy_interior(const Geo &g, C5 src, F5 dst, hipStream_t s);
+// returns the number of partial blocks written to `partials` (5 per block)
+int launch_update_norm(const Geo &g, const Phys &p, F5 U_, const double *J, C5 NR, F5 Res, int write_res,
+                       int dplusgs, double *partials, hipStream_t s);
+void launch_finish_norm(int nblocks, const double *partials, double DN, int squared_twice, double *er_dev,
+                        hipStream_t s);
+void launch_bc_y(const Geo &g, const Phys &p, int gstart, F5 U, const double *J, double T_low,
+                 double T_high, int heated_lo, int heated_hi, hipStream_t s);
+void launch_bc_z(const Geo &g, F5 U, hipStream_t s);
+void launch_bc_x_periodic(const Geo &g, F5 U, hipStream_t s);
+// zero fill by a kernel (safe inside a captured graph, unlike a memset node)
+void launch_zero(double *p, long n, hipStream_t s);
+
+}  // namespace cgli

@@ -1,0 +1,645 @@
+// implicit.hip -- the dual-time implicit update on gfx950 (FP64).
+//
+// Dual_time_stepping.cpp:18-1794 split into device passes:
+//   k_zero_ghosts     NL/NR ghost planes (198-323)
+//   k_sponge_*        absorbing-zone sources (503-664), optional
+//   k_rhs_diag        BDF2 + flux divergence + viscous + body force, Gamma^-1,
+//                     spectral-radius diagonal, NL = NR = D MR (737-908)
+//   k_lusgs_plane     one i+j+k hyperplane of the forward / backward sweep
+//                     (983-1300, 1312-1664)
+//   k_update_norm     primitive update and per-block partial norms (1702-1782)
+//   k_finish_norm     fixed-order sum of the partials, sqrt, scale
+// plus the Jacobi sweep of DPLUSGS.cpp:544-1125.
+#include "cgli_internal.hpp"
+
+namespace cgli {
+
+struct CP { double rho, u, v, w, VV, P, C, T, H, beta; };
+
+// Dual_time_stepping.cpp:759-769
+__device__ __forceinline__ CP cell_prim(const C5 &U, const double *__restrict__ J, long c, double K, double e)
+{
+    CP q;
+    const double U1 = U.v[0][c], Jc = J[c];
+    q.rho = U1 * Jc;
+    q.u = U.v[1][c] / U1;
+    q.v = U.v[2][c] / U1;
+    q.w = U.v[3][c] / U1;
+    q.VV = q.u * q.u + q.v * q.v + q.w * q.w;
+    q.P = (U.v[4][c] * Jc - 0.5 * q.rho * q.VV) * (K - 1);
+    q.C = K * q.P / q.rho;
+    q.T = q.P / q.rho;
+    q.H = 0.5 * q.VV + q.C / (K - 1);
+    const double r = q.VV / q.C;
+    q.beta = (r > e) ? r : e;
+    return q;
+}
+
+// split coefficients of one neighbour along one direction
+// (Dual_time_stepping.cpp:1054-1170); late52: B/C ordering of x52..x54
+struct SC { double p11, p12, p13, p14, p21, p22, p31, p41, p51, p52, p53, p54, m11, m22; };
+
+__device__ __forceinline__ SC split(const CP &q, double mx, double my, double mz, double K, bool late52)
+{
+    SC s;
+    const double b = q.beta;
+    const double Un = mx * q.u + my * q.v + mz * q.w;
+    const double S = sqrt(Un * Un * (b - 1) * (b - 1) + 4 * b * q.C * (mx * mx + my * my + mz * mz));
+    const double lam = ((b + 1) * fabs(Un) + S) / 2;
+    const double d = fabs(lam);
+    s.p11 = 0.5 * (b * Un + d);
+    s.p12 = 0.5 * q.P * K * b * mx;
+    s.p13 = 0.5 * q.P * K * b * my;
+    s.p14 = 0.5 * q.P * K * b * mz;
+    s.p21 = 0.5 * mx / q.rho;
+    s.p22 = 0.5 * (Un + d);   // also x33, x44, x55
+    s.p31 = 0.5 * my / q.rho;
+    s.p41 = 0.5 * mz / q.rho;
+    s.p51 = 0.5 * Un * (K - 1) * (b - 1) / q.rho / K;
+    if (!late52) {
+        s.p52 = 0.5 * mx * b * q.T * (K - 1);
+        s.p53 = 0.5 * my * b * q.T * (K - 1);
+        s.p54 = 0.5 * mz * b * q.T * (K - 1);
+    } else {
+        s.p52 = 0.5 * b * q.T * (K - 1) * mx;
+        s.p53 = 0.5 * b * q.T * (K - 1) * my;
+        s.p54 = 0.5 * b * q.T * (K - 1) * mz;
+    }
+    s.m11 = 0.5 * (b * Un - d);
+    s.m22 = 0.5 * (Un - d);
+    return s;
+}
+
+struct DG { double d11, d22, d51; };
+
+// Dual_time_stepping.cpp:856-895
+__device__ __forceinline__ DG diag(const CP &q, const double m[9], const Phys &p, double deltaT)
+{
+    const double K = p.K, b = q.beta;
+    const double U = m[0] * q.u + m[1] * q.v + m[2] * q.w;
+    const double V = m[3] * q.u + m[4] * q.v + m[5] * q.w;
+    const double W = m[6] * q.u + m[7] * q.v + m[8] * q.w;
+    const double Sx = sqrt(U * U * (b - 1) * (b - 1) + 4 * b * q.C * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2]));
+    const double Sy = sqrt(V * V * (b - 1) * (b - 1) + 4 * b * q.C * (m[3] * m[3] + m[4] * m[4] + m[5] * m[5]));
+    const double Sz = sqrt(W * W * (b - 1) * (b - 1) + 4 * b * q.C * (m[6] * m[6] + m[7] * m[7] + m[8] * m[8]));
+    const double lX = ((b + 1) * fabs(U) + Sx) / 2;
+    const double lY = ((b + 1) * fabs(V) + Sy) / 2;
+    const double lZ = ((b + 1) * fabs(W) + Sz) / 2;
+    const double lambda = lX / p.dxi + lY / p.deta + lZ / p.dzeta;
+    DG D;
+    D.d11 = 2 * deltaT / (3 * b + 2 * lambda * deltaT);
+    D.d22 = 2 * deltaT / (3 + 2 * lambda * deltaT);
+    const double t1 = 6 * (K - 1) * (b - 1) * deltaT;
+    const double t2 = K * (3 + 2 * lambda * deltaT) * (3 * b + 2 * lambda * deltaT) * q.rho;
+    D.d51 = -t1 / t2;
+    return D;
+}
+
+__device__ __forceinline__ void load9(const M9 &M, long c, double m[9])
+{
+#pragma unroll
+    for (int q = 0; q < 9; q++) m[q] = M.m[q][c];
+}
+
+// ------------------------------------------------------------------ ghosts
+__global__ void k_zero_ghosts(Geo g, F5 NL, F5 NR)
+{
+    // every (i,j,k) of i=2..gend+1, j=1..NYY, k=1..NZZ that is a ghost of
+    // the increment arrays: x planes (first/last), j=1/NYY, k=1/NZZ
+    const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = 2 + blockIdx.z;
+    if (k > g.NZZ || j > g.NYY) return;
+    const bool xg = (g.first && i == 2) || (g.last && i == g.gend + 1);
+    const bool yg = (j == 1 || j == g.NYY);
+    const bool zg = (k == 1 || k == g.NZZ);
+    if (!(xg || yg || zg)) return;
+    const long c = ix(g, i, j, k);
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        NL.v[v][c] = 0;
+        NR.v[v][c] = 0;
+    }
+}
+
+void launch_zero_increment_ghosts(const Geo &g, F5 NL, F5 NR, hipStream_t s)
+{
+    dim3 b(64, 4);
+    dim3 gr((g.NZZ + b.x - 1) / b.x, (g.NYY + b.y - 1) / b.y, g.gend);
+    hipLaunchKernelGGL(k_zero_ghosts, gr, b, 0, s, g, NL, NR);
+}
+
+__global__ void k_zero(double *p, long n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0.0;
+}
+
+void launch_zero(double *p, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+}
+
+// ------------------------------------------------------------ reductions
+// Block sums in a fixed order: wave shuffles then a serial pass over the
+// waves, so two runs give identical bits.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&x)[NV], double *out)
+{
+    __shared__ double red[NV][16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        double a = x[v];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+        if (lane == 0) red[v][wid] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            double s = 0;
+            for (int w = 0; w < nw; w++) s += red[v][w];
+            out[v] = s;
+        }
+    }
+}
+
+// ----------------------------------------------------------------- sponge
+// targetU integral over the inlet plane i=3 (Dual_time_stepping.cpp:503-514)
+__global__ void __launch_bounds__(256) k_sponge_target(Geo g, Phys p, C5 U, const double *__restrict__ etdy,
+                                                       double *partials)
+{
+    const int nk = g.NZZ - 1, nj = g.NYY - 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double x[1] = {0.0};
+    if (t < (long)nk * nj) {
+        const int j = 2 + (int)(t / nk), k = 2 + (int)(t % nk);
+        const long c = ix(g, 3, j, k);
+        const double u = U.v[1][c] / U.v[0][c];
+        x[0] = u * p.deta / etdy[c] * p.dzeta;
+    }
+    double o[1];
+    block_sum<1>(x, o);
+    if (threadIdx.x == 0) partials[blockIdx.x] = o[0];
+}
+
+__global__ void k_sponge_finish(int nb, const double *partials, double DZ, double *targetU)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double z = 0.0;
+        for (int b = 0; b < nb; b++) z += partials[b];
+        *targetU = z * DZ;
+    }
+}
+
+struct SpongeArgs {
+    int gstart, gend0, nx_global, nx_inlet, nx_outlet;
+    int in_lo, in_hi, out_lo, out_hi;   // i ranges (empty when lo > hi)
+};
+
+__global__ void __launch_bounds__(256) k_sponge_res(Geo g, Phys p, double e, SpongeArgs a, C5 U,
+                                                   const double *__restrict__ J, const double *targetU_p, F5 Res)
+{
+    const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = 3 + blockIdx.z;
+    if (k > g.NZ || j > g.NY) return;
+    const long c = ix(g, i, j, k);
+    const bool out = (i >= a.out_lo && i <= a.out_hi);
+    const bool in = (i >= a.in_lo && i <= a.in_hi);
+    if (!out && !in) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) Res.v[v][c] = 0;
+        return;
+    }
+    const double K = p.K;
+    const long im = c - g.Ym * g.Zm;
+    const double Jc = J[c];
+    const double rho = U.v[0][c] * Jc;
+    const double u = U.v[1][c] / U.v[0][c];
+    const double v = U.v[2][c] / U.v[0][c];
+    const double w = U.v[3][c] / U.v[0][c];
+    const double VV = u * u + v * v + w * w;
+    const double P = (U.v[4][c] * Jc - 0.5 * rho * VV) * (K - 1);
+    const double C = K * P / rho;
+    const double r = VV / C;
+    const double beta = (r > e) ? r : e;
+    const double C_plan = 0.5 * sqrt(u * u * (beta - 1) * (beta - 1) + 4 * beta * C);
+    double s1, sg, t[5];
+    if (out) {   // the outlet loop runs after the inlet loop and overwrites it
+        const double targetU = *targetU_p;
+        const int nxr = a.nx_global + 1;
+        s1 = pow((((i + a.gstart - 0.5 - (nxr - a.nx_outlet)) * 1.0) / a.nx_outlet), 3.0) * 1.0 * C_plan;
+        sg = 1.25 * s1 / p.high;
+        t[0] = rho - p.rho_ref;
+        t[1] = rho * u - p.rho_ref * targetU;
+        t[2] = rho * v;
+        t[3] = rho * w;
+        t[4] = U.v[4][c] * Jc - p.E_ref - 0.5 * p.rho_ref * targetU * targetU;
+    } else {
+        s1 = pow((((a.nx_inlet - a.gstart - 0.5) * 1.0) / a.nx_inlet), 3.0) * 1.0 * C_plan;
+        sg = 0.035 * s1 / p.high;
+        t[0] = rho - p.rho_ref;
+        t[1] = rho * u;
+        t[2] = rho * v;
+        t[3] = rho * w;
+        t[4] = U.v[4][c] * Jc - p.E_ref;
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) Res.v[q][c] = -s1 * (U.v[q][c] * Jc - U.v[q][im] * J[im]) / p.dxi - sg * t[q];
+}
+
+void launch_sponge(const Geo &g, const Phys &p, double e, int gstart, int gend0, int nx_global, int nx_inlet,
+                   int nx_outlet, C5 U, const double *J, const double *etdy, F5 Res, double *scratch,
+                   hipStream_t s)
+{
+    const long n = (long)(g.NZZ - 1) * (g.NYY - 1);
+    const int nb = (int)((n + 255) / 256);
+    double *targetU = scratch;
+    double *partials = scratch + 1;
+    if (g.first) {
+        hipLaunchKernelGGL(k_sponge_target, dim3(nb), dim3(256), 0, s, g, p, U, etdy, partials);
+        const double DZ = 1. / (p.high * p.dzeta * g.nz);
+        hipLaunchKernelGGL(k_sponge_finish, dim3(1), dim3(64), 0, s, nb, partials, DZ, targetU);
+    } else {
+        launch_zero(targetU, 1, s);
+    }
+    SpongeArgs a{gstart, gend0, nx_global, nx_inlet, nx_outlet, 1, 0, 1, 0};
+    if (gstart < nx_inlet) {
+        a.in_lo = 3;
+        a.in_hi = (gend0 < nx_inlet) ? g.gend : nx_inlet - gstart + 2;
+    }
+    if (gend0 > (nx_global - nx_outlet - 1)) {
+        a.out_lo = (gstart > (nx_global - nx_outlet - 1)) ? 3 : (nx_global - nx_outlet) - gstart + 2;
+        a.out_hi = g.gend;
+    }
+    dim3 b(64, 4);
+    dim3 gr((g.NZ - 1 + b.x - 1) / b.x, (g.NY - 1 + b.y - 1) / b.y, g.nx);
+    hipLaunchKernelGGL(k_sponge_res, gr, b, 0, s, g, p, e, a, U, J, targetU, Res);
+}
+
+// --------------------------------------------------------------- RHS+diag
+__global__ void __launch_bounds__(256) k_rhs_diag(Geo g, Phys p, double deltaT, double e, int with_res, C5 U_,
+                                                  C5 U, C5 Uq, const double *__restrict__ J, M9 met, C5 Fx,
+                                                  C5 Fy, C5 Fz, C4 vF, C5 Res, F5 MR, F5 NL, F5 NR)
+{
+    const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = 3 + blockIdx.z;
+    if (k > g.NZ || j > g.NY) return;
+    const double K = p.K;
+    const long c = ix(g, i, j, k);
+    const long sx = g.Ym * g.Zm, sy = g.Zm;
+    const CP q = cell_prim(U_, J, c, K, e);
+    const double rho = q.rho, u = q.u, v = q.v, w = q.w, VV = q.VV, T = q.T, H = q.H, beta = q.beta;
+    const double temp = rho * K;
+    double d[5][5];
+    d[0][0] = -(H * (K - 1) + VV - K * (T + VV)) * beta;
+    d[0][1] = -(K - 1) * u * beta;
+    d[0][2] = -(K - 1) * v * beta;
+    d[0][3] = -(K - 1) * w * beta;
+    d[0][4] = (K - 1) * beta;
+    d[1][0] = -u / rho; d[1][1] = 1 / rho; d[1][2] = 0; d[1][3] = 0; d[1][4] = 0;
+    d[2][0] = -v / rho; d[2][1] = 0; d[2][2] = 1 / rho; d[2][3] = 0; d[2][4] = 0;
+    d[3][0] = -w / rho; d[3][1] = 0; d[3][2] = 0; d[3][3] = 1 / rho; d[3][4] = 0;
+    d[4][0] = (K - 1) * (VV + K * T * beta + (K - 1) * VV * beta + H * (-1 + beta - K * beta)) / temp;
+    d[4][1] = -(K - 1) * u * (1 + (K - 1) * beta) / temp;
+    d[4][2] = -(K - 1) * v * (1 + (K - 1) * beta) / temp;
+    d[4][3] = -(K - 1) * w * (1 + (K - 1) * beta) / temp;
+    d[4][4] = (K - 1) * (1 + (K - 1) * beta) / temp;
+    // flux face indices: x faces of cell (i,j,k) at [i][j-1][k-1] and
+    // [i-1][j-1][k-1]; y at [i-1][j][k-1] and [i-1][j-1][k-1]; z at
+    // [i-1][j-1][k] and [i-1][j-1][k-1] (Dual_time_stepping.cpp:810-828)
+    const long f0 = c - sx - sy - 1;
+    const double f = -(rho - p.rho_ref) * p.grav + p.f_const;
+    const double Jc = J[c];
+    double Rk[5];
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+        const double Rp = -(3 * U_.v[n][c] - 4 * U.v[n][c] + Uq.v[n][c]) / (2 * deltaT);
+        const double Rf = -((Fx.v[n][f0 + sx] - Fx.v[n][f0]) / p.dxi + (Fy.v[n][f0 + sy] - Fy.v[n][f0]) / p.deta +
+                            (Fz.v[n][f0 + 1] - Fz.v[n][f0]) / p.dzeta);
+        double r = with_res ? Res.v[n][c] / Jc + Rp + Rf : Rp + Rf;
+        if (n >= 1) r = r + vF.v[n - 1][c];
+        if (n == 1) r = r + f / Jc;
+        if (n == 4) r = r + f * u / Jc;
+        Rk[n] = r;
+    }
+    double mr[5];
+#pragma unroll
+    for (int a = 0; a < 5; a++)
+        mr[a] = (d[a][0] * Rk[0] + d[a][1] * Rk[1] + d[a][2] * Rk[2] + d[a][3] * Rk[3] + d[a][4] * Rk[4]);
+    double m[9];
+    load9(met, c, m);
+    const DG D = diag(q, m, p, deltaT);
+    double nl[5];
+    nl[0] = D.d11 * mr[0];
+    nl[1] = D.d22 * mr[1];
+    nl[2] = D.d22 * mr[2];
+    nl[3] = D.d22 * mr[3];
+    nl[4] = D.d51 * mr[0] + D.d22 * mr[4];
+#pragma unroll
+    for (int a = 0; a < 5; a++) {
+        MR.v[a][c] = mr[a];
+        NL.v[a][c] = nl[a];
+        NR.v[a][c] = nl[a];
+    }
+}
+
+void launch_rhs_diag(const Geo &g, const Phys &p, double deltaT, double e, int with_res, C5 U_, C5 U, C5 Uq,
+                     const double *J, M9 met, C5 Fx, C5 Fy, C5 Fz, C4 vF, C5 Res, F5 MR, F5 NL, F5 NR,
+                     hipStream_t s)
+{
+    dim3 b(64, 4);
+    dim3 gr((g.NZ - 1 + b.x - 1) / b.x, (g.NY - 1 + b.y - 1) / b.y, g.nx);
+    hipLaunchKernelGGL(k_rhs_diag, gr, b, 0, s, g, p, deltaT, e, with_res, U_, U, Uq, J, met, Fx, Fy, Fz, vF,
+                       Res, MR, NL, NR);
+}
+
+// ------------------------------------------------------------- LU-SGS
+__device__ __forceinline__ SC nb_split(const C5 &U_, const double *J, const M9 &met, long c, int dir,
+                                       double K, double e)
+{
+    const CP q = cell_prim(U_, J, c, K, e);
+    return split(q, met.m[3 * dir][c], met.m[3 * dir + 1][c], met.m[3 * dir + 2][c], K, dir != 0);
+}
+
+// forward: NL = D (MR - L NL), L from the lower neighbours (1172-1293)
+__device__ __forceinline__ void fwd_cell(const Geo &g, const Phys &p, double deltaT, double e, long c,
+                                         const C5 &U_, const double *J, const M9 &met, const C5 &MR, F5 &NL)
+{
+    const double K = p.K;
+    const long ia = c - g.Ym * g.Zm, jb = c - g.Zm, kc = c - 1;
+    const SC A = nb_split(U_, J, met, ia, 0, K, e);
+    const SC B = nb_split(U_, J, met, jb, 1, K, e);
+    const SC Cc = nb_split(U_, J, met, kc, 2, K, e);
+    const double *X0 = NL.v[0], *X1 = NL.v[1], *X2 = NL.v[2], *X3 = NL.v[3], *X4 = NL.v[4];
+    const double LL0 = (-A.p11 * X0[ia] - A.p12 * X1[ia] - A.p13 * X2[ia] - A.p14 * X3[ia]) / p.dxi +
+                       (-B.p11 * X0[jb] - B.p12 * X1[jb] - B.p13 * X2[jb] - B.p14 * X3[jb]) / p.deta +
+                       (-Cc.p11 * X0[kc] - Cc.p12 * X1[kc] - Cc.p13 * X2[kc] - Cc.p14 * X3[kc]) / p.dzeta;
+    const double LL1 = (-A.p21 * X0[ia] - A.p22 * X1[ia]) / p.dxi + (-B.p21 * X0[jb] - B.p22 * X1[jb]) / p.deta +
+                       (-Cc.p21 * X0[kc] - Cc.p22 * X1[kc]) / p.dzeta;
+    const double LL2 = (-A.p31 * X0[ia] - A.p22 * X2[ia]) / p.dxi + (-B.p31 * X0[jb] - B.p22 * X2[jb]) / p.deta +
+                       (-Cc.p31 * X0[kc] - Cc.p22 * X2[kc]) / p.dzeta;
+    const double LL3 = (-A.p41 * X0[ia] - A.p22 * X3[ia]) / p.dxi + (-B.p41 * X0[jb] - B.p22 * X3[jb]) / p.deta +
+                       (-Cc.p41 * X0[kc] - Cc.p22 * X3[kc]) / p.dzeta;
+    const double LL4 =
+        (-A.p51 * X0[ia] - A.p52 * X1[ia] - A.p53 * X2[ia] - A.p54 * X3[ia] - A.p22 * X4[ia]) / p.dxi +
+        (-B.p51 * X0[jb] - B.p52 * X1[jb] - B.p53 * X2[jb] - B.p54 * X3[jb] - B.p22 * X4[jb]) / p.deta +
+        (-Cc.p51 * X0[kc] - Cc.p52 * X1[kc] - Cc.p53 * X2[kc] - Cc.p54 * X3[kc] - Cc.p22 * X4[kc]) / p.dzeta;
+    const CP q = cell_prim(U_, J, c, K, e);
+    double m[9];
+    load9(met, c, m);
+    const DG D = diag(q, m, p, deltaT);
+    NL.v[0][c] = D.d11 * (MR.v[0][c] - LL0);
+    NL.v[1][c] = D.d22 * (MR.v[1][c] - LL1);
+    NL.v[2][c] = D.d22 * (MR.v[2][c] - LL2);
+    NL.v[3][c] = D.d22 * (MR.v[3][c] - LL3);
+    NL.v[4][c] = D.d51 * (MR.v[0][c] - LL0) + D.d22 * (MR.v[4][c] - LL4);
+}
+
+// backward: NR = NL - D U NR, U from the upper neighbours (1355-1655)
+__device__ __forceinline__ void bwd_cell(const Geo &g, const Phys &p, double deltaT, double e, long c,
+                                         const C5 &U_, const double *J, const M9 &met, const F5 &NL, F5 &NR)
+{
+    const double K = p.K;
+    const long ia = c + g.Ym * g.Zm, jb = c + g.Zm, kc = c + 1;
+    const SC A = nb_split(U_, J, met, ia, 0, K, e);
+    const SC B = nb_split(U_, J, met, jb, 1, K, e);
+    const SC Cc = nb_split(U_, J, met, kc, 2, K, e);
+    const double *X0 = NR.v[0], *X1 = NR.v[1], *X2 = NR.v[2], *X3 = NR.v[3], *X4 = NR.v[4];
+    const double UU0 = (+A.m11 * X0[ia] + A.p12 * X1[ia] + A.p13 * X2[ia] + A.p14 * X3[ia]) / p.dxi +
+                       (+B.m11 * X0[jb] + B.p12 * X1[jb] + B.p13 * X2[jb] + B.p14 * X3[jb]) / p.deta +
+                       (+Cc.m11 * X0[kc] + Cc.p12 * X1[kc] + Cc.p13 * X2[kc] + Cc.p14 * X3[kc]) / p.dzeta;
+    const double UU1 = (+A.p21 * X0[ia] + A.m22 * X1[ia]) / p.dxi + (+B.p21 * X0[jb] + B.m22 * X1[jb]) / p.deta +
+                       (+Cc.p21 * X0[kc] + Cc.m22 * X1[kc]) / p.dzeta;
+    const double UU2 = (+A.p31 * X0[ia] + A.m22 * X2[ia]) / p.dxi + (+B.p31 * X0[jb] + B.m22 * X2[jb]) / p.deta +
+                       (+Cc.p31 * X0[kc] + Cc.m22 * X2[kc]) / p.dzeta;
+    const double UU3 = (+A.p41 * X0[ia] + A.m22 * X3[ia]) / p.dxi + (+B.p41 * X0[jb] + B.m22 * X3[jb]) / p.deta +
+                       (+Cc.p41 * X0[kc] + Cc.m22 * X3[kc]) / p.dzeta;
+    const double UU4 =
+        (+A.p51 * X0[ia] + A.p52 * X1[ia] + A.p53 * X2[ia] + A.p54 * X3[ia] + A.m22 * X4[ia]) / p.dxi +
+        (+B.p51 * X0[jb] + B.p52 * X1[jb] + B.p53 * X2[jb] + B.p54 * X3[jb] + B.m22 * X4[jb]) / p.deta +
+        (+Cc.p51 * X0[kc] + Cc.p52 * X1[kc] + Cc.p53 * X2[kc] + Cc.p54 * X3[kc] + Cc.m22 * X4[kc]) / p.dzeta;
+    const CP q = cell_prim(U_, J, c, K, e);
+    double m[9];
+    load9(met, c, m);
+    const DG D = diag(q, m, p, deltaT);
+    NR.v[0][c] = NL.v[0][c] - D.d11 * UU0;
+    NR.v[1][c] = NL.v[1][c] - D.d22 * UU1;
+    NR.v[2][c] = NL.v[2][c] - D.d22 * UU2;
+    NR.v[3][c] = NL.v[3][c] - D.d22 * UU3;
+    NR.v[4][c] = NL.v[4][c] - (D.d51 * UU0 + D.d22 * UU4);
+}
+
+__global__ void __launch_bounds__(256) k_lusgs_plane(Geo g, Phys p, double deltaT, double e, int forward,
+                                                     int first_cell, int count, const int *__restrict__ ijk,
+                                                     C5 U_, const double *__restrict__ J, M9 met, C5 MR, F5 NL,
+                                                     F5 NR)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int mi = first_cell + t;
+    const int i = ijk[3 * mi], j = ijk[3 * mi + 1], k = ijk[3 * mi + 2];
+    const long c = ix(g, i, j, k);
+    if (forward) fwd_cell(g, p, deltaT, e, c, U_, J, met, MR, NL);
+    else bwd_cell(g, p, deltaT, e, c, U_, J, met, NL, NR);
+}
+
+void launch_lusgs_plane(const Geo &g, const Phys &p, double deltaT, double e, int forward, int m, int first_cell,
+                        int count, const int *ijk, C5 U_, const double *J, M9 met, C5 MR, F5 NL, F5 NR,
+                        hipStream_t s)
+{
+    (void)m;
+    const int b = 128;
+    hipLaunchKernelGGL(k_lusgs_plane, dim3((count + b - 1) / b), dim3(b), 0, s, g, p, deltaT, e, forward,
+                       first_cell, count, ijk, U_, J, met, MR, NL, NR);
+}
+
+// ------------------------------------------------------------ DP-LUR
+// NRnew = D (MR - L NLold - U NLold) (DPLUSGS.cpp:575-1125), using the
+// evident neighbour j+1 / k+1 for the upper part (the reference reads
+// never-assigned j_ / k_ there)
+__global__ void __launch_bounds__(256) k_dplur(Geo g, Phys p, double deltaT, double e, C5 U_,
+                                               const double *__restrict__ J, M9 met, C5 MR, C5 NLo, F5 NRn)
+{
+    const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = 3 + blockIdx.z;
+    if (k > g.NZ || j > g.NY) return;
+    const double K = p.K;
+    const long c = ix(g, i, j, k);
+    const long sx = g.Ym * g.Zm, sy = g.Zm;
+    const double *X0 = NLo.v[0], *X1 = NLo.v[1], *X2 = NLo.v[2], *X3 = NLo.v[3], *X4 = NLo.v[4];
+    double LL[5], UU[5];
+    {
+        const long ia = c - sx, jb = c - sy, kc = c - 1;
+        const SC A = nb_split(U_, J, met, ia, 0, K, e);
+        const SC B = nb_split(U_, J, met, jb, 1, K, e);
+        const SC Cc = nb_split(U_, J, met, kc, 2, K, e);
+        LL[0] = (-A.p11 * X0[ia] - A.p12 * X1[ia] - A.p13 * X2[ia] - A.p14 * X3[ia]) / p.dxi +
+                (-B.p11 * X0[jb] - B.p12 * X1[jb] - B.p13 * X2[jb] - B.p14 * X3[jb]) / p.deta +
+                (-Cc.p11 * X0[kc] - Cc.p12 * X1[kc] - Cc.p13 * X2[kc] - Cc.p14 * X3[kc]) / p.dzeta;
+        LL[1] = (-A.p21 * X0[ia] - A.p22 * X1[ia]) / p.dxi + (-B.p21 * X0[jb] - B.p22 * X1[jb]) / p.deta +
+                (-Cc.p21 * X0[kc] - Cc.p22 * X1[kc]) / p.dzeta;
+        LL[2] = (-A.p31 * X0[ia] - A.p22 * X2[ia]) / p.dxi + (-B.p31 * X0[jb] - B.p22 * X2[jb]) / p.deta +
+                (-Cc.p31 * X0[kc] - Cc.p22 * X2[kc]) / p.dzeta;
+        LL[3] = (-A.p41 * X0[ia] - A.p22 * X3[ia]) / p.dxi + (-B.p41 * X0[jb] - B.p22 * X3[jb]) / p.deta +
+                (-Cc.p41 * X0[kc] - Cc.p22 * X3[kc]) / p.dzeta;
+        LL[4] = (-A.p51 * X0[ia] - A.p52 * X1[ia] - A.p53 * X2[ia] - A.p54 * X3[ia] - A.p22 * X4[ia]) / p.dxi +
+                (-B.p51 * X0[jb] - B.p52 * X1[jb] - B.p53 * X2[jb] - B.p54 * X3[jb] - B.p22 * X4[jb]) / p.deta +
+                (-Cc.p51 * X0[kc] - Cc.p52 * X1[kc] - Cc.p53 * X2[kc] - Cc.p54 * X3[kc] - Cc.p22 * X4[kc]) /
+                    p.dzeta;
+    }
+    {
+        const long ia = c + sx, jb = c + sy, kc = c + 1;
+        const SC A = nb_split(U_, J, met, ia, 0, K, e);
+        const SC B = nb_split(U_, J, met, jb, 1, K, e);
+        const SC Cc = nb_split(U_, J, met, kc, 2, K, e);
+        UU[0] = (+A.m11 * X0[ia] + A.p12 * X1[ia] + A.p13 * X2[ia] + A.p14 * X3[ia]) / p.dxi +
+                (+B.m11 * X0[jb] + B.p12 * X1[jb] + B.p13 * X2[jb] + B.p14 * X3[jb]) / p.deta +
+                (+Cc.m11 * X0[kc] + Cc.p12 * X1[kc] + Cc.p13 * X2[kc] + Cc.p14 * X3[kc]) / p.dzeta;
+        UU[1] = (+A.p21 * X0[ia] + A.m22 * X1[ia]) / p.dxi + (+B.p21 * X0[jb] + B.m22 * X1[jb]) / p.deta +
+                (+Cc.p21 * X0[kc] + Cc.m22 * X1[kc]) / p.dzeta;
+        UU[2] = (+A.p31 * X0[ia] + A.m22 * X2[ia]) / p.dxi + (+B.p31 * X0[jb] + B.m22 * X2[jb]) / p.deta +
+                (+Cc.p31 * X0[kc] + Cc.m22 * X2[kc]) / p.dzeta;
+        UU[3] = (+A.p41 * X0[ia] + A.m22 * X3[ia]) / p.dxi + (+B.p41 * X0[jb] + B.m22 * X3[jb]) / p.deta +
+                (+Cc.p41 * X0[kc] + Cc.m22 * X3[kc]) / p.dzeta;
+        UU[4] = (+A.p51 * X0[ia] + A.p52 * X1[ia] + A.p53 * X2[ia] + A.p54 * X3[ia] + A.m22 * X4[ia]) / p.dxi +
+                (+B.p51 * X0[jb] + B.p52 * X1[jb] + B.p53 * X2[jb] + B.p54 * X3[jb] + B.m22 * X4[jb]) / p.deta +
+                (+Cc.p51 * X0[kc] + Cc.p52 * X1[kc] + Cc.p53 * X2[kc] + Cc.p54 * X3[kc] + Cc.m22 * X4[kc]) /
+                    p.dzeta;
+    }
+    const CP q = cell_prim(U_, J, c, K, e);
+    double m[9];
+    load9(met, c, m);
+    const DG D = diag(q, m, p, deltaT);
+    NRn.v[0][c] = D.d11 * (MR.v[0][c] - LL[0] - UU[0]);
+    NRn.v[1][c] = D.d22 * (MR.v[1][c] - LL[1] - UU[1]);
+    NRn.v[2][c] = D.d22 * (MR.v[2][c] - LL[2] - UU[2]);
+    NRn.v[3][c] = D.d22 * (MR.v[3][c] - LL[3] - UU[3]);
+    NRn.v[4][c] = D.d51 * (MR.v[0][c] - LL[0] - UU[0]) + D.d22 * (MR.v[4][c] - LL[4] - UU[4]);
+}
+
+void launch_dplur_sweep(const Geo &g, const Phys &p, double deltaT, double e, C5 U_, const double *J, M9 met,
+                        C5 MR, C5 NLold, F5 NRnew, hipStream_t s)
+{
+    dim3 b(64, 4);
+    dim3 gr((g.NZ - 1 + b.x - 1) / b.x, (g.NY - 1 + b.y - 1) / b.y, g.nx);
+    hipLaunchKernelGGL(k_dplur, gr, b, 0, s, g, p, deltaT, e, U_, J, met, MR, NLold, NRnew);
+}
+
+__global__ void k_copy_interior(Geo g, C5 src, F5 dst)
+{
+    const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = 3 + blockIdx.z;
+    if (k > g.NZ || j > g.NY) return;
+    const long c = ix(g, i, j, k);
+#pragma unroll
+    for (int v = 0; v < 5; v++) dst.v[v][c] = src.v[v][c];
+}
+
+void launch_copy_interior(const Geo &g, C5 src, F5 dst, hipStream_t s)
+{
+    dim3 b(64, 4);
+    dim3 gr((g.NZ - 1 + b.x - 1) / b.x, (g.NY - 1 + b.y - 1) / b.y, g.nx);
+    hipLaunchKernelGGL(k_copy_interior, gr, b, 0, s, g, src, dst);
+}
+
+// ------------------------------------------------------ update + norm
+// Dual_time_stepping.cpp:1702-1743 (DPLUSGS.cpp:1163-1202 measures the
+// pressure increment and squares again).  Thread t covers interior cell t
+// in (i, j, k) order; each block writes 5 partial sums.
+__global__ void __launch_bounds__(256) k_update_norm(Geo g, Phys p, F5 U_, const double *__restrict__ J, C5 NR,
+                                                     F5 Res, int write_res, int dplusgs, double *partials)
+{
+    const long ncell = (long)g.nx * g.ny * g.nz;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double x[5] = {0, 0, 0, 0, 0};
+    if (t < ncell) {
+        const int k = 2 + (int)(t % g.nz);
+        const int j = 2 + (int)((t / g.nz) % g.ny);
+        const int i = 3 + (int)(t / ((long)g.nz * g.ny));
+        const long c = ix(g, i, j, k);
+        const double K = p.K;
+        const double Jc = J[c];
+        const double U1 = U_.v[0][c];
+        const double rhoold = U1 * Jc;
+        const double Uold = U_.v[1][c] / U1;
+        const double Vold = U_.v[2][c] / U1;
+        const double Wold = U_.v[3][c] / U1;
+        const double VVold = Uold * Uold + Vold * Vold + Wold * Wold;
+        const double Pold = (U_.v[4][c] * Jc - 0.5 * rhoold * VVold) * (K - 1);
+        const double Told = Pold / rhoold;
+        const double P = Pold + NR.v[0][c] * Jc;
+        const double Uu = Uold + NR.v[1][c] * Jc;
+        const double V = Vold + NR.v[2][c] * Jc;
+        const double W = Wold + NR.v[3][c] * Jc;
+        const double T = Told + NR.v[4][c] * Jc;
+        const double rho = P / T;
+        U_.v[0][c] = rho / Jc;
+        U_.v[1][c] = rho * Uu / Jc;
+        U_.v[2][c] = rho * V / Jc;
+        U_.v[3][c] = rho * W / Jc;
+        U_.v[4][c] = (P / (K - 1) + 0.5 * rho * (Uu * Uu + V * V + W * W)) / Jc;
+        x[0] = dplusgs ? (P - Pold) * (P - Pold) : (rho - rhoold) * (rho - rhoold);
+        x[1] = (Uu - Uold) * (Uu - Uold);
+        x[2] = (V - Vold) * (V - Vold);
+        x[3] = (W - Wold) * (W - Wold);
+        x[4] = (T - Told) * (T - Told);
+        if (write_res) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) Res.v[v][c] = x[v];
+        }
+        if (dplusgs) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) x[v] = x[v] * x[v];
+        }
+    }
+    double o[5];
+    block_sum<5>(x, o);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int v = 0; v < 5; v++) partials[5 * (long)blockIdx.x + v] = o[v];
+}
+
+int launch_update_norm(const Geo &g, const Phys &p, F5 U_, const double *J, C5 NR, F5 Res, int write_res,
+                       int dplusgs, double *partials, hipStream_t s)
+{
+    const long ncell = (long)g.nx * g.ny * g.nz;
+    const int nb = (int)((ncell + 255) / 256);
+    hipLaunchKernelGGL(k_update_norm, dim3(nb), dim3(256), 0, s, g, p, U_, J, NR, Res, write_res, dplusgs,
+                       partials);
+    return nb;
+}
+
+// one block: each thread sums a contiguous run of partials in order, then
+// the block combines the runs in thread order
+__global__ void __launch_bounds__(256) k_finish_norm(int nb, const double *__restrict__ partials, double DN,
+                                                     double *er)
+{
+    double x[5] = {0, 0, 0, 0, 0};
+    const int per = (nb + blockDim.x - 1) / blockDim.x;
+    const int b0 = threadIdx.x * per;
+    for (int b = b0; b < b0 + per && b < nb; b++)
+#pragma unroll
+        for (int v = 0; v < 5; v++) x[v] += partials[5 * (long)b + v];
+    double o[5];
+    block_sum<5>(x, o);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int v = 0; v < 5; v++) er[v] = sqrt(o[v]) * DN;
+}
+
+void launch_finish_norm(int nblocks, const double *partials, double DN, int squared_twice, double *er_dev,
+                        hipStream_t s)
+{
+    (void)squared_twice;
+    hipLaunchKernelGGL(k_finish_norm, dim3(1), dim3(256), 0, s, nblocks, partials, DN, er_dev);
+}
+
+}  // namespace cgli

@@ -1,0 +1,198 @@
+"""ctypes binding of libcgli.so (include/cgli.h).
+
+The HIP library is the only compute path: if it is missing or no device is
+present, the calls raise.  There is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libcgli.so")
+
+# field ids (include/cgli.h)
+U_, U, UQ, J = 0, 5, 10, 15
+MET, J_U, MET_U, J_V, MET_V, J_W, MET_W = 16, 25, 26, 35, 36, 45, 46
+INFX, INFY, INFZ, VF, MR, NL, NR, RES = 55, 60, 65, 70, 74, 79, 84, 89
+EPX, EPY, EPZ, YPOINT, NFIELDS = 94, 95, 96, 97, 98
+
+METRIC_NAMES = ["xidx", "xidy", "xidz", "etdx", "etdy", "etdz", "ztdx", "ztdy", "ztdz"]
+
+ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "device allocation failed",
+          -4: "non-finite residual norm", -5: "a required input field was never set", -6: "no HIP device",
+          -7: "bounded device wait expired"}
+
+
+class CgliError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__("%s: %s (%d)" % (what, ERRORS.get(code, "unknown"), code))
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [(n, C.c_int) for n in (
+        "nx", "ny", "nz", "first", "last", "gstart", "gend0", "nx_global",
+        "nx_inlet", "nx_outlet", "sponge", "xi_mode")] + [(n, C.c_double) for n in (
+        "dxi", "deta", "dzeta", "high", "K", "R", "Cv", "mu_L", "Pr_L", "suth_T",
+        "suth_S", "rho_ref", "grav", "f_const", "f_dplusgs", "E_ref")]
+
+
+_lib = None
+
+
+def lib():
+    """Load libcgli.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libcgli.so is not built (run __graft_entry__.build() or "
+                              "python -m cgli_fu_flow_amd.build): %s" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        L.cgli_config_reference.argtypes = [C.POINTER(Config), C.c_int, C.c_int, C.c_int]
+        L.cgli_config_reference.restype = None
+        L.cgli_create.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(vp)]
+        L.cgli_destroy.argtypes = [vp]
+        L.cgli_error_string.restype = C.c_char_p
+        L.cgli_set_field.argtypes = [vp, C.c_int, dp]
+        L.cgli_get_field.argtypes = [vp, C.c_int, dp]
+        L.cgli_field_device_ptr.argtypes = [vp, C.c_int, C.POINTER(vp)]
+        L.cgli_synchronize.argtypes = [vp]
+        for n in ("cgli_flux_x", "cgli_flux_y", "cgli_flux_z"):
+            getattr(L, n).argtypes = [vp, C.c_double, C.c_double]
+        L.cgli_viscous.argtypes = [vp]
+        L.cgli_hyperplane.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.cgli_dual_time_stepping.argtypes = [vp, C.c_double, C.c_double, dp, C.c_int, C.c_int]
+        L.cgli_dplusgs.argtypes = [vp, C.c_double, dp, C.c_double, C.c_int]
+        L.cgli_bc_y.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int]
+        L.cgli_bc_z.argtypes = [vp]
+        L.cgli_bc_x_periodic.argtypes = [vp]
+        L.cgli_timing_enable.argtypes = [vp, C.c_int]
+        L.cgli_timing_reset.argtypes = [vp]
+        L.cgli_timing_get.argtypes = [vp, C.c_int, dp, C.POINTER(C.c_long)]
+        _lib = L
+    return _lib
+
+
+def reference_config(nx, ny, nz):
+    c = Config()
+    lib().cgli_config_reference(C.byref(c), nx, ny, nz)
+    return c
+
+
+def _check(code, what):
+    if code != 0:
+        raise CgliError(code, what)
+
+
+class Context:
+    """One slab resident on one device (cgli_ctx)."""
+
+    def __init__(self, cfg, device=0):
+        self.cfg = cfg
+        self.shape = (cfg.nx + 6, cfg.ny + 4, cfg.nz + 4)
+        self.n = int(np.prod(self.shape))
+        h = C.c_void_p()
+        _check(lib().cgli_create(C.byref(cfg), device, C.byref(h)), "cgli_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().cgli_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- fields
+    def set(self, fid, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.size != self.n:
+            raise ValueError("field %d: expected %d values, got %d" % (fid, self.n, a.size))
+        _check(lib().cgli_set_field(self.h, fid, a.ctypes.data_as(C.POINTER(C.c_double))), "set_field")
+
+    def get(self, fid):
+        a = np.empty(self.shape)
+        _check(lib().cgli_get_field(self.h, fid, a.ctypes.data_as(C.POINTER(C.c_double))), "get_field")
+        return a
+
+    def set_many(self, fid, arrs):
+        for q, a in enumerate(arrs):
+            self.set(fid + q, a)
+
+    def get_many(self, fid, count):
+        return [self.get(fid + q) for q in range(count)]
+
+    def sync(self):
+        _check(lib().cgli_synchronize(self.h), "synchronize")
+
+    # -- hot path
+    def flux_x(self, Ep, e):
+        _check(lib().cgli_flux_x(self.h, Ep, e), "flux_x")
+
+    def flux_y(self, Ep, e):
+        _check(lib().cgli_flux_y(self.h, Ep, e), "flux_y")
+
+    def flux_z(self, Ep, e):
+        _check(lib().cgli_flux_z(self.h, Ep, e), "flux_z")
+
+    def viscous(self):
+        _check(lib().cgli_viscous(self.h), "viscous")
+
+    def dual_time_stepping(self, deltaT, e, sweeps=1, write_residual=False):
+        er = (C.c_double * 5)()
+        _check(lib().cgli_dual_time_stepping(self.h, deltaT, e, er, sweeps, int(write_residual)),
+               "dual_time_stepping")
+        return np.array(list(er))
+
+    def dplusgs(self, deltaT, e, sweeps=4):
+        er = (C.c_double * 5)()
+        _check(lib().cgli_dplusgs(self.h, deltaT, er, e, sweeps), "dplusgs")
+        return np.array(list(er))
+
+    def hyperplane(self):
+        c = self.cfg
+        rows = c.nx + 2 + c.ny + 1 + c.nz + 1 + 8
+        cells = c.nx * c.ny * c.nz
+        N = np.zeros((rows, 2), np.int32)
+        I = np.zeros((cells + 1, 3), np.int32)
+        _check(lib().cgli_hyperplane(self.h, N.ctypes.data_as(C.POINTER(C.c_int)),
+                                     I.ctypes.data_as(C.POINTER(C.c_int))), "hyperplane")
+        return N, I
+
+    def bc_y(self, T_low, T_high, heated_lo=-(2 ** 31), heated_hi=2 ** 31 - 1):
+        _check(lib().cgli_bc_y(self.h, T_low, T_high, heated_lo, heated_hi), "bc_y")
+
+    def bc_z(self):
+        _check(lib().cgli_bc_z(self.h), "bc_z")
+
+    def bc_x_periodic(self):
+        _check(lib().cgli_bc_x_periodic(self.h), "bc_x_periodic")
+
+    # -- timing (HIP events around each launch class)
+    TIMING_CLASSES = ["flux_x", "flux_y", "flux_z", "viscous", "rhs_diag", "lusgs_fwd", "lusgs_bwd",
+                      "update_norm"]
+
+    def timing(self, on=True):
+        _check(lib().cgli_timing_enable(self.h, int(on)), "timing_enable")
+        _check(lib().cgli_timing_reset(self.h), "timing_reset")
+
+    def timing_report(self):
+        out = {}
+        for i, name in enumerate(self.TIMING_CLASSES):
+            t = C.c_double()
+            n = C.c_long()
+            _check(lib().cgli_timing_get(self.h, i, C.byref(t), C.byref(n)), "timing_get")
+            out[name] = (t.value, n.value)
+        return out

@@ -1,0 +1,219 @@
+"""GPU (gfx950) kernels against the CPU oracle, called through the C ABI.
+
+Bar: max|gpu - oracle| / max|oracle| <= 1e-12 per conserved variable (FP64).
+The oracle is itself bit-identical to the reference build (test_oracle.py).
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle_harness import Case, Slab, dp, load_oracle, make_inputs, ptrs, regions, run_chain, IP
+from gpu_harness import config_from, rel, upload_inputs
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+MANIFEST = json.load(open(os.path.join(GOLD, "manifest.json")))
+SEED = MANIFEST["seed"]
+
+CASES = {
+    "ref_8x10x6_both_ends": (8, 10, 6, 1, 3, 3, 0, (0.001, 0.005, 0.002)),
+    "ref_24x10x6_interior": (24, 10, 6, 3, 3, 3, 1, (0.001, 0.005, 0.002)),
+    "ref_12x16x10": (12, 16, 10, 1, 4, 4, 0, (0.001, 0.005, 0.002)),
+    "ragged_20x24x17": (20, 24, 17, 1, 5, 6, 0, (0.002, 0.004, 0.003)),
+    "wide_40x12x70": (40, 12, 70, 1, 8, 9, 0, (0.001, 0.005, 0.002)),
+}
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return load_oracle()
+
+
+@pytest.fixture(scope="module")
+def nv():
+    from cgli_fu_flow_amd import native
+    native.lib()
+    return native
+
+
+def make_case(name):
+    X, Y, Z, NP, nin, nout, myid, d = CASES[name]
+    return Case(X, Y, Z, NP, nin, nout, myid, d)
+
+
+def open_ctx(nv, orc, case, xi_mode=0, **over):
+    cfg = case.cfg(orc, xi_mode=xi_mode)
+    for k, v in over.items():
+        setattr(cfg, k, v)
+    case.overrides = dict(over)
+    return nv.Context(config_from(cfg))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("xi_mode", [0, 1])
+def test_fluxes_match_oracle(nv, orc, name, xi_mode):
+    print("case", name, "xi_mode", xi_mode, flush=True)
+    case = make_case(name)
+    f = make_inputs(orc, case.slab, SEED)
+    case_cfg = case.cfg(orc, xi_mode=xi_mode)
+    want = {}
+    sl = case.slab
+    for d, tag in enumerate(("_u", "_v", "_w")):
+        U_ = [a.copy() for a in f["U_"]]
+        MR, ML, F = sl.zeros(5), sl.zeros(5), sl.zeros(5)
+        fn = (orc.orc_flux_x, orc.orc_flux_y, orc.orc_flux_z)[d]
+        from oracle_harness import met9
+        fn(C.byref(case_cfg), C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]),
+           dp(f["J" + tag]), ptrs(met9(f, tag)), ptrs(F))
+        want["inF" + "xyz"[d]] = F
+    with open_ctx(nv, orc, case, xi_mode=xi_mode) as ctx:
+        upload_inputs(ctx, f, with_levels=False)
+        ctx.flux_x(1.0, 1e-4)
+        ctx.flux_y(1.0, 1e-4)
+        ctx.flux_z(1.0, 1e-4)
+        reg = regions(sl)
+        for d, fid in enumerate((nv.INFX, nv.INFY, nv.INFZ)):
+            key = "inF" + "xyz"[d]
+            got = ctx.get_many(fid, 5)
+            for v in range(5):
+                g, w = got[v][reg[key]], want[key][v][reg[key]]
+                assert np.all(np.isfinite(g))
+                assert rel(g, w) <= TOL, (key, v, rel(g, w))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_viscous_matches_oracle(nv, orc, name):
+    print("case", name, flush=True)
+    case = make_case(name)
+    f = make_inputs(orc, case.slab, SEED + 1)
+    want = run_chain("orc", orc, case, f, dts=False)["vF"]
+    with open_ctx(nv, orc, case) as ctx:
+        upload_inputs(ctx, f, with_levels=False)
+        ctx.viscous()
+        got = ctx.get_many(nv.VF, 4)
+    reg = regions(case.slab)["vF"]
+    for v in range(4):
+        assert rel(got[v][reg], want[v][reg]) <= TOL, (v, rel(got[v][reg], want[v][reg]))
+
+
+@pytest.mark.parametrize("name", [n for n in sorted(CASES) if CASES[n][3] == 1])
+@pytest.mark.parametrize("sponge", [1, 0])
+def test_dual_time_stepping_matches_oracle(nv, orc, name, sponge):
+    print("case", name, "sponge", sponge, flush=True)
+    case = make_case(name)
+    f = make_inputs(orc, case.slab, SEED + 2)
+    case.overrides = {"sponge": sponge}
+    want = run_chain("orc", orc, case, f, dts=True, sweeps=2)
+    with open_ctx(nv, orc, case, sponge=sponge) as ctx:
+        upload_inputs(ctx, f)
+        ctx.flux_x(1.0, 1e-4)
+        ctx.flux_y(1.0, 1e-4)
+        ctx.flux_z(1.0, 1e-4)
+        ctx.viscous()
+        er = ctx.dual_time_stepping(2e-4, 1e-4, sweeps=2, write_residual=True)
+        U_ = ctx.get_many(nv.U_, 5)
+        NR = ctx.get_many(nv.NR, 5)
+        MR = ctx.get_many(nv.MR, 5)
+    reg = regions(case.slab)["cell"]
+    for v in range(5):
+        assert rel(MR[v][reg], want["MR"][v][reg]) <= TOL, ("MR", v)
+        assert rel(NR[v][reg], want["NR"][v][reg]) <= TOL, ("NR", v, rel(NR[v][reg], want["NR"][v][reg]))
+        assert rel(U_[v][reg], want["U_"][v][reg]) <= TOL, ("U_", v)
+        assert abs(er[v] - want["er"][v]) <= TOL * abs(want["er"][v]), ("er", v, er[v], want["er"][v])
+
+
+@pytest.mark.parametrize("name", [n for n in sorted(CASES) if CASES[n][3] == 1])
+def test_dplusgs_matches_oracle(nv, orc, name):
+    print("case", name, flush=True)
+    case = make_case(name)
+    f = make_inputs(orc, case.slab, SEED + 3)
+    sl = case.slab
+    chain = run_chain("orc", orc, case, f, dts=False)
+    cfg = case.cfg(orc)
+    U_ = [a.copy() for a in f["U_"]]
+    MR, NR, NL, Res = sl.zeros(5), sl.zeros(5), sl.zeros(5), sl.zeros(5)
+    er = (C.c_double * 5)()
+    from oracle_harness import met9
+    orc.orc_dplusgs(C.byref(cfg), C.c_double(2e-4), er, C.c_double(1e-4), ptrs(U_), ptrs(f["U"]), ptrs(f["Uq"]),
+                    dp(f["J"]), ptrs(met9(f)), ptrs(chain["inFx"]), ptrs(chain["inFy"]), ptrs(chain["inFz"]),
+                    ptrs(chain["vF"]), ptrs(MR), ptrs(NR), ptrs(NL), ptrs(Res), 4)
+    with open_ctx(nv, orc, case) as ctx:
+        upload_inputs(ctx, f)
+        ctx.flux_x(1.0, 1e-4)
+        ctx.flux_y(1.0, 1e-4)
+        ctx.flux_z(1.0, 1e-4)
+        ctx.viscous()
+        ger = ctx.dplusgs(2e-4, 1e-4, sweeps=4)
+        gU = ctx.get_many(nv.U_, 5)
+        gNR = ctx.get_many(nv.NR, 5)
+    reg = regions(sl)["cell"]
+    for v in range(5):
+        assert rel(gNR[v][reg], NR[v][reg]) <= TOL, ("NR", v)
+        assert rel(gU[v][reg], U_[v][reg]) <= TOL, ("U_", v)
+        assert abs(ger[v] - er[v]) <= TOL * abs(er[v])
+
+
+@pytest.mark.parametrize("name", [n for n in sorted(CASES) if CASES[n][3] == 1])
+def test_ghost_fills_match_oracle(nv, orc, name):
+    case = make_case(name)
+    f = make_inputs(orc, case.slab, SEED + 4)
+    cfg = case.cfg(orc)
+    lo, hi = case.nin + 2, case.X - case.nout + 3
+    U_ = [a.copy() for a in f["U_"]]
+    orc.orc_bc_y(C.byref(cfg), ptrs(U_), dp(f["J"]), C.c_double(343.0592), C.c_double(298.0592), lo, hi)
+    orc.orc_bc_z(C.byref(cfg), ptrs(U_))
+    orc.orc_bc_x_periodic(C.byref(cfg), ptrs(U_))
+    with open_ctx(nv, orc, case) as ctx:
+        ctx.set_many(nv.U_, f["U_"])
+        ctx.set(nv.J, f["J"])
+        ctx.bc_y(343.0592, 298.0592, lo, hi)
+        ctx.bc_z()
+        ctx.bc_x_periodic()
+        got = ctx.get_many(nv.U_, 5)
+    for v in range(5):
+        assert rel(got[v], U_[v]) <= TOL
+
+
+def test_hyperplane_matches_oracle(nv, orc):
+    case = make_case("ragged_20x24x17")
+    cfg = case.cfg(orc)
+    sl = case.slab
+    rows = sl.gend + sl.NY + sl.NZ + 8
+    cells = sl.nx * sl.ny * sl.nz
+    N = np.zeros((rows, 2), np.int32)
+    I = np.zeros((cells + 1, 3), np.int32)
+    orc.orc_hyperplane(C.byref(cfg), N.ctypes.data_as(IP), I.ctypes.data_as(IP))
+    with open_ctx(nv, orc, case) as ctx:
+        gN, gI = ctx.hyperplane()
+    assert np.array_equal(gN, N) and np.array_equal(gI, I)
+
+
+def test_dual_time_stepping_is_deterministic(nv, orc):
+    case = make_case("ragged_20x24x17")
+    f = make_inputs(orc, case.slab, SEED + 5)
+    outs = []
+    for _ in range(2):
+        with open_ctx(nv, orc, case) as ctx:
+            upload_inputs(ctx, f)
+            ctx.flux_x(1.0, 1e-4)
+            ctx.flux_y(1.0, 1e-4)
+            ctx.flux_z(1.0, 1e-4)
+            ctx.viscous()
+            er = ctx.dual_time_stepping(2e-4, 1e-4)
+            outs.append((er, ctx.get_many(nv.U_, 5)))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    for v in range(5):
+        assert np.array_equal(outs[0][1][v], outs[1][1][v])
+
+
+def test_missing_input_is_an_error(nv, orc):
+    case = make_case("ref_8x10x6_both_ends")
+    with open_ctx(nv, orc, case) as ctx:
+        with pytest.raises(nv.CgliError) as ei:
+            ctx.flux_x(1.0, 1e-4)
+        assert ei.value.code == -5
