@@ -128,7 +128,7 @@ void cgli_config_reference(cgli_config *c, int nx, int ny, int nz)
     c->first = 1; c->last = 1;
     c->gstart = 0; c->gend0 = nx - 1; c->nx_global = nx;
     c->nx_inlet = 100; c->nx_outlet = 150;
-    c->sponge = 1; c->xi_mode = 0;
+    c->sponge = 1; c->xi_mode = 0; c->x_periodic = 0; c->precision = 0;
     c->dxi = 0.001; c->deta = 0.005; c->dzeta = 0.002;
     c->high = 0.05;
     c->K = 1.4; c->R = 287;
@@ -176,7 +176,7 @@ int cgli_create(const cgli_config *cfg, int device, cgli_ctx **out)
     g.N = (long)(cfg->nx + 6) * g.Ym * g.Zm;
     g.gend = cfg->nx + 2; g.NY = cfg->ny + 1; g.NZ = cfg->nz + 1;
     g.NYY = g.NY + 1; g.NZZ = g.NZ + 1;
-    g.first = cfg->first; g.last = cfg->last;
+    g.first = cfg->first; g.last = cfg->last; g.xper = cfg->x_periodic;
     Phys &p = c->p;
     p.K = cfg->K; p.R = cfg->R; p.Cv = cfg->Cv; p.mu_L = cfg->mu_L; p.Pr_L = cfg->Pr_L;
     p.suth_T = cfg->suth_T; p.suth_S = cfg->suth_S;

@@ -18,6 +18,7 @@ struct Geo {
     int nx, ny, nz;
     int gend, NY, NZ, NYY, NZZ;
     int first, last;
+    int xper;   // single periodic slab
 };
 
 struct Phys {

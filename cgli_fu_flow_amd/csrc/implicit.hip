@@ -110,7 +110,7 @@ __global__ void k_zero_ghosts(Geo g, F5 NL, F5 NR)
     const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
     const int i = 2 + blockIdx.z;
     if (k > g.NZZ || j > g.NYY) return;
-    const bool xg = (g.first && i == 2) || (g.last && i == g.gend + 1);
+    const bool xg = ((g.first || g.xper) && i == 2) || ((g.last || g.xper) && i == g.gend + 1);
     const bool yg = (j == 1 || j == g.NYY);
     const bool zg = (k == 1 || k == g.NZZ);
     if (!(xg || yg || zg)) return;

@@ -41,6 +41,8 @@ typedef struct {
     int nx_inlet, nx_outlet; /* absorbing-zone widths (Resolution.h:1-2) */
     int sponge;              /* 1: absorbing-zone sources of Dual_time_stepping.cpp:532-664 */
     int xi_mode;             /* Flux_Y/Z pressure-jump normal: 0 = as the reference (none), 1 = face normal */
+    int x_periodic;          /* 1: one periodic slab: x ghosts filled periodically, zero implicit x ghosts */
+    int precision;           /* 0: FP64 residual kernels, 1: FP32 residual kernels (FP64 update) */
     double dxi, deta, dzeta; /* deltaXI, deltaET, deltaZT */
     double high;             /* prm.h:16 */
     double K, R;             /* ratio of specific heats, gas constant */

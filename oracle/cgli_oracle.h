@@ -34,6 +34,8 @@ typedef struct {
     int nx_inlet, nx_outlet; /* sponge widths (Resolution.h:1-2) */
     int sponge;              /* 1: reference absorbing zones (Dual_time_stepping.cpp:532-664) */
     int xi_mode;             /* Flux_Y/Z Fav2..4 normal: 0 = zero (UB compat), 1 = face unit normal */
+    int x_periodic;          /* one periodic slab: implicit x ghosts are zero as at nproc=1 */
+    int precision;           /* unused by the oracle (FP64 only) */
     double dxi, deta, dzeta; /* deltaXI, deltaET, deltaZT */
     double high;             /* channel height used by the sponge integral (prm.h:16) */
     double K, R;             /* gamma, gas constant */

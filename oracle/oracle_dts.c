@@ -126,10 +126,10 @@ static void zero_increment_ghosts(const orc_cfg *c, double *NR[5], double *NL[5]
 {
     DIMS(c);
     for (int v = 0; v < 5; v++) {
-        if (c->first)
+        if (c->first || c->x_periodic)
             for (int j = 1; j <= NYY; j++)
                 for (int k = 1; k <= NZZ; k++) NL[v][ID(2, j, k)] = NR[v][ID(2, j, k)] = 0;
-        if (c->last)
+        if (c->last || c->x_periodic)
             for (int j = 1; j <= NYY; j++)
                 for (int k = 1; k <= NZZ; k++) NL[v][ID(gend + 1, j, k)] = NR[v][ID(gend + 1, j, k)] = 0;
         for (int i = 2; i <= gend + 1; i++)

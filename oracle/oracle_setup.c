@@ -15,7 +15,7 @@ void orc_cfg_reference(orc_cfg *c, int nx, int ny, int nz)
     c->first = 1; c->last = 1;
     c->gstart = 0; c->gend0 = nx - 1; c->nx_global = nx;
     c->nx_inlet = 100; c->nx_outlet = 150;         /* Resolution.h:1-2 */
-    c->sponge = 1; c->xi_mode = 0;
+    c->sponge = 1; c->xi_mode = 0; c->x_periodic = 0; c->precision = 0;
     c->dxi = 0.001; c->deta = 0.005; c->dzeta = 0.002;  /* Resolution.h:19-21 */
     c->high = 0.05;                                 /* prm.h:16 */
     c->K = 1.4; c->R = 287;                         /* prm.h:7-8 */

@@ -24,7 +24,7 @@ IP = C.POINTER(C.c_int)
 class OrcCfg(C.Structure):
     _fields_ = [(n, C.c_int) for n in (
         "nx", "ny", "nz", "first", "last", "gstart", "gend0", "nx_global",
-        "nx_inlet", "nx_outlet", "sponge", "xi_mode")] + [(n, C.c_double) for n in (
+        "nx_inlet", "nx_outlet", "sponge", "xi_mode", "x_periodic", "precision")] + [(n, C.c_double) for n in (
         "dxi", "deta", "dzeta", "high", "K", "R", "Cv", "mu_L", "Pr_L", "suth_T",
         "suth_S", "rho_ref", "grav", "f_const", "f_dplusgs", "E_ref")]
 
