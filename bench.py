@@ -1,0 +1,181 @@
+"""Benchmark: physical steps/s of the compressible channel at a 1e-6 inner drop.
+
+Workload (BASELINE config "plain FP64 GPU path on a curvilinear grid"):
+channel 128x128x128, periodic x (4 pi) and z (2 pi), isothermal walls,
+tanh stretching beta=2.2, skew x' = x + 0.05 sin(2 pi y) sin(2 pi z/Lz),
+Re_tau=180, Mach 0.5, acoustic CFL 0.02, laminar Poiseuille + seeded +-10%
+noise.  Every physical step iterates the dual-time LU-SGS inner loop until
+max_v er_v/er_v(first) <= 1e-6 (or --max-inner).  State is resident in HBM
+before the timed region starts.
+
+Prints ONE JSON line.  `roofline` is the dominant kernel class measured
+with HIP events on the library's stream over the timed region;
+`cpu_baseline` is the oracle (C restatement, OpenMP) on this host for a
+bounded number of inner iterations at the same size.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "physical steps/s and Mcell-updates/s at 1e-6 inner-residual drop; plain vs mixed+learned"
+HBM_PEAK_GBS = 8000.0
+# algorithmic FP64 bytes per interior cell per launch of each kernel class
+# (each input element read once, each output written once)
+ALGO_BYTES = {"flux_x": 120, "flux_y": 120, "flux_z": 120, "viscous": 320, "rhs_diag": 472,
+              "lusgs_fwd": 240, "lusgs_bwd": 240, "update_norm": 208}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nx", type=int, default=128)
+    ap.add_argument("--ny", type=int, default=128)
+    ap.add_argument("--nz", type=int, default=128)
+    ap.add_argument("--beta", type=float, default=2.2)
+    ap.add_argument("--skew", type=float, default=0.05)
+    ap.add_argument("--re-tau", type=float, default=180.0)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--max-inner", type=int, default=200)
+    ap.add_argument("--cfl", type=float, default=0.02)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(ctx, nv, cfg, dt, inner_per_step, budget_s):
+    """Oracle inner iterations on the host, same grid and state as the GPU run."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_harness import OrcCfg, dp, load_oracle, ptrs
+    orc = load_oracle()
+    oc = OrcCfg()
+    for name, _ in OrcCfg._fields_:
+        setattr(oc, name, getattr(cfg, name))
+    f = {}
+    for base, n, key in ((nv.U_, 5, "U_"), (nv.U, 5, "U"), (nv.UQ, 5, "Uq"), (nv.MET, 9, "met"),
+                         (nv.MET_U, 9, "met_u"), (nv.MET_V, 9, "met_v"), (nv.MET_W, 9, "met_w")):
+        f[key] = ctx.get_many(base, n)
+    for fid, key in ((nv.J, "J"), (nv.J_U, "J_u"), (nv.J_V, "J_v"), (nv.J_W, "J_w")):
+        f[key] = ctx.get(fid)
+    N, I = ctx.hyperplane()
+    z = lambda: [np.zeros(ctx.shape) for _ in range(5)]
+    MR, ML, Fx, Fy, Fz, NL, NR, Res = z(), z(), z(), z(), z(), z(), z(), z()
+    vF = [np.zeros(ctx.shape) for _ in range(4)]
+    LL, MLv, NLv = z(), z(), z()
+    er = (C.c_double * 5)()
+    U_ = f["U_"]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    iters, t_total = 0, 0.0
+    while True:
+        t0 = time.perf_counter()
+        orc.orc_bc_y(C.byref(oc), ptrs(U_), dp(f["J"]), C.c_double(1.0), C.c_double(1.0), -(2 ** 31), 2 ** 31 - 1)
+        orc.orc_bc_z(C.byref(oc), ptrs(U_))
+        orc.orc_bc_x_periodic(C.byref(oc), ptrs(U_))
+        orc.orc_flux_x(C.byref(oc), C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]),
+                       dp(f["J_u"]), ptrs(f["met_u"]), ptrs(Fx))
+        orc.orc_flux_y(C.byref(oc), C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]),
+                       dp(f["J_v"]), ptrs(f["met_v"]), ptrs(Fy))
+        orc.orc_flux_z(C.byref(oc), C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]),
+                       dp(f["J_w"]), ptrs(f["met_w"]), ptrs(Fz))
+        orc.orc_viscous(C.byref(oc), ptrs(U_), ptrs(vF), dp(f["J"]), dp(f["J_u"]), ptrs(f["met_u"]), dp(f["J_v"]),
+                        ptrs(f["met_v"]), dp(f["J_w"]), ptrs(f["met_w"]), ptrs(LL), ptrs(MLv), ptrs(NLv))
+        orc.orc_dual_time_stepping(C.byref(oc), C.c_double(dt), C.c_double(1e-4), er, N.ctypes.data_as(C.POINTER(C.c_int)),
+                                   I.ctypes.data_as(C.POINTER(C.c_int)), ptrs(U_), ptrs(f["U"]), ptrs(f["Uq"]),
+                                   dp(f["J"]), ptrs(f["met"]), ptrs(Fx), ptrs(Fy), ptrs(Fz), ptrs(vF), ptrs(MR),
+                                   ptrs(NR), ptrs(NL), ptrs(Res), dp(f["met_v"][4]), dp(f["J_v"]), 1)
+        t_total += time.perf_counter() - t0
+        iters += 1
+        if t_total >= budget_s or iters >= 50:
+            break
+    t_inner = t_total / iters
+    steps_per_s = 1.0 / (t_inner * inner_per_step)
+    ncell = cfg.nx * cfg.ny * cfg.nz
+    return {"value": steps_per_s, "unit": "physical steps/s", "cores": threads, "kind": "port",
+            "mcell_updates_per_s": ncell * steps_per_s / 1e6,
+            "sample": "%d inner iterations (ghosts, fluxes, viscous, dual-time LU-SGS) of the oracle at %dx%dx%d "
+                      "on %d OpenMP threads, %.3f s each, scaled by the GPU run's %.2f inner iterations per "
+                      "physical step" % (iters, cfg.nx, cfg.ny, cfg.nz, threads, t_inner, inner_per_step)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py runs on exactly one GPU: this build has no inter-GPU decomposition "
+              "(requested --gpus %d)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    from cgli_fu_flow_amd import native as nv
+    ch = nv.channel(a.re_tau, beta=a.beta, skew=a.skew, cfl=a.cfl)
+    cfg = nv.channel_config(ch, a.nx, a.ny, a.nz)
+    ctx = nv.Context(cfg, device=0)
+    dt = ctx.channel_init(ch)
+    print("setup done: dt=%.4e" % dt, file=sys.stderr, flush=True)
+    for s in range(a.warmup):
+        it, _, _ = ctx.channel_step(dt, a.tol, a.max_inner)
+        print("warmup step %d: %d inner iterations" % (s + 1, it), file=sys.stderr, flush=True)
+    ctx.timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    inner = []
+    for s in range(a.steps):
+        it, e1, en = ctx.channel_step(dt, a.tol, a.max_inner)
+        inner.append(it)
+        print("step %d: %d inner iterations, drop %.2e" % (s + 1, it, float(np.max(en / e1))), file=sys.stderr,
+              flush=True)
+    ctx.sync()
+    t1 = time.perf_counter()
+    rep = ctx.timing_report()
+    ctx.timing(False)
+    elapsed = t1 - t0
+    ncell = a.nx * a.ny * a.nz
+    steps_per_s = a.steps / elapsed
+    kernels = {}
+    for name, (tot, cnt) in rep.items():
+        if cnt:
+            avg = tot / cnt
+            kernels[name] = {"avg_ms": avg, "launches": cnt, "total_ms": tot,
+                             "GBps": ALGO_BYTES[name] * ncell / (avg * 1e-3) / 1e9}
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    achieved = kernels[dom]["GBps"]
+    out = {
+        "metric": METRIC,
+        "value": steps_per_s,
+        "unit": "physical steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: laminar Poiseuille + seeded +-10%% velocity noise (splitmix64 seed %d)" % ch.seed,
+        "config": {"workload": "compressible channel %dx%dx%d, beta=%.1f, skew=%.2f, Re_tau=%g, Mach %.1f, "
+                               "acoustic CFL %.2f, plain FP64 dual-time LU-SGS to a %.0e inner drop"
+                               % (a.nx, a.ny, a.nz, a.beta, a.skew, a.re_tau, ch.mach, ch.cfl, a.tol),
+                   "cells": ncell, "deltaT": dt, "max_inner": a.max_inner},
+        "mcell_updates_per_s": ncell * steps_per_s / 1e6,
+        "inner_iterations_per_step": float(np.mean(inner)),
+        "mcell_inner_iterations_per_s": ncell * float(np.sum(inner)) / elapsed / 1e6,
+        "kernels": kernels,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": ALGO_BYTES[dom] * ncell},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(ctx, nv, cfg, dt, float(np.mean(inner)), a.cpu_seconds)
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

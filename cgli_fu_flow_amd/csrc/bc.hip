@@ -89,4 +89,29 @@ void launch_bc_x_periodic(const Geo &g, F5 U, hipStream_t s)
     hipLaunchKernelGGL(k_bc_x_periodic, gr, b, 0, s, g, U);
 }
 
+__global__ void k_replicate(double *dst, long plane, int nplanes)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= plane) return;
+    const double v = dst[t];
+    for (int p = 1 + blockIdx.y; p < nplanes; p += gridDim.y) dst[(long)p * plane + t] = v;
+}
+
+void launch_replicate_planes(double *dst, long plane, int nplanes, hipStream_t s)
+{
+    dim3 gr((unsigned)((plane + 255) / 256), 16);
+    hipLaunchKernelGGL(k_replicate, gr, dim3(256), 0, s, dst, plane, nplanes);
+}
+
+__global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src, long n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[t];
+}
+
+void launch_copy_field(double *dst, const double *src, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, n);
+}
+
 }  // namespace cgli

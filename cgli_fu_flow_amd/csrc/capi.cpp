@@ -119,6 +119,18 @@ static void tend(cgli_ctx *c, int cls)
     c->ev_used[cls]++;
 }
 
+namespace cgli {
+int ctx_field(cgli_ctx *c, int id, double **dptr)
+{
+    int r = ensure(c, id);
+    if (!r) *dptr = c->f[id];
+    return r;
+}
+hipStream_t ctx_stream(cgli_ctx *c) { return c->stream; }
+const cgli_config *ctx_config(cgli_ctx *c) { return &c->cfg; }
+void ctx_mark(cgli_ctx *c, int id, int n) { mark(c, id, n); }
+}  // namespace cgli
+
 extern "C" {
 
 void cgli_config_reference(cgli_config *c, int nx, int ny, int nz)

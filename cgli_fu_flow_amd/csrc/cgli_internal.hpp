@@ -72,5 +72,14 @@ void launch_bc_z(const Geo &g, F5 U, hipStream_t s);
 void launch_bc_x_periodic(const Geo &g, F5 U, hipStream_t s);
 // zero fill by a kernel (safe inside a captured graph, unlike a memset node)
 void launch_zero(double *p, long n, hipStream_t s);
+// copy plane 0 of dst to planes 1..nplanes-1
+void launch_replicate_planes(double *dst, long plane, int nplanes, hipStream_t s);
+void launch_copy_field(double *dst, const double *src, long n, hipStream_t s);
+
+// context access for host modules other than capi.cpp
+int ctx_field(cgli_ctx *c, int id, double **dptr);   // allocates on first use
+hipStream_t ctx_stream(cgli_ctx *c);
+const cgli_config *ctx_config(cgli_ctx *c);
+void ctx_mark(cgli_ctx *c, int id, int n);
 
 }  // namespace cgli

@@ -38,6 +38,11 @@ class Config(C.Structure):
         "suth_S", "rho_ref", "grav", "f_const", "f_dplusgs", "E_ref")]
 
 
+class Channel(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("Lx", "Lz", "beta", "skew", "mach", "re_tau", "re_c", "Pr", "suth_S",
+                                          "cfl", "noise")] + [("seed", C.c_ulonglong), ("xi_mode", C.c_int)]
+
+
 _lib = None
 
 
@@ -69,6 +74,12 @@ def lib():
         L.cgli_bc_y.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int]
         L.cgli_bc_z.argtypes = [vp]
         L.cgli_bc_x_periodic.argtypes = [vp]
+        L.cgli_channel_defaults.argtypes = [C.POINTER(Channel), C.c_double]
+        L.cgli_channel_defaults.restype = None
+        L.cgli_channel_config.argtypes = [C.POINTER(Channel), C.c_int, C.c_int, C.c_int, C.POINTER(Config)]
+        L.cgli_channel_init.argtypes = [vp, C.POINTER(Channel), dp]
+        L.cgli_channel_ghosts.argtypes = [vp]
+        L.cgli_channel_step.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), dp, dp]
         L.cgli_timing_enable.argtypes = [vp, C.c_int]
         L.cgli_timing_reset.argtypes = [vp]
         L.cgli_timing_get.argtypes = [vp, C.c_int, dp, C.POINTER(C.c_long)]
@@ -79,6 +90,21 @@ def lib():
 def reference_config(nx, ny, nz):
     c = Config()
     lib().cgli_config_reference(C.byref(c), nx, ny, nz)
+    return c
+
+
+def channel(re_tau=180.0, **kw):
+    """Channel parameters (cgli_channel) with defaults for re_tau, overridden by kw."""
+    ch = Channel()
+    lib().cgli_channel_defaults(C.byref(ch), re_tau)
+    for k, v in kw.items():
+        setattr(ch, k, v)
+    return ch
+
+
+def channel_config(ch, nx, ny, nz):
+    c = Config()
+    _check(lib().cgli_channel_config(C.byref(ch), nx, ny, nz, C.byref(c)), "channel_config")
     return c
 
 
@@ -179,6 +205,23 @@ class Context:
 
     def bc_x_periodic(self):
         _check(lib().cgli_bc_x_periodic(self.h), "bc_x_periodic")
+
+    # -- channel workload
+    def channel_init(self, ch):
+        dt = C.c_double()
+        _check(lib().cgli_channel_init(self.h, C.byref(ch), C.byref(dt)), "channel_init")
+        return dt.value
+
+    def channel_ghosts(self):
+        _check(lib().cgli_channel_ghosts(self.h), "channel_ghosts")
+
+    def channel_step(self, deltaT, tol=1e-6, max_inner=100, sweeps=1):
+        it = C.c_int()
+        e1 = (C.c_double * 5)()
+        en = (C.c_double * 5)()
+        _check(lib().cgli_channel_step(self.h, deltaT, tol, max_inner, sweeps, C.byref(it), e1, en),
+               "channel_step")
+        return it.value, np.array(list(e1)), np.array(list(en))
 
     # -- timing (HIP events around each launch class)
     TIMING_CLASSES = ["flux_x", "flux_y", "flux_z", "viscous", "rhs_diag", "lusgs_fwd", "lusgs_bwd",

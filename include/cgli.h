@@ -122,6 +122,39 @@ int cgli_bc_z(cgli_ctx *ctx);
 /* three periodic ghost planes at each x end of a single periodic slab */
 int cgli_bc_x_periodic(cgli_ctx *ctx);
 
+/* ---- compressible turbulent channel (the benchmark workload) ---------- */
+/* Periodic x and z, isothermal no-slip walls at y = +-1 with tanh
+ * stretching, constant driving pressure gradient, nondimensional with
+ * rho = T = U_centreline = 1.  Replaces the reference driver's setup
+ * (Grid.cpp, Jacobian*.cpp, Initial_condition.cpp), which hard-codes a
+ * vertical heated channel. */
+typedef struct {
+    double Lx, Lz;           /* domain lengths */
+    double beta;             /* tanh stretching of the wall-normal grid */
+    double skew;             /* a of x' = x + a sin(2 pi y) sin(2 pi z / Lz) */
+    double mach;             /* centreline Mach number */
+    double re_tau, re_c;     /* friction and centreline Reynolds numbers */
+    double Pr, suth_S;       /* Prandtl number; Sutherland constant / T_ref */
+    double cfl;              /* physical CFL number that sets deltaT */
+    double noise;            /* uniform velocity perturbation, fraction of U_c */
+    unsigned long long seed; /* splitmix64 seed of the perturbation */
+    int xi_mode;             /* as cgli_config.xi_mode */
+} cgli_channel;
+
+void cgli_channel_defaults(cgli_channel *ch, double re_tau);
+/* configuration of a single periodic slab of nx*ny*nz cells */
+int cgli_channel_config(const cgli_channel *ch, int nx, int ny, int nz, cgli_config *cfg);
+/* metrics, grid and the laminar + noise initial state on all time levels;
+ * *deltaT receives the CFL-limited physical step */
+int cgli_channel_init(cgli_ctx *ctx, const cgli_channel *ch, double *deltaT);
+/* wall, z and x ghost fills of U_ */
+int cgli_channel_ghosts(cgli_ctx *ctx);
+/* one physical step: inner iterations (ghosts, fluxes, viscous terms,
+ * dual-time LU-SGS) until max_v er_v / er_v(first) <= tol or max_inner,
+ * then the time-level shift.  *inner receives the iteration count. */
+int cgli_channel_step(cgli_ctx *ctx, double deltaT, double tol, int max_inner, int sweeps, int *inner,
+                      double er_first[5], double er_last[5]);
+
 /* ---- timing ---------------------------------------------------------- */
 /* Average device time (ms) of the last `n` launches of a kernel class
  * recorded since cgli_timing_reset; classes: 0 flux_x 1 flux_y 2 flux_z

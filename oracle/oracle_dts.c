@@ -225,6 +225,7 @@ static void rhs_diag(const orc_cfg *c, double deltaT, double e, int with_res,
 {
     DIMS(c);
     const double K = c->K;
+#pragma omp parallel for schedule(static)
     for (int i = 3; i <= gend; i++)
         for (int j = 2; j < NYY; j++)
             for (int k = 2; k < NZZ; k++) {
@@ -330,6 +331,7 @@ static void update_norm(const orc_cfg *c, double *U_[5], const double *J, double
 {
     DIMS(c);
     const double K = c->K;
+#pragma omp parallel for schedule(static)
     for (int i = 3; i <= gend; i++)
         for (int j = 2; j < NYY; j++)
             for (int k = 2; k < NZZ; k++) {
@@ -390,6 +392,7 @@ void orc_dual_time_stepping(const orc_cfg *c, double deltaT, double e, double er
 
     for (int sweep = 1; sweep <= sweeps; sweep++) {
         for (int m = 7; m <= gend + NY + NZ; m++) {
+#pragma omp parallel for schedule(static)
             for (int mi = Nijk[2 * m] - Nijk[2 * m + 1] + 1; mi <= Nijk[2 * m]; mi++) {
                 int i = ijk[3 * mi], j = ijk[3 * mi + 1], k = ijk[3 * mi + 2];
                 long id = ID(i, j, k), ia = ID(i - 1, j, k), jb = ID(i, j - 1, k), kc = ID(i, j, k - 1);
@@ -411,6 +414,7 @@ void orc_dual_time_stepping(const orc_cfg *c, double deltaT, double e, double er
             }
         }
         for (int m = gend + NY + NZ; m >= 7; m--) {
+#pragma omp parallel for schedule(static)
             for (int mi = Nijk[2 * m] - Nijk[2 * m + 1] + 1; mi <= Nijk[2 * m]; mi++) {
                 int i = ijk[3 * mi], j = ijk[3 * mi + 1], k = ijk[3 * mi + 2];
                 long id = ID(i, j, k), ia = ID(i + 1, j, k), jb = ID(i, j + 1, k), kc = ID(i, j, k + 1);
@@ -456,6 +460,7 @@ void orc_dplusgs(const orc_cfg *c, double deltaT, double er[5], double e,
     zero_increment_ghosts(c, NR, NL);
     rhs_diag(&cc, deltaT, e, 0, U_, U, Uq, J, mt, inFx, inFy, inFz, vF, MR, NR, NL, Res);
     for (int sweep = 1; sweep <= sweeps; sweep++) {
+#pragma omp parallel for schedule(static)
         for (int i = 3; i <= gend; i++)
             for (int j = 2; j < NYY; j++)
                 for (int k = 2; k < NZZ; k++) {

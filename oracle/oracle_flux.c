@@ -124,6 +124,7 @@ void orc_flux_x(const orc_cfg *c, double Ep, double e, double *const U_[5],
     (void)Ep;
     int istart = c->first ? 4 : 2;
     int iend = c->last ? gend : gend + 1;
+    #pragma omp parallel for schedule(static)
     for (int i = istart; i <= iend; i++)
         for (int j = 2; j <= NY; j++)
             for (int k = 2; k <= NZ; k++)
@@ -151,6 +152,7 @@ void orc_flux_x(const orc_cfg *c, double Ep, double e, double *const U_[5],
                     MR[v][ID(i, j - 1, k - 1)] = rec2(U_[v], J, ID(i, j, k), ID(i + 1, j, k));
                 }
 
+    #pragma omp parallel for schedule(static)
     for (int i = 2; i <= gend; i++)
         for (int j = 1; j < NY; j++)
             for (int k = 1; k < NZ; k++) {
@@ -172,6 +174,7 @@ void orc_flux_y(const orc_cfg *c, double Ep, double e, double *const U_[5],
     DIMS(c);
     int istart = c->first ? 2 : 3;
     int iend = gend;
+    #pragma omp parallel for schedule(static)
     for (int i = istart; i <= iend; i++)
         for (int j = 4; j <= NY - 2; j++)
             for (int k = 2; k <= NZ; k++)
@@ -181,6 +184,7 @@ void orc_flux_y(const orc_cfg *c, double Ep, double e, double *const U_[5],
                     MR[v][ID(i - 1, j - 1, k - 1)] = rec5R(U_[v], J, ID(i, j - 2, k), ID(i, j - 1, k), ID(i, j, k),
                                                            ID(i, j + 1, k), ID(i, j + 2, k));
                 }
+    #pragma omp parallel for schedule(static)
     for (int i = istart; i <= iend; i++)
         for (int k = 2; k <= NZ; k++)
             for (int v = 0; v < 5; v++) {
@@ -202,6 +206,7 @@ void orc_flux_y(const orc_cfg *c, double Ep, double e, double *const U_[5],
                 MR[v][ID(i - 1, j, k - 1)] = rec2(U, J, ID(i, j, k), ID(i, j + 1, k));
             }
 
+    #pragma omp parallel for schedule(static)
     for (int i = 2; i <= gend - 1; i++)
         for (int j = 1; j < NYY; j++)
             for (int k = 1; k < NZ; k++) {
@@ -232,6 +237,7 @@ void orc_flux_z(const orc_cfg *c, double Ep, double e, double *const U_[5],
     (void)Ep;
     int istart = c->first ? 2 : 3;
     int iend = gend;
+    #pragma omp parallel for schedule(static)
     for (int i = istart; i <= iend; i++)
         for (int j = 2; j <= NY; j++)
             for (int k = 3; k <= NZ - 1; k++)
@@ -241,6 +247,7 @@ void orc_flux_z(const orc_cfg *c, double Ep, double e, double *const U_[5],
                     MR[v][ID(i - 1, j - 1, k - 1)] = rec5R(U_[v], J, ID(i, j, k - 2), ID(i, j, k - 1), ID(i, j, k),
                                                            ID(i, j, k + 1), ID(i, j, k + 2));
                 }
+    #pragma omp parallel for schedule(static)
     for (int i = istart; i <= iend; i++)
         for (int j = 2; j <= NY; j++)
             for (int v = 0; v < 5; v++) {
@@ -256,6 +263,7 @@ void orc_flux_z(const orc_cfg *c, double Ep, double e, double *const U_[5],
                 MR[v][ID(i - 1, j - 1, k)] = rec2(U, J, ID(i, j, k), ID(i, j, k + 1));
             }
 
+    #pragma omp parallel for schedule(static)
     for (int i = 2; i <= gend - 1; i++)
         for (int j = 1; j < NY; j++)
             for (int k = 1; k < NZZ; k++) {

@@ -141,6 +141,7 @@ void orc_viscous(const orc_cfg *c, double *const U_[5], double *vF[4],
     const double inv4XI = 1. / (4 * c->dxi), inv4ET = 1. / (4 * c->deta), inv4ZT = 1. / (4 * c->dzeta);
     const double CvK = c->Cv * c->K, Pr = c->Pr_L;
 
+    #pragma omp parallel for schedule(static)
     for (int i = 3; i <= gend + 1; i++)
         for (int j = 2; j <= NYY; j++)
             for (int k = 2; k <= NZZ; k++) {
@@ -252,6 +253,7 @@ void orc_viscous(const orc_cfg *c, double *const U_[5], double *vF[4],
                 }
             }
 
+    #pragma omp parallel for schedule(static)
     for (int i = 3; i <= gend; i++)
         for (int j = 2; j <= NY; j++)
             for (int k = 2; k <= NZ; k++)
