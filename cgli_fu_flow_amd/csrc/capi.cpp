@@ -14,7 +14,7 @@ using namespace cgli;
 
 namespace {
 
-enum { T_FLUX_X, T_FLUX_Y, T_FLUX_Z, T_VISCOUS, T_RHS, T_FWD, T_BWD, T_UPDATE, T_NCLASS };
+enum { T_FLUX_X, T_FLUX_Y, T_FLUX_Z, T_VISCOUS, T_RHS, T_FWD, T_BWD, T_UPDATE, T_COEF, T_NCLASS };
 
 struct EventPair { hipEvent_t a, b; };
 
@@ -36,6 +36,15 @@ struct cgli_ctx {
     int *ijk_dev = nullptr;
     std::vector<int> Nijk;       // (gend+NY+NZ+8) x 2
     std::vector<int> ijk;        // (cells+1) x 3
+    int lusgs_mode = 1;          // 0: one launch per hyperplane, 1: tiled wavefront
+    double *coef = nullptr;      // 46 doubles per cell: split coefficients and diagonal
+    double *xbuf = nullptr;      // 5 fields: tile-edge exchange of the wavefront
+    double *trash = nullptr;     // write-only scratch of the wavefront
+    int *order_dev = nullptr;    // wavefront tile order
+    int ntiles = 0;
+    int *counters = nullptr;     // ticket, error flag
+    int *counters_host = nullptr;
+    int max_spin = 1000000;
     bool timing = false;
     std::vector<EventPair> ev[T_NCLASS];
     size_t ev_used[T_NCLASS] = {};
@@ -217,6 +226,12 @@ int cgli_destroy(cgli_ctx *c)
     if (c->faces12) hipFree(c->faces12);
     if (c->scratch) hipFree(c->scratch);
     if (c->ijk_dev) hipFree(c->ijk_dev);
+    if (c->coef) hipFree(c->coef);
+    if (c->xbuf) hipFree(c->xbuf);
+    if (c->trash) hipFree(c->trash);
+    if (c->order_dev) hipFree(c->order_dev);
+    if (c->counters) hipFree(c->counters);
+    if (c->counters_host) hipHostFree(c->counters_host);
     if (c->er_host) hipHostFree(c->er_host);
     for (int k = 0; k < T_NCLASS; k++)
         for (auto &e : c->ev[k]) {
@@ -384,20 +399,59 @@ static int implicit_inputs(cgli_ctx *c)
     return ensure_range(c, CGLI_RES, 5);
 }
 
-static int finish_norm(cgli_ctx *c, int nb, double er[5])
+static int finish_norm(cgli_ctx *c, int nb, double er[5], bool check_wave = false)
 {
     const double DN = 1. / (double)((long)c->cfg.nx_global * c->cfg.ny * c->cfg.nz);
     double *er_dev = c->scratch + c->scratch_len - 8;
     launch_finish_norm(nb, c->scratch, DN, 0, er_dev, c->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->er_host, er_dev, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
+    if (check_wave) HIPCHK(hipMemcpyAsync(c->counters_host, c->counters, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (check_wave && c->counters_host[1] != 0) return CGLI_E_TIMEOUT;
     bool finite = true;
     for (int v = 0; v < 5; v++) {
         er[v] = c->er_host[v];
         if (!std::isfinite(er[v])) finite = false;
     }
     return finite ? CGLI_OK : CGLI_E_NONFINITE;
+}
+
+static int wave_setup(cgli_ctx *c)
+{
+    if (c->coef) return CGLI_OK;
+    void *p = nullptr;
+    if (hipMalloc(&p, sizeof(double) * 46 * c->g.N) != hipSuccess) return CGLI_E_NOMEM;
+    c->coef = static_cast<double *>(p);
+    if (hipMalloc(&p, sizeof(double) * 5 * c->g.N) != hipSuccess) return CGLI_E_NOMEM;
+    c->xbuf = static_cast<double *>(p);
+    HIPCHK(hipMemset(c->xbuf, 0, sizeof(double) * 5 * c->g.N));   // ghost entries stay zero
+    if (hipMalloc(&p, sizeof(double) * WAVE_TRASH) != hipSuccess) return CGLI_E_NOMEM;
+    c->trash = static_cast<double *>(p);
+    std::vector<int> order;
+    c->ntiles = wave_tiles(c->g, WAVE_TJ, WAVE_TK, order);
+    if (hipMalloc(&p, sizeof(int) * order.size()) != hipSuccess) return CGLI_E_NOMEM;
+    c->order_dev = static_cast<int *>(p);
+    HIPCHK(hipMemcpy(c->order_dev, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+    if (hipMalloc(&p, sizeof(int) * 4) != hipSuccess) return CGLI_E_NOMEM;
+    c->counters = static_cast<int *>(p);
+    if (hipHostMalloc(&p, sizeof(int) * 4) != hipSuccess) return CGLI_E_NOMEM;
+    c->counters_host = static_cast<int *>(p);
+    return CGLI_OK;
+}
+
+int cgli_set_lusgs_mode(cgli_ctx *c, int mode)
+{
+    if (!c || mode < 0 || mode > 1) return CGLI_E_ARG;
+    c->lusgs_mode = mode;
+    return CGLI_OK;
+}
+
+int cgli_set_spin_limit(cgli_ctx *c, int max_spin)
+{
+    if (!c || max_spin < 1) return CGLI_E_ARG;
+    c->max_spin = max_spin;
+    return CGLI_OK;
 }
 
 int cgli_dual_time_stepping(cgli_ctx *c, double deltaT, double e, double er[5], int sweeps, int write_residual)
@@ -428,25 +482,44 @@ int cgli_dual_time_stepping(cgli_ctx *c, double deltaT, double e, double er[5], 
                     c5(c, CGLI_RES), f5(c, CGLI_MR), f5(c, CGLI_NL), f5(c, CGLI_NR), s);
     tend(c, T_RHS);
     const int mlast = g.gend + g.NY + g.NZ;
+    if (c->lusgs_mode == 1) {
+        r = wave_setup(c);
+        if (r) return r;
+        launch_reset_ints(c->counters, 2, s);
+        tbegin(c, T_COEF);
+        launch_split_coef(g, c->p, deltaT, e, c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c->coef, s);
+        tend(c, T_COEF);
+    }
     for (int sw = 0; sw < sweeps; sw++) {
-        tbegin(c, T_FWD);
-        for (int m = 7; m <= mlast; m++) {
-            const int cnt = c->Nijk[2 * m + 1];
-            if (!cnt) continue;
-            launch_lusgs_plane(g, c->p, deltaT, e, 1, m, c->Nijk[2 * m] - cnt + 1, cnt, c->ijk_dev,
-                               c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_MR), f5(c, CGLI_NL),
-                               f5(c, CGLI_NR), s);
+        if (c->lusgs_mode == 1) {
+            tbegin(c, T_FWD);
+            launch_lusgs_wave(g, 1, c->coef, c5(c, CGLI_MR), f5(c, CGLI_NL), c->xbuf, c->trash, c->order_dev,
+                              c->ntiles, c->counters, c->max_spin, s);
+            tend(c, T_FWD);
+            tbegin(c, T_BWD);
+            launch_lusgs_wave(g, 0, c->coef, c5(c, CGLI_NL), f5(c, CGLI_NR), c->xbuf, c->trash, c->order_dev,
+                              c->ntiles, c->counters, c->max_spin, s);
+            tend(c, T_BWD);
+        } else {
+            tbegin(c, T_FWD);
+            for (int m = 7; m <= mlast; m++) {
+                const int cnt = c->Nijk[2 * m + 1];
+                if (!cnt) continue;
+                launch_lusgs_plane(g, c->p, deltaT, e, 1, m, c->Nijk[2 * m] - cnt + 1, cnt, c->ijk_dev,
+                                   c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_MR), f5(c, CGLI_NL),
+                                   f5(c, CGLI_NR), s);
+            }
+            tend(c, T_FWD);
+            tbegin(c, T_BWD);
+            for (int m = mlast; m >= 7; m--) {
+                const int cnt = c->Nijk[2 * m + 1];
+                if (!cnt) continue;
+                launch_lusgs_plane(g, c->p, deltaT, e, 0, m, c->Nijk[2 * m] - cnt + 1, cnt, c->ijk_dev,
+                                   c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_MR), f5(c, CGLI_NL),
+                                   f5(c, CGLI_NR), s);
+            }
+            tend(c, T_BWD);
         }
-        tend(c, T_FWD);
-        tbegin(c, T_BWD);
-        for (int m = mlast; m >= 7; m--) {
-            const int cnt = c->Nijk[2 * m + 1];
-            if (!cnt) continue;
-            launch_lusgs_plane(g, c->p, deltaT, e, 0, m, c->Nijk[2 * m] - cnt + 1, cnt, c->ijk_dev,
-                               c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_MR), f5(c, CGLI_NL),
-                               f5(c, CGLI_NR), s);
-        }
-        tend(c, T_BWD);
         launch_copy_interior(g, c5(c, CGLI_NR), f5(c, CGLI_NL), s);
     }
     tbegin(c, T_UPDATE);
@@ -455,7 +528,7 @@ int cgli_dual_time_stepping(cgli_ctx *c, double deltaT, double e, double er[5], 
     tend(c, T_UPDATE);
     HIPCHK(hipGetLastError());
     mark(c, CGLI_MR, 15);
-    return finish_norm(c, nblk, er);
+    return finish_norm(c, nblk, er, c->lusgs_mode == 1);
 }
 
 int cgli_dplusgs(cgli_ctx *c, double deltaT, double er[5], double e, int sweeps)

@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 
 #include "cgli.h"
 
@@ -75,6 +76,18 @@ void launch_zero(double *p, long n, hipStream_t s);
 // copy plane 0 of dst to planes 1..nplanes-1
 void launch_replicate_planes(double *dst, long plane, int nplanes, hipStream_t s);
 void launch_copy_field(double *dst, const double *src, long n, hipStream_t s);
+
+// wavefront LU-SGS (implicit.hip)
+constexpr int WAVE_TJ = 8, WAVE_TK = 16;
+constexpr int WAVE_TRASH = 1024;   // doubles of write-only scratch of the wavefront sweep
+int wave_tiles(const Geo &g, int tj, int tk, std::vector<int> &order);
+void launch_split_coef(const Geo &g, const Phys &p, double deltaT, double e, C5 U_, const double *J, M9 met,
+                       double *coef, hipStream_t s);
+// coef: 46 doubles per cell (coefficient records), xbuf: 5 per cell (tile-edge exchange),
+// trash: WAVE_TRASH doubles
+void launch_lusgs_wave(const Geo &g, int forward, const double *coef, C5 R, F5 X, double *xbuf, double *trash,
+                       const int *order, int ntiles, int *counters, int max_spin, hipStream_t s);
+void launch_reset_ints(int *p, int n, hipStream_t s);
 
 // context access for host modules other than capi.cpp
 int ctx_field(cgli_ctx *c, int id, double **dptr);   // allocates on first use

@@ -10,6 +10,8 @@
 //   k_update_norm     primitive update and per-block partial norms (1702-1782)
 //   k_finish_norm     fixed-order sum of the partials, sqrt, scale
 // plus the Jacobi sweep of DPLUSGS.cpp:544-1125.
+#include <vector>
+
 #include "cgli_internal.hpp"
 
 namespace cgli {
@@ -640,6 +642,383 @@ void launch_finish_norm(int nblocks, const double *partials, double DN, int squa
 {
     (void)squared_twice;
     hipLaunchKernelGGL(k_finish_norm, dim3(1), dim3(256), 0, s, nblocks, partials, DN, er_dev);
+}
+
+
+// =================================================================
+// Wavefront LU-SGS.
+//
+// Pass 1 (k_split_coef, fully parallel): one record of NREC doubles per
+// cell: the cell's split coefficients along x, y and z (14 values each,
+// already divided by the grid spacing of that direction) and, for interior
+// cells, the diagonal (d11, d22, d51).  Records are contiguous per cell so
+// a sweep reads a neighbour's values with a few 16-byte loads.
+//
+// Pass 2 (k_lusgs_wave, one launch per sweep): the (j,k) plane is cut into
+// TJ x TK tiles; a workgroup owns one tile, one thread per (j,k) column,
+// and marches i.  At local step s thread (jl,kl) updates i = s - jl - kl,
+// so its i-1, j-1 and k-1 neighbours were updated at step s-1: inside the
+// tile they come from LDS, across tile edges (and from the ghost planes)
+// from an exchange buffer.  Before the sweep every interior exchange entry
+// holds a sentinel NaN; a tile-edge thread writes its result with an
+// agent-scope atomic store and the consumer spins on an atomic load until
+// the sentinel is gone, so each value is validated on its own and no
+// ordering between locations is needed.  Tiles are claimed through a
+// ticket in topological order (anti-diagonals), so a workgroup only ever
+// waits on tiles held by workgroups that are already running.  Spins are
+// bounded; an expired spin sets *err and the kernel still runs to
+// completion.  The coefficients of the next cell of each column are loaded
+// one step ahead into one of two register sets used alternately.
+//
+// Compared with the plane-by-plane kernels (which follow the reference's
+// expressions literally) the only arithmetic difference is that the
+// division by dxi/deta/dzeta is applied to each coefficient instead of to
+// each partial sum: results agree to rounding (a few ulp).
+// =================================================================
+constexpr int NREC = 46;   // 3 x 14 split values, d11, d22, d51, pad
+// slot order of one direction: the ten values both sweeps use, then the
+// forward pair (p11, p22), then the backward pair (m11, m22)
+enum { S_P12, S_P13, S_P14, S_P21, S_P31, S_P41, S_P51, S_P52, S_P53, S_P54, S_P11, S_P22, S_M11, S_M22 };
+constexpr unsigned long long SENTINEL = 0x7FF4DEADBEEF0001ULL;
+
+__global__ void __launch_bounds__(256) k_split_coef(Geo g, Phys p, double deltaT, double e, C5 U_,
+                                                    const double *__restrict__ J, M9 met, double *__restrict__ coef)
+{
+    // box i=2..gend+1, j=1..NYY, k=1..NZZ
+    const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = 2 + blockIdx.z;
+    if (k > g.NZZ || j > g.NYY) return;
+    const long c = ix(g, i, j, k);
+    double *o = coef + c * NREC;
+    const CP q = cell_prim(U_, J, c, p.K, e);
+    const double dd[3] = {p.dxi, p.deta, p.dzeta};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const SC s = split(q, met.m[3 * d][c], met.m[3 * d + 1][c], met.m[3 * d + 2][c], p.K, d != 0);
+        const double h = dd[d];
+        double2 *b = reinterpret_cast<double2 *>(o + 14 * d);
+        b[0] = make_double2(s.p12 / h, s.p13 / h);
+        b[1] = make_double2(s.p14 / h, s.p21 / h);
+        b[2] = make_double2(s.p31 / h, s.p41 / h);
+        b[3] = make_double2(s.p51 / h, s.p52 / h);
+        b[4] = make_double2(s.p53 / h, s.p54 / h);
+        b[5] = make_double2(s.p11 / h, s.p22 / h);
+        b[6] = make_double2(s.m11 / h, s.m22 / h);
+    }
+    if (i >= 3 && i <= g.gend && j >= 2 && j <= g.NY && k >= 2 && k <= g.NZ) {
+        double m[9];
+        load9(met, c, m);
+        const DG D = diag(q, m, p, deltaT);
+        double2 *b = reinterpret_cast<double2 *>(o + 42);
+        b[0] = make_double2(D.d11, D.d22);
+        b[1] = make_double2(D.d51, 0.0);
+    }
+}
+
+// the sentinel into the exchange entry of every interior cell
+__global__ void k_fill_sentinel(Geo g, double *__restrict__ xbuf)
+{
+    const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = 3 + blockIdx.z;
+    if (k > g.NZ || j > g.NY) return;
+    const long c = ix(g, i, j, k);
+#pragma unroll
+    for (int v = 0; v < 5; v++) xbuf[5 * c + v] = __longlong_as_double((long long)SENTINEL);
+}
+
+__global__ void k_reset_ints(int *p, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// one direction's 12 values for a sweep: P12..P54, then g11, g22 (the
+// p- or m-pair)
+struct Dir12 { double v[12]; };
+
+template <bool FWD>
+__device__ __forceinline__ void load_dir(Dir12 &s, const double *__restrict__ r)
+{
+    const double2 *b = reinterpret_cast<const double2 *>(r);
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        const double2 t = b[q];
+        s.v[2 * q] = t.x;
+        s.v[2 * q + 1] = t.y;
+    }
+    const double2 t = b[FWD ? 5 : 6];
+    s.v[10] = t.x;
+    s.v[11] = t.y;
+}
+
+struct Pref { Dir12 A, B, C; double d11, d22, d51; double r[5]; };
+
+// rec: the own cell's record; oA/oB/oC: offsets of the upstream neighbours'
+// records along i, j, k
+template <bool FWD>
+__device__ __forceinline__ void load_pref(Pref &P, const double *__restrict__ rec, long oA, long oB, long oC,
+                                          const C5 &R, long c)
+{
+    load_dir<FWD>(P.A, rec + oA);
+    load_dir<FWD>(P.B, rec + oB + 14);
+    load_dir<FWD>(P.C, rec + oC + 28);
+    const double2 *b = reinterpret_cast<const double2 *>(rec + 42);
+    const double2 t0 = b[0], t1 = b[1];
+    P.d11 = t0.x;
+    P.d22 = t0.y;
+    P.d51 = t1.x;
+#pragma unroll
+    for (int v = 0; v < 5; v++) P.r[v] = R.v[v][c];
+}
+
+__device__ __forceinline__ unsigned long long aload(const double *p)
+{
+    return __hip_atomic_load(reinterpret_cast<const unsigned long long *>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// issue the five exchange loads of one neighbour without waiting
+__device__ __forceinline__ void issue5(const double *xbuf, long c, unsigned long long b[5])
+{
+#pragma unroll
+    for (int v = 0; v < 5; v++) b[v] = aload(xbuf + 5 * c + v);
+}
+
+// where `need`, re-poll the entries that still hold the sentinel (bounded)
+// and take them as the neighbour's values
+__device__ __forceinline__ void settle5(const double *xbuf, long c, bool need, unsigned long long b[5],
+                                        double out[5], int max_spin, int *err)
+{
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        int n = 0;
+        while (need && b[v] == SENTINEL && n < max_spin) {
+            __builtin_amdgcn_s_sleep(1);
+            b[v] = aload(xbuf + 5 * c + v);
+            n++;
+        }
+        if (need && b[v] == SENTINEL) atomicOr(err, 1);
+        if (need) out[v] = __longlong_as_double((long long)b[v]);
+    }
+}
+
+// per-lane constants of the march down one (j,k) column
+struct Lane {
+    int tid, jl, kl;
+    bool col, edge_out;
+    long c0;                // cell (i = 0, j, k), clamped into the domain
+};
+
+struct WaveArgs {
+    Geo g;
+    const double *coef;
+    C5 R;
+    F5 X;
+    double *xbuf, *trash;
+    int *err;
+    int max_spin;
+    long sxs, oA, oB, oC, d;
+};
+
+// cell of local step il (clamped to the column, so every lane has a valid
+// address at every step)
+template <bool FWD>
+__device__ __forceinline__ long cell_at(const WaveArgs &w, const Lane &L, int il)
+{
+    const int c = il < 0 ? 0 : (il >= w.g.nx ? w.g.nx - 1 : il);
+    return L.c0 + (long)(FWD ? 3 + c : w.g.gend - c) * w.sxs;
+}
+
+// FWD: X = NL from R = MR, NL = D (MR - L NL).  BWD: X = NR from R = NL,
+// NR = NL - D U NR.  One step of the march, the same instruction stream
+// on every lane: loads are issued unconditionally (clamped addresses) and
+// lanes with nothing to do store into a trash buffer, so no branch
+// separates a prefetch from its use.  PAR is the step's parity (which LDS
+// half is written); Pc/nbc/ncc hold the values loaded for this step,
+// Pn/nbn/ncn receive the next step's.
+template <int TJ, int TK, bool FWD, int PAR>
+__device__ __forceinline__ void wave_step(const WaveArgs &w, const Lane &L, int s, double prev[5],
+                                          double (*sx)[5][TJ * TK], Pref &Pc, Pref &Pn, unsigned long long *nbc,
+                                          unsigned long long *nbn, unsigned long long *ncc, unsigned long long *ncn)
+{
+    const int il = s - L.jl - L.kl;
+    const bool act = L.col && il >= 0 && il < w.g.nx;
+    __syncthreads();
+    const long c = cell_at<FWD>(w, L, il);
+    double xb[5], xc[5];
+    const int tb = L.tid >= TK ? L.tid - TK : L.tid, tc = L.tid >= 1 ? L.tid - 1 : L.tid;
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        xb[v] = sx[PAR ^ 1][v][tb];
+        xc[v] = sx[PAR ^ 1][v][tc];
+    }
+    settle5(w.xbuf, c + w.oB / NREC, act && L.jl == 0, nbc, xb, w.max_spin, w.err);
+    settle5(w.xbuf, c + w.oC / NREC, act && L.kl == 0, ncc, xc, w.max_spin, w.err);
+    {   // prefetch for the next step
+        const long cn = cell_at<FWD>(w, L, il + 1);
+        load_pref<FWD>(Pn, w.coef + cn * NREC, w.oA, w.oB, w.oC, w.R, cn);
+        issue5(w.xbuf, cn + w.oB / NREC, nbn);
+        issue5(w.xbuf, cn + w.oC / NREC, ncn);
+    }
+    const double *A = Pc.A.v, *B = Pc.B.v, *Cc = Pc.C.v;
+    const double *xa = prev;
+    double out[5];
+    if (FWD) {
+        const double LL0 = (-A[10] * xa[0] - A[0] * xa[1] - A[1] * xa[2] - A[2] * xa[3]) +
+                           (-B[10] * xb[0] - B[0] * xb[1] - B[1] * xb[2] - B[2] * xb[3]) +
+                           (-Cc[10] * xc[0] - Cc[0] * xc[1] - Cc[1] * xc[2] - Cc[2] * xc[3]);
+        const double LL1 = (-A[3] * xa[0] - A[11] * xa[1]) + (-B[3] * xb[0] - B[11] * xb[1]) +
+                           (-Cc[3] * xc[0] - Cc[11] * xc[1]);
+        const double LL2 = (-A[4] * xa[0] - A[11] * xa[2]) + (-B[4] * xb[0] - B[11] * xb[2]) +
+                           (-Cc[4] * xc[0] - Cc[11] * xc[2]);
+        const double LL3 = (-A[5] * xa[0] - A[11] * xa[3]) + (-B[5] * xb[0] - B[11] * xb[3]) +
+                           (-Cc[5] * xc[0] - Cc[11] * xc[3]);
+        const double LL4 = (-A[6] * xa[0] - A[7] * xa[1] - A[8] * xa[2] - A[9] * xa[3] - A[11] * xa[4]) +
+                           (-B[6] * xb[0] - B[7] * xb[1] - B[8] * xb[2] - B[9] * xb[3] - B[11] * xb[4]) +
+                           (-Cc[6] * xc[0] - Cc[7] * xc[1] - Cc[8] * xc[2] - Cc[9] * xc[3] - Cc[11] * xc[4]);
+        out[0] = Pc.d11 * (Pc.r[0] - LL0);
+        out[1] = Pc.d22 * (Pc.r[1] - LL1);
+        out[2] = Pc.d22 * (Pc.r[2] - LL2);
+        out[3] = Pc.d22 * (Pc.r[3] - LL3);
+        out[4] = Pc.d51 * (Pc.r[0] - LL0) + Pc.d22 * (Pc.r[4] - LL4);
+    } else {
+        const double UU0 = (+A[10] * xa[0] + A[0] * xa[1] + A[1] * xa[2] + A[2] * xa[3]) +
+                           (+B[10] * xb[0] + B[0] * xb[1] + B[1] * xb[2] + B[2] * xb[3]) +
+                           (+Cc[10] * xc[0] + Cc[0] * xc[1] + Cc[1] * xc[2] + Cc[2] * xc[3]);
+        const double UU1 = (+A[3] * xa[0] + A[11] * xa[1]) + (+B[3] * xb[0] + B[11] * xb[1]) +
+                           (+Cc[3] * xc[0] + Cc[11] * xc[1]);
+        const double UU2 = (+A[4] * xa[0] + A[11] * xa[2]) + (+B[4] * xb[0] + B[11] * xb[2]) +
+                           (+Cc[4] * xc[0] + Cc[11] * xc[2]);
+        const double UU3 = (+A[5] * xa[0] + A[11] * xa[3]) + (+B[5] * xb[0] + B[11] * xb[3]) +
+                           (+Cc[5] * xc[0] + Cc[11] * xc[3]);
+        const double UU4 = (+A[6] * xa[0] + A[7] * xa[1] + A[8] * xa[2] + A[9] * xa[3] + A[11] * xa[4]) +
+                           (+B[6] * xb[0] + B[7] * xb[1] + B[8] * xb[2] + B[9] * xb[3] + B[11] * xb[4]) +
+                           (+Cc[6] * xc[0] + Cc[7] * xc[1] + Cc[8] * xc[2] + Cc[9] * xc[3] + Cc[11] * xc[4]);
+        out[0] = Pc.r[0] - Pc.d11 * UU0;
+        out[1] = Pc.r[1] - Pc.d22 * UU1;
+        out[2] = Pc.r[2] - Pc.d22 * UU2;
+        out[3] = Pc.r[3] - Pc.d22 * UU3;
+        out[4] = Pc.r[4] - (Pc.d51 * UU0 + Pc.d22 * UU4);
+    }
+    const bool pub = act && L.edge_out;
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        double *t = w.trash + v * (TJ * TK) + L.tid;
+        double *px = act ? w.X.v[v] + c : t;
+        double *pe = pub ? w.xbuf + 5 * c + v : t;
+        __hip_atomic_store(reinterpret_cast<unsigned long long *>(pe),
+                           (unsigned long long)__double_as_longlong(out[v]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        *px = out[v];
+        sx[PAR][v][L.tid] = out[v];
+        prev[v] = act ? out[v] : prev[v];
+    }
+}
+
+// trash: WAVE_TRASH doubles of scratch that no one reads
+template <int TJ, int TK, bool FWD>
+__global__ void __launch_bounds__(TJ *TK) k_lusgs_wave(Geo g, const double *__restrict__ coef, C5 R, F5 X,
+                                                     double *xbuf, double *trash, const int *__restrict__ order,
+                                                     int *ticket, int *err, int max_spin)
+{
+    static_assert(5 * TJ * TK <= WAVE_TRASH, "trash buffer too small");
+    __shared__ int s_t;
+    __shared__ double sx[2][5][TJ * TK];
+    Lane L;
+    L.tid = threadIdx.x;
+    L.kl = L.tid % TK;
+    L.jl = L.tid / TK;
+    if (L.tid == 0) s_t = atomicAdd(ticket, 1);
+    __syncthreads();
+    const int packed = order[s_t];
+    const int TJi = packed >> 16, TKi = packed & 0xffff;
+    const int jo = TJi * TJ + L.jl, ko = TKi * TK + L.kl;
+    L.col = (jo < g.ny) && (ko < g.nz);
+    const int joc = jo < g.ny ? jo : g.ny - 1, koc = ko < g.nz ? ko : g.nz - 1;
+    const int j = FWD ? 2 + joc : g.NY - joc;
+    const int k = FWD ? 2 + koc : g.NZ - koc;
+    L.c0 = ix(g, 0, j, k);
+    L.edge_out = (L.jl == TJ - 1) || (L.kl == TK - 1);
+    WaveArgs w;
+    w.g = g;
+    w.coef = coef;
+    w.R = R;
+    w.X = X;
+    w.xbuf = xbuf;
+    w.trash = trash;
+    w.err = err;
+    w.max_spin = max_spin;
+    w.sxs = (long)g.Ym * g.Zm;
+    w.d = FWD ? -1 : 1;   // towards the upstream neighbours
+    w.oA = w.d * w.sxs * NREC;
+    w.oB = w.d * g.Zm * NREC;
+    w.oC = w.d * NREC;
+    const int S = g.nx + TJ + TK - 2;
+    double prev[5];
+    Pref P0, P1;
+    unsigned long long nb0[5], nb1[5], nc0[5], nc1[5];
+    {
+        const long c = cell_at<FWD>(w, L, 0);
+#pragma unroll
+        for (int v = 0; v < 5; v++) prev[v] = X.v[v][c + w.d * w.sxs];   // ghost plane (zero)
+        const long cs = cell_at<FWD>(w, L, -(L.jl + L.kl));               // the cell of step 0
+        load_pref<FWD>(P0, coef + cs * NREC, w.oA, w.oB, w.oC, R, cs);
+        issue5(xbuf, cs + w.oB / NREC, nb0);
+        issue5(xbuf, cs + w.oC / NREC, nc0);
+    }
+    for (int s = 0; s < S; s += 2) {
+        wave_step<TJ, TK, FWD, 0>(w, L, s, prev, sx, P0, P1, nb0, nb1, nc0, nc1);
+        if (s + 1 < S) wave_step<TJ, TK, FWD, 1>(w, L, s + 1, prev, sx, P1, P0, nb1, nb0, nc1, nc0);
+    }
+}
+
+// tile order: anti-diagonals J+K ascending, J ascending inside
+static std::vector<int> tile_order(int ntj, int ntk)
+{
+    std::vector<int> o;
+    for (int s = 0; s <= ntj + ntk - 2; s++)
+        for (int J = 0; J < ntj; J++) {
+            const int K = s - J;
+            if (K >= 0 && K < ntk) o.push_back((J << 16) | K);
+        }
+    return o;
+}
+
+int wave_tiles(const Geo &g, int tj, int tk, std::vector<int> &order)
+{
+    const int ntj = (g.ny + tj - 1) / tj, ntk = (g.nz + tk - 1) / tk;
+    order = tile_order(ntj, ntk);
+    return ntj * ntk;
+}
+
+void launch_split_coef(const Geo &g, const Phys &p, double deltaT, double e, C5 U_, const double *J, M9 met,
+                       double *coef, hipStream_t s)
+{
+    dim3 b(64, 4);
+    dim3 gr((g.NZZ + b.x - 1) / b.x, (g.NYY + b.y - 1) / b.y, g.gend);   // i=2..gend+1
+    hipLaunchKernelGGL(k_split_coef, gr, b, 0, s, g, p, deltaT, e, U_, J, met, coef);
+}
+
+void launch_lusgs_wave(const Geo &g, int forward, const double *coef, C5 R, F5 X, double *xbuf, double *trash,
+                       const int *order, int ntiles, int *counters, int max_spin, hipStream_t s)
+{
+    // counters[0]: ticket, counters[1]: error flag (cleared by the caller)
+    dim3 bf(64, 4);
+    dim3 gf((g.NZ - 1 + bf.x - 1) / bf.x, (g.NY - 1 + bf.y - 1) / bf.y, g.nx);
+    hipLaunchKernelGGL(k_fill_sentinel, gf, bf, 0, s, g, xbuf);
+    hipLaunchKernelGGL(k_reset_ints, dim3(1), dim3(64), 0, s, counters, 1);
+    if (forward)
+        hipLaunchKernelGGL((k_lusgs_wave<WAVE_TJ, WAVE_TK, true>), dim3(ntiles), dim3(WAVE_TJ * WAVE_TK), 0, s, g,
+                           coef, R, X, xbuf, trash, order, counters, counters + 1, max_spin);
+    else
+        hipLaunchKernelGGL((k_lusgs_wave<WAVE_TJ, WAVE_TK, false>), dim3(ntiles), dim3(WAVE_TJ * WAVE_TK), 0, s, g,
+                           coef, R, X, xbuf, trash, order, counters, counters + 1, max_spin);
+}
+
+void launch_reset_ints(int *p, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reset_ints, dim3(1), dim3(64), 0, s, p, n);
 }
 
 }  // namespace cgli

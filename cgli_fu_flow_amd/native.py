@@ -80,6 +80,8 @@ def lib():
         L.cgli_channel_init.argtypes = [vp, C.POINTER(Channel), dp]
         L.cgli_channel_ghosts.argtypes = [vp]
         L.cgli_channel_step.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), dp, dp]
+        L.cgli_set_lusgs_mode.argtypes = [vp, C.c_int]
+        L.cgli_set_spin_limit.argtypes = [vp, C.c_int]
         L.cgli_timing_enable.argtypes = [vp, C.c_int]
         L.cgli_timing_reset.argtypes = [vp]
         L.cgli_timing_get.argtypes = [vp, C.c_int, dp, C.POINTER(C.c_long)]
@@ -225,7 +227,14 @@ class Context:
 
     # -- timing (HIP events around each launch class)
     TIMING_CLASSES = ["flux_x", "flux_y", "flux_z", "viscous", "rhs_diag", "lusgs_fwd", "lusgs_bwd",
-                      "update_norm"]
+                      "update_norm", "split_coef"]
+
+    def set_lusgs_mode(self, mode):
+        """0: one launch per hyperplane, 1: tiled wavefront (default)."""
+        _check(lib().cgli_set_lusgs_mode(self.h, mode), "set_lusgs_mode")
+
+    def set_spin_limit(self, n):
+        _check(lib().cgli_set_spin_limit(self.h, n), "set_spin_limit")
 
     def timing(self, on=True):
         _check(lib().cgli_timing_enable(self.h, int(on)), "timing_enable")

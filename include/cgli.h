@@ -109,6 +109,12 @@ int cgli_hyperplane(cgli_ctx *ctx, int *Nijk, int *ijk);
  * write_residual != 0 also stores the squared increments in Residual. */
 int cgli_dual_time_stepping(cgli_ctx *ctx, double deltaT, double e, double er[5], int sweeps,
                             int write_residual);
+/* LU-SGS schedule: 0 = one launch per i+j+k hyperplane (the reference's
+ * order), 1 = tiled wavefront in one launch per sweep (default).  Both
+ * give identical bits.  The wavefront's bounded waits give up after
+ * max_spin polls each (then CGLI_E_TIMEOUT). */
+int cgli_set_lusgs_mode(cgli_ctx *ctx, int mode);
+int cgli_set_spin_limit(cgli_ctx *ctx, int max_spin);
 /* DPLUSGS.cpp:18-1238: Jacobi (DP-LUR) variant with `sweeps` sweeps (the
  * reference runs 4). */
 int cgli_dplusgs(cgli_ctx *ctx, double deltaT, double er[5], double e, int sweeps);
@@ -158,7 +164,7 @@ int cgli_channel_step(cgli_ctx *ctx, double deltaT, double tol, int max_inner, i
 /* ---- timing ---------------------------------------------------------- */
 /* Average device time (ms) of the last `n` launches of a kernel class
  * recorded since cgli_timing_reset; classes: 0 flux_x 1 flux_y 2 flux_z
- * 3 viscous 4 rhs_diag 5 lusgs_fwd 6 lusgs_bwd 7 update_norm. */
+ * 3 viscous 4 rhs_diag 5 lusgs_fwd 6 lusgs_bwd 7 update_norm 8 split_coef. */
 int cgli_timing_enable(cgli_ctx *ctx, int on);
 int cgli_timing_reset(cgli_ctx *ctx);
 int cgli_timing_get(cgli_ctx *ctx, int cls, double *total_ms, long *count);

@@ -217,3 +217,31 @@ def test_missing_input_is_an_error(nv, orc):
         with pytest.raises(nv.CgliError) as ei:
             ctx.flux_x(1.0, 1e-4)
         assert ei.value.code == -5
+
+
+@pytest.mark.parametrize("name", [n for n in sorted(CASES) if CASES[n][3] == 1])
+def test_wavefront_and_plane_schedules_agree(nv, orc, name):
+    """The tiled wavefront visits cells in another topological order of the
+    same dependencies as the reference's plane order, with the grid-spacing
+    division folded into its coefficients: results agree to rounding."""
+    print("case", name, flush=True)
+    case = make_case(name)
+    f = make_inputs(orc, case.slab, SEED + 6)
+    outs = []
+    for mode in (0, 1):
+        with open_ctx(nv, orc, case) as ctx:
+            ctx.set_lusgs_mode(mode)
+            upload_inputs(ctx, f)
+            ctx.flux_x(1.0, 1e-4)
+            ctx.flux_y(1.0, 1e-4)
+            ctx.flux_z(1.0, 1e-4)
+            ctx.viscous()
+            er = ctx.dual_time_stepping(2e-4, 1e-4, sweeps=2)
+            outs.append((er, ctx.get_many(nv.NL, 5), ctx.get_many(nv.NR, 5), ctx.get_many(nv.U_, 5)))
+    reg = regions(case.slab)["cell"]
+    for v in range(5):
+        assert abs(outs[0][0][v] - outs[1][0][v]) <= 1e-13 * abs(outs[0][0][v])
+        for q in (1, 2, 3):
+            a, b = outs[0][q][v][reg], outs[1][q][v][reg]
+            assert np.all(np.isfinite(b))
+            assert rel(b, a) <= 1e-13, (q, v, rel(b, a))
