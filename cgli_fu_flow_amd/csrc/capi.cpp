@@ -37,7 +37,7 @@ struct cgli_ctx {
     std::vector<int> Nijk;       // (gend+NY+NZ+8) x 2
     std::vector<int> ijk;        // (cells+1) x 3
     int lusgs_mode = 1;          // 0: one launch per hyperplane, 1: tiled wavefront
-    double *coef = nullptr;      // 46 doubles per cell: split coefficients and diagonal
+    double *coef = nullptr;      // coefficient blocks of the wavefront: split coefficients and diagonal
     double *xbuf = nullptr;      // 5 fields: tile-edge exchange of the wavefront
     double *trash = nullptr;     // write-only scratch of the wavefront
     int *order_dev = nullptr;    // wavefront tile order
@@ -421,15 +421,15 @@ static int wave_setup(cgli_ctx *c)
 {
     if (c->coef) return CGLI_OK;
     void *p = nullptr;
-    if (hipMalloc(&p, sizeof(double) * 46 * c->g.N) != hipSuccess) return CGLI_E_NOMEM;
+    if (hipMalloc(&p, sizeof(double) * coef_doubles(c->g)) != hipSuccess) return CGLI_E_NOMEM;
     c->coef = static_cast<double *>(p);
     if (hipMalloc(&p, sizeof(double) * 5 * c->g.N) != hipSuccess) return CGLI_E_NOMEM;
     c->xbuf = static_cast<double *>(p);
     HIPCHK(hipMemset(c->xbuf, 0, sizeof(double) * 5 * c->g.N));   // ghost entries stay zero
-    if (hipMalloc(&p, sizeof(double) * WAVE_TRASH) != hipSuccess) return CGLI_E_NOMEM;
-    c->trash = static_cast<double *>(p);
     std::vector<int> order;
     c->ntiles = wave_tiles(c->g, WAVE_TJ, WAVE_TK, order);
+    if (hipMalloc(&p, sizeof(double) * 256 * c->ntiles) != hipSuccess) return CGLI_E_NOMEM;
+    c->trash = static_cast<double *>(p);
     if (hipMalloc(&p, sizeof(int) * order.size()) != hipSuccess) return CGLI_E_NOMEM;
     c->order_dev = static_cast<int *>(p);
     HIPCHK(hipMemcpy(c->order_dev, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));

@@ -78,13 +78,14 @@ void launch_replicate_planes(double *dst, long plane, int nplanes, hipStream_t s
 void launch_copy_field(double *dst, const double *src, long n, hipStream_t s);
 
 // wavefront LU-SGS (implicit.hip)
-constexpr int WAVE_TJ = 8, WAVE_TK = 16;
-constexpr int WAVE_TRASH = 1024;   // doubles of write-only scratch of the wavefront sweep
+constexpr int WAVE_TJ = 4, WAVE_TK = 16;
 int wave_tiles(const Geo &g, int tj, int tk, std::vector<int> &order);
 void launch_split_coef(const Geo &g, const Phys &p, double deltaT, double e, C5 U_, const double *J, M9 met,
                        double *coef, hipStream_t s);
-// coef: 46 doubles per cell (coefficient records), xbuf: 5 per cell (tile-edge exchange),
-// trash: WAVE_TRASH doubles
+// doubles of the coefficient blocks of the wavefront sweep
+long coef_doubles(const Geo &g);
+// coef: coefficient blocks, xbuf: 5 fields (tile-edge exchange),
+// trash: 256 doubles per tile
 void launch_lusgs_wave(const Geo &g, int forward, const double *coef, C5 R, F5 X, double *xbuf, double *trash,
                        const int *order, int ntiles, int *counters, int max_spin, hipStream_t s);
 void launch_reset_ints(int *p, int n, hipStream_t s);

@@ -648,11 +648,14 @@ void launch_finish_norm(int nblocks, const double *partials, double DN, int squa
 // =================================================================
 // Wavefront LU-SGS.
 //
-// Pass 1 (k_split_coef, fully parallel): one record of NREC doubles per
-// cell: the cell's split coefficients along x, y and z (14 values each,
-// already divided by the grid spacing of that direction) and, for interior
-// cells, the diagonal (d11, d22, d51).  Records are contiguous per cell so
-// a sweep reads a neighbour's values with a few 16-byte loads.
+// Pass 1 (k_split_coef, fully parallel): per cell 23 pairs of doubles: the
+// cell's split coefficients along x, y and z (14 values each, already
+// divided by the grid spacing of that direction) and, for interior cells,
+// the diagonal (d11, d22, d51).  Cells are grouped in blocks of CB
+// consecutive k (one tile row); inside a block pair p of all CB cells is
+// contiguous, so a tile row reads one pair with a single 16-byte load per
+// lane over 2*CB*8 contiguous bytes, and all data of a cell's block lies
+// within a few kilobytes.
 //
 // Pass 2 (k_lusgs_wave, one launch per sweep): the (j,k) plane is cut into
 // TJ x TK tiles; a workgroup owns one tile, one thread per (j,k) column,
@@ -675,7 +678,21 @@ void launch_finish_norm(int nblocks, const double *partials, double DN, int squa
 // division by dxi/deta/dzeta is applied to each coefficient instead of to
 // each partial sum: results agree to rounding (a few ulp).
 // =================================================================
-constexpr int NREC = 46;   // 3 x 14 split values, d11, d22, d51, pad
+constexpr int NPAIR = 23;              // 3 x 7 split pairs, (d11, d22), (d51, pad)
+constexpr int CB = WAVE_TK;            // cells per block along k
+constexpr int CBLK = NPAIR * CB * 2;   // doubles per block
+constexpr int CKOFF = CB - 2;          // the first interior k starts a block
+constexpr long CPAIR = 2 * CB;         // doubles from one pair to the next
+
+// blocks per (i,j) row
+__host__ __device__ inline long coef_nkb(const Geo &g) { return (g.Zm + CKOFF + CB - 1) / CB; }
+
+// offset of pair 0 of cell (i,j,k)
+__device__ __forceinline__ long coef_at(const Geo &g, long nkb, int i, int j, int k)
+{
+    const int kk = k + CKOFF;
+    return (((long)i * g.Ym + j) * nkb + kk / CB) * CBLK + (kk % CB) * 2;
+}
 // slot order of one direction: the ten values both sweeps use, then the
 // forward pair (p11, p22), then the backward pair (m11, m22)
 enum { S_P12, S_P13, S_P14, S_P21, S_P31, S_P41, S_P51, S_P52, S_P53, S_P54, S_P11, S_P22, S_M11, S_M22 };
@@ -690,29 +707,28 @@ __global__ void __launch_bounds__(256) k_split_coef(Geo g, Phys p, double deltaT
     const int i = 2 + blockIdx.z;
     if (k > g.NZZ || j > g.NYY) return;
     const long c = ix(g, i, j, k);
-    double *o = coef + c * NREC;
+    double2 *o = reinterpret_cast<double2 *>(coef + coef_at(g, coef_nkb(g), i, j, k));
     const CP q = cell_prim(U_, J, c, p.K, e);
     const double dd[3] = {p.dxi, p.deta, p.dzeta};
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const SC s = split(q, met.m[3 * d][c], met.m[3 * d + 1][c], met.m[3 * d + 2][c], p.K, d != 0);
         const double h = dd[d];
-        double2 *b = reinterpret_cast<double2 *>(o + 14 * d);
-        b[0] = make_double2(s.p12 / h, s.p13 / h);
-        b[1] = make_double2(s.p14 / h, s.p21 / h);
-        b[2] = make_double2(s.p31 / h, s.p41 / h);
-        b[3] = make_double2(s.p51 / h, s.p52 / h);
-        b[4] = make_double2(s.p53 / h, s.p54 / h);
-        b[5] = make_double2(s.p11 / h, s.p22 / h);
-        b[6] = make_double2(s.m11 / h, s.m22 / h);
+        double2 *b = o + 7 * d * CB;
+        b[0 * CB] = make_double2(s.p12 / h, s.p13 / h);
+        b[1 * CB] = make_double2(s.p14 / h, s.p21 / h);
+        b[2 * CB] = make_double2(s.p31 / h, s.p41 / h);
+        b[3 * CB] = make_double2(s.p51 / h, s.p52 / h);
+        b[4 * CB] = make_double2(s.p53 / h, s.p54 / h);
+        b[5 * CB] = make_double2(s.p11 / h, s.p22 / h);
+        b[6 * CB] = make_double2(s.m11 / h, s.m22 / h);
     }
     if (i >= 3 && i <= g.gend && j >= 2 && j <= g.NY && k >= 2 && k <= g.NZ) {
         double m[9];
         load9(met, c, m);
         const DG D = diag(q, m, p, deltaT);
-        double2 *b = reinterpret_cast<double2 *>(o + 42);
-        b[0] = make_double2(D.d11, D.d22);
-        b[1] = make_double2(D.d51, 0.0);
+        o[21 * CB] = make_double2(D.d11, D.d22);
+        o[22 * CB] = make_double2(D.d51, 0.0);
     }
 }
 
@@ -725,7 +741,7 @@ __global__ void k_fill_sentinel(Geo g, double *__restrict__ xbuf)
     if (k > g.NZ || j > g.NY) return;
     const long c = ix(g, i, j, k);
 #pragma unroll
-    for (int v = 0; v < 5; v++) xbuf[5 * c + v] = __longlong_as_double((long long)SENTINEL);
+    for (int v = 0; v < 5; v++) xbuf[v * g.N + c] = __longlong_as_double((long long)SENTINEL);
 }
 
 __global__ void k_reset_ints(int *p, int n)
@@ -743,11 +759,11 @@ __device__ __forceinline__ void load_dir(Dir12 &s, const double *__restrict__ r)
     const double2 *b = reinterpret_cast<const double2 *>(r);
 #pragma unroll
     for (int q = 0; q < 5; q++) {
-        const double2 t = b[q];
+        const double2 t = b[q * CB];
         s.v[2 * q] = t.x;
         s.v[2 * q + 1] = t.y;
     }
-    const double2 t = b[FWD ? 5 : 6];
+    const double2 t = b[(FWD ? 5 : 6) * CB];
     s.v[10] = t.x;
     s.v[11] = t.y;
 }
@@ -761,10 +777,10 @@ __device__ __forceinline__ void load_pref(Pref &P, const double *__restrict__ re
                                           const C5 &R, long c)
 {
     load_dir<FWD>(P.A, rec + oA);
-    load_dir<FWD>(P.B, rec + oB + 14);
-    load_dir<FWD>(P.C, rec + oC + 28);
-    const double2 *b = reinterpret_cast<const double2 *>(rec + 42);
-    const double2 t0 = b[0], t1 = b[1];
+    load_dir<FWD>(P.B, rec + oB + 7 * CPAIR);
+    load_dir<FWD>(P.C, rec + oC + 14 * CPAIR);
+    const double2 *b = reinterpret_cast<const double2 *>(rec + 21 * CPAIR);
+    const double2 t0 = b[0], t1 = b[CB];
     P.d11 = t0.x;
     P.d22 = t0.y;
     P.d51 = t1.x;
@@ -778,16 +794,17 @@ __device__ __forceinline__ unsigned long long aload(const double *p)
                              __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// issue the five exchange loads of one neighbour without waiting
-__device__ __forceinline__ void issue5(const double *xbuf, long c, unsigned long long b[5])
+// issue the five exchange loads of one neighbour without waiting (the
+// exchange buffer is five fields of N cells)
+__device__ __forceinline__ void issue5(const double *xbuf, long N, long c, unsigned long long b[5])
 {
 #pragma unroll
-    for (int v = 0; v < 5; v++) b[v] = aload(xbuf + 5 * c + v);
+    for (int v = 0; v < 5; v++) b[v] = aload(xbuf + v * N + c);
 }
 
 // where `need`, re-poll the entries that still hold the sentinel (bounded)
 // and take them as the neighbour's values
-__device__ __forceinline__ void settle5(const double *xbuf, long c, bool need, unsigned long long b[5],
+__device__ __forceinline__ void settle5(const double *xbuf, long N, long c, bool need, unsigned long long b[5],
                                         double out[5], int max_spin, int *err)
 {
 #pragma unroll
@@ -795,7 +812,7 @@ __device__ __forceinline__ void settle5(const double *xbuf, long c, bool need, u
         int n = 0;
         while (need && b[v] == SENTINEL && n < max_spin) {
             __builtin_amdgcn_s_sleep(1);
-            b[v] = aload(xbuf + 5 * c + v);
+            b[v] = aload(xbuf + v * N + c);
             n++;
         }
         if (need && b[v] == SENTINEL) atomicOr(err, 1);
@@ -808,6 +825,8 @@ struct Lane {
     int tid, jl, kl;
     bool col, edge_out;
     long c0;                // cell (i = 0, j, k), clamped into the domain
+    long r0;                // coefficient offset of the column's first cell
+    long tline;             // this wave's line of the trash buffer
 };
 
 struct WaveArgs {
@@ -818,23 +837,29 @@ struct WaveArgs {
     double *xbuf, *trash;
     int *err;
     int max_spin;
-    long sxs, oA, oB, oC, d;
+    long sxs, oA, oB, oC, d, dcol;   // dcol: coefficient offset of the next cell of a column
 };
 
-// cell of local step il (clamped to the column, so every lane has a valid
-// address at every step)
+// local step il clamped to the column, so every lane has a valid address
+// at every step
+__device__ __forceinline__ int icl(const WaveArgs &w, int il)
+{
+    return il < 0 ? 0 : (il >= w.g.nx ? w.g.nx - 1 : il);
+}
+
+// cell of local step il
 template <bool FWD>
 __device__ __forceinline__ long cell_at(const WaveArgs &w, const Lane &L, int il)
 {
-    const int c = il < 0 ? 0 : (il >= w.g.nx ? w.g.nx - 1 : il);
+    const int c = icl(w, il);
     return L.c0 + (long)(FWD ? 3 + c : w.g.gend - c) * w.sxs;
 }
 
 // FWD: X = NL from R = MR, NL = D (MR - L NL).  BWD: X = NR from R = NL,
-// NR = NL - D U NR.  One step of the march, the same instruction stream
-// on every lane: loads are issued unconditionally (clamped addresses) and
-// lanes with nothing to do store into a trash buffer, so no branch
-// separates a prefetch from its use.  PAR is the step's parity (which LDS
+// NR = NL - D U NR.  One step of the march.  Coefficient loads are issued
+// by every lane (clamped addresses) and lanes with nothing to do store X
+// into a trash line, so no branch separates a prefetch from its use; the
+// tile-edge loads and stores are confined to the lanes that need them.  PAR is the step's parity (which LDS
 // half is written); Pc/nbc/ncc hold the values loaded for this step,
 // Pn/nbn/ncn receive the next step's.
 template <int TJ, int TK, bool FWD, int PAR>
@@ -853,13 +878,14 @@ __device__ __forceinline__ void wave_step(const WaveArgs &w, const Lane &L, int 
         xb[v] = sx[PAR ^ 1][v][tb];
         xc[v] = sx[PAR ^ 1][v][tc];
     }
-    settle5(w.xbuf, c + w.oB / NREC, act && L.jl == 0, nbc, xb, w.max_spin, w.err);
-    settle5(w.xbuf, c + w.oC / NREC, act && L.kl == 0, ncc, xc, w.max_spin, w.err);
-    {   // prefetch for the next step
+    settle5(w.xbuf, w.g.N, c + w.d * w.g.Zm, act && L.jl == 0, nbc, xb, w.max_spin, w.err);
+    settle5(w.xbuf, w.g.N, c + w.d, act && L.kl == 0, ncc, xc, w.max_spin, w.err);
+    {   // prefetch for the next step: the tile-edge values first (only the
+        // lanes that consume them), then the coefficients of every lane
         const long cn = cell_at<FWD>(w, L, il + 1);
-        load_pref<FWD>(Pn, w.coef + cn * NREC, w.oA, w.oB, w.oC, w.R, cn);
-        issue5(w.xbuf, cn + w.oB / NREC, nbn);
-        issue5(w.xbuf, cn + w.oC / NREC, ncn);
+        if (L.jl == 0) issue5(w.xbuf, w.g.N, cn + w.d * w.g.Zm, nbn);
+        if (L.kl == 0) issue5(w.xbuf, w.g.N, cn + w.d, ncn);
+        load_pref<FWD>(Pn, w.coef + L.r0 + (long)icl(w, il + 1) * w.dcol, w.oA, w.oB, w.oC, w.R, cn);
     }
     const double *A = Pc.A.v, *B = Pc.B.v, *Cc = Pc.C.v;
     const double *xa = prev;
@@ -901,28 +927,31 @@ __device__ __forceinline__ void wave_step(const WaveArgs &w, const Lane &L, int 
         out[3] = Pc.r[3] - Pc.d22 * UU3;
         out[4] = Pc.r[4] - (Pc.d51 * UU0 + Pc.d22 * UU4);
     }
-    const bool pub = act && L.edge_out;
 #pragma unroll
     for (int v = 0; v < 5; v++) {
-        double *t = w.trash + v * (TJ * TK) + L.tid;
-        double *px = act ? w.X.v[v] + c : t;
-        double *pe = pub ? w.xbuf + 5 * c + v : t;
-        __hip_atomic_store(reinterpret_cast<unsigned long long *>(pe),
-                           (unsigned long long)__double_as_longlong(out[v]), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        double *px = act ? w.X.v[v] + c : w.trash + L.tline + v;
         *px = out[v];
         sx[PAR][v][L.tid] = out[v];
         prev[v] = act ? out[v] : prev[v];
     }
+    if (act && L.edge_out) {
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            __hip_atomic_store(reinterpret_cast<unsigned long long *>(w.xbuf + v * w.g.N + c),
+                               (unsigned long long)__double_as_longlong(out[v]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
-// trash: WAVE_TRASH doubles of scratch that no one reads
+// trash: 256 doubles per tile of scratch that no one reads; lanes of one
+// wave with nothing to store all write the same 128-byte line, so a store
+// instruction costs one extra line and waves never share one
 template <int TJ, int TK, bool FWD>
 __global__ void __launch_bounds__(TJ *TK) k_lusgs_wave(Geo g, const double *__restrict__ coef, C5 R, F5 X,
                                                      double *xbuf, double *trash, const int *__restrict__ order,
                                                      int *ticket, int *err, int max_spin)
 {
-    static_assert(5 * TJ * TK <= WAVE_TRASH, "trash buffer too small");
+    static_assert((TJ * TK + 63) / 64 <= 16, "one trash line per wave, at most 16 waves");
     __shared__ int s_t;
     __shared__ double sx[2][5][TJ * TK];
     Lane L;
@@ -940,6 +969,7 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_wave(Geo g, const double *__re
     const int k = FWD ? 2 + koc : g.NZ - koc;
     L.c0 = ix(g, 0, j, k);
     L.edge_out = (L.jl == TJ - 1) || (L.kl == TK - 1);
+    L.tline = ((long)blockIdx.x * 16 + L.tid / 64) * 16;
     WaveArgs w;
     w.g = g;
     w.coef = coef;
@@ -951,9 +981,15 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_wave(Geo g, const double *__re
     w.max_spin = max_spin;
     w.sxs = (long)g.Ym * g.Zm;
     w.d = FWD ? -1 : 1;   // towards the upstream neighbours
-    w.oA = w.d * w.sxs * NREC;
-    w.oB = w.d * g.Zm * NREC;
-    w.oC = w.d * NREC;
+    {
+        const long nkb = coef_nkb(g);
+        const int i0 = FWD ? 3 : g.gend;
+        w.dcol = -w.d * g.Ym * nkb * CBLK;
+        w.oA = -w.dcol;
+        w.oB = w.d * nkb * CBLK;
+        L.r0 = coef_at(g, nkb, i0, j, k);
+        w.oC = coef_at(g, nkb, i0, j, k + (int)w.d) - L.r0;
+    }
     const int S = g.nx + TJ + TK - 2;
     double prev[5];
     Pref P0, P1;
@@ -963,9 +999,9 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_wave(Geo g, const double *__re
 #pragma unroll
         for (int v = 0; v < 5; v++) prev[v] = X.v[v][c + w.d * w.sxs];   // ghost plane (zero)
         const long cs = cell_at<FWD>(w, L, -(L.jl + L.kl));               // the cell of step 0
-        load_pref<FWD>(P0, coef + cs * NREC, w.oA, w.oB, w.oC, R, cs);
-        issue5(xbuf, cs + w.oB / NREC, nb0);
-        issue5(xbuf, cs + w.oC / NREC, nc0);
+        if (L.jl == 0) issue5(xbuf, g.N, cs + w.d * g.Zm, nb0);
+        if (L.kl == 0) issue5(xbuf, g.N, cs + w.d, nc0);
+        load_pref<FWD>(P0, coef + L.r0 + (long)icl(w, -(L.jl + L.kl)) * w.dcol, w.oA, w.oB, w.oC, R, cs);
     }
     for (int s = 0; s < S; s += 2) {
         wave_step<TJ, TK, FWD, 0>(w, L, s, prev, sx, P0, P1, nb0, nb1, nc0, nc1);
@@ -984,6 +1020,8 @@ static std::vector<int> tile_order(int ntj, int ntk)
         }
     return o;
 }
+
+long coef_doubles(const Geo &g) { return (long)(g.nx + 6) * g.Ym * coef_nkb(g) * CBLK; }
 
 int wave_tiles(const Geo &g, int tj, int tk, std::vector<int> &order)
 {
