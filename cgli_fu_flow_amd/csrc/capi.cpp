@@ -14,7 +14,7 @@ using namespace cgli;
 
 namespace {
 
-enum { T_FLUX_X, T_FLUX_Y, T_FLUX_Z, T_VISCOUS, T_RHS, T_FWD, T_BWD, T_UPDATE, T_COEF, T_NCLASS };
+enum { T_FLUX_X, T_FLUX_Y, T_FLUX_Z, T_VISCOUS, T_RHS, T_FWD, T_BWD, T_UPDATE, T_COEF, T_FILTER, T_NCLASS };
 
 struct EventPair { hipEvent_t a, b; };
 
@@ -43,6 +43,7 @@ struct cgli_ctx {
     int *order_dev = nullptr;    // wavefront tile order
     int ntiles = 0;
     int *counters = nullptr;     // ticket, error flag
+    double *filt = nullptr;      // filter work arrays (allocated on first use)
     int *counters_host = nullptr;
     int max_spin = 1000000;
     bool timing = false;
@@ -229,6 +230,7 @@ int cgli_destroy(cgli_ctx *c)
     if (c->coef) hipFree(c->coef);
     if (c->xbuf) hipFree(c->xbuf);
     if (c->trash) hipFree(c->trash);
+    if (c->filt) hipFree(c->filt);
     if (c->order_dev) hipFree(c->order_dev);
     if (c->counters) hipFree(c->counters);
     if (c->counters_host) hipHostFree(c->counters_host);
@@ -342,6 +344,28 @@ int cgli_viscous(cgli_ctx *c)
     tend(c, T_VISCOUS);
     HIPCHK(hipGetLastError());
     mark(c, CGLI_VF, 4);
+    return CGLI_OK;
+}
+
+int cgli_filter(cgli_ctx *c, double Roe_criterion, double E_high, double E_low)
+{
+    if (!c) return CGLI_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    if (!have(c, CGLI_U_, 4) || !have(c, CGLI_J, 1) || !have(c, CGLI_YPOINT, 1)) return CGLI_E_STATE;
+    if (c->g.nx < 3) return CGLI_E_ARG;
+    int r = ensure_range(c, CGLI_EPX, 3);
+    if (r) return r;
+    if (!c->filt) {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(double) * filter_scratch_doubles(c->g)) != hipSuccess) return CGLI_E_NOMEM;
+        c->filt = static_cast<double *>(p);
+    }
+    tbegin(c, T_FILTER);
+    launch_filter(c->g, Roe_criterion, E_high, E_low, c5(c, CGLI_U_), c->f[CGLI_J], c->f[CGLI_YPOINT], c->filt,
+                  c->f[CGLI_EPX], c->f[CGLI_EPY], c->f[CGLI_EPZ], c->stream);
+    tend(c, T_FILTER);
+    HIPCHK(hipGetLastError());
+    mark(c, CGLI_EPX, 3);
     return CGLI_OK;
 }
 

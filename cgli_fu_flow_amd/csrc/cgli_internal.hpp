@@ -77,6 +77,11 @@ void launch_zero(double *p, long n, hipStream_t s);
 void launch_replicate_planes(double *dst, long plane, int nplanes, hipStream_t s);
 void launch_copy_field(double *dst, const double *src, long n, hipStream_t s);
 
+// filter sensor (filter.hip); work holds filter_scratch_doubles(g) doubles
+long filter_scratch_doubles(const Geo &g);
+void launch_filter(const Geo &g, double crit, double E_high, double E_low, C5 U_, const double *J,
+                   const double *Y, double *work, double *EpX, double *EpY, double *EpZ, hipStream_t s);
+
 // wavefront LU-SGS (implicit.hip)
 constexpr int WAVE_TJ = 4, WAVE_TK = 16;
 int wave_tiles(const Geo &g, int tj, int tk, std::vector<int> &order);

@@ -68,6 +68,7 @@ def lib():
         for n in ("cgli_flux_x", "cgli_flux_y", "cgli_flux_z"):
             getattr(L, n).argtypes = [vp, C.c_double, C.c_double]
         L.cgli_viscous.argtypes = [vp]
+        L.cgli_filter.argtypes = [vp, C.c_double, C.c_double, C.c_double]
         L.cgli_hyperplane.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.cgli_dual_time_stepping.argtypes = [vp, C.c_double, C.c_double, dp, C.c_int, C.c_int]
         L.cgli_dplusgs.argtypes = [vp, C.c_double, dp, C.c_double, C.c_int]
@@ -178,6 +179,11 @@ class Context:
     def viscous(self):
         _check(lib().cgli_viscous(self.h), "viscous")
 
+    def filter(self, Roe_criterion=0.007, E_high=1.0, E_low=0.1):
+        """Filter.cpp:12-1720: EpX/EpY/EpZ from U_, J and Y_point."""
+        _check(lib().cgli_filter(self.h, C.c_double(Roe_criterion), C.c_double(E_high), C.c_double(E_low)),
+               "filter")
+
     def dual_time_stepping(self, deltaT, e, sweeps=1, write_residual=False):
         er = (C.c_double * 5)()
         _check(lib().cgli_dual_time_stepping(self.h, deltaT, e, er, sweeps, int(write_residual)),
@@ -227,7 +233,7 @@ class Context:
 
     # -- timing (HIP events around each launch class)
     TIMING_CLASSES = ["flux_x", "flux_y", "flux_z", "viscous", "rhs_diag", "lusgs_fwd", "lusgs_bwd",
-                      "update_norm", "split_coef"]
+                      "update_norm", "split_coef", "filter"]
 
     def set_lusgs_mode(self, mode):
         """0: one launch per hyperplane, 1: tiled wavefront (default)."""

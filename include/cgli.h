@@ -99,6 +99,11 @@ int cgli_flux_y(cgli_ctx *ctx, double Ep, double e);
 int cgli_flux_z(cgli_ctx *ctx, double Ep, double e);
 /* Viscous_terms.cpp:15-908: reads U_, J and the face metrics; writes vF. */
 int cgli_viscous(cgli_ctx *ctx);
+/* Filter.cpp:12-1720: the multi-scale filter sensor.  Reads U_ (U1..U4),
+ * J and Y_point; writes EpX/EpY/EpZ = E_high where the h/2h filtered
+ * kinetic-energy ratio is >= Roe_criterion, else E_low (the reference's
+ * defaults are 0.007, 1, 0.1, main.cpp:60-62).  Needs nx >= 3. */
+int cgli_filter(cgli_ctx *ctx, double Roe_criterion, double E_high, double E_low);
 /* Hyperplane.cpp:18-81: writes the reference's 1-based plane lists to host
  * arrays Nijk[(gend+ny+nz+8)][2] and ijk[nx*ny*nz+1][3]. */
 int cgli_hyperplane(cgli_ctx *ctx, int *Nijk, int *ijk);
@@ -164,7 +169,8 @@ int cgli_channel_step(cgli_ctx *ctx, double deltaT, double tol, int max_inner, i
 /* ---- timing ---------------------------------------------------------- */
 /* Average device time (ms) of the last `n` launches of a kernel class
  * recorded since cgli_timing_reset; classes: 0 flux_x 1 flux_y 2 flux_z
- * 3 viscous 4 rhs_diag 5 lusgs_fwd 6 lusgs_bwd 7 update_norm 8 split_coef. */
+ * 3 viscous 4 rhs_diag 5 lusgs_fwd 6 lusgs_bwd 7 update_norm 8 split_coef
+ * 9 filter. */
 int cgli_timing_enable(cgli_ctx *ctx, int on);
 int cgli_timing_reset(cgli_ctx *ctx);
 int cgli_timing_get(cgli_ctx *ctx, int cls, double *total_ms, long *count);

@@ -100,6 +100,9 @@ void orc_dplusgs(const orc_cfg *c, double deltaT, double er[5], double e,
 void orc_filter(const orc_cfg *c, double Roe_criterion, double E_high, double E_low,
                 double *const U_[5], const double *Y_point,
                 double *EpX, double *EpY, double *EpZ, const double *J);
+/* E_h and E_2h (with their ghost cells) that the sensor compares */
+void orc_filter_energy(const orc_cfg *c, double *const U_[5], const double *Y_point, const double *J,
+                       double *Eh, double *E2h);
 
 /* --- metrics (a12) ----------------------------------------------------- */
 /* Reference analytic tanh grid (Jacobian*.cpp), gamma1 = 2.8. met = 9

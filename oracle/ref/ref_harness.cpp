@@ -3,7 +3,8 @@
 // C entry points around the reference solver's own routines, compiled from
 // the reference sources where they lie (see build_ref.sh).  It lets the
 // golden-vector script call Flux_X/Y/Z, Viscous_terms, Dual_time_stepping,
-// Hyperplane, Jacobian*, Y_/Z_boundary_condition on arrays it owns.  The
+// Hyperplane, Jacobian*, Y_/Z_boundary_condition and Filter on arrays it
+// owns.  The
 // reference's grid sizes are compile-time macros (Resolution.h), so one
 // library is built per size; ref_dims() reports them.
 #include <mpi.h>
@@ -124,6 +125,14 @@ void ref_bc_yz(int which, int xnp, int myid, double deltaT, double e, double **U
                              a(Uq[0]), a(Uq[1]), a(Uq[2]), a(Uq[3]), a(Uq[4]), a(etdy), a(J));
     else
         Z_boundary_condition(myid, a(U_[0]), a(U_[1]), a(U_[2]), a(U_[3]), a(U_[4]), a(J));
+}
+
+void ref_filter(int xnp, int myid, double Roe_criterion, double E_high, double E_low, double **U_,
+                double *Y_point, double *EpX, double *EpY, double *EpZ, double *J)
+{
+    X_np = xnp;
+    Filter(myid, Roe_criterion, E_high, E_low, a(U_[0]), a(U_[1]), a(U_[2]), a(U_[3]), a(U_[4]), a(Y_point),
+           a(EpX), a(EpY), a(EpZ), a(J));
 }
 
 // X_out Y_out Z_out Y_m Z_m np nx_inlet nx_outlet.  MPI_prm.h is a

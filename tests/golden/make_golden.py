@@ -6,7 +6,7 @@ It builds (if needed) oracle/_ref/libref_<size>.so from the reference sources
 with oracle/ref/build_ref.sh, fills seeded inputs with the oracle library's
 splitmix64 generator, runs the reference routines (Flux_X/Y/Z,
 Viscous_terms, Hyperplane, Dual_time_stepping, Jacobian*,
-Y_/Z_boundary_condition) and stores their outputs on the index windows they
+Y_/Z_boundary_condition, Filter) and stores their outputs on the index windows they
 define.  Only data is written: golden_<size>.npz plus manifest.json with the
 inputs' seed and SHA-256 of every fixture.
 """
@@ -21,8 +21,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
-from oracle_harness import (Case, REF_DIR, ORACLE_DIR, dp, load_oracle, make_inputs, met9, ptrs,  # noqa: E402
-                            ref_cfg_from, regions, run_chain)
+from oracle_harness import (Case, FILTER_CRITERIA, REF_DIR, ORACLE_DIR, dp, load_oracle, make_inputs,  # noqa: E402
+                            met9, ptrs, ref_cfg_from, regions, run_chain, run_filter)
 
 SEED = 20261203
 # (X_out, Y_out, Z_out, np, nx_inlet, nx_outlet, myid, run Dual_time_stepping)
@@ -84,6 +84,12 @@ def main():
             ref.ref_bc_yz(1, case.xnp, myid, C.c_double(2e-4), C.c_double(1e-4), ptrs(U_), ptrs(f["U"]),
                           ptrs(f["Uq"]), dp(f["etdy"]), dp(f["J"]))
             fx["bc_U_"] = np.stack(U_)
+            # the filter sensor (Filter.cpp): single-slab builds only, an
+            # interior slab's halo exchange needs its MPI neighbours
+            for q, crit in enumerate(FILTER_CRITERIA):
+                Ep = run_filter("ref", ref, case, f, crit)
+                for d, nm in enumerate(("EpX", "EpY", "EpZ")):
+                    fx["%s_%d" % (nm, q)] = Ep[d][reg[nm]]
         # full outputs are pinned by SHA-256; the npz keeps values on windows
         # small enough to commit (all of them for the small slabs)
         digests = {k: sha(v) for k, v in fx.items()}
@@ -96,7 +102,7 @@ def main():
                 sl = case.slab
                 keep["bc_U_jplanes"] = v[:, :, [1, 2, sl.NY, sl.NY + 1], :]
                 keep["bc_U_kplanes"] = v[:, :, :, [1, 2, sl.NZ, sl.NZ + 1]]
-            elif small or k in ("er", "Nijk"):
+            elif small or k in ("er", "Nijk") or k.startswith("Ep"):
                 keep[k] = v
             elif k != "ijk":
                 keep[k] = v[:, :2]                           # first two i planes

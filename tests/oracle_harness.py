@@ -114,6 +114,10 @@ def make_inputs(orc, slab, seed, K=1.4, dimensional=True):
             f[nme + tag] = fill(orc, sh, seed + base + q, lo, hi)
         if tag:
             f["J" + tag] = fill(orc, sh, seed + base + 9, 0.7, 1.3)
+    # grid y coordinate (Filter input): strictly increasing in j with a
+    # seeded jitter, so one-sided spacings differ from cell to cell
+    jj = np.arange(sh[1], dtype=float)[None, :, None]
+    f["Y_point"] = 0.1 * jj + 0.03 * fill(orc, sh, seed + 90, -1.0, 1.0)
     if dimensional:
         r0, r1, ua, p0, p1 = 1.05, 1.35, 30.0, 95000.0, 107000.0
     else:
@@ -244,10 +248,30 @@ def run_chain(backend, lib, case, f, orc=None, Ep=1.0, e=1e-4, deltaT=2e-4, dts=
     return out
 
 
+FILTER_CRITERIA = (0.007, 0.024)   # the reference default, and the median E_h/E_2h of the seeded data
+
+
+def run_filter(backend, lib, case, f, criterion, E_high=1.0, E_low=0.1):
+    """EpX, EpY, EpZ of Filter.cpp on the seeded inputs."""
+    slab = case.slab
+    Ep = [np.full(slab.shape, -7.0) for _ in range(3)]
+    if backend == "ref":
+        lib.ref_filter(case.xnp, case.myid, C.c_double(criterion), C.c_double(E_high), C.c_double(E_low),
+                       ptrs(f["U_"]), dp(f["Y_point"]), dp(Ep[0]), dp(Ep[1]), dp(Ep[2]), dp(f["J"]))
+    else:
+        cfg = case.cfg(lib)
+        lib.orc_filter(C.byref(cfg), C.c_double(criterion), C.c_double(E_high), C.c_double(E_low), ptrs(f["U_"]),
+                       dp(f["Y_point"]), dp(Ep[0]), dp(Ep[1]), dp(Ep[2]), dp(f["J"]))
+    return Ep
+
+
 def regions(slab):
     """Index windows each output is defined on (reference loop ranges)."""
     g, NY, NZ = slab.gend, slab.NY, slab.NZ
     return {
+        "EpX": (slice(2, g + 1), slice(1, NY), slice(1, NZ)),         # Filter.cpp:1564-1583
+        "EpY": (slice(2, g), slice(1, NY + 1), slice(1, NZ)),         # Filter.cpp:1593-1614
+        "EpZ": (slice(2, g), slice(1, NY), slice(1, NZ + 1)),         # Filter.cpp:1624-1643
         "inFx": (slice(2, g + 1), slice(1, NY), slice(1, NZ)),        # Flux_X.cpp:229-245
         "inFy": (slice(2, g), slice(1, NY + 1), slice(1, NZ)),        # Flux_Y.cpp:272-289
         "inFz": (slice(2, g), slice(1, NY), slice(1, NZ + 1)),        # Flux_Z.cpp:230-248

@@ -6,6 +6,7 @@ terms, dual-time LU-SGS, time shift), compared per conserved variable at
 library's setup and are handed to the oracle unchanged.
 """
 import ctypes as C
+from types import SimpleNamespace
 
 import numpy as np
 import pytest
@@ -97,3 +98,33 @@ def test_channel_converges_to_tolerance():
             assert np.all(np.isfinite(en))
             assert np.max(en / e1) <= 1e-6
             print("inner iterations", it, flush=True)
+
+
+def test_channel_filter_sensor_matches_oracle():
+    """The sensor on the channel's own grid (Y_point = wall-normal
+    coordinate) and seeded state, periodic slab semantics."""
+    from cgli_fu_flow_amd import native as nv
+    from oracle_harness import Slab, run_filter
+    orc = load_oracle()
+    ch = nv.channel(180.0, beta=2.2, skew=0.05)
+    cfg = nv.channel_config(ch, 16, 24, 12)
+    with nv.Context(cfg) as ctx:
+        ctx.channel_init(ch)
+        ctx.channel_ghosts()
+        f = {"U_": ctx.get_many(nv.U_, 5), "J": ctx.get(nv.J), "Y_point": ctx.get(nv.YPOINT)}
+        oc = OrcCfg()
+        for name, _ in OrcCfg._fields_:
+            setattr(oc, name, getattr(cfg, name))
+
+        case = SimpleNamespace(slab=Slab(cfg.nx, cfg.ny, cfg.nz), cfg=lambda lib: oc)
+        for crit in (0.007, 0.05):
+            want = run_filter("orc", orc, case, f, crit)
+            ctx.filter(crit, 1.0, 0.1)
+            got = ctx.get_many(nv.EPX, 3)
+            g = cfg.nx + 2
+            NY, NZ = cfg.ny + 1, cfg.nz + 1
+            wins = [(slice(2, g + 1), slice(1, NY), slice(1, NZ)), (slice(2, g), slice(1, NY + 1), slice(1, NZ)),
+                    (slice(2, g), slice(1, NY), slice(1, NZ + 1))]
+            for d in range(3):
+                assert np.array_equal(got[d][wins[d]], want[d][wins[d]]), (d, crit)
+            print("crit", crit, "fraction high", [float(np.mean(got[d][wins[d]] == 1.0)) for d in range(3)])

@@ -10,7 +10,8 @@ import os
 import numpy as np
 import pytest
 
-from oracle_harness import Case, Slab, dp, load_oracle, make_inputs, ptrs, regions, run_chain, IP
+from oracle_harness import (FILTER_CRITERIA, Case, Slab, dp, load_oracle, make_inputs, ptrs, regions, run_chain,
+                            run_filter, IP)
 from gpu_harness import config_from, rel, upload_inputs
 
 pytestmark = pytest.mark.gpu
@@ -245,3 +246,37 @@ def test_wavefront_and_plane_schedules_agree(nv, orc, name):
             a, b = outs[0][q][v][reg], outs[1][q][v][reg]
             assert np.all(np.isfinite(b))
             assert rel(b, a) <= 1e-13, (q, v, rel(b, a))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("periodic", [0, 1])
+def test_filter_matches_oracle(nv, orc, name, periodic):
+    """Filter sensor: identical E_high/E_low decisions on every face of the
+    reference's windows, for both thresholds (the second splits the faces
+    roughly in half)."""
+    print("case", name, "periodic", periodic, flush=True)
+    case = make_case(name)
+    f = make_inputs(orc, case.slab, SEED + 7)
+    over = {"x_periodic": 1, "first": 0, "last": 0} if periodic else {}
+    case.overrides = dict(over)
+    reg = regions(case.slab)
+    with open_ctx(nv, orc, case, **over) as ctx:
+        ctx.set_many(nv.U_, f["U_"])
+        ctx.set(nv.J, f["J"])
+        ctx.set(nv.YPOINT, f["Y_point"])
+        for crit in FILTER_CRITERIA:
+            want = run_filter("orc", orc, case, f, crit)
+            ctx.filter(crit, 1.0, 0.1)
+            got = ctx.get_many(nv.EPX, 3)
+            for d, nm in enumerate(("EpX", "EpY", "EpZ")):
+                g, w = got[d][reg[nm]], want[d][reg[nm]]
+                assert np.array_equal(g, w), (nm, crit, int(np.sum(g != w)))
+
+
+def test_filter_needs_its_inputs(nv, orc):
+    case = make_case("ref_12x16x10")
+    with open_ctx(nv, orc, case) as ctx:
+        ctx.set_many(nv.U_, make_inputs(orc, case.slab, SEED)["U_"])
+        with pytest.raises(nv.CgliError) as ei:
+            ctx.filter()
+        assert ei.value.code == -5

@@ -13,8 +13,8 @@ import os
 import numpy as np
 import pytest
 
-from oracle_harness import (Case, REF_DIR, Slab, dp, hyperplane_sizes, load_oracle, make_inputs, met9,
-                            ptrs, ref_cfg, regions, run_chain, IP)
+from oracle_harness import (Case, FILTER_CRITERIA, REF_DIR, Slab, dp, hyperplane_sizes, load_oracle, make_inputs,
+                            met9, ptrs, ref_cfg, regions, run_chain, run_filter, IP)
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 MANIFEST = json.load(open(os.path.join(GOLD, "manifest.json")))
@@ -226,3 +226,95 @@ def test_hyperplane_lists_every_cell_once_in_plane_order(orc):
     assert sums == sorted(sums)
     for m in range(7, sl.gend + sl.NY + sl.NZ + 1):
         assert N[m, 1] == sum(1 for s in sums if s == m)
+
+
+# ------------------------------------------------------------ filter sensor
+
+@pytest.mark.parametrize("tag", [t for t in sorted(MANIFEST["cases"]) if "np1" in t])
+def test_filter_bit_identical_to_reference(orc, tag):
+    """Filter.cpp on the seeded inputs, two thresholds (both branches)."""
+    entry = MANIFEST["cases"][tag]
+    case = case_from(tag, entry)
+    gold = np.load(os.path.join(GOLD, entry["file"]))
+    f = make_inputs(orc, case.slab, SEED)
+    reg = regions(case.slab)
+    for q, crit in enumerate(FILTER_CRITERIA):
+        Ep = run_filter("orc", orc, case, f, crit)
+        for d, nm in enumerate(("EpX", "EpY", "EpZ")):
+            key = "%s_%d" % (nm, q)
+            got = Ep[d][reg[nm]]
+            assert sha(got) == entry["sha256"][key], key
+            assert np.array_equal(got, gold[key]), key
+            # untouched outside the reference's loop window
+            mask = np.ones(case.slab.shape, bool)
+            mask[reg[nm]] = False
+            assert np.all(Ep[d][mask] == -7.0), key
+    # both branches of the sensor are exercised by the second threshold
+    hi = np.mean(gold["EpX_1"] == 1.0)
+    assert 0.2 < hi < 0.8
+
+
+def filter_energy(orc, cfg, U_, Y, J, shape):
+    Eh, E2h = np.zeros(shape), np.zeros(shape)
+    orc.orc_filter_energy(C.byref(cfg), ptrs(U_), dp(Y), dp(J), dp(Eh), dp(E2h))
+    return Eh, E2h
+
+
+def periodic_slab(orc, nx, ny, nz):
+    c = ref_cfg(orc, nx, ny, nz)
+    c.first = c.last = 0
+    c.x_periodic = 1
+    return c
+
+
+def test_filter_fixed_point_linear_in_y(orc):
+    """rho = 1 and u, v, w linear in the (jittered) y coordinate, stored as
+    Q/J with J varying in y: every level of both scales returns its input
+    (the y pass filters J*G with one-sided spacings, exact for linear
+    data), so E_h = E_2h = 0 everywhere the sensor reads."""
+    nx, ny, nz = 6, 9, 5
+    sl = Slab(nx, ny, nz)
+    cfg = periodic_slab(orc, nx, ny, nz)
+    rng = np.random.default_rng(3)
+    yj = np.cumsum(0.05 + 0.1 * rng.random(sl.shape[1]))
+    Y = np.broadcast_to(yj[None, :, None], sl.shape).copy()
+    J = np.broadcast_to((0.8 + 0.3 * rng.random(sl.shape[1]))[None, :, None], sl.shape).copy()
+    vel = [0.3 + 2.0 * Y, -1.0 + 0.5 * Y, 0.7 - 1.5 * Y]
+    U_ = [1.0 / J] + [v / J for v in vel] + [np.full(sl.shape, 2.5) / J]
+    Eh, E2h = filter_energy(orc, cfg, U_, Y, J, sl.shape)
+    inner = (slice(2, sl.gend + 2), slice(1, sl.NY + 2), slice(1, sl.NZ + 2))
+    assert np.max(Eh[inner]) < 1e-24 and np.max(E2h[inner]) < 1e-24
+    # a dropped J weight or swapped one-sided spacing breaks the fixed point
+    Eh2, _ = filter_energy(orc, cfg, U_, Y, np.ones(sl.shape), sl.shape)
+    assert np.max(Eh2[inner]) > 1e-8
+
+
+@pytest.mark.parametrize("m", [1, 2, 3])
+def test_filter_transfer_function_of_x_mode(orc, m):
+    """u = cos(2 pi m i / nx) on a periodic slab, uniform in y and z, rho = 1:
+    the h level is G = (6 + 2 cos t)/8 and the 2h level (2 + 2 cos t)/4 per
+    pass along x, the other passes are the identity, and the six-level
+    combination is 1-(1-G)^6, so E = ((1-G)^6 u)^2 in closed form.  This
+    pins the pass weights, the level combination and the periodic GG
+    exchange.  The j and k ghosts of every level hold the unfiltered state
+    (the reference's semantics), so the closed form holds beyond the six
+    levels' reach of them: j, k at least 7 cells from a ghost."""
+    nx, ny, nz = 12, 16, 15
+    sl = Slab(nx, ny, nz)
+    cfg = periodic_slab(orc, nx, ny, nz)
+    i = np.arange(sl.shape[0]) - 3                                  # x ghosts carry the periodic images
+    t = 2 * np.pi * m / nx
+    u = np.cos(t * i)[:, None, None] * np.ones(sl.shape)
+    Y = np.broadcast_to(np.arange(sl.shape[1], dtype=float)[None, :, None], sl.shape).copy()
+    J = np.ones(sl.shape)
+    U_ = [np.ones(sl.shape), u, np.zeros(sl.shape), np.zeros(sl.shape), np.full(sl.shape, 2.5)]
+    Eh, E2h = filter_energy(orc, cfg, U_, Y, J, sl.shape)
+    Gh, G2h = (6 + 2 * np.cos(t)) / 8, (2 + 2 * np.cos(t)) / 4
+    core = (slice(3, sl.gend + 1), slice(8, sl.NY - 5), slice(8, sl.NZ - 5))
+    assert Eh[core].size >= 12 * 4 * 3
+    # sqrt(E) = |u - u_f| is a difference of O(1) numbers: absolute rounding
+    # of a few hundred ulp through the six-level combination
+    np.testing.assert_allclose(np.sqrt(Eh[core]), np.abs((1 - Gh) ** 6 * u[core]), rtol=0, atol=1e-13)
+    np.testing.assert_allclose(np.sqrt(E2h[core]), np.abs((1 - G2h) ** 6 * u[core]), rtol=0, atol=1e-13)
+    # and the signal is well above that rounding for these modes
+    assert np.max(np.abs((1 - Gh) ** 6 * u[core])) > 1e-9
