@@ -50,3 +50,24 @@ def test_config_struct_matches_header():
             fields += [(n.strip(), m.group(1)) for n in m.group(2).split(",")]
     ours = [(n, "int" if t is C.c_int else "double") for n, t in native.Config._fields_]
     assert ours == fields
+
+
+def test_reference_signature_shims_compile_as_a_reference_unit(tmp_path):
+    """include/cgli_reference_api.hpp defines the reference's routine names
+    with its parameter types; a unit that defines Y_m/Z_m (Resolution.h)
+    compiles against it, one without them is refused."""
+    import subprocess
+    hdr = os.path.join(ROOT, "include", "cgli_reference_api.hpp")
+    text = open(hdr).read()
+    for name in ("Flux_X", "Flux_Y", "Flux_Z", "Viscous_terms", "Dual_time_stepping", "DPLUSGS", "Filter",
+                 "Hyperplane"):
+        assert re.search(r"inline void %s\(int myid" % name, text), name
+    probe = os.path.join(ROOT, "tests", "ref_api", "shim_probe.cpp")
+    ok = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                         "-DY_m=20", "-DZ_m=14", probe], capture_output=True, text=True)
+    assert ok.returncode == 0, ok.stderr
+    src = tmp_path / "no_dims.cpp"
+    src.write_text('#include "cgli_reference_api.hpp"\n')
+    bad = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I" + os.path.join(ROOT, "include"), str(src)],
+                         capture_output=True, text=True)
+    assert bad.returncode != 0 and "Resolution.h" in bad.stderr
