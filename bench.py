@@ -1,17 +1,19 @@
 """Benchmark: physical steps/s of the compressible channel at a 1e-6 inner drop.
 
-Workload (BASELINE config "plain FP64 GPU path on a curvilinear grid"):
+Workload (the plain FP64 GPU path on a curvilinear grid):
 channel 128x128x128, periodic x (4 pi) and z (2 pi), isothermal walls,
 tanh stretching beta=2.2, skew x' = x + 0.05 sin(2 pi y) sin(2 pi z/Lz),
 Re_tau=180, Mach 0.5, acoustic CFL 0.02, laminar Poiseuille + seeded +-10%
 noise.  Every physical step iterates the dual-time LU-SGS inner loop until
-max_v er_v/er_v(first) <= 1e-6 (or --max-inner).  State is resident in HBM
-before the timed region starts.
+max_v er_v/er_v(first) <= 1e-6 (or --max-inner), after one evaluation of
+the filter sensor.  State is resident in HBM before the timed region starts.
 
 Prints ONE JSON line.  `roofline` is the dominant kernel class measured
 with HIP events on the library's stream over the timed region;
-`cpu_baseline` is the oracle (C restatement, OpenMP) on this host for a
-bounded number of inner iterations at the same size.
+`traffic` is the HBM bytes per launch of that kernel measured by two
+rocprofv3 counter passes (profiles/channel_128_traffic.md) for the default
+workload; `cpu_baseline` is the oracle (C restatement, OpenMP) on this host
+for a bounded number of inner iterations at the same size.
 """
 import argparse
 import ctypes as C
@@ -30,7 +32,8 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic FP64 bytes per interior cell per launch of each kernel class
 # (each input element read once, each output written once)
 ALGO_BYTES = {"flux_x": 120, "flux_y": 120, "flux_z": 120, "viscous": 320, "rhs_diag": 472,
-              "lusgs_fwd": 240, "lusgs_bwd": 240, "update_norm": 128, "split_coef": 480}
+              "lusgs_fwd": 240, "lusgs_bwd": 240, "update_norm": 128, "split_coef": 480, "filter": 72}
+TRAFFIC = os.path.join(ROOT, "profiles", "channel_128_traffic.json")
 
 
 def parse():
@@ -49,11 +52,13 @@ def parse():
     ap.add_argument("--cfl", type=float, default=0.02)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-filter", action="store_true", help="skip the per-step filter sensor")
     return ap.parse_args()
 
 
-def cpu_baseline(ctx, nv, cfg, dt, inner_per_step, budget_s):
-    """Oracle inner iterations on the host, same grid and state as the GPU run."""
+def cpu_baseline(ctx, nv, cfg, dt, inner_per_step, budget_s, with_filter):
+    """Oracle inner iterations on the host, same grid and state as the GPU run
+    (plus one filter evaluation per physical step when the GPU run has one)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_harness import OrcCfg, dp, load_oracle, ptrs
     orc = load_oracle()
@@ -97,13 +102,22 @@ def cpu_baseline(ctx, nv, cfg, dt, inner_per_step, budget_s):
         if t_total >= budget_s or iters >= 50:
             break
     t_inner = t_total / iters
-    steps_per_s = 1.0 / (t_inner * inner_per_step)
+    t_filter = 0.0
+    if with_filter:
+        Y = ctx.get(nv.YPOINT)
+        Ep = [np.zeros(ctx.shape) for _ in range(3)]
+        t0 = time.perf_counter()
+        orc.orc_filter(C.byref(oc), C.c_double(0.007), C.c_double(1.0), C.c_double(0.1), ptrs(U_), dp(Y),
+                       dp(Ep[0]), dp(Ep[1]), dp(Ep[2]), dp(f["J"]))
+        t_filter = time.perf_counter() - t0
+    steps_per_s = 1.0 / (t_inner * inner_per_step + t_filter)
     ncell = cfg.nx * cfg.ny * cfg.nz
     return {"value": steps_per_s, "unit": "physical steps/s", "cores": threads, "kind": "port",
             "mcell_updates_per_s": ncell * steps_per_s / 1e6,
             "sample": "%d inner iterations (ghosts, fluxes, viscous, dual-time LU-SGS) of the oracle at %dx%dx%d "
                       "on %d OpenMP threads, %.3f s each, scaled by the GPU run's %.2f inner iterations per "
-                      "physical step" % (iters, cfg.nx, cfg.ny, cfg.nz, threads, t_inner, inner_per_step)}
+                      "physical step, plus %.3f s for one filter evaluation"
+                      % (iters, cfg.nx, cfg.ny, cfg.nz, threads, t_inner, inner_per_step, t_filter)}
 
 
 def main():
@@ -119,6 +133,8 @@ def main():
     dt = ctx.channel_init(ch)
     print("setup done: dt=%.4e" % dt, file=sys.stderr, flush=True)
     for s in range(a.warmup):
+        if not a.no_filter:
+            ctx.filter()
         it, _, _ = ctx.channel_step(dt, a.tol, a.max_inner)
         print("warmup step %d: %d inner iterations" % (s + 1, it), file=sys.stderr, flush=True)
     ctx.timing(True)
@@ -126,6 +142,8 @@ def main():
     t0 = time.perf_counter()
     inner = []
     for s in range(a.steps):
+        if not a.no_filter:
+            ctx.filter()
         it, e1, en = ctx.channel_step(dt, a.tol, a.max_inner)
         inner.append(it)
         print("step %d: %d inner iterations, drop %.2e" % (s + 1, it, float(np.max(en / e1))), file=sys.stderr,
@@ -145,6 +163,9 @@ def main():
                              "GBps": ALGO_BYTES[name] * ncell / (avg * 1e-3) / 1e9}
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     achieved = kernels[dom]["GBps"]
+    traffic = None
+    if (a.nx, a.ny, a.nz) == (128, 128, 128) and os.path.exists(TRAFFIC):
+        traffic = json.load(open(TRAFFIC))["bytes_per_launch"].get(dom)
     out = {
         "metric": METRIC,
         "value": steps_per_s,
@@ -161,18 +182,18 @@ def main():
         "config": {"workload": "compressible channel %dx%dx%d, beta=%.1f, skew=%.2f, Re_tau=%g, Mach %.1f, "
                                "acoustic CFL %.2f, plain FP64 dual-time LU-SGS to a %.0e inner drop"
                                % (a.nx, a.ny, a.nz, a.beta, a.skew, a.re_tau, ch.mach, ch.cfl, a.tol),
-                   "cells": ncell, "deltaT": dt, "max_inner": a.max_inner},
+                   "cells": ncell, "deltaT": dt, "max_inner": a.max_inner, "filter_every_step": not a.no_filter},
         "mcell_updates_per_s": ncell * steps_per_s / 1e6,
         "inner_iterations_per_step": float(np.mean(inner)),
         "mcell_inner_iterations_per_s": ncell * float(np.sum(inner)) / elapsed / 1e6,
         "kernels": kernels,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": ALGO_BYTES[dom] * ncell},
         "cpu_baseline": None,
     }
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(ctx, nv, cfg, dt, float(np.mean(inner)), a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(ctx, nv, cfg, dt, float(np.mean(inner)), a.cpu_seconds, not a.no_filter)
     ctx.close()
     print(json.dumps(out))
 

@@ -516,10 +516,12 @@ int cgli_dual_time_stepping(cgli_ctx *c, double deltaT, double e, double er[5], 
     }
     for (int sw = 0; sw < sweeps; sw++) {
         if (c->lusgs_mode == 1) {
+            launch_lusgs_wave_prep(g, c->xbuf, c->counters, s);
             tbegin(c, T_FWD);
             launch_lusgs_wave(g, 1, c->coef, c5(c, CGLI_MR), f5(c, CGLI_NL), c->xbuf, c->trash, c->order_dev,
                               c->ntiles, c->counters, c->max_spin, s);
             tend(c, T_FWD);
+            launch_lusgs_wave_prep(g, c->xbuf, c->counters, s);
             tbegin(c, T_BWD);
             launch_lusgs_wave(g, 0, c->coef, c5(c, CGLI_NL), f5(c, CGLI_NR), c->xbuf, c->trash, c->order_dev,
                               c->ntiles, c->counters, c->max_spin, s);

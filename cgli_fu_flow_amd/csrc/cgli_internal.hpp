@@ -91,6 +91,8 @@ void launch_split_coef(const Geo &g, const Phys &p, double deltaT, double e, C5 
 long coef_doubles(const Geo &g);
 // coef: coefficient blocks, xbuf: 5 fields (tile-edge exchange),
 // trash: 256 doubles per tile
+// before each sweep: exchange sentinels and the tile ticket
+void launch_lusgs_wave_prep(const Geo &g, double *xbuf, int *counters, hipStream_t s);
 void launch_lusgs_wave(const Geo &g, int forward, const double *coef, C5 R, F5 X, double *xbuf, double *trash,
                        const int *order, int ntiles, int *counters, int max_spin, hipStream_t s);
 void launch_reset_ints(int *p, int n, hipStream_t s);

@@ -1038,14 +1038,19 @@ void launch_split_coef(const Geo &g, const Phys &p, double deltaT, double e, C5 
     hipLaunchKernelGGL(k_split_coef, gr, b, 0, s, g, p, deltaT, e, U_, J, met, coef);
 }
 
-void launch_lusgs_wave(const Geo &g, int forward, const double *coef, C5 R, F5 X, double *xbuf, double *trash,
-                       const int *order, int ntiles, int *counters, int max_spin, hipStream_t s)
+void launch_lusgs_wave_prep(const Geo &g, double *xbuf, int *counters, hipStream_t s)
 {
-    // counters[0]: ticket, counters[1]: error flag (cleared by the caller)
+    // sentinels into the exchange buffer; counters[0] (ticket) back to 0
     dim3 bf(64, 4);
     dim3 gf((g.NZ - 1 + bf.x - 1) / bf.x, (g.NY - 1 + bf.y - 1) / bf.y, g.nx);
     hipLaunchKernelGGL(k_fill_sentinel, gf, bf, 0, s, g, xbuf);
     hipLaunchKernelGGL(k_reset_ints, dim3(1), dim3(64), 0, s, counters, 1);
+}
+
+void launch_lusgs_wave(const Geo &g, int forward, const double *coef, C5 R, F5 X, double *xbuf, double *trash,
+                       const int *order, int ntiles, int *counters, int max_spin, hipStream_t s)
+{
+    // counters[0]: ticket, counters[1]: error flag (cleared by the caller)
     if (forward)
         hipLaunchKernelGGL((k_lusgs_wave<WAVE_TJ, WAVE_TK, true>), dim3(ntiles), dim3(WAVE_TJ * WAVE_TK), 0, s, g,
                            coef, R, X, xbuf, trash, order, counters, counters + 1, max_spin);
