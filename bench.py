@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-filter", action="store_true", help="skip the per-step filter sensor")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="mixed: FP32 residual kernels until the FP32 floor, FP32 off-diagonal sweep")
     return ap.parse_args()
 
 
@@ -129,6 +131,7 @@ def main():
     from cgli_fu_flow_amd import native as nv
     ch = nv.channel(a.re_tau, beta=a.beta, skew=a.skew, cfl=a.cfl)
     cfg = nv.channel_config(ch, a.nx, a.ny, a.nz)
+    cfg.precision = 1 if a.precision == "mixed" else 0
     ctx = nv.Context(cfg, device=0)
     dt = ctx.channel_init(ch)
     print("setup done: dt=%.4e" % dt, file=sys.stderr, flush=True)

@@ -37,13 +37,16 @@ struct cgli_ctx {
     std::vector<int> Nijk;       // (gend+NY+NZ+8) x 2
     std::vector<int> ijk;        // (cells+1) x 3
     int lusgs_mode = 1;          // 0: one launch per hyperplane, 1: tiled wavefront
-    double *coef = nullptr;      // coefficient blocks of the wavefront: split coefficients and diagonal
+    double *coef = nullptr;      // split-coefficient blocks of the wavefront (float in the mixed mode)
+    double *dgn = nullptr;       // diagonal blocks of the wavefront (FP64)
     double *xbuf = nullptr;      // 5 fields: tile-edge exchange of the wavefront
     double *trash = nullptr;     // write-only scratch of the wavefront
     int *order_dev = nullptr;    // wavefront tile order
     int ntiles = 0;
     int *counters = nullptr;     // ticket, error flag
     double *filt = nullptr;      // filter work arrays (allocated on first use)
+    double *prof = nullptr;      // 11 x ny wall-normal profiles
+    int last_fp32 = 0;           // FP32-residual inner iterations of the last channel step
     int *counters_host = nullptr;
     int max_spin = 1000000;
     bool timing = false;
@@ -138,6 +141,10 @@ int ctx_field(cgli_ctx *c, int id, double **dptr)
 }
 hipStream_t ctx_stream(cgli_ctx *c) { return c->stream; }
 const cgli_config *ctx_config(cgli_ctx *c) { return &c->cfg; }
+
+void ctx_set_fp32(cgli_ctx *c, bool on) { c->p.fp32 = on ? 1 : 0; }
+
+void ctx_set_last_fp32(cgli_ctx *c, int n) { c->last_fp32 = n; }
 void ctx_mark(cgli_ctx *c, int id, int n) { mark(c, id, n); }
 }  // namespace cgli
 
@@ -205,6 +212,7 @@ int cgli_create(const cgli_config *cfg, int device, cgli_ctx **out)
     p.dxi = cfg->dxi; p.deta = cfg->deta; p.dzeta = cfg->dzeta;
     p.rho_ref = cfg->rho_ref; p.grav = cfg->grav; p.f_const = cfg->f_const; p.E_ref = cfg->E_ref;
     p.high = cfg->high;
+    p.fp32 = cfg->precision == 1;
     void *h = nullptr;
     if (hipHostMalloc(&h, sizeof(double) * 8) != hipSuccess) {
         hipStreamDestroy(c->stream);
@@ -228,9 +236,11 @@ int cgli_destroy(cgli_ctx *c)
     if (c->scratch) hipFree(c->scratch);
     if (c->ijk_dev) hipFree(c->ijk_dev);
     if (c->coef) hipFree(c->coef);
+    if (c->dgn) hipFree(c->dgn);
     if (c->xbuf) hipFree(c->xbuf);
     if (c->trash) hipFree(c->trash);
     if (c->filt) hipFree(c->filt);
+    if (c->prof) hipFree(c->prof);
     if (c->order_dev) hipFree(c->order_dev);
     if (c->counters) hipFree(c->counters);
     if (c->counters_host) hipHostFree(c->counters_host);
@@ -369,6 +379,31 @@ int cgli_filter(cgli_ctx *c, double Roe_criterion, double E_high, double E_low)
     return CGLI_OK;
 }
 
+int cgli_channel_last_step(cgli_ctx *c, int *inner_fp32)
+{
+    if (!c || !inner_fp32) return CGLI_E_ARG;
+    *inner_fp32 = c->last_fp32;
+    return CGLI_OK;
+}
+
+int cgli_profiles(cgli_ctx *c, double *out)
+{
+    if (!c || !out) return CGLI_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    if (!have(c, CGLI_U_, 5) || !have(c, CGLI_J, 1)) return CGLI_E_STATE;
+    const size_t bytes = sizeof(double) * 11 * c->g.ny;
+    if (!c->prof) {
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return CGLI_E_NOMEM;
+        c->prof = static_cast<double *>(p);
+    }
+    launch_profiles(c->g, c->p, c5(c, CGLI_U_), c->f[CGLI_J], c->prof, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, c->prof, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CGLI_OK;
+}
+
 // Hyperplane.cpp:38-78 ordering: planes m ascending; inside a plane i, j, k
 // ascending.  Built in O(cells) and uploaded once.
 static int build_planes(cgli_ctx *c)
@@ -447,6 +482,8 @@ static int wave_setup(cgli_ctx *c)
     void *p = nullptr;
     if (hipMalloc(&p, sizeof(double) * coef_doubles(c->g)) != hipSuccess) return CGLI_E_NOMEM;
     c->coef = static_cast<double *>(p);
+    if (hipMalloc(&p, sizeof(double) * diag_doubles(c->g)) != hipSuccess) return CGLI_E_NOMEM;
+    c->dgn = static_cast<double *>(p);
     if (hipMalloc(&p, sizeof(double) * 5 * c->g.N) != hipSuccess) return CGLI_E_NOMEM;
     c->xbuf = static_cast<double *>(p);
     HIPCHK(hipMemset(c->xbuf, 0, sizeof(double) * 5 * c->g.N));   // ghost entries stay zero
@@ -511,19 +548,20 @@ int cgli_dual_time_stepping(cgli_ctx *c, double deltaT, double e, double er[5], 
         if (r) return r;
         launch_reset_ints(c->counters, 2, s);
         tbegin(c, T_COEF);
-        launch_split_coef(g, c->p, deltaT, e, c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c->coef, s);
+        launch_split_coef(g, c->p, deltaT, e, c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c->coef, c->dgn,
+                          c->cfg.precision == 1, s);
         tend(c, T_COEF);
     }
     for (int sw = 0; sw < sweeps; sw++) {
         if (c->lusgs_mode == 1) {
             launch_lusgs_wave_prep(g, c->xbuf, c->counters, s);
             tbegin(c, T_FWD);
-            launch_lusgs_wave(g, 1, c->coef, c5(c, CGLI_MR), f5(c, CGLI_NL), c->xbuf, c->trash, c->order_dev,
+            launch_lusgs_wave(g, 1, c->coef, c->dgn, c->cfg.precision == 1, c5(c, CGLI_MR), f5(c, CGLI_NL), c->xbuf, c->trash, c->order_dev,
                               c->ntiles, c->counters, c->max_spin, s);
             tend(c, T_FWD);
             launch_lusgs_wave_prep(g, c->xbuf, c->counters, s);
             tbegin(c, T_BWD);
-            launch_lusgs_wave(g, 0, c->coef, c5(c, CGLI_NL), f5(c, CGLI_NR), c->xbuf, c->trash, c->order_dev,
+            launch_lusgs_wave(g, 0, c->coef, c->dgn, c->cfg.precision == 1, c5(c, CGLI_NL), f5(c, CGLI_NR), c->xbuf, c->trash, c->order_dev,
                               c->ntiles, c->counters, c->max_spin, s);
             tend(c, T_BWD);
         } else {

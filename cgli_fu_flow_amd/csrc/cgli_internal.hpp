@@ -26,6 +26,7 @@ struct Phys {
     double K, R, Cv, mu_L, Pr_L, suth_T, suth_S;
     double dxi, deta, dzeta;
     double rho_ref, grav, f_const, E_ref, high;
+    int fp32;   // residual kernels (fluxes, viscous terms) in FP32 arithmetic
 };
 
 struct F5 { double *v[5]; };
@@ -82,19 +83,25 @@ long filter_scratch_doubles(const Geo &g);
 void launch_filter(const Geo &g, double crit, double E_high, double E_low, C5 U_, const double *J,
                    const double *Y, double *work, double *EpX, double *EpY, double *EpZ, hipStream_t s);
 
+// wall-normal profiles (stats.hip): out = 11 x ny doubles on the device
+void launch_profiles(const Geo &g, const Phys &p, C5 U, const double *J, double *out, hipStream_t s);
+
 // wavefront LU-SGS (implicit.hip)
 constexpr int WAVE_TJ = 4, WAVE_TK = 16;
 int wave_tiles(const Geo &g, int tj, int tk, std::vector<int> &order);
 void launch_split_coef(const Geo &g, const Phys &p, double deltaT, double e, C5 U_, const double *J, M9 met,
-                       double *coef, hipStream_t s);
+                       void *coef, double *dgn, int fp32, hipStream_t s);
 // doubles of the coefficient blocks of the wavefront sweep
 long coef_doubles(const Geo &g);
+long diag_doubles(const Geo &g);
 // coef: coefficient blocks, xbuf: 5 fields (tile-edge exchange),
 // trash: 256 doubles per tile
 // before each sweep: exchange sentinels and the tile ticket
 void launch_lusgs_wave_prep(const Geo &g, double *xbuf, int *counters, hipStream_t s);
-void launch_lusgs_wave(const Geo &g, int forward, const double *coef, C5 R, F5 X, double *xbuf, double *trash,
-                       const int *order, int ntiles, int *counters, int max_spin, hipStream_t s);
+// coef: split blocks (float when fp32), dgn: FP64 diagonal blocks
+void launch_lusgs_wave(const Geo &g, int forward, const void *coef, const double *dgn, int fp32, C5 R, F5 X,
+                       double *xbuf, double *trash, const int *order, int ntiles, int *counters, int max_spin,
+                       hipStream_t s);
 void launch_reset_ints(int *p, int n, hipStream_t s);
 
 // context access for host modules other than capi.cpp
@@ -102,5 +109,7 @@ int ctx_field(cgli_ctx *c, int id, double **dptr);   // allocates on first use
 hipStream_t ctx_stream(cgli_ctx *c);
 const cgli_config *ctx_config(cgli_ctx *c);
 void ctx_mark(cgli_ctx *c, int id, int n);
+void ctx_set_fp32(cgli_ctx *c, bool on);          // residual kernels in FP32 from now on
+void ctx_set_last_fp32(cgli_ctx *c, int n);       // FP32 inner iterations of the last step
 
 }  // namespace cgli

@@ -648,14 +648,17 @@ void launch_finish_norm(int nblocks, const double *partials, double DN, int squa
 // =================================================================
 // Wavefront LU-SGS.
 //
-// Pass 1 (k_split_coef, fully parallel): per cell 23 pairs of doubles: the
-// cell's split coefficients along x, y and z (14 values each, already
-// divided by the grid spacing of that direction) and, for interior cells,
-// the diagonal (d11, d22, d51).  Cells are grouped in blocks of CB
-// consecutive k (one tile row); inside a block pair p of all CB cells is
-// contiguous, so a tile row reads one pair with a single 16-byte load per
-// lane over 2*CB*8 contiguous bytes, and all data of a cell's block lies
-// within a few kilobytes.
+// Pass 1 (k_split_coef, fully parallel): per cell 21 pairs of the cell's
+// split coefficients along x, y and z (14 values each, already divided by
+// the grid spacing of that direction), stored as CT (double, or float for
+// the mixed-precision sweep), and for interior cells the diagonal (d11,
+// d22, d51) in two FP64 pairs of a separate array.  Cells are grouped in
+// blocks of CB consecutive k (one tile row); inside a block pair p of all
+// CB cells is contiguous, so a tile row reads one pair with a single load
+// per lane over contiguous bytes, and all data of a cell's block lies
+// within a few kilobytes.  The sweep is only the preconditioner of the
+// inner iteration: an FP32 split (with the FP64 diagonal) changes the
+// iteration's convergence path, not the state it converges to.
 //
 // Pass 2 (k_lusgs_wave, one launch per sweep): the (j,k) plane is cut into
 // TJ x TK tiles; a workgroup owns one tile, one thread per (j,k) column,
@@ -678,28 +681,42 @@ void launch_finish_norm(int nblocks, const double *partials, double DN, int squa
 // division by dxi/deta/dzeta is applied to each coefficient instead of to
 // each partial sum: results agree to rounding (a few ulp).
 // =================================================================
-constexpr int NPAIR = 23;              // 3 x 7 split pairs, (d11, d22), (d51, pad)
+constexpr int NSPLIT = 21;             // 3 x 7 split pairs per cell
 constexpr int CB = WAVE_TK;            // cells per block along k
-constexpr int CBLK = NPAIR * CB * 2;   // doubles per block
+constexpr int CBLK = NSPLIT * CB * 2;  // split elements per block
+constexpr int DBLK = 2 * CB * 2;       // diagonal doubles per block: (d11, d22), (d51, pad)
 constexpr int CKOFF = CB - 2;          // the first interior k starts a block
-constexpr long CPAIR = 2 * CB;         // doubles from one pair to the next
+constexpr long CPAIR = 2 * CB;         // elements from one pair to the next
 
 // blocks per (i,j) row
 __host__ __device__ inline long coef_nkb(const Geo &g) { return (g.Zm + CKOFF + CB - 1) / CB; }
 
-// offset of pair 0 of cell (i,j,k)
-__device__ __forceinline__ long coef_at(const Geo &g, long nkb, int i, int j, int k)
+// offset (in elements) of pair 0 of cell (i,j,k) in a blocked array of
+// `blk` elements per block
+__device__ __forceinline__ long block_at(const Geo &g, long nkb, long blk, int i, int j, int k)
 {
     const int kk = k + CKOFF;
-    return (((long)i * g.Ym + j) * nkb + kk / CB) * CBLK + (kk % CB) * 2;
+    return (((long)i * g.Ym + j) * nkb + kk / CB) * blk + (kk % CB) * 2;
 }
+
+template <typename CT> struct Pair2;
+template <> struct Pair2<double> {
+    typedef double2 t;
+    static __device__ __forceinline__ double2 make(double a, double b) { return make_double2(a, b); }
+};
+template <> struct Pair2<float> {
+    typedef float2 t;
+    static __device__ __forceinline__ float2 make(double a, double b) { return make_float2((float)a, (float)b); }
+};
 // slot order of one direction: the ten values both sweeps use, then the
 // forward pair (p11, p22), then the backward pair (m11, m22)
 enum { S_P12, S_P13, S_P14, S_P21, S_P31, S_P41, S_P51, S_P52, S_P53, S_P54, S_P11, S_P22, S_M11, S_M22 };
 constexpr unsigned long long SENTINEL = 0x7FF4DEADBEEF0001ULL;
 
+template <typename CT>
 __global__ void __launch_bounds__(256) k_split_coef(Geo g, Phys p, double deltaT, double e, C5 U_,
-                                                    const double *__restrict__ J, M9 met, double *__restrict__ coef)
+                                                    const double *__restrict__ J, M9 met, CT *__restrict__ coef,
+                                                    double *__restrict__ dgn)
 {
     // box i=2..gend+1, j=1..NYY, k=1..NZZ
     const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
@@ -707,28 +724,31 @@ __global__ void __launch_bounds__(256) k_split_coef(Geo g, Phys p, double deltaT
     const int i = 2 + blockIdx.z;
     if (k > g.NZZ || j > g.NYY) return;
     const long c = ix(g, i, j, k);
-    double2 *o = reinterpret_cast<double2 *>(coef + coef_at(g, coef_nkb(g), i, j, k));
+    typedef typename Pair2<CT>::t V2;
+    const long nkb = coef_nkb(g);
+    V2 *o = reinterpret_cast<V2 *>(coef + block_at(g, nkb, CBLK, i, j, k));
     const CP q = cell_prim(U_, J, c, p.K, e);
     const double dd[3] = {p.dxi, p.deta, p.dzeta};
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const SC s = split(q, met.m[3 * d][c], met.m[3 * d + 1][c], met.m[3 * d + 2][c], p.K, d != 0);
         const double h = dd[d];
-        double2 *b = o + 7 * d * CB;
-        b[0 * CB] = make_double2(s.p12 / h, s.p13 / h);
-        b[1 * CB] = make_double2(s.p14 / h, s.p21 / h);
-        b[2 * CB] = make_double2(s.p31 / h, s.p41 / h);
-        b[3 * CB] = make_double2(s.p51 / h, s.p52 / h);
-        b[4 * CB] = make_double2(s.p53 / h, s.p54 / h);
-        b[5 * CB] = make_double2(s.p11 / h, s.p22 / h);
-        b[6 * CB] = make_double2(s.m11 / h, s.m22 / h);
+        V2 *b = o + 7 * d * CB;
+        b[0 * CB] = Pair2<CT>::make(s.p12 / h, s.p13 / h);
+        b[1 * CB] = Pair2<CT>::make(s.p14 / h, s.p21 / h);
+        b[2 * CB] = Pair2<CT>::make(s.p31 / h, s.p41 / h);
+        b[3 * CB] = Pair2<CT>::make(s.p51 / h, s.p52 / h);
+        b[4 * CB] = Pair2<CT>::make(s.p53 / h, s.p54 / h);
+        b[5 * CB] = Pair2<CT>::make(s.p11 / h, s.p22 / h);
+        b[6 * CB] = Pair2<CT>::make(s.m11 / h, s.m22 / h);
     }
     if (i >= 3 && i <= g.gend && j >= 2 && j <= g.NY && k >= 2 && k <= g.NZ) {
         double m[9];
         load9(met, c, m);
         const DG D = diag(q, m, p, deltaT);
-        o[21 * CB] = make_double2(D.d11, D.d22);
-        o[22 * CB] = make_double2(D.d51, 0.0);
+        double2 *od = reinterpret_cast<double2 *>(dgn + block_at(g, nkb, DBLK, i, j, k));
+        od[0] = make_double2(D.d11, D.d22);
+        od[CB] = make_double2(D.d51, 0.0);
     }
 }
 
@@ -751,35 +771,38 @@ __global__ void k_reset_ints(int *p, int n)
 
 // one direction's 12 values for a sweep: P12..P54, then g11, g22 (the
 // p- or m-pair)
-struct Dir12 { double v[12]; };
+template <typename CT>
+struct Dir12 { CT v[12]; };
 
-template <bool FWD>
-__device__ __forceinline__ void load_dir(Dir12 &s, const double *__restrict__ r)
+template <bool FWD, typename CT>
+__device__ __forceinline__ void load_dir(Dir12<CT> &s, const CT *__restrict__ r)
 {
-    const double2 *b = reinterpret_cast<const double2 *>(r);
+    typedef typename Pair2<CT>::t V2;
+    const V2 *b = reinterpret_cast<const V2 *>(r);
 #pragma unroll
     for (int q = 0; q < 5; q++) {
-        const double2 t = b[q * CB];
+        const V2 t = b[q * CB];
         s.v[2 * q] = t.x;
         s.v[2 * q + 1] = t.y;
     }
-    const double2 t = b[(FWD ? 5 : 6) * CB];
+    const V2 t = b[(FWD ? 5 : 6) * CB];
     s.v[10] = t.x;
     s.v[11] = t.y;
 }
 
-struct Pref { Dir12 A, B, C; double d11, d22, d51; double r[5]; };
+template <typename CT>
+struct Pref { Dir12<CT> A, B, C; double d11, d22, d51; double r[5]; };
 
-// rec: the own cell's record; oA/oB/oC: offsets of the upstream neighbours'
-// records along i, j, k
-template <bool FWD>
-__device__ __forceinline__ void load_pref(Pref &P, const double *__restrict__ rec, long oA, long oB, long oC,
-                                          const C5 &R, long c)
+// rec: the own cell's split record; oA/oB/oC: offsets of the upstream
+// neighbours' records along i, j, k; drec: the own cell's diagonal
+template <bool FWD, typename CT>
+__device__ __forceinline__ void load_pref(Pref<CT> &P, const CT *__restrict__ rec, long oA, long oB, long oC,
+                                          const double *__restrict__ drec, const C5 &R, long c)
 {
     load_dir<FWD>(P.A, rec + oA);
     load_dir<FWD>(P.B, rec + oB + 7 * CPAIR);
     load_dir<FWD>(P.C, rec + oC + 14 * CPAIR);
-    const double2 *b = reinterpret_cast<const double2 *>(rec + 21 * CPAIR);
+    const double2 *b = reinterpret_cast<const double2 *>(drec);
     const double2 t0 = b[0], t1 = b[CB];
     P.d11 = t0.x;
     P.d22 = t0.y;
@@ -825,31 +848,34 @@ struct Lane {
     int tid, jl, kl;
     bool col, edge_out;
     long c0;                // cell (i = 0, j, k), clamped into the domain
-    long r0;                // coefficient offset of the column's first cell
+    long r0, d0;            // split and diagonal offsets of the column's first cell
     long tline;             // this wave's line of the trash buffer
 };
 
+template <typename CT>
 struct WaveArgs {
     Geo g;
-    const double *coef;
+    const CT *coef;
+    const double *dgn;
     C5 R;
     F5 X;
     double *xbuf, *trash;
     int *err;
     int max_spin;
-    long sxs, oA, oB, oC, d, dcol;   // dcol: coefficient offset of the next cell of a column
+    long sxs, oA, oB, oC, d, dcol, ddcol;   // dcol/ddcol: split/diagonal offset of the next cell of a column
 };
 
 // local step il clamped to the column, so every lane has a valid address
 // at every step
-__device__ __forceinline__ int icl(const WaveArgs &w, int il)
+template <typename CT>
+__device__ __forceinline__ int icl(const WaveArgs<CT> &w, int il)
 {
     return il < 0 ? 0 : (il >= w.g.nx ? w.g.nx - 1 : il);
 }
 
 // cell of local step il
-template <bool FWD>
-__device__ __forceinline__ long cell_at(const WaveArgs &w, const Lane &L, int il)
+template <bool FWD, typename CT>
+__device__ __forceinline__ long cell_at(const WaveArgs<CT> &w, const Lane &L, int il)
 {
     const int c = icl(w, il);
     return L.c0 + (long)(FWD ? 3 + c : w.g.gend - c) * w.sxs;
@@ -862,10 +888,11 @@ __device__ __forceinline__ long cell_at(const WaveArgs &w, const Lane &L, int il
 // tile-edge loads and stores are confined to the lanes that need them.  PAR is the step's parity (which LDS
 // half is written); Pc/nbc/ncc hold the values loaded for this step,
 // Pn/nbn/ncn receive the next step's.
-template <int TJ, int TK, bool FWD, int PAR>
-__device__ __forceinline__ void wave_step(const WaveArgs &w, const Lane &L, int s, double prev[5],
-                                          double (*sx)[5][TJ * TK], Pref &Pc, Pref &Pn, unsigned long long *nbc,
-                                          unsigned long long *nbn, unsigned long long *ncc, unsigned long long *ncn)
+template <int TJ, int TK, bool FWD, int PAR, typename CT>
+__device__ __forceinline__ void wave_step(const WaveArgs<CT> &w, const Lane &L, int s, double prev[5],
+                                          double (*sx)[5][TJ * TK], Pref<CT> &Pc, Pref<CT> &Pn,
+                                          unsigned long long *nbc, unsigned long long *nbn, unsigned long long *ncc,
+                                          unsigned long long *ncn)
 {
     const int il = s - L.jl - L.kl;
     const bool act = L.col && il >= 0 && il < w.g.nx;
@@ -885,47 +912,56 @@ __device__ __forceinline__ void wave_step(const WaveArgs &w, const Lane &L, int 
         const long cn = cell_at<FWD>(w, L, il + 1);
         if (L.jl == 0) issue5(w.xbuf, w.g.N, cn + w.d * w.g.Zm, nbn);
         if (L.kl == 0) issue5(w.xbuf, w.g.N, cn + w.d, ncn);
-        load_pref<FWD>(Pn, w.coef + L.r0 + (long)icl(w, il + 1) * w.dcol, w.oA, w.oB, w.oC, w.R, cn);
+        const long ic = icl(w, il + 1);
+        load_pref<FWD>(Pn, w.coef + L.r0 + ic * w.dcol, w.oA, w.oB, w.oC, w.dgn + L.d0 + ic * w.ddcol, w.R, cn);
     }
-    const double *A = Pc.A.v, *B = Pc.B.v, *Cc = Pc.C.v;
-    const double *xa = prev;
+    const CT *A = Pc.A.v, *B = Pc.B.v, *Cc = Pc.C.v;
+    // the off-diagonal products in the coefficients' precision (the FP64
+    // path converts nothing)
+    CT xa[5], xbc[5], xcc[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        xa[v] = (CT)prev[v];
+        xbc[v] = (CT)xb[v];
+        xcc[v] = (CT)xc[v];
+    }
     double out[5];
     if (FWD) {
-        const double LL0 = (-A[10] * xa[0] - A[0] * xa[1] - A[1] * xa[2] - A[2] * xa[3]) +
-                           (-B[10] * xb[0] - B[0] * xb[1] - B[1] * xb[2] - B[2] * xb[3]) +
-                           (-Cc[10] * xc[0] - Cc[0] * xc[1] - Cc[1] * xc[2] - Cc[2] * xc[3]);
-        const double LL1 = (-A[3] * xa[0] - A[11] * xa[1]) + (-B[3] * xb[0] - B[11] * xb[1]) +
-                           (-Cc[3] * xc[0] - Cc[11] * xc[1]);
-        const double LL2 = (-A[4] * xa[0] - A[11] * xa[2]) + (-B[4] * xb[0] - B[11] * xb[2]) +
-                           (-Cc[4] * xc[0] - Cc[11] * xc[2]);
-        const double LL3 = (-A[5] * xa[0] - A[11] * xa[3]) + (-B[5] * xb[0] - B[11] * xb[3]) +
-                           (-Cc[5] * xc[0] - Cc[11] * xc[3]);
-        const double LL4 = (-A[6] * xa[0] - A[7] * xa[1] - A[8] * xa[2] - A[9] * xa[3] - A[11] * xa[4]) +
-                           (-B[6] * xb[0] - B[7] * xb[1] - B[8] * xb[2] - B[9] * xb[3] - B[11] * xb[4]) +
-                           (-Cc[6] * xc[0] - Cc[7] * xc[1] - Cc[8] * xc[2] - Cc[9] * xc[3] - Cc[11] * xc[4]);
-        out[0] = Pc.d11 * (Pc.r[0] - LL0);
-        out[1] = Pc.d22 * (Pc.r[1] - LL1);
-        out[2] = Pc.d22 * (Pc.r[2] - LL2);
-        out[3] = Pc.d22 * (Pc.r[3] - LL3);
-        out[4] = Pc.d51 * (Pc.r[0] - LL0) + Pc.d22 * (Pc.r[4] - LL4);
+        const CT LL0 = (-A[10] * xa[0] - A[0] * xa[1] - A[1] * xa[2] - A[2] * xa[3]) +
+                           (-B[10] * xbc[0] - B[0] * xbc[1] - B[1] * xbc[2] - B[2] * xbc[3]) +
+                           (-Cc[10] * xcc[0] - Cc[0] * xcc[1] - Cc[1] * xcc[2] - Cc[2] * xcc[3]);
+        const CT LL1 = (-A[3] * xa[0] - A[11] * xa[1]) + (-B[3] * xbc[0] - B[11] * xbc[1]) +
+                           (-Cc[3] * xcc[0] - Cc[11] * xcc[1]);
+        const CT LL2 = (-A[4] * xa[0] - A[11] * xa[2]) + (-B[4] * xbc[0] - B[11] * xbc[2]) +
+                           (-Cc[4] * xcc[0] - Cc[11] * xcc[2]);
+        const CT LL3 = (-A[5] * xa[0] - A[11] * xa[3]) + (-B[5] * xbc[0] - B[11] * xbc[3]) +
+                           (-Cc[5] * xcc[0] - Cc[11] * xcc[3]);
+        const CT LL4 = (-A[6] * xa[0] - A[7] * xa[1] - A[8] * xa[2] - A[9] * xa[3] - A[11] * xa[4]) +
+                           (-B[6] * xbc[0] - B[7] * xbc[1] - B[8] * xbc[2] - B[9] * xbc[3] - B[11] * xbc[4]) +
+                           (-Cc[6] * xcc[0] - Cc[7] * xcc[1] - Cc[8] * xcc[2] - Cc[9] * xcc[3] - Cc[11] * xcc[4]);
+        out[0] = Pc.d11 * (Pc.r[0] - (double)LL0);
+        out[1] = Pc.d22 * (Pc.r[1] - (double)LL1);
+        out[2] = Pc.d22 * (Pc.r[2] - (double)LL2);
+        out[3] = Pc.d22 * (Pc.r[3] - (double)LL3);
+        out[4] = Pc.d51 * (Pc.r[0] - (double)LL0) + Pc.d22 * (Pc.r[4] - (double)LL4);
     } else {
-        const double UU0 = (+A[10] * xa[0] + A[0] * xa[1] + A[1] * xa[2] + A[2] * xa[3]) +
-                           (+B[10] * xb[0] + B[0] * xb[1] + B[1] * xb[2] + B[2] * xb[3]) +
-                           (+Cc[10] * xc[0] + Cc[0] * xc[1] + Cc[1] * xc[2] + Cc[2] * xc[3]);
-        const double UU1 = (+A[3] * xa[0] + A[11] * xa[1]) + (+B[3] * xb[0] + B[11] * xb[1]) +
-                           (+Cc[3] * xc[0] + Cc[11] * xc[1]);
-        const double UU2 = (+A[4] * xa[0] + A[11] * xa[2]) + (+B[4] * xb[0] + B[11] * xb[2]) +
-                           (+Cc[4] * xc[0] + Cc[11] * xc[2]);
-        const double UU3 = (+A[5] * xa[0] + A[11] * xa[3]) + (+B[5] * xb[0] + B[11] * xb[3]) +
-                           (+Cc[5] * xc[0] + Cc[11] * xc[3]);
-        const double UU4 = (+A[6] * xa[0] + A[7] * xa[1] + A[8] * xa[2] + A[9] * xa[3] + A[11] * xa[4]) +
-                           (+B[6] * xb[0] + B[7] * xb[1] + B[8] * xb[2] + B[9] * xb[3] + B[11] * xb[4]) +
-                           (+Cc[6] * xc[0] + Cc[7] * xc[1] + Cc[8] * xc[2] + Cc[9] * xc[3] + Cc[11] * xc[4]);
-        out[0] = Pc.r[0] - Pc.d11 * UU0;
-        out[1] = Pc.r[1] - Pc.d22 * UU1;
-        out[2] = Pc.r[2] - Pc.d22 * UU2;
-        out[3] = Pc.r[3] - Pc.d22 * UU3;
-        out[4] = Pc.r[4] - (Pc.d51 * UU0 + Pc.d22 * UU4);
+        const CT UU0 = (+A[10] * xa[0] + A[0] * xa[1] + A[1] * xa[2] + A[2] * xa[3]) +
+                           (+B[10] * xbc[0] + B[0] * xbc[1] + B[1] * xbc[2] + B[2] * xbc[3]) +
+                           (+Cc[10] * xcc[0] + Cc[0] * xcc[1] + Cc[1] * xcc[2] + Cc[2] * xcc[3]);
+        const CT UU1 = (+A[3] * xa[0] + A[11] * xa[1]) + (+B[3] * xbc[0] + B[11] * xbc[1]) +
+                           (+Cc[3] * xcc[0] + Cc[11] * xcc[1]);
+        const CT UU2 = (+A[4] * xa[0] + A[11] * xa[2]) + (+B[4] * xbc[0] + B[11] * xbc[2]) +
+                           (+Cc[4] * xcc[0] + Cc[11] * xcc[2]);
+        const CT UU3 = (+A[5] * xa[0] + A[11] * xa[3]) + (+B[5] * xbc[0] + B[11] * xbc[3]) +
+                           (+Cc[5] * xcc[0] + Cc[11] * xcc[3]);
+        const CT UU4 = (+A[6] * xa[0] + A[7] * xa[1] + A[8] * xa[2] + A[9] * xa[3] + A[11] * xa[4]) +
+                           (+B[6] * xbc[0] + B[7] * xbc[1] + B[8] * xbc[2] + B[9] * xbc[3] + B[11] * xbc[4]) +
+                           (+Cc[6] * xcc[0] + Cc[7] * xcc[1] + Cc[8] * xcc[2] + Cc[9] * xcc[3] + Cc[11] * xcc[4]);
+        out[0] = Pc.r[0] - Pc.d11 * (double)UU0;
+        out[1] = Pc.r[1] - Pc.d22 * (double)UU1;
+        out[2] = Pc.r[2] - Pc.d22 * (double)UU2;
+        out[3] = Pc.r[3] - Pc.d22 * (double)UU3;
+        out[4] = Pc.r[4] - (Pc.d51 * (double)UU0 + Pc.d22 * (double)UU4);
     }
 #pragma unroll
     for (int v = 0; v < 5; v++) {
@@ -946,10 +982,11 @@ __device__ __forceinline__ void wave_step(const WaveArgs &w, const Lane &L, int 
 // trash: 256 doubles per tile of scratch that no one reads; lanes of one
 // wave with nothing to store all write the same 128-byte line, so a store
 // instruction costs one extra line and waves never share one
-template <int TJ, int TK, bool FWD>
-__global__ void __launch_bounds__(TJ *TK) k_lusgs_wave(Geo g, const double *__restrict__ coef, C5 R, F5 X,
-                                                     double *xbuf, double *trash, const int *__restrict__ order,
-                                                     int *ticket, int *err, int max_spin)
+template <int TJ, int TK, bool FWD, typename CT>
+__global__ void __launch_bounds__(TJ *TK) k_lusgs_wave(Geo g, const CT *__restrict__ coef,
+                                                     const double *__restrict__ dgn, C5 R, F5 X, double *xbuf,
+                                                     double *trash, const int *__restrict__ order, int *ticket,
+                                                     int *err, int max_spin)
 {
     static_assert((TJ * TK + 63) / 64 <= 16, "one trash line per wave, at most 16 waves");
     __shared__ int s_t;
@@ -970,9 +1007,10 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_wave(Geo g, const double *__re
     L.c0 = ix(g, 0, j, k);
     L.edge_out = (L.jl == TJ - 1) || (L.kl == TK - 1);
     L.tline = ((long)blockIdx.x * 16 + L.tid / 64) * 16;
-    WaveArgs w;
+    WaveArgs<CT> w;
     w.g = g;
     w.coef = coef;
+    w.dgn = dgn;
     w.R = R;
     w.X = X;
     w.xbuf = xbuf;
@@ -985,14 +1023,16 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_wave(Geo g, const double *__re
         const long nkb = coef_nkb(g);
         const int i0 = FWD ? 3 : g.gend;
         w.dcol = -w.d * g.Ym * nkb * CBLK;
+        w.ddcol = -w.d * g.Ym * nkb * DBLK;
         w.oA = -w.dcol;
         w.oB = w.d * nkb * CBLK;
-        L.r0 = coef_at(g, nkb, i0, j, k);
-        w.oC = coef_at(g, nkb, i0, j, k + (int)w.d) - L.r0;
+        L.r0 = block_at(g, nkb, CBLK, i0, j, k);
+        L.d0 = block_at(g, nkb, DBLK, i0, j, k);
+        w.oC = block_at(g, nkb, CBLK, i0, j, k + (int)w.d) - L.r0;
     }
     const int S = g.nx + TJ + TK - 2;
     double prev[5];
-    Pref P0, P1;
+    Pref<CT> P0, P1;
     unsigned long long nb0[5], nb1[5], nc0[5], nc1[5];
     {
         const long c = cell_at<FWD>(w, L, 0);
@@ -1001,7 +1041,8 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_wave(Geo g, const double *__re
         const long cs = cell_at<FWD>(w, L, -(L.jl + L.kl));               // the cell of step 0
         if (L.jl == 0) issue5(xbuf, g.N, cs + w.d * g.Zm, nb0);
         if (L.kl == 0) issue5(xbuf, g.N, cs + w.d, nc0);
-        load_pref<FWD>(P0, coef + L.r0 + (long)icl(w, -(L.jl + L.kl)) * w.dcol, w.oA, w.oB, w.oC, R, cs);
+        const long ic = icl(w, -(L.jl + L.kl));
+        load_pref<FWD>(P0, coef + L.r0 + ic * w.dcol, w.oA, w.oB, w.oC, dgn + L.d0 + ic * w.ddcol, R, cs);
     }
     for (int s = 0; s < S; s += 2) {
         wave_step<TJ, TK, FWD, 0>(w, L, s, prev, sx, P0, P1, nb0, nb1, nc0, nc1);
@@ -1021,7 +1062,10 @@ static std::vector<int> tile_order(int ntj, int ntk)
     return o;
 }
 
+// split blocks sized for FP64 elements (an FP32 sweep uses half), and the
+// diagonal blocks
 long coef_doubles(const Geo &g) { return (long)(g.nx + 6) * g.Ym * coef_nkb(g) * CBLK; }
+long diag_doubles(const Geo &g) { return (long)(g.nx + 6) * g.Ym * coef_nkb(g) * DBLK; }
 
 int wave_tiles(const Geo &g, int tj, int tk, std::vector<int> &order)
 {
@@ -1031,11 +1075,16 @@ int wave_tiles(const Geo &g, int tj, int tk, std::vector<int> &order)
 }
 
 void launch_split_coef(const Geo &g, const Phys &p, double deltaT, double e, C5 U_, const double *J, M9 met,
-                       double *coef, hipStream_t s)
+                       void *coef, double *dgn, int fp32, hipStream_t s)
 {
     dim3 b(64, 4);
     dim3 gr((g.NZZ + b.x - 1) / b.x, (g.NYY + b.y - 1) / b.y, g.gend);   // i=2..gend+1
-    hipLaunchKernelGGL(k_split_coef, gr, b, 0, s, g, p, deltaT, e, U_, J, met, coef);
+    if (fp32)
+        hipLaunchKernelGGL(k_split_coef<float>, gr, b, 0, s, g, p, deltaT, e, U_, J, met,
+                           static_cast<float *>(coef), dgn);
+    else
+        hipLaunchKernelGGL(k_split_coef<double>, gr, b, 0, s, g, p, deltaT, e, U_, J, met,
+                           static_cast<double *>(coef), dgn);
 }
 
 void launch_lusgs_wave_prep(const Geo &g, double *xbuf, int *counters, hipStream_t s)
@@ -1047,16 +1096,26 @@ void launch_lusgs_wave_prep(const Geo &g, double *xbuf, int *counters, hipStream
     hipLaunchKernelGGL(k_reset_ints, dim3(1), dim3(64), 0, s, counters, 1);
 }
 
-void launch_lusgs_wave(const Geo &g, int forward, const double *coef, C5 R, F5 X, double *xbuf, double *trash,
-                       const int *order, int ntiles, int *counters, int max_spin, hipStream_t s)
+void launch_lusgs_wave(const Geo &g, int forward, const void *coef, const double *dgn, int fp32, C5 R, F5 X,
+                       double *xbuf, double *trash, const int *order, int ntiles, int *counters, int max_spin,
+                       hipStream_t s)
 {
     // counters[0]: ticket, counters[1]: error flag (cleared by the caller)
-    if (forward)
-        hipLaunchKernelGGL((k_lusgs_wave<WAVE_TJ, WAVE_TK, true>), dim3(ntiles), dim3(WAVE_TJ * WAVE_TK), 0, s, g,
-                           coef, R, X, xbuf, trash, order, counters, counters + 1, max_spin);
+    const dim3 gr(ntiles), bl(WAVE_TJ * WAVE_TK);
+    const float *cf = static_cast<const float *>(coef);
+    const double *cd = static_cast<const double *>(coef);
+    if (forward && fp32)
+        hipLaunchKernelGGL((k_lusgs_wave<WAVE_TJ, WAVE_TK, true, float>), gr, bl, 0, s, g, cf, dgn, R, X, xbuf, trash,
+                           order, counters, counters + 1, max_spin);
+    else if (forward)
+        hipLaunchKernelGGL((k_lusgs_wave<WAVE_TJ, WAVE_TK, true, double>), gr, bl, 0, s, g, cd, dgn, R, X, xbuf,
+                           trash, order, counters, counters + 1, max_spin);
+    else if (fp32)
+        hipLaunchKernelGGL((k_lusgs_wave<WAVE_TJ, WAVE_TK, false, float>), gr, bl, 0, s, g, cf, dgn, R, X, xbuf,
+                           trash, order, counters, counters + 1, max_spin);
     else
-        hipLaunchKernelGGL((k_lusgs_wave<WAVE_TJ, WAVE_TK, false>), dim3(ntiles), dim3(WAVE_TJ * WAVE_TK), 0, s, g,
-                           coef, R, X, xbuf, trash, order, counters, counters + 1, max_spin);
+        hipLaunchKernelGGL((k_lusgs_wave<WAVE_TJ, WAVE_TK, false, double>), gr, bl, 0, s, g, cd, dgn, R, X, xbuf,
+                           trash, order, counters, counters + 1, max_spin);
 }
 
 void launch_reset_ints(int *p, int n, hipStream_t s)

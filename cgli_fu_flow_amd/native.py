@@ -69,6 +69,8 @@ def lib():
             getattr(L, n).argtypes = [vp, C.c_double, C.c_double]
         L.cgli_viscous.argtypes = [vp]
         L.cgli_filter.argtypes = [vp, C.c_double, C.c_double, C.c_double]
+        L.cgli_profiles.argtypes = [vp, dp]
+        L.cgli_channel_last_step.argtypes = [vp, C.POINTER(C.c_int)]
         L.cgli_hyperplane.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.cgli_dual_time_stepping.argtypes = [vp, C.c_double, C.c_double, dp, C.c_int, C.c_int]
         L.cgli_dplusgs.argtypes = [vp, C.c_double, dp, C.c_double, C.c_int]
@@ -178,6 +180,20 @@ class Context:
 
     def viscous(self):
         _check(lib().cgli_viscous(self.h), "viscous")
+
+    def last_step_fp32(self):
+        """FP32-residual inner iterations of the last channel step."""
+        n = C.c_int()
+        _check(lib().cgli_channel_last_step(self.h, C.byref(n)), "channel_last_step")
+        return n.value
+
+    PROFILE_NAMES = ["U", "V", "W", "T", "UU", "VV", "WW", "TT", "UV", "UT", "VT"]
+
+    def profiles(self):
+        """Statistic.cpp:130-164 quantities averaged over x and z: (11, ny)."""
+        out = np.zeros(11 * self.cfg.ny)
+        _check(lib().cgli_profiles(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "profiles")
+        return out.reshape(11, self.cfg.ny)
 
     def filter(self, Roe_criterion=0.007, E_high=1.0, E_low=0.1):
         """Filter.cpp:12-1720: EpX/EpY/EpZ from U_, J and Y_point."""

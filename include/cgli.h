@@ -104,6 +104,11 @@ int cgli_viscous(cgli_ctx *ctx);
  * kinetic-energy ratio is >= Roe_criterion, else E_low (the reference's
  * defaults are 0.007, 1, 0.1, main.cpp:60-62).  Needs nx >= 3. */
 int cgli_filter(cgli_ctx *ctx, double Roe_criterion, double E_high, double E_low);
+/* Statistic.cpp:130-164: the mean and raw second-moment profiles U V W T
+ * UU VV WW TT UV UT VT of U_, averaged over x and z (the channel's
+ * homogeneous directions) for every interior j; out[11*ny] on the host,
+ * out[q*ny + (j-2)].  FP64 whatever the residual precision. */
+int cgli_profiles(cgli_ctx *ctx, double *out);
 /* Hyperplane.cpp:18-81: writes the reference's 1-based plane lists to host
  * arrays Nijk[(gend+ny+nz+8)][2] and ijk[nx*ny*nz+1][3]. */
 int cgli_hyperplane(cgli_ctx *ctx, int *Nijk, int *ijk);
@@ -165,6 +170,12 @@ int cgli_channel_ghosts(cgli_ctx *ctx);
  * then the time-level shift.  *inner receives the iteration count. */
 int cgli_channel_step(cgli_ctx *ctx, double deltaT, double tol, int max_inner, int sweeps, int *inner,
                       double er_first[5], double er_last[5]);
+
+/* With cfg.precision = 1 the step runs its inner iterations with FP32
+ * residual kernels until the drop reaches 100 x tol or stops falling, then
+ * in FP64 down to tol; *inner_fp32 receives how many of the last step's
+ * iterations used FP32 residuals. */
+int cgli_channel_last_step(cgli_ctx *ctx, int *inner_fp32);
 
 /* ---- timing ---------------------------------------------------------- */
 /* Average device time (ms) of the last `n` launches of a kernel class

@@ -104,6 +104,10 @@ void orc_filter(const orc_cfg *c, double Roe_criterion, double E_high, double E_
 void orc_filter_energy(const orc_cfg *c, double *const U_[5], const double *Y_point, const double *J,
                        double *Eh, double *E2h);
 
+/* --- wall-normal profiles (statistics of the channel) ------------------ */
+/* out[11*ny]: U V W T UU VV WW TT UV UT VT averaged over x and z per j */
+void orc_profiles(const orc_cfg *c, double *const U_[5], const double *J, double *out);
+
 /* --- metrics (a12) ----------------------------------------------------- */
 /* Reference analytic tanh grid (Jacobian*.cpp), gamma1 = 2.8. met = 9
  * arrays xidx..ztdz; `cell` is the centre set used by Jacobian_v's wall

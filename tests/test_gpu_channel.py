@@ -128,3 +128,37 @@ def test_channel_filter_sensor_matches_oracle():
             for d in range(3):
                 assert np.array_equal(got[d][wins[d]], want[d][wins[d]]), (d, crit)
             print("crit", crit, "fraction high", [float(np.mean(got[d][wins[d]] == 1.0)) for d in range(3)])
+
+
+def test_fp32_residuals_keep_the_mean_profiles():
+    """The channel in the mixed mode (FP32 residual kernels until the drop
+    nears the FP32 floor, then FP64; FP32 off-diagonal sweep; FP64 update,
+    diagonal and norm) against the same run in FP64: both reach the 1e-6
+    inner drop every step, and after the steps the mean-velocity and
+    Reynolds-stress profiles differ by at most 1e-4 relative (the target for
+    the mixed path)."""
+    from cgli_fu_flow_amd import native as nv
+    ch = nv.channel(180.0, beta=2.2, skew=0.05)
+    res = {}
+    for prec in (0, 1):
+        cfg = nv.channel_config(ch, 16, 24, 12)
+        cfg.precision = prec
+        with nv.Context(cfg) as ctx:
+            dt = ctx.channel_init(ch)
+            its = []
+            for _ in range(4):
+                it, e1, en = ctx.channel_step(dt, tol=1e-6, max_inner=200)
+                assert np.max(en / e1) <= 1e-6, (prec, np.max(en / e1))
+                its.append((it, ctx.last_step_fp32()))
+            res[prec] = (ctx.profiles(), ctx.get_many(nv.U_, 5), its)
+    P64, P32 = res[0][0], res[1][0]
+    U = P64[0]
+    err_u = np.max(np.abs(P32[0] - U)) / np.max(np.abs(U))
+    # Reynolds stresses <u'u'>, <v'v'>, <w'w'>, <u'v'> from the raw moments
+    st64 = [P64[4] - P64[0] ** 2, P64[5] - P64[1] ** 2, P64[6] - P64[2] ** 2, P64[8] - P64[0] * P64[1]]
+    st32 = [P32[4] - P32[0] ** 2, P32[5] - P32[1] ** 2, P32[6] - P32[2] ** 2, P32[8] - P32[0] * P32[1]]
+    err_s = max(np.max(np.abs(a - b)) / np.max(np.abs(a)) for a, b in zip(st64, st32))
+    state = max(float(np.max(np.abs(a - b)) / np.max(np.abs(a))) for a, b in zip(res[0][1], res[1][1]))
+    print("inner iterations fp64", res[0][2], "fp32", res[1][2], flush=True)
+    print("mean-u error %.2e, Reynolds-stress error %.2e, state error %.2e" % (err_u, err_s, state), flush=True)
+    assert err_u <= 1e-4 and err_s <= 1e-4

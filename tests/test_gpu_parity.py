@@ -280,3 +280,86 @@ def test_filter_needs_its_inputs(nv, orc):
         with pytest.raises(nv.CgliError) as ei:
             ctx.filter()
         assert ei.value.code == -5
+
+
+@pytest.mark.parametrize("name", ["ref_12x16x10", "ragged_20x24x17"])
+def test_profiles_match_oracle(nv, orc, name):
+    case = make_case(name)
+    f = make_inputs(orc, case.slab, SEED + 8)
+    cfg = case.cfg(orc)
+    want = np.zeros(11 * case.slab.ny)
+    orc.orc_profiles(C.byref(cfg), ptrs(f["U_"]), dp(f["J"]), dp(want))
+    want = want.reshape(11, case.slab.ny)
+    with open_ctx(nv, orc, case) as ctx:
+        ctx.set_many(nv.U_, f["U_"])
+        ctx.set(nv.J, f["J"])
+        got = ctx.profiles()
+        again = ctx.profiles()
+    assert np.array_equal(got, again)                 # fixed-order reduction
+    for q in range(11):
+        assert rel(got[q], want[q]) <= TOL, (q, rel(got[q], want[q]))
+
+
+# FP32 residual kernels (cfg.precision = 1) against the FP64 oracle: the
+# bar is single-precision rounding of the stencils, max-norm relative per
+# variable
+FP32_TOL = {"inF": 5e-5, "vF": 1e-4}
+
+
+@pytest.mark.parametrize("name", ["ref_12x16x10", "ragged_20x24x17", "wide_40x12x70"])
+def test_fp32_residuals_track_the_fp64_oracle(nv, orc, name):
+    print("case", name, flush=True)
+    case = make_case(name)
+    f = make_inputs(orc, case.slab, SEED + 9)
+    want = run_chain("orc", orc, case, f, dts=False)
+    reg = regions(case.slab)
+    with open_ctx(nv, orc, case, precision=1) as ctx:
+        upload_inputs(ctx, f, with_levels=False)
+        ctx.flux_x(1.0, 1e-4)
+        ctx.flux_y(1.0, 1e-4)
+        ctx.flux_z(1.0, 1e-4)
+        ctx.viscous()
+        worst = {}
+        for d, fid in enumerate((nv.INFX, nv.INFY, nv.INFZ)):
+            key = "inF" + "xyz"[d]
+            got = ctx.get_many(fid, 5)
+            for v in range(5):
+                r = rel(got[v][reg[key]], want[key][v][reg[key]])
+                worst[(key, v)] = r
+                assert r <= FP32_TOL["inF"], (key, v, r)
+                assert r > 0.0   # really single precision
+        got = ctx.get_many(nv.VF, 4)
+        for v in range(4):
+            r = rel(got[v][reg["vF"]], want["vF"][v][reg["vF"]])
+            worst[("vF", v)] = r
+            assert r <= FP32_TOL["vF"], ("vF", v, r)
+    print("worst FP32 relative error", max(worst.values()), flush=True)
+
+
+@pytest.mark.parametrize("name", ["ref_12x16x10", "ragged_20x24x17"])
+def test_mixed_precision_iteration_tracks_the_fp64_oracle(nv, orc, name):
+    """One inner iteration with cfg.precision = 1 (FP32 residual kernels and
+    an FP32 off-diagonal sweep around the FP64 diagonal, update and norm):
+    the increments agree with the FP64 oracle to single-precision rounding."""
+    print("case", name, flush=True)
+    case = make_case(name)
+    f = make_inputs(orc, case.slab, SEED + 10)
+    case.overrides = {"sponge": 0}
+    want = run_chain("orc", orc, case, f, dts=True, sweeps=1)
+    with open_ctx(nv, orc, case, sponge=0, precision=1) as ctx:
+        upload_inputs(ctx, f)
+        ctx.flux_x(1.0, 1e-4)
+        ctx.flux_y(1.0, 1e-4)
+        ctx.flux_z(1.0, 1e-4)
+        ctx.viscous()
+        er = ctx.dual_time_stepping(2e-4, 1e-4, sweeps=1)
+        NR = ctx.get_many(nv.NR, 5)
+    reg = regions(case.slab)["cell"]
+    worst = 0.0
+    for v in range(5):
+        r = rel(NR[v][reg], want["NR"][v][reg])
+        worst = max(worst, r)
+        assert r <= 1e-4, ("NR", v, r)
+        assert abs(er[v] - want["er"][v]) <= 1e-4 * abs(want["er"][v])
+    print("worst mixed-precision increment error", worst, flush=True)
+    assert worst > 1e-12   # the FP32 kernels really ran

@@ -318,3 +318,52 @@ def test_filter_transfer_function_of_x_mode(orc, m):
     np.testing.assert_allclose(np.sqrt(E2h[core]), np.abs((1 - G2h) ** 6 * u[core]), rtol=0, atol=1e-13)
     # and the signal is well above that rounding for these modes
     assert np.max(np.abs((1 - Gh) ** 6 * u[core])) > 1e-9
+
+
+# ------------------------------------------------------------ profiles
+
+def test_profiles_of_a_laminar_state_are_exact(orc):
+    """rho = 1, u = 1 - y^2, v = w = 0, T = 1 (p = R), stored as Q/J with J
+    varying in y: the x-z means are the point values and the second
+    moments their products (Statistic.cpp:137-160 quantities)."""
+    nx, ny, nz = 4, 7, 5
+    sl = Slab(nx, ny, nz)
+    c = ref_cfg(orc, nx, ny, nz)
+    yj = np.linspace(-0.9, 0.9, sl.shape[1])
+    J = np.broadcast_to((0.8 + 0.4 * np.abs(yj))[None, :, None], sl.shape).copy()
+    u = np.broadcast_to((1 - yj ** 2)[None, :, None], sl.shape)
+    K, R = c.K, c.R
+    U_ = [1.0 / J, u / J, np.zeros(sl.shape), np.zeros(sl.shape), (R / (K - 1) + 0.5 * u * u) / J]
+    out = np.zeros(11 * ny)
+    orc.orc_profiles(C.byref(c), ptrs(U_), dp(J), dp(out))
+    P = out.reshape(11, ny)
+    uj = (1 - yj ** 2)[2: ny + 2]
+    np.testing.assert_allclose(P[0], uj, rtol=1e-14)
+    np.testing.assert_allclose(P[3], 1.0, rtol=1e-12)               # T
+    np.testing.assert_allclose(P[4], uj * uj, rtol=1e-14)           # UU
+    np.testing.assert_allclose(P[7], 1.0, rtol=1e-12)               # TT
+    np.testing.assert_allclose(P[9], uj, rtol=1e-12)                # UT
+    for q in (1, 2, 5, 6, 8, 10):
+        assert np.all(P[q] == 0.0)
+
+
+def test_profiles_second_moments_bound_the_means(orc):
+    """On a seeded random state: <UU> >= <U>^2 etc. (variances are not
+    negative), and the profile of a state mirrored u -> -u keeps the second
+    moments and flips the first."""
+    case = Case(6, 8, 5, 1, 2, 2, 0, (0.01, 0.02, 0.015))
+    f = make_inputs(orc, case.slab, SEED + 12, dimensional=False)
+    c = case.cfg(orc)
+    ny = case.slab.ny
+    out = np.zeros(11 * ny)
+    orc.orc_profiles(C.byref(c), ptrs(f["U_"]), dp(f["J"]), dp(out))
+    P = out.reshape(11, ny)
+    for m, s in ((0, 4), (1, 5), (2, 6), (3, 7)):
+        assert np.all(P[s] - P[m] ** 2 >= -1e-15 * P[s])
+    U2 = [f["U_"][0], -f["U_"][1], f["U_"][2], f["U_"][3], f["U_"][4]]
+    out2 = np.zeros(11 * ny)
+    orc.orc_profiles(C.byref(c), ptrs(U2), dp(f["J"]), dp(out2))
+    P2 = out2.reshape(11, ny)
+    np.testing.assert_array_equal(P2[0], -P[0])
+    np.testing.assert_array_equal(P2[4], P[4])
+    np.testing.assert_array_equal(P2[8], -P[8])
