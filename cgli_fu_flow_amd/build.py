@@ -14,7 +14,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libcgli.so")
-SOURCES = ["flux.hip", "viscous.hip", "implicit.hip", "filter.hip", "stats.hip", "bc.hip", "capi.cpp", "channel.cpp"]
+SOURCES = ["flux.hip", "viscous.hip", "implicit.hip", "filter.hip", "stats.hip", "bc.hip", "cnn.hip", "capi.cpp", "channel.cpp",
+           "learned.cpp"]
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CGLI_ARCH", "gfx950")
 

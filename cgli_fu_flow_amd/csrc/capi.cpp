@@ -47,6 +47,7 @@ struct cgli_ctx {
     double *filt = nullptr;      // filter work arrays (allocated on first use)
     double *prof = nullptr;      // 11 x ny wall-normal profiles
     int last_fp32 = 0;           // FP32-residual inner iterations of the last channel step
+    ChannelState chan;           // rule, guess and network of the channel driver
     int *counters_host = nullptr;
     int max_spin = 1000000;
     bool timing = false;
@@ -146,6 +147,8 @@ void ctx_set_fp32(cgli_ctx *c, bool on) { c->p.fp32 = on ? 1 : 0; }
 
 void ctx_set_last_fp32(cgli_ctx *c, int n) { c->last_fp32 = n; }
 void ctx_mark(cgli_ctx *c, int id, int n) { mark(c, id, n); }
+const Geo &ctx_geo(cgli_ctx *c) { return c->g; }
+ChannelState &ctx_channel(cgli_ctx *c) { return c->chan; }
 }  // namespace cgli
 
 extern "C" {
@@ -517,6 +520,17 @@ int cgli_set_spin_limit(cgli_ctx *c, int max_spin)
 
 int cgli_dual_time_stepping(cgli_ctx *c, double deltaT, double e, double er[5], int sweeps, int write_residual)
 {
+    return ctx_dts(c, deltaT, e, er, sweeps, write_residual, false);
+}
+
+}  // extern "C"
+
+namespace cgli {
+
+// rhs_ready: MR, NL, NR already hold rhs_diag's output for the current U_
+// (ctx_rhs_sumsq was the last call on it), so rhs_diag is not repeated
+int ctx_dts(cgli_ctx *c, double deltaT, double e, double er[5], int sweeps, int write_residual, bool rhs_ready)
+{
     if (!c || !er || sweeps < 1) return CGLI_E_ARG;
     HIPCHK(hipSetDevice(c->device));
     int r = implicit_inputs(c);
@@ -536,12 +550,14 @@ int cgli_dual_time_stepping(cgli_ctx *c, double deltaT, double e, double er[5], 
         launch_sponge(g, c->p, e, c->cfg.gstart, c->cfg.gend0, c->cfg.nx_global, c->cfg.nx_inlet,
                       c->cfg.nx_outlet, c5(c, CGLI_U_), c->f[CGLI_J], c->f[CGLI_MET + 4], f5(c, CGLI_RES),
                       c->scratch, s);
-    tbegin(c, T_RHS);
-    launch_rhs_diag(g, c->p, deltaT, e, c->cfg.sponge, c5(c, CGLI_U_), c5(c, CGLI_U), c5(c, CGLI_UQ),
-                    c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_INFX), c5(c, CGLI_INFY), c5(c, CGLI_INFZ),
-                    C4{{c->f[CGLI_VF], c->f[CGLI_VF + 1], c->f[CGLI_VF + 2], c->f[CGLI_VF + 3]}},
-                    c5(c, CGLI_RES), f5(c, CGLI_MR), f5(c, CGLI_NL), f5(c, CGLI_NR), s);
-    tend(c, T_RHS);
+    if (!rhs_ready || c->cfg.sponge) {
+        tbegin(c, T_RHS);
+        launch_rhs_diag(g, c->p, deltaT, e, c->cfg.sponge, c5(c, CGLI_U_), c5(c, CGLI_U), c5(c, CGLI_UQ),
+                        c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_INFX), c5(c, CGLI_INFY), c5(c, CGLI_INFZ),
+                        C4{{c->f[CGLI_VF], c->f[CGLI_VF + 1], c->f[CGLI_VF + 2], c->f[CGLI_VF + 3]}},
+                        c5(c, CGLI_RES), f5(c, CGLI_MR), f5(c, CGLI_NL), f5(c, CGLI_NR), s);
+        tend(c, T_RHS);
+    }
     const int mlast = g.gend + g.NY + g.NZ;
     if (c->lusgs_mode == 1) {
         r = wave_setup(c);
@@ -594,6 +610,10 @@ int cgli_dual_time_stepping(cgli_ctx *c, double deltaT, double e, double er[5], 
     mark(c, CGLI_MR, 15);
     return finish_norm(c, nblk, er, c->lusgs_mode == 1);
 }
+
+}  // namespace cgli
+
+extern "C" {
 
 int cgli_dplusgs(cgli_ctx *c, double deltaT, double er[5], double e, int sweeps)
 {
@@ -690,3 +710,36 @@ int cgli_timing_get(cgli_ctx *c, int cls, double *total_ms, long *count)
 }
 
 }  // extern "C"
+
+namespace cgli {
+
+// without the sponge term (the channel has none)
+int ctx_rhs_sumsq(cgli_ctx *c, double deltaT, double e, double out[5])
+{
+    HIPCHK(hipSetDevice(c->device));
+    int r = implicit_inputs(c);
+    if (r) return r;
+    const Geo &g = c->g;
+    const long ncell = (long)g.nx * g.ny * g.nz;
+    const long nb = (ncell + 255) / 256;
+    r = scratch(c, 5 * nb + (long)g.NYY * g.NZZ + 64);
+    if (!r) r = scratch(c, 5L * g.nx * g.ny + 16);
+    if (r) return r;
+    const hipStream_t s = c->stream;
+    launch_rhs_diag(g, c->p, deltaT, e, 0, c5(c, CGLI_U_), c5(c, CGLI_U), c5(c, CGLI_UQ),
+                    c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_INFX), c5(c, CGLI_INFY), c5(c, CGLI_INFZ),
+                    C4{{c->f[CGLI_VF], c->f[CGLI_VF + 1], c->f[CGLI_VF + 2], c->f[CGLI_VF + 3]}},
+                    c5(c, CGLI_RES), f5(c, CGLI_MR), f5(c, CGLI_NL), f5(c, CGLI_NR), s);
+    double *dev = c->scratch + c->scratch_len - 8;
+    launch_sumsq5(g, c5(c, CGLI_MR), c->scratch, dev, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->er_host, dev, sizeof(double) * 5, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int q = 0; q < 5; q++) {
+        out[q] = c->er_host[q];
+        if (!std::isfinite(out[q])) return CGLI_E_NONFINITE;
+    }
+    return CGLI_OK;
+}
+
+}  // namespace cgli

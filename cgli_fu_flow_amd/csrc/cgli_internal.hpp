@@ -104,6 +104,31 @@ void launch_lusgs_wave(const Geo &g, int forward, const void *coef, const double
                        hipStream_t s);
 void launch_reset_ints(int *p, int n, hipStream_t s);
 
+// learned initial guess (cnn.hip): see that file for layouts
+struct NetDims { int B, D, H, W; };
+constexpr int NET_NSCALE = 15;   // inverse scales per record: 1/s1, 1/sd, 1/s2
+int net_conv(int cin_pad, int cout_pad, bool act, const void *X, const void *W, const float *bias, float *Z, void *A,
+             const NetDims &m, int cout, int per, hipStream_t s);
+void net_gelu_back(const float *dA, const float *Z, void *dZ, float *dZf, long n, hipStream_t s);
+void net_to_bf16(const float *x, void *y, long n, hipStream_t s);
+void net_transpose_w(const void *W, void *WT, int cin_pad, int cout_pad, hipStream_t s);
+int net_wgrad(int cin_pad, int cout_pad, const void *dZ, const void *X, const NetDims &m, int nblk, float *part,
+              int cout, int cin, float *dW, hipStream_t s);
+void net_bias_grad(const float *dZ, long nvox, int cpad, int cout, float *db, hipStream_t s);
+void net_mse(const float *Y, const float *T, const unsigned char *mask, long nvox, float inv, float *dY,
+             double *partial, int nblk, double *loss, double loss_scale, hipStream_t s);
+void net_adam(float *w, float *m1, float *m2, const float *g, void *wb, long n, float lr, float b1, float b2,
+              float eps, float c1, float c2, hipStream_t s);
+void net_store(const Geo &g, C5 U, C5 Uq, float *S, hipStream_t s);
+void net_gather(const float *S, const float *rs, int cap, long vol, int nx, int ny, int nz, const int *org,
+                const NetDims &m, int halo, void *X, float *T, unsigned char *mask, hipStream_t s);
+void net_pack_input(const float *x, void *X, long nv, hipStream_t s);
+void net_record_scales(const float *Sa, const float *Sb, const float *Sc, long vol, double *partial, int nblk,
+                       float *rs, hipStream_t s);
+void net_guess(const Geo &g, C5 U, C5 Uq, const float *prev, const float *Y, const float *rs, F5 U_, hipStream_t s);
+// sum over the interior cells of f_q^2 for q < 5 into out_dev[5] (stats.hip)
+void launch_sumsq5(const Geo &g, C5 f, double *partials, double *out_dev, hipStream_t s);
+
 // context access for host modules other than capi.cpp
 int ctx_field(cgli_ctx *c, int id, double **dptr);   // allocates on first use
 hipStream_t ctx_stream(cgli_ctx *c);
@@ -111,5 +136,31 @@ const cgli_config *ctx_config(cgli_ctx *c);
 void ctx_mark(cgli_ctx *c, int id, int n);
 void ctx_set_fp32(cgli_ctx *c, bool on);          // residual kernels in FP32 from now on
 void ctx_set_last_fp32(cgli_ctx *c, int n);       // FP32 inner iterations of the last step
+const Geo &ctx_geo(cgli_ctx *c);
+// residual MR of the current U_ (rhs_diag after the caller's flux and
+// viscous calls); out[5] = its interior sum of squares per variable
+int ctx_rhs_sumsq(cgli_ctx *c, double deltaT, double e, double out[5]);
+// one inner iteration (cgli_dual_time_stepping); rhs_ready: the last call
+// on U_ was ctx_rhs_sumsq, whose right-hand side is reused
+int ctx_dts(cgli_ctx *c, double deltaT, double e, double er[5], int sweeps, int write_residual, bool rhs_ready);
+// per-context state of the channel driver (channel.cpp)
+struct ChannelState {
+    int rule = 0;            // 0: per-step drop, 1: running maximum
+    double E[5] = {0, 0, 0, 0, 0};
+    int guess = 0;           // 0: U^n, 1: linear, 2: network, 3: quadratic extrapolation
+    cgli_net *net = nullptr;
+    int used_net = 0;        // last step: 1 if the network's guess was taken
+    double r_net[5] = {0, 0, 0, 0, 0}, r_ext[5] = {0, 0, 0, 0, 0};
+};
+ChannelState &ctx_channel(cgli_ctx *c);
+// network side of the channel driver (learned.cpp)
+int net_record(cgli_net *net, cgli_ctx *c);                 // append U - Uq to the ring
+int net_ready(cgli_net *net, cgli_ctx *c);                  // trained, 3 records of this grid
+// whole-grid prediction on the latest records: Y [cell][16] and the
+// latest record's inverse scales rs[NET_NSCALE], both on the device
+int net_predict(cgli_net *net, hipStream_t s, const float **Y, const float **rs);
+// the ring's record before the latest ([cell][5], device), or null when
+// the ring does not hold two records of this grid
+const float *net_prev_record(cgli_net *net, cgli_ctx *c);
 
 }  // namespace cgli

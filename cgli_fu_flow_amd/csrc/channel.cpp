@@ -237,14 +237,134 @@ int cgli_channel_ghosts(cgli_ctx *ctx)
     return cgli_bc_x_periodic(ctx);
 }
 
+}  // extern "C"
+
+namespace {
+
+// residual sums of squares of the current U_ (ghosts, fluxes, viscous
+// terms and the right-hand side of the inner iteration)
+int guess_residual(cgli_ctx *ctx, double deltaT, double out[5])
+{
+    int r = cgli_channel_ghosts(ctx);
+    if (!r) r = cgli_flux_x(ctx, 1.0, 1e-4);
+    if (!r) r = cgli_flux_y(ctx, 1.0, 1e-4);
+    if (!r) r = cgli_flux_z(ctx, 1.0, 1e-4);
+    if (!r) r = cgli_viscous(ctx);
+    if (!r) r = ctx_rhs_sumsq(ctx, deltaT, 1e-4, out);
+    return r;
+}
+
+// U_ at the start of a step (after the time shift U_ = U = U^n).  *ready
+// is set when the last evaluation left fluxes and right-hand side of the
+// chosen guess in place, so that the first inner iteration can reuse them.
+int initial_guess(cgli_ctx *ctx, double deltaT, ChannelState &cs, bool *ready)
+{
+    cs.used_net = 0;
+    *ready = false;
+    if (cs.guess == 0) return CGLI_OK;
+    const Geo &g = ctx_geo(ctx);
+    const hipStream_t s = ctx_stream(ctx);
+    double *u_[5], *u[5], *uq[5];
+    for (int q = 0; q < 5; q++)
+        if (ctx_field(ctx, CGLI_U_ + q, &u_[q]) || ctx_field(ctx, CGLI_U + q, &u[q]) ||
+            ctx_field(ctx, CGLI_UQ + q, &uq[q]))
+            return CGLI_E_NOMEM;
+    const F5 U_{{u_[0], u_[1], u_[2], u_[3], u_[4]}};
+    const C5 U{{u[0], u[1], u[2], u[3], u[4]}}, Uq{{uq[0], uq[1], uq[2], uq[3], uq[4]}};
+    const float *prev = cs.guess >= 2 ? net_prev_record(cs.net, ctx) : nullptr;
+    if (cs.guess == 2 && prev && net_ready(cs.net, ctx)) {
+        const float *Y = nullptr, *rs = nullptr;
+        int r = net_predict(cs.net, s, &Y, &rs);
+        if (r) return r;
+        // the extrapolated guess first, the network's last: when the
+        // network's is taken its right-hand side is still in place
+        net_guess(g, U, Uq, nullptr, nullptr, nullptr, U_, s);
+        r = guess_residual(ctx, deltaT, cs.r_ext);
+        if (r) return r;
+        net_guess(g, U, Uq, prev, Y, rs, U_, s);
+        r = guess_residual(ctx, deltaT, cs.r_net);
+        if (r) return r;
+        bool worse = false;
+        for (int q = 0; q < 5; q++) worse = worse || cs.r_net[q] > cs.r_ext[q];
+        if (worse) {
+            net_guess(g, U, Uq, nullptr, nullptr, nullptr, U_, s);
+        } else {
+            cs.used_net = 1;
+            *ready = true;
+        }
+    } else if (cs.guess == 3 && prev) {
+        net_guess(g, U, Uq, prev, nullptr, nullptr, U_, s);
+    } else {
+        net_guess(g, U, Uq, nullptr, nullptr, nullptr, U_, s);
+    }
+    return hipGetLastError() == hipSuccess ? CGLI_OK : CGLI_E_HIP;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cgli_channel_set_rule(cgli_ctx *ctx, int rule)
+{
+    if (!ctx || rule < 0 || rule > 1) return CGLI_E_ARG;
+    ChannelState &cs = ctx_channel(ctx);
+    cs.rule = rule;
+    for (int v = 0; v < 5; v++) cs.E[v] = 0.0;
+    return CGLI_OK;
+}
+
+int cgli_channel_rule_state(cgli_ctx *ctx, double E[5], int set)
+{
+    if (!ctx || !E) return CGLI_E_ARG;
+    ChannelState &cs = ctx_channel(ctx);
+    for (int v = 0; v < 5; v++) {
+        if (set) cs.E[v] = E[v];
+        else E[v] = cs.E[v];
+    }
+    return CGLI_OK;
+}
+
+int cgli_channel_set_guess(cgli_ctx *ctx, int guess)
+{
+    if (!ctx || guess < 0 || guess > 3) return CGLI_E_ARG;
+    ctx_channel(ctx).guess = guess;
+    return CGLI_OK;
+}
+
+int cgli_channel_attach_net(cgli_ctx *ctx, cgli_net *net)
+{
+    if (!ctx) return CGLI_E_ARG;
+    ctx_channel(ctx).net = net;
+    return CGLI_OK;
+}
+
+int cgli_channel_guess_info(cgli_ctx *ctx, int *used_net, double r_net[5], double r_ext[5])
+{
+    if (!ctx) return CGLI_E_ARG;
+    const ChannelState &cs = ctx_channel(ctx);
+    if (used_net) *used_net = cs.used_net;
+    for (int v = 0; v < 5; v++) {
+        if (r_net) r_net[v] = cs.r_net[v];
+        if (r_ext) r_ext[v] = cs.r_ext[v];
+    }
+    return CGLI_OK;
+}
+
 int cgli_channel_step(cgli_ctx *ctx, double deltaT, double tol, int max_inner, int sweeps, int *inner,
                       double er_first[5], double er_last[5])
 {
     if (!ctx || max_inner < 1) return CGLI_E_ARG;
     const cgli_config *cfg = ctx_config(ctx);
     const hipStream_t s = ctx_stream(ctx);
+    ChannelState &cs = ctx_channel(ctx);
     double e1[5] = {0, 0, 0, 0, 0}, er[5] = {0, 0, 0, 0, 0};
     int it = 0;
+    bool ready = false;
+    {
+        ctx_set_fp32(ctx, cfg->precision == 1);
+        const int r = initial_guess(ctx, deltaT, cs, &ready);
+        if (r) return r;
+    }
     // Mixed precision (cfg->precision == 1): FP32 residual kernels while the
     // drop is above 100 x tol and still falling, FP64 afterwards.  The
     // residual's rounding bounds the drop an iteration can reach (about
@@ -256,17 +376,29 @@ int cgli_channel_step(cgli_ctx *ctx, double deltaT, double tol, int max_inner, i
     for (it = 1; it <= max_inner; it++) {
         ctx_set_fp32(ctx, fp32);
         if (fp32) n32++;
-        int r = cgli_channel_ghosts(ctx);
-        if (!r) r = cgli_flux_x(ctx, 1.0, 1e-4);
-        if (!r) r = cgli_flux_y(ctx, 1.0, 1e-4);
-        if (!r) r = cgli_flux_z(ctx, 1.0, 1e-4);
-        if (!r) r = cgli_viscous(ctx);
-        if (!r) r = cgli_dual_time_stepping(ctx, deltaT, 1e-4, er, sweeps, 0);
+        // the first iteration reuses the residual evaluation of the guess
+        // check when the network's guess was taken
+        const bool reuse = ready && it == 1;
+        int r = CGLI_OK;
+        if (!reuse) {
+            r = cgli_channel_ghosts(ctx);
+            if (!r) r = cgli_flux_x(ctx, 1.0, 1e-4);
+            if (!r) r = cgli_flux_y(ctx, 1.0, 1e-4);
+            if (!r) r = cgli_flux_z(ctx, 1.0, 1e-4);
+            if (!r) r = cgli_viscous(ctx);
+        }
+        if (!r) r = ctx_dts(ctx, deltaT, 1e-4, er, sweeps, 0, reuse);
         if (r) return r;
         if (it == 1)
             for (int v = 0; v < 5; v++) e1[v] = er[v];
+        // rule 1 normalises by the running maximum (main.cpp:287-304)
+        const double *ref = e1;
+        if (cs.rule == 1) {
+            for (int v = 0; v < 5; v++) cs.E[v] = std::max(cs.E[v], er[v]);
+            ref = cs.E;
+        }
         double drop = 0.0;
-        for (int v = 0; v < 5; v++) drop = std::max(drop, e1[v] > 0 ? er[v] / e1[v] : 0.0);
+        for (int v = 0; v < 5; v++) drop = std::max(drop, ref[v] > 0 ? er[v] / ref[v] : 0.0);
         if (drop <= tol) break;
         if (fp32 && (drop <= 100 * tol || (it > 2 && drop > 0.85 * prev))) fp32 = false;
         prev = drop;
@@ -283,6 +415,10 @@ int cgli_channel_step(cgli_ctx *ctx, double deltaT, double tol, int max_inner, i
         ctx_field(ctx, CGLI_UQ + q, &uq);
         launch_copy_field(uq, u, N, s);
         launch_copy_field(u, u_, N, s);
+    }
+    if (cs.net) {
+        const int r = net_record(cs.net, ctx);
+        if (r) return r;
     }
     if (hipStreamSynchronize(s) != hipSuccess) return CGLI_E_HIP;
     if (inner) *inner = it;

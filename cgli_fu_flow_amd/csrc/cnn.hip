@@ -1,0 +1,576 @@
+// cnn.hip -- kernels of the learned initial guess: a 3-D convolutional
+// network (15 -> 32 -> 32 -> 32 -> 5 channels, 3x3x3 kernels, GELU between
+// layers) in BF16 with FP32 accumulation on the gfx950 matrix cores, its
+// training kernels (input and weight gradients, masked MSE, Adam), and the
+// snapshot ring it learns from.
+//
+// Activations are channel-last, [voxel][C] with C padded to 16 or 32 and
+// voxel = ((b*D + d)*H + h)*W + w, W a multiple of 16.  Weights are
+// [Cout_pad][27 taps][Cin_pad], tap = (dd+1)*9 + (dh+1)*3 + (dw+1) for the
+// offsets dd, dh, dw in -1..1 (cross-correlation, as torch's conv3d with
+// padding 1).  Outside a sample the input is zero, or, for the whole-field
+// pass, periodic in D and W (x and z of the channel) and zero in H (y).
+//
+// A convolution is an implicit GEMM on v_mfma_f32_16x16x32_bf16:
+//   C[co][n] += sum_k A[co][k] B[k][n],  k = (tap, ci),  n = 16 voxels of a row.
+// Lane l holds row l%16 of A and column l%16 of B over the k values
+// 8*(l/16)..+7 (16 contiguous bytes of one weight row, of one voxel's
+// channels); the sum over k does not depend on how k is split across lanes as
+// long as A and B use the same split.  The result holds rows 4*(l/16)..+3 of
+// column l%16.
+//   forward          z = conv(x, W) + b (FP32), a = GELU(z) rounded to BF16
+//   input gradient   dx = conv(dz, W^T),  W^T[ci][26-tap][co] = W[co][tap][ci]
+//   weight gradient  dW[co][tap][ci] = sum_v dz[v][co] x[v + off(tap)][ci]:
+//                    per tap a GEMM over voxels with K = voxels, partial sums
+//                    per block, then a fixed-order FP64 reduction, so the
+//                    bits do not depend on scheduling
+#include "cgli_internal.hpp"
+
+namespace cgli {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+// FP32 -> BF16 bits, round to nearest even (a NaN stays a quiet NaN)
+__device__ __forceinline__ u16 f2bf(float f)
+{
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (u16)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (u16)(u >> 16);
+}
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z)
+{
+    const float Phi = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
+    const float phi = 0.39894228040143268f * expf(-0.5f * z * z);
+    return Phi + z * phi;
+}
+
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u)); }
+
+// 8 channels c0..c0+7 of voxel (b, d, h, w) of a [voxel][C] BF16 tensor;
+// zero outside the sample unless `per` (then periodic in d and w)
+__device__ __forceinline__ bf16x8 frag(const u16 *__restrict__ X, int C, const NetDims &m, int b, int d, int h,
+                                       int w, int c0, int per)
+{
+    if (per) {
+        d = d < 0 ? d + m.D : d >= m.D ? d - m.D : d;
+        w = w < 0 ? w + m.W : w >= m.W ? w - m.W : w;
+    }
+    if (d < 0 || d >= m.D || h < 0 || h >= m.H || w < 0 || w >= m.W) return zero8();
+    const long v = (((long)b * m.D + d) * m.H + h) * m.W + w;
+    return *reinterpret_cast<const bf16x8 *>(X + v * C + c0);
+}
+
+// y = conv(x, W) + bias for all voxels.  One wave per 16-voxel row tile and
+// all COUT = 16*MT outputs.  Outputs (either may be null): Z FP32
+// [v][COUT] (the pre-activation), A BF16 [v][COUT] = GELU(Z) when ACT, else
+// Z; channels co >= cout are written as 0.
+template <int CIN, int MT, bool ACT>
+__global__ void __launch_bounds__(256) k_conv(const u16 *__restrict__ X, const u16 *__restrict__ Wt,
+                                              const float *__restrict__ bias, float *__restrict__ Z,
+                                              u16 *__restrict__ A, NetDims m, int cout, long ntiles, int per)
+{
+    constexpr int COUT = 16 * MT;
+    constexpr int TPS = 32 / CIN;                 // taps per K step of 32
+    constexpr int NK = (27 + TPS - 1) / TPS;
+    const int lane = threadIdx.x & 63;
+    const long tile = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= ntiles) return;
+    const int rowsW = m.W / 16;
+    const long row = tile / rowsW;                // (b, d, h)
+    const int w0 = (int)(tile - row * rowsW) * 16;
+    const int h = (int)(row % m.H), d = (int)((row / m.H) % m.D), b = (int)(row / ((long)m.H * m.D));
+    const int n = lane & 15, kq = lane >> 4;
+    f32x4 acc[MT];
+#pragma unroll
+    for (int t = 0; t < MT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int kk = 8 * kq;
+    const int tsub = kk / CIN, c0 = kk % CIN;
+#pragma unroll 2
+    for (int ks = 0; ks < NK; ks++) {
+        const int tap = ks * TPS + tsub;
+        bf16x8 bfr = zero8();
+        if (tap < 27) {
+            const int dd = tap / 9 - 1, dh = (tap / 3) % 3 - 1, dw = tap % 3 - 1;
+            bfr = frag(X, CIN, m, b, d + dd, h + dh, w0 + n + dw, c0, per);
+        }
+#pragma unroll
+        for (int t = 0; t < MT; t++) {
+            bf16x8 afr = zero8();
+            if (tap < 27) afr = *reinterpret_cast<const bf16x8 *>(Wt + ((long)(16 * t + n) * 27 + tap) * CIN + c0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr, acc[t], 0, 0, 0);
+        }
+    }
+    const long v = (((long)b * m.D + d) * m.H + h) * m.W + w0 + n;
+#pragma unroll
+    for (int t = 0; t < MT; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int co = 16 * t + 4 * kq + r;
+            const bool live = co < cout;
+            const float z = live ? acc[t][r] + (bias ? bias[co] : 0.0f) : 0.0f;
+            if (Z) Z[v * COUT + co] = z;
+            if (A) A[v * COUT + co] = live ? f2bf(ACT ? gelu(z) : z) : (u16)0;
+        }
+}
+
+// dz = da * GELU'(z): BF16 for the next convolutions, FP32 for the bias
+__global__ void k_gelu_back(const float *__restrict__ dA, const float *__restrict__ Z, u16 *__restrict__ dZ,
+                            float *__restrict__ dZf, long n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float g = dA[t] * gelu_grad(Z[t]);
+    dZ[t] = f2bf(g);
+    dZf[t] = g;
+}
+
+__global__ void k_to_bf16(const float *__restrict__ x, u16 *__restrict__ y, long n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) y[t] = f2bf(x[t]);
+}
+
+// W^T[ci][26-tap][co] = W[co][tap][ci]   (Cin_pad x 27 x Cout_pad)
+__global__ void k_transpose_w(const u16 *__restrict__ W, u16 *__restrict__ WT, int cin_pad, int cout_pad)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cin_pad * 27 * cout_pad) return;
+    const int co = t % cout_pad, tap = (t / cout_pad) % 27, ci = t / (cout_pad * 27);
+    WT[t] = W[((long)co * 27 + (26 - tap)) * cin_pad + ci];
+}
+
+// Weight-gradient partials.  Block (bx, tap) sums dz[v][co] x[v + off(tap)][ci]
+// over its voxel range [bx*per, (bx+1)*per) in chunks of CH voxels, staged
+// transposed in LDS ([channel][voxel]) so that a lane's 8 consecutive voxels
+// of one channel are 16 contiguous bytes.  One wave per 16x16 (co, ci) tile.
+// part[bx][tap][co][ci].
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) k_wgrad(const u16 *__restrict__ dZ, const u16 *__restrict__ X, NetDims m,
+                                               long nvox, long per, float *__restrict__ part)
+{
+    constexpr int CH = 128, LD = CH + 8;
+    constexpr int MT = COUT / 16, NT = CIN / 16, NW = MT * NT;
+    constexpr int VZ = COUT / 8, VX = CIN / 8;    // 16-byte vectors per voxel
+    __shared__ __align__(16) u16 sdz[COUT * LD];
+    __shared__ __align__(16) u16 sx[CIN * LD];
+    __shared__ long src[CH];
+    const int tap = blockIdx.y;
+    const int dd = tap / 9 - 1, dh = (tap / 3) % 3 - 1, dw = tap % 3 - 1;
+    const long v0 = (long)blockIdx.x * per;
+    const long v1 = v0 + per < nvox ? v0 + per : nvox;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int mt = wave / NT, nt = wave % NT;
+    const int n = lane & 15, kq = lane >> 4;
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (long vb = v0; vb < v1; vb += CH) {
+        __syncthreads();
+        if (threadIdx.x < CH) {
+            const long v = vb + threadIdx.x;
+            long sidx = -1;
+            if (v < v1) {
+                const int w = (int)(v % m.W), h = (int)((v / m.W) % m.H);
+                const int d = (int)((v / ((long)m.W * m.H)) % m.D), b = (int)(v / ((long)m.W * m.H * m.D));
+                const int d2 = d + dd, h2 = h + dh, w2 = w + dw;
+                if (d2 >= 0 && d2 < m.D && h2 >= 0 && h2 < m.H && w2 >= 0 && w2 < m.W)
+                    sidx = (((long)b * m.D + d2) * m.H + h2) * m.W + w2;
+            }
+            src[threadIdx.x] = sidx;
+        }
+        __syncthreads();
+        // BF16 data moves as 32-bit words (two channels each): element
+        // access to __bf16 vectors is avoided
+        for (int t = threadIdx.x; t < CH * (VZ + VX); t += blockDim.x) {
+            const int vv = t % CH, q = t / CH;
+            const long v = vb + vv;
+            uint4 val = make_uint4(0u, 0u, 0u, 0u);
+            const u16 *srcp = nullptr;
+            u16 *dst;
+            if (q < VZ) {
+                if (v < v1) srcp = dZ + v * COUT + 8 * q;
+                dst = sdz + 8 * q * LD + vv;
+            } else {
+                const long s = src[vv];
+                if (s >= 0) srcp = X + s * CIN + 8 * (q - VZ);
+                dst = sx + 8 * (q - VZ) * LD + vv;
+            }
+            if (srcp) val = *reinterpret_cast<const uint4 *>(srcp);
+            const unsigned wd[4] = {val.x, val.y, val.z, val.w};
+#pragma unroll
+            for (int e = 0; e < 8; e++) dst[e * LD] = (u16)(wd[e >> 1] >> (16 * (e & 1)));
+        }
+        __syncthreads();
+        if (wave < NW) {
+#pragma unroll
+            for (int kc = 0; kc < CH / 32; kc++) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&sdz[(16 * mt + n) * LD + 32 * kc + 8 * kq]);
+                const bf16x8 bb = *reinterpret_cast<const bf16x8 *>(&sx[(16 * nt + n) * LD + 32 * kc + 8 * kq]);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc, 0, 0, 0);
+            }
+        }
+    }
+    if (wave < NW) {
+        float *o = part + ((long)blockIdx.x * 27 + tap) * COUT * CIN;
+#pragma unroll
+        for (int r = 0; r < 4; r++) o[(16 * mt + 4 * kq + r) * CIN + 16 * nt + n] = acc[r];
+    }
+}
+
+// dW[co][tap][ci] = sum over blocks, in block order; padded entries are 0
+__global__ void k_wgrad_reduce(const float *__restrict__ part, int nblk, int cout_pad, int cin_pad, int cout,
+                               int cin, float *__restrict__ dW)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cout_pad * 27 * cin_pad) return;
+    const int ci = t % cin_pad, tap = (t / cin_pad) % 27, co = t / (cin_pad * 27);
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += part[(((long)b * 27 + tap) * cout_pad + co) * cin_pad + ci];
+    dW[t] = (co < cout && ci < cin) ? (float)s : 0.0f;
+}
+
+// db[co] = sum_v dz[v][co]: one block per channel, fixed reduction tree
+__global__ void __launch_bounds__(256) k_bias_grad(const float *__restrict__ dZ, long nvox, int cpad, int cout,
+                                                   float *__restrict__ db)
+{
+    __shared__ double sh[256];
+    const int co = blockIdx.x;
+    double s = 0.0;
+    for (long v = threadIdx.x; v < nvox; v += 256) s += dZ[v * cpad + co];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[co] = co < cout ? (float)sh[0] : 0.0f;
+}
+
+// Masked mean-square error over the first 5 of 16 output channels:
+// partial[block] = sum of e^2, dY = 2 e inv on masked voxels (0 elsewhere),
+// inv = 1 / (5 x masked voxels)
+__global__ void __launch_bounds__(256) k_mse(const float *__restrict__ Y, const float *__restrict__ T,
+                                             const unsigned char *__restrict__ mask, long nvox, float inv,
+                                             float *__restrict__ dY, double *__restrict__ partial)
+{
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < nvox * 16; t += (long)gridDim.x * 256) {
+        const int c = (int)(t & 15);
+        const long v = t >> 4;
+        float g = 0.0f;
+        if (c < 5 && mask[v]) {
+            const float e = Y[t] - T[v * 5 + c];
+            s += (double)e * e;
+            g = 2.0f * e * inv;
+        }
+        dY[t] = g;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// out = scale x sum of partial[0..n) in index order (one thread)
+__global__ void k_sum_partials(const double *__restrict__ partial, int n, double scale, double *__restrict__ out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int b = 0; b < n; b++) s += partial[b];
+        *out = s * scale;
+    }
+}
+
+// Adam on the FP32 master parameters; refreshes their BF16 copy
+__global__ void k_adam(float *__restrict__ w, float *__restrict__ m1, float *__restrict__ m2,
+                       const float *__restrict__ g, u16 *__restrict__ wb, long n, float lr, float b1, float b2,
+                       float eps, float c1, float c2)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float gt = g[t];
+    const float a = b1 * m1[t] + (1.0f - b1) * gt;
+    const float b = b2 * m2[t] + (1.0f - b2) * gt * gt;
+    m1[t] = a;
+    m2[t] = b;
+    const float nw = w[t] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+    w[t] = nw;
+    wb[t] = f2bf(nw);
+}
+
+// ---------------------------------------------------------------- snapshots
+// Ring slot = the physical-step increment dq = U - Uq over the interior
+// cells, FP32, [cell][5] with cell = ((i-3)*ny + (j-2))*nz + (k-2).
+__global__ void k_store(Geo g, C5 U, C5 Uq, float *__restrict__ S)
+{
+    const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = 3 + blockIdx.z;
+    if (k > g.NZ || j > g.NY) return;
+    const long c = ix(g, i, j, k);
+    const long v = ((long)(i - 3) * g.ny + (j - 2)) * g.nz + (k - 2);
+#pragma unroll
+    for (int q = 0; q < 5; q++) S[v * 5 + q] = (float)(U.v[q][c] - Uq.v[q][c]);
+}
+
+// Network input and target of sample b.  org[b] = (t, i0, j0, k0): record t
+// of the ring (slot t % cap), crop origin (i0, j0, k0), wrapping in i and k.
+// rs[slot][15] holds each record's inverse scales 1/s1, 1/sd, 1/s2 (five
+// each): the field RMS of dq_t, of d1_t = dq_t - dq_{t-1} and of
+// d2_t = d1_t - d1_{t-1}, per variable.  With record t's scales:
+//   x[0..4]   = dq_t / s1
+//   x[5..9]   = d1_t / sd
+//   x[10..14] = d2_t / s2,   x[15] = 0
+//   target    = d2_{t+1} / s2 = ((dq_{t+1} - dq_t) - d1_t) / s2  (T not null)
+// so that a zero output is the quadratic extrapolation dq_{t+1} = dq_t + d1_t.
+// mask[v] (when not null): 1 where the voxel's `halo`-voxel neighbourhood
+// lies in the crop or beyond a wall (j outside 0..ny-1), else 0.
+__global__ void k_gather(const float *__restrict__ S, const float *__restrict__ rs, int cap, long vol, int nx,
+                         int ny, int nz, const int *__restrict__ org, NetDims m, int halo, u16 *__restrict__ X,
+                         float *__restrict__ T, unsigned char *__restrict__ mask)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nv = (long)m.B * m.D * m.H * m.W;
+    if (t >= nv) return;
+    const int w = (int)(t % m.W), h = (int)((t / m.W) % m.H), d = (int)((t / ((long)m.W * m.H)) % m.D);
+    const int b = (int)(t / ((long)m.W * m.H * m.D));
+    const int *o = org + 4 * b;
+    const int i = (o[1] + d) % nx, k = (o[3] + w) % nz, j = o[2] + h;
+    const long cell = ((long)i * ny + j) * nz + k;
+    const int rec = o[0];
+    const float *sc = rs + (long)(rec % cap) * NET_NSCALE;
+    const float *s0 = S + ((long)(rec % cap) * vol + cell) * 5;
+    const float *s1 = S + ((long)((rec - 1) % cap) * vol + cell) * 5;
+    const float *s2 = S + ((long)((rec - 2) % cap) * vol + cell) * 5;
+    u16 *x = X + t * 16;
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        const float a = s0[q], bq = s1[q], c = s2[q];
+        const float d1 = a - bq, d1p = bq - c;
+        x[q] = f2bf(a * sc[q]);
+        x[5 + q] = f2bf(d1 * sc[5 + q]);
+        x[10 + q] = f2bf((d1 - d1p) * sc[10 + q]);
+    }
+    x[15] = 0;
+    if (T) {
+        const float *s3 = S + ((long)((rec + 1) % cap) * vol + cell) * 5;
+#pragma unroll
+        for (int q = 0; q < 5; q++) T[t * 5 + q] = ((s3[q] - s0[q]) - (s0[q] - s1[q])) * sc[10 + q];
+    }
+    if (mask) {
+        const bool okd = d >= halo && d < m.D - halo;
+        const bool okw = w >= halo && w < m.W - halo;
+        const bool okh = (h >= halo || o[2] == 0) && (h < m.H - halo || o[2] + m.H == ny);
+        mask[t] = (okd && okw && okh) ? 1 : 0;
+    }
+}
+
+// Host-supplied FP32 input [v][15] -> BF16 [v][16]
+__global__ void k_pack_input(const float *__restrict__ x, u16 *__restrict__ X, long nv)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv) return;
+#pragma unroll
+    for (int q = 0; q < 15; q++) X[t * 16 + q] = f2bf(x[t * 15 + q]);
+    X[t * 16 + 15] = 0;
+}
+
+// Per-block sums over vol cells of dq_a^2, d1^2 and d2^2 (five each) with
+// d1 = dq_a - dq_b, d2 = d1 - (dq_b - dq_c), FP32 differences, FP64 sums
+__global__ void __launch_bounds__(256) k_record_sums(const float *__restrict__ Sa, const float *__restrict__ Sb,
+                                                     const float *__restrict__ Sc, long vol,
+                                                     double *__restrict__ partial)
+{
+    __shared__ double sh[NET_NSCALE][256];
+    double s[NET_NSCALE];
+#pragma unroll
+    for (int q = 0; q < NET_NSCALE; q++) s[q] = 0.0;
+    for (long v = (long)blockIdx.x * 256 + threadIdx.x; v < vol; v += (long)gridDim.x * 256)
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            const float a = Sa[v * 5 + q], b = Sb[v * 5 + q], c = Sc[v * 5 + q];
+            const float d1 = a - b, d2 = d1 - (b - c);
+            s[q] += (double)a * a;
+            s[5 + q] += (double)d1 * d1;
+            s[10 + q] += (double)d2 * d2;
+        }
+#pragma unroll
+    for (int q = 0; q < NET_NSCALE; q++) sh[q][threadIdx.x] = s[q];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+#pragma unroll
+            for (int q = 0; q < NET_NSCALE; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < NET_NSCALE) partial[blockIdx.x * NET_NSCALE + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// Inverse scales of a record from the block sums, in block order: 1/RMS, or
+// 1 where an RMS is 0 or its earlier records do not exist (nprev < 1 for
+// d1, < 2 for d2)
+__global__ void k_record_scales(const double *__restrict__ partial, int nblk, long vol, int nprev,
+                                float *__restrict__ rs)
+{
+    const int q = threadIdx.x;
+    if (q >= NET_NSCALE) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += partial[b * NET_NSCALE + q];
+    const double rms = sqrt(s / (double)vol);
+    rs[q] = (rms > 0.0 && nprev >= q / 5) ? (float)(1.0 / rms) : 1.0f;
+}
+
+// Initial guess U_ = U + dq (+ (dq - dq_prev) + s2 Y when Y is given), with
+// dq = U - Uq in FP64 and dq_prev the ring's previous record (FP32): the
+// linear extrapolation, or the quadratic one plus the network's correction
+// of the second difference (Y [cell][16], first 5 channels; rs = the
+// latest record's inverse scales).  Y without rs: quadratic only (Y unused).
+__global__ void k_guess(Geo g, C5 U, C5 Uq, const float *__restrict__ prev, const float *__restrict__ Y,
+                        const float *__restrict__ rs, F5 U_)
+{
+    const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = 3 + blockIdx.z;
+    if (k > g.NZ || j > g.NY) return;
+    const long c = ix(g, i, j, k);
+    const long v = ((long)(i - 3) * g.ny + (j - 2)) * g.nz + (k - 2);
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        const double u = U.v[q][c];
+        const double dq = u - Uq.v[q][c];
+        double next = dq;
+        if (prev) {
+            next += dq - (double)prev[v * 5 + q];
+            if (rs) next += (double)Y[v * 16 + q] / (double)rs[10 + q];
+        }
+        U_.v[q][c] = u + next;
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+static unsigned blocks_of(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+int net_conv(int cin_pad, int cout_pad, bool act, const void *X, const void *W, const float *bias, float *Z, void *A,
+             const NetDims &m, int cout, int per, hipStream_t s)
+{
+    if (m.W % 16) return -1;
+    const long nt = (long)m.B * m.D * m.H * (m.W / 16);
+    const dim3 gr(blocks_of(nt, 4)), bl(256);
+    const u16 *x = static_cast<const u16 *>(X), *w = static_cast<const u16 *>(W);
+    u16 *a = static_cast<u16 *>(A);
+    if (cin_pad == 16 && cout_pad == 32 && act)
+        hipLaunchKernelGGL((k_conv<16, 2, true>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nt, per);
+    else if (cin_pad == 32 && cout_pad == 32 && act)
+        hipLaunchKernelGGL((k_conv<32, 2, true>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nt, per);
+    else if (cin_pad == 32 && cout_pad == 16 && !act)
+        hipLaunchKernelGGL((k_conv<32, 1, false>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nt, per);
+    else if (cin_pad == 16 && cout_pad == 32 && !act)
+        hipLaunchKernelGGL((k_conv<16, 2, false>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nt, per);
+    else if (cin_pad == 32 && cout_pad == 32 && !act)
+        hipLaunchKernelGGL((k_conv<32, 2, false>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nt, per);
+    else
+        return -1;
+    return 0;
+}
+
+void net_gelu_back(const float *dA, const float *Z, void *dZ, float *dZf, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gelu_back, dim3(blocks_of(n, 256)), dim3(256), 0, s, dA, Z, static_cast<u16 *>(dZ), dZf, n);
+}
+
+void net_to_bf16(const float *x, void *y, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks_of(n, 256)), dim3(256), 0, s, x, static_cast<u16 *>(y), n);
+}
+
+void net_transpose_w(const void *W, void *WT, int cin_pad, int cout_pad, hipStream_t s)
+{
+    const long n = (long)cin_pad * 27 * cout_pad;
+    hipLaunchKernelGGL(k_transpose_w, dim3(blocks_of(n, 256)), dim3(256), 0, s, static_cast<const u16 *>(W),
+                       static_cast<u16 *>(WT), cin_pad, cout_pad);
+}
+
+int net_wgrad(int cin_pad, int cout_pad, const void *dZ, const void *X, const NetDims &m, int nblk, float *part,
+              int cout, int cin, float *dW, hipStream_t s)
+{
+    const long nvox = (long)m.B * m.D * m.H * m.W;
+    const long per = (nvox + nblk - 1) / nblk;
+    const dim3 gr(nblk, 27), bl(256);
+    const u16 *dz = static_cast<const u16 *>(dZ), *x = static_cast<const u16 *>(X);
+    if (cin_pad == 16 && cout_pad == 32)
+        hipLaunchKernelGGL((k_wgrad<16, 32>), gr, bl, 0, s, dz, x, m, nvox, per, part);
+    else if (cin_pad == 32 && cout_pad == 32)
+        hipLaunchKernelGGL((k_wgrad<32, 32>), gr, bl, 0, s, dz, x, m, nvox, per, part);
+    else if (cin_pad == 32 && cout_pad == 16)
+        hipLaunchKernelGGL((k_wgrad<32, 16>), gr, bl, 0, s, dz, x, m, nvox, per, part);
+    else
+        return -1;
+    const long n = (long)cout_pad * 27 * cin_pad;
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks_of(n, 256)), dim3(256), 0, s, part, nblk, cout_pad, cin_pad, cout,
+                       cin, dW);
+    return 0;
+}
+
+void net_bias_grad(const float *dZ, long nvox, int cpad, int cout, float *db, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bias_grad, dim3(cpad), dim3(256), 0, s, dZ, nvox, cpad, cout, db);
+}
+
+void net_mse(const float *Y, const float *T, const unsigned char *mask, long nvox, float inv, float *dY,
+             double *partial, int nblk, double *loss, double loss_scale, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mse, dim3(nblk), dim3(256), 0, s, Y, T, mask, nvox, inv, dY, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, partial, nblk, loss_scale, loss);
+}
+
+void net_adam(float *w, float *m1, float *m2, const float *g, void *wb, long n, float lr, float b1, float b2,
+              float eps, float c1, float c2, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adam, dim3(blocks_of(n, 256)), dim3(256), 0, s, w, m1, m2, g, static_cast<u16 *>(wb), n,
+                       lr, b1, b2, eps, c1, c2);
+}
+
+static dim3 interior_grid(const Geo &g) { return dim3((g.nz + 63) / 64, (g.ny + 3) / 4, g.nx); }
+
+void net_store(const Geo &g, C5 U, C5 Uq, float *S, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_store, interior_grid(g), dim3(64, 4), 0, s, g, U, Uq, S);
+}
+
+void net_gather(const float *S, const float *rs, int cap, long vol, int nx, int ny, int nz, const int *org,
+                const NetDims &m, int halo, void *X, float *T, unsigned char *mask, hipStream_t s)
+{
+    const long nv = (long)m.B * m.D * m.H * m.W;
+    hipLaunchKernelGGL(k_gather, dim3(blocks_of(nv, 256)), dim3(256), 0, s, S, rs, cap, vol, nx, ny, nz, org, m,
+                       halo, static_cast<u16 *>(X), T, mask);
+}
+
+void net_pack_input(const float *x, void *X, long nv, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pack_input, dim3(blocks_of(nv, 256)), dim3(256), 0, s, x, static_cast<u16 *>(X), nv);
+}
+
+// rs[NET_NSCALE] = the inverse scales of record a given the two before it
+// (b, c; null where they do not exist)
+void net_record_scales(const float *Sa, const float *Sb, const float *Sc, long vol, double *partial, int nblk,
+                       float *rs, hipStream_t s)
+{
+    const int nprev = Sb ? (Sc ? 2 : 1) : 0;
+    hipLaunchKernelGGL(k_record_sums, dim3(nblk), dim3(256), 0, s, Sa, Sb ? Sb : Sa, Sc ? Sc : (Sb ? Sb : Sa), vol,
+                       partial);
+    hipLaunchKernelGGL(k_record_scales, dim3(1), dim3(64), 0, s, partial, nblk, vol, nprev, rs);
+}
+
+void net_guess(const Geo &g, C5 U, C5 Uq, const float *prev, const float *Y, const float *rs, F5 U_, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_guess, interior_grid(g), dim3(64, 4), 0, s, g, U, Uq, prev, Y, rs, U_);
+}
+
+}  // namespace cgli

@@ -54,4 +54,45 @@ void launch_profiles(const Geo &g, const Phys &p, C5 U, const double *J, double 
     hipLaunchKernelGGL(k_profiles, dim3(g.NY - 1), dim3(256), 0, s, g, p.K, p.R, U, J, out);
 }
 
+// Per-block sums of f_q^2 over the interior cells (one block per (i, j)
+// row of k), then a one-thread sum over blocks in index order.
+__global__ void __launch_bounds__(256) k_sumsq5_rows(Geo g, C5 f, double *__restrict__ partial)
+{
+    __shared__ double sh[5][256];
+    const int j = 2 + blockIdx.x % (g.NY - 1), i = 3 + blockIdx.x / (g.NY - 1);
+    double s[5] = {0, 0, 0, 0, 0};
+    for (int k = 2 + threadIdx.x; k <= g.NZ; k += 256) {
+        const long c = ix(g, i, j, k);
+#pragma unroll
+        for (int q = 0; q < 5; q++) s[q] += f.v[q][c] * f.v[q][c];
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) sh[q][threadIdx.x] = s[q];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+#pragma unroll
+            for (int q = 0; q < 5; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) partial[(long)blockIdx.x * 5 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_sumsq5_final(const double *__restrict__ partial, long nblk, double *__restrict__ out)
+{
+    const int q = threadIdx.x;
+    if (q >= 5) return;
+    double s = 0.0;
+    for (long b = 0; b < nblk; b++) s += partial[b * 5 + q];
+    out[q] = s;
+}
+
+// partials: 5 x nx x ny doubles
+void launch_sumsq5(const Geo &g, C5 f, double *partials, double *out_dev, hipStream_t s)
+{
+    const long nblk = (long)g.nx * (g.NY - 1);
+    hipLaunchKernelGGL(k_sumsq5_rows, dim3((unsigned)nblk), dim3(256), 0, s, g, f, partials);
+    hipLaunchKernelGGL(k_sumsq5_final, dim3(1), dim3(64), 0, s, partials, nblk, out_dev);
+}
+
 }  // namespace cgli

@@ -88,6 +88,26 @@ def lib():
         L.cgli_timing_enable.argtypes = [vp, C.c_int]
         L.cgli_timing_reset.argtypes = [vp]
         L.cgli_timing_get.argtypes = [vp, C.c_int, dp, C.POINTER(C.c_long)]
+        L.cgli_channel_set_rule.argtypes = [vp, C.c_int]
+        L.cgli_channel_rule_state.argtypes = [vp, dp, C.c_int]
+        L.cgli_channel_set_guess.argtypes = [vp, C.c_int]
+        L.cgli_channel_attach_net.argtypes = [vp, vp]
+        L.cgli_channel_guess_info.argtypes = [vp, C.POINTER(C.c_int), dp, dp]
+        fp = C.POINTER(C.c_float)
+        up = C.POINTER(C.c_ubyte)
+        L.cgli_net_create.argtypes = [C.c_int, C.c_ulonglong, C.POINTER(vp)]
+        L.cgli_net_destroy.argtypes = [vp]
+        L.cgli_net_params.argtypes = [vp, fp, C.c_int]
+        L.cgli_net_eval.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, fp, fp, up, fp, dp, fp]
+        L.cgli_net_adam.argtypes = [vp, C.c_float]
+        L.cgli_net_ring.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.cgli_net_push.argtypes = [vp, fp]
+        L.cgli_net_records.argtypes = [vp, C.POINTER(C.c_long)]
+        L.cgli_net_rewind.argtypes = [vp, C.c_long]
+        L.cgli_net_record_scales.argtypes = [vp, C.c_long, fp]
+        L.cgli_net_train.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_ulonglong, fp]
+        L.cgli_net_predict.argtypes = [vp, fp, fp]
+        L.cgli_net_timing.argtypes = [vp, dp, C.POINTER(C.c_long), dp, C.POINTER(C.c_long)]
         _lib = L
     return _lib
 
@@ -247,6 +267,34 @@ class Context:
                "channel_step")
         return it.value, np.array(list(e1)), np.array(list(en))
 
+    # -- stopping rule and initial guess of channel_step
+    def set_rule(self, rule):
+        """0: per-step drop er/er(1); 1: running maximum er/E (main.cpp:287-304)."""
+        _check(lib().cgli_channel_set_rule(self.h, rule), "channel_set_rule")
+
+    def rule_state(self, E=None):
+        """Return the running maxima E[5]; overwrite them first when E is given."""
+        buf = (C.c_double * 5)(*(E if E is not None else [0.0] * 5))
+        _check(lib().cgli_channel_rule_state(self.h, buf, 1 if E is not None else 0), "channel_rule_state")
+        return np.array(list(buf))
+
+    GUESSES = {"previous": 0, "extrapolate": 1, "network": 2, "quadratic": 3}
+
+    def set_guess(self, guess):
+        """'previous' (U^n), 'extrapolate' (2U^n - U^{n-1}), 'quadratic' or 'network'."""
+        _check(lib().cgli_channel_set_guess(self.h, self.GUESSES.get(guess, guess)), "channel_set_guess")
+
+    def attach_net(self, net):
+        _check(lib().cgli_channel_attach_net(self.h, net.h if net is not None else None), "channel_attach_net")
+        self._net = net   # keep it alive while attached
+
+    def guess_info(self):
+        used = C.c_int()
+        rn = (C.c_double * 5)()
+        re = (C.c_double * 5)()
+        _check(lib().cgli_channel_guess_info(self.h, C.byref(used), rn, re), "channel_guess_info")
+        return bool(used.value), np.array(list(rn)), np.array(list(re))
+
     # -- timing (HIP events around each launch class)
     TIMING_CLASSES = ["flux_x", "flux_y", "flux_z", "viscous", "rhs_diag", "lusgs_fwd", "lusgs_bwd",
                       "update_norm", "split_coef", "filter"]
@@ -270,3 +318,104 @@ class Context:
             _check(lib().cgli_timing_get(self.h, i, C.byref(t), C.byref(n)), "timing_get")
             out[name] = (t.value, n.value)
         return out
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Net:
+    """The learned initial guess's network (cgli_net)."""
+
+    NPARAM = 83056
+
+    def __init__(self, seed=1, device=0):
+        h = C.c_void_p()
+        _check(lib().cgli_net_create(device, seed, C.byref(h)), "cgli_net_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cgli_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self, new=None):
+        """FP32 master parameters (padded layout); set them first when new is given."""
+        if new is not None:
+            a = np.ascontiguousarray(new, dtype=np.float32)
+            assert a.size == self.NPARAM
+            _check(lib().cgli_net_params(self.h, _fp(a), 1), "net_params")
+            return a.copy()
+        a = np.zeros(self.NPARAM, np.float32)
+        _check(lib().cgli_net_params(self.h, _fp(a), 0), "net_params")
+        return a
+
+    def eval(self, x, t=None, mask=None, periodic=False, want_grad=True):
+        """x[B,D,H,W,15] -> y[B,D,H,W,5]; with t and mask also (loss, grad)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        B, D, H, W, c = x.shape
+        assert c == 15
+        y = np.zeros((B, D, H, W, 5), np.float32)
+        loss = C.c_double()
+        g = np.zeros(self.NPARAM, np.float32) if (t is not None and want_grad) else None
+        tt = np.ascontiguousarray(t, dtype=np.float32) if t is not None else None
+        mm = np.ascontiguousarray(mask, dtype=np.uint8) if mask is not None else None
+        _check(lib().cgli_net_eval(self.h, B, D, H, W, int(periodic), _fp(x),
+                                   _fp(tt) if tt is not None else None,
+                                   mm.ctypes.data_as(C.POINTER(C.c_ubyte)) if mm is not None else None,
+                                   _fp(y), C.byref(loss) if t is not None else None,
+                                   _fp(g) if g is not None else None), "net_eval")
+        if t is None:
+            return y
+        return y, loss.value, g
+
+    def adam(self, lr=1e-3):
+        _check(lib().cgli_net_adam(self.h, lr), "net_adam")
+
+    def ring(self, capacity, nx, ny, nz):
+        _check(lib().cgli_net_ring(self.h, capacity, nx, ny, nz), "net_ring")
+        self.grid = (nx, ny, nz)
+
+    def push(self, dq):
+        a = np.ascontiguousarray(dq, dtype=np.float32)
+        _check(lib().cgli_net_push(self.h, _fp(a)), "net_push")
+
+    def rewind(self, nrec):
+        _check(lib().cgli_net_rewind(self.h, nrec), "net_rewind")
+
+    def records(self):
+        n = C.c_long()
+        _check(lib().cgli_net_records(self.h, C.byref(n)), "net_records")
+        return n.value
+
+    def record_scales(self, rec):
+        """inverse scales of record `rec`: (1/s1, 1/sd, 1/s2), five each."""
+        a = np.zeros(15, np.float32)
+        _check(lib().cgli_net_record_scales(self.h, rec, _fp(a)), "net_record_scales")
+        return a[:5], a[5:10], a[10:]
+
+    def train(self, steps, batch=8, crop=(32, 32, 32), lr=1e-3, seed=1):
+        loss = np.zeros(steps, np.float32)
+        _check(lib().cgli_net_train(self.h, steps, batch, crop[0], crop[1], crop[2], lr, seed, _fp(loss)),
+               "net_train")
+        return loss
+
+    def predict(self):
+        """(x, y) of a whole-grid pass on the latest three records."""
+        nx, ny, nz = self.grid
+        x = np.zeros((nx, ny, nz, 15), np.float32)
+        y = np.zeros((nx, ny, nz, 5), np.float32)
+        _check(lib().cgli_net_predict(self.h, _fp(x), _fp(y)), "net_predict")
+        return x, y
+
+    def timing(self):
+        pm, tm = C.c_double(), C.c_double()
+        pn, tn = C.c_long(), C.c_long()
+        _check(lib().cgli_net_timing(self.h, C.byref(pm), C.byref(pn), C.byref(tm), C.byref(tn)), "net_timing")
+        return {"predict_ms": pm.value, "predicts": pn.value, "train_ms": tm.value, "train_steps": tn.value}

@@ -30,6 +30,7 @@ extern "C" {
 #define CGLI_E_TIMEOUT (-7)    /* a bounded device wait gave up */
 
 typedef struct cgli_ctx cgli_ctx;
+typedef struct cgli_net cgli_net;
 
 /* Runtime form of the reference's compile-time configuration
  * (Resolution.h, prm.h, Viscous_terms.h, Mpi_division.h). */
@@ -165,9 +166,12 @@ int cgli_channel_config(const cgli_channel *ch, int nx, int ny, int nz, cgli_con
 int cgli_channel_init(cgli_ctx *ctx, const cgli_channel *ch, double *deltaT);
 /* wall, z and x ghost fills of U_ */
 int cgli_channel_ghosts(cgli_ctx *ctx);
-/* one physical step: inner iterations (ghosts, fluxes, viscous terms,
- * dual-time LU-SGS) until max_v er_v / er_v(first) <= tol or max_inner,
- * then the time-level shift.  *inner receives the iteration count. */
+/* one physical step: the initial guess (cgli_channel_set_guess), inner
+ * iterations (ghosts, fluxes, viscous terms, dual-time LU-SGS) until the
+ * stopping rule (cgli_channel_set_rule; default max_v er_v / er_v(first)
+ * <= tol) holds or max_inner, then the time-level shift and, with a network
+ * attached, the record of the step's increment.  *inner receives the
+ * iteration count. */
 int cgli_channel_step(cgli_ctx *ctx, double deltaT, double tol, int max_inner, int sweeps, int *inner,
                       double er_first[5], double er_last[5]);
 
@@ -176,6 +180,88 @@ int cgli_channel_step(cgli_ctx *ctx, double deltaT, double tol, int max_inner, i
  * in FP64 down to tol; *inner_fp32 receives how many of the last step's
  * iterations used FP32 residuals. */
 int cgli_channel_last_step(cgli_ctx *ctx, int *inner_fp32);
+
+/* ---- inner-iteration rule and initial guess of the channel step ------ */
+/* No counterpart in the reference's file set; the reference stops on
+ * max_v er_v / E_v < 1e-4 with E_v the running maximum of er_v over the
+ * whole run (main.cpp:287-304) and starts each step from U_ = U^n
+ * (main.cpp:378-382).
+ * rule 0: max_v er_v / er_v(first iteration of the step) <= tol (default);
+ * rule 1: max_v er_v / E_v <= tol, E_v = running maximum of er_v over every
+ *         inner iteration since the rule was set. */
+int cgli_channel_set_rule(cgli_ctx *ctx, int rule);
+/* read (set = 0) or overwrite (set = 1) the running maxima E[5] */
+int cgli_channel_rule_state(cgli_ctx *ctx, double E[5], int set);
+/* guess 0: U_ = U^n; 1: U_ = U^n + dq^n (linear extrapolation), with
+ * dq^n = U^n - U^{n-1}; 3: U_ = U^n + 2 dq^n - dq^{n-1} (quadratic
+ * extrapolation, dq^{n-1} from the attached network's ring); 2: the
+ * quadratic extrapolation plus s2 y, y the attached network's output on its
+ * latest records and s2 their scale (cgli_net_record_scales), unless the
+ * residual at that guess is larger (any variable) than at guess 1, which is
+ * then used.  Modes 2 and 3 fall back to guess 1 while the network is not
+ * ready (trained, three records of this grid) or the ring lacks dq^{n-1}. */
+int cgli_channel_set_guess(cgli_ctx *ctx, int guess);
+/* record every step's dq = U^{n+1} - U^n into the network's ring (net may
+ * be NULL to detach) */
+int cgli_channel_attach_net(cgli_ctx *ctx, cgli_net *net);
+/* last step: *used_net = 1 if the network's guess was taken; residual sums
+ * of squares per variable at the network's and the extrapolated guess */
+int cgli_channel_guess_info(cgli_ctx *ctx, int *used_net, double r_net[5], double r_ext[5]);
+
+/* ---- learned initial guess: 3-D CNN (no counterpart in the reference) --
+ * 15 -> 32 -> 32 -> 32 -> 5 channels, 3x3x3 kernels, GELU after the first
+ * three, BF16 weights and activations, FP32 accumulation and pre-
+ * activations, FP32 master weights trained with Adam.  Parameters, FP32, in
+ * the padded layout: per layer W[Cout_pad][27][Cin_pad] then b[Cout_pad],
+ * (Cin, Cout) = (15, 32) (32, 32) (32, 32) (32, 5) padded to (16, 32) (32,
+ * 32) (32, 32) (32, 16); tap = (dd+1)*9 + (dh+1)*3 + (dw+1) for offsets dd,
+ * dh, dw along D, H, W.  Padded entries are zero and stay zero. */
+#define CGLI_NET_NPARAM 83056
+int cgli_net_create(int device, unsigned long long seed, cgli_net **out);
+int cgli_net_destroy(cgli_net *net);
+/* FP32 master parameters: copy out (set = 0) or in (set = 1) */
+int cgli_net_params(cgli_net *net, float *params, int set);
+/* One pass over host arrays, channel-last: x[B][D][H][W][15] (rounded to
+ * BF16 on the device), W a multiple of 16; periodic != 0 wraps D and W
+ * (else zero padding).  y[..][5] (may be NULL) receives the output.  With
+ * t[..][5] and mask[B*D*H*W] (1 = counted), *loss = mean over counted
+ * voxels and the 5 channels of (y - t)^2 and grad[CGLI_NET_NPARAM] (may be
+ * NULL) its gradient; the gradient is also kept for cgli_net_adam. */
+int cgli_net_eval(cgli_net *net, int B, int D, int H, int W, int periodic, const float *x, const float *t,
+                  const unsigned char *mask, float *y, double *loss, float *grad);
+/* one Adam step (beta 0.9, 0.999, eps 1e-8) with the kept gradient */
+int cgli_net_adam(cgli_net *net, float lr);
+/* snapshot ring of `capacity` records of an nx*ny*nz grid (FP32 dq per
+ * interior cell, [cell][5], cell = (i*ny + j)*nz + k); clears it */
+int cgli_net_ring(cgli_net *net, int capacity, int nx, int ny, int nz);
+/* append a record from the host */
+int cgli_net_push(cgli_net *net, const float *dq);
+/* forget the records appended after the first nrec (at most capacity - 3
+ * of them), to replay steps from an earlier state */
+int cgli_net_rewind(cgli_net *net, long nrec);
+/* records appended since cgli_net_ring */
+int cgli_net_records(cgli_net *net, long *n);
+/* inverse scales of a record held in the ring, as the kernels use them:
+ * inv[0..4] = 1 / field RMS of dq_t, inv[5..9] = 1 / field RMS of d1_t =
+ * dq_t - dq_{t-1}, inv[10..14] = 1 / field RMS of d2_t = d1_t - d1_{t-1}
+ * (1 where an RMS is 0 or the earlier records are missing); computed on
+ * the device when the record is appended */
+int cgli_net_record_scales(cgli_net *net, long rec, float inv[15]);
+/* Train with Adam on `steps` batches of `batch` random crops (D, H, W) =
+ * (crop_d, crop_h, crop_w) of the ring.  For record t with scales s1, sd,
+ * s2: input x0 = dq_t / s1, x1 = d1_t / sd, x2 = d2_t / s2, target
+ * d2_{t+1} / s2, so that a zero output is the quadratic extrapolation
+ * dq_{t+1} = dq_t + d1_t; crops wrap in x and z; the loss counts voxels
+ * whose 4-voxel neighbourhood lies inside the crop or beyond a wall.
+ * loss[steps] (may be NULL) receives each step's loss. */
+int cgli_net_train(cgli_net *net, int steps, int batch, int crop_d, int crop_h, int crop_w, float lr,
+                   unsigned long long seed, float *loss);
+/* whole-grid pass on the latest three records (periodic in x and z):
+ * x[cells][15] and y[cells][5] (either may be NULL) as the network sees
+ * and returns them (x after BF16 rounding) */
+int cgli_net_predict(cgli_net *net, float *x, float *y);
+/* device time (ms) and count: whole-grid predictions, training steps */
+int cgli_net_timing(cgli_net *net, double *predict_ms, long *n_predict, double *train_ms, long *n_train);
 
 /* ---- timing ---------------------------------------------------------- */
 /* Average device time (ms) of the last `n` launches of a kernel class

@@ -1,0 +1,201 @@
+"""The learned initial guess on the GPU against the network oracle.
+
+The oracle (oracle/oracle_cnn.py, pinned to torch and finite differences on
+the CPU) is run in its emulating mode: the same BF16 rounding points as the
+gfx950 kernels, sums in FP64.  What remains between the two is FP32
+accumulation order inside the matrix-core sums and the occasional BF16
+rounding that flips with it, hence these bounds:
+  forward output      max |y - y_oracle| <= 2e-3 max |y_oracle|
+  loss                relative 1e-4
+  gradient per layer  || g - g_oracle || <= 2e-3 || g_oracle ||  (W and b)
+  Adam step           |w - w_oracle| <= 1e-3 lr
+  record scales       relative 1e-6 (FP64 sums in another order)
+  features            bit-identical given the record's scales
+"""
+import numpy as np
+import pytest
+
+from cgli_fu_flow_amd import native as nv
+from oracle import oracle_cnn as oc
+
+pytestmark = pytest.mark.gpu
+
+
+def random_params(seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    layers = []
+    for cin, _, cout, _, _, _ in oc.LAYERS:
+        layers.append((rng.standard_normal((cout, 27, cin)) * scale * np.sqrt(2.0 / (27 * cin)),
+                       rng.standard_normal(cout) * 0.1))
+    return oc.pack(layers).astype(np.float32)
+
+
+def layer_errors(g, ref):
+    out = []
+    for (gW, gb), (rW, rb) in zip(oc.unpack(g), oc.unpack(ref)):
+        out.append((np.linalg.norm(gW - rW) / np.linalg.norm(rW), np.linalg.norm(gb - rb) / np.linalg.norm(rb)))
+    return out
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_forward_matches_oracle(periodic):
+    p = random_params(1)
+    x = np.random.default_rng(2).standard_normal((2, 5, 6, 16, 15)).astype(np.float32)
+    net = nv.Net(seed=3)
+    net.params(p)
+    y = net.eval(x, periodic=periodic)
+    ref, _ = oc.forward(p.astype(np.float64), x, periodic=periodic, emulate=True)
+    err = np.max(np.abs(y - ref)) / np.max(np.abs(ref))
+    print("forward periodic=%d max rel %.3e" % (periodic, err))
+    assert err <= 2e-3
+    net.close()
+
+
+def test_gradient_matches_oracle():
+    p = random_params(4)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((2, 6, 8, 16, 15)).astype(np.float32)
+    t = (rng.standard_normal((2, 6, 8, 16, 5)) * 0.5).astype(np.float32)
+    mask = (rng.random((2, 6, 8, 16)) < 0.6).astype(np.uint8)
+    net = nv.Net(seed=6)
+    net.params(p)
+    y, loss, g = net.eval(x, t, mask)
+    rl, ry, rg = oc.loss_grad(p.astype(np.float64), x, t, mask, emulate=True)
+    print("loss %.8e oracle %.8e" % (loss, rl))
+    assert abs(loss - rl) <= 1e-4 * rl
+    errs = layer_errors(g.astype(np.float64), rg)
+    print("gradient errors per layer (W, b):", ["%.2e %.2e" % e for e in errs])
+    assert all(ew <= 2e-3 and eb <= 2e-3 for ew, eb in errs)
+    # padded entries stay exactly zero
+    full = oc.pack(oc.unpack(np.ones(oc.NPARAM)))
+    assert np.all(g[full == 0] == 0)
+    # the BF16 network is close to the FP64 one (not a parity bound)
+    _, _, g64 = oc.loss_grad(p.astype(np.float64), x, t, mask)
+    print("vs FP64 network:", ["%.2e %.2e" % e for e in layer_errors(g.astype(np.float64), g64)])
+    net.close()
+
+
+def test_adam_matches_oracle():
+    p = random_params(7)
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((1, 4, 6, 16, 15)).astype(np.float32)
+    t = (rng.standard_normal((1, 4, 6, 16, 5)) * 0.5).astype(np.float32)
+    mask = np.ones((1, 4, 6, 16), np.uint8)
+    net = nv.Net(seed=9)
+    net.params(p)
+    w = p.astype(np.float64)
+    m1 = np.zeros_like(w)
+    m2 = np.zeros_like(w)
+    lr = 1e-3
+    for step in range(1, 4):
+        _, _, g = net.eval(x, t, mask)
+        net.adam(lr)
+        w, m1, m2 = oc.adam(w, m1, m2, g.astype(np.float64), step, lr)
+        got = net.params()
+        err = np.max(np.abs(got - w))
+        print("adam step %d max |dw| %.3e" % (step, err))
+        assert err <= 1e-3 * lr
+        w = got.astype(np.float64)   # continue from the device's rounding
+    net.close()
+
+
+def test_features_and_whole_grid_prediction():
+    nx, ny, nz = 6, 8, 16
+    rng = np.random.default_rng(10)
+    net = nv.Net(seed=11)
+    net.params(random_params(12))
+    net.ring(4, nx, ny, nz)
+    amp = np.array([1e-3, 2e-3, 5e-4, 1e-3, 3e-3])
+    recs = [(rng.standard_normal((nx, ny, nz, 5)) * amp).astype(np.float32) for _ in range(5)]
+    for r in recs:
+        net.push(r)   # 5 records into a ring of 4: the oldest is overwritten
+    for t in range(2, 5):
+        got = net.record_scales(t)
+        ref = oc.record_scales(recs[t], recs[t - 1], recs[t - 2])
+        assert all(np.allclose(a, b, rtol=1e-6) for a, b in zip(got, ref))
+    x, y = net.predict()
+    ref_x = oc.features(recs[4], recs[3], recs[2], *net.record_scales(4))
+    assert np.array_equal(x.view(np.uint32), ref_x.view(np.uint32))
+    ref_y, _ = oc.forward(net.params().astype(np.float64), x[None], periodic=True, emulate=True)
+    err = np.max(np.abs(y - ref_y[0])) / np.max(np.abs(ref_y))
+    print("whole-grid prediction max rel %.3e" % err)
+    assert err <= 2e-3
+    net.close()
+
+
+def test_training_reduces_the_loss():
+    # records of a field whose increments are quadratic in time: the target
+    # d2_{t+1} / s2 equals the input channels 10..14
+    nx, ny, nz = 16, 12, 32
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1).astype(np.float64)
+    base = np.stack([np.sin(2 * np.pi * (g[..., 0] / nx + q / 5.0)) * np.cos(np.pi * g[..., 1] / ny) *
+                     np.sin(2 * np.pi * g[..., 2] / nz + q) for q in range(5)], -1)
+    slope = np.roll(base, 3, axis=0) * 0.3
+    curv = np.roll(base, 5, axis=2) * 0.2
+    net = nv.Net(seed=13)
+    net.ring(24, nx, ny, nz)
+    for t in range(24):
+        tau = t / 24.0
+        net.push((1e-3 * (base + slope * tau + curv * tau * tau)).astype(np.float32))
+    loss = net.train(150, batch=2, crop=(12, 12, 16), lr=1e-3, seed=14)
+    first, last = float(np.mean(loss[:10])), float(np.mean(loss[-10:]))
+    print("training loss %.4e -> %.4e" % (first, last))
+    assert np.all(np.isfinite(loss)) and last < 0.2 * first
+    net.close()
+
+
+def interior(a, nx, ny, nz):
+    return a[3:nx + 3, 2:ny + 2, 2:nz + 2]
+
+
+def test_network_guess_with_zero_output_is_the_quadratic_extrapolation():
+    """A network whose output layer is zero predicts d2_{n+1} = 0, the
+    quadratic extrapolation.  Each step of the network run (guess, residual
+    check against the linear extrapolation, first iteration on the check's
+    right-hand side when the network's guess is taken) is bit-identical to a
+    step started from the guess it chose, and the ring holds exactly the
+    steps' increments."""
+    nx, ny, nz = 12, 16, 16
+    nsteps = 7
+    ch = nv.channel(180.0, beta=2.2, skew=0.05)
+    cfg = nv.channel_config(ch, nx, ny, nz)
+    p = random_params(15)
+    cin, cinp, cout, coutp, wo, bo = oc.LAYERS[3]
+    p[wo:] = 0.0
+    runs = {}
+    for mode in ("network", "polynomial"):
+        net = nv.Net(seed=16)
+        net.params(p)
+        net.ring(8, nx, ny, nz)
+        with nv.Context(cfg) as ctx:
+            dt = ctx.channel_init(ch)
+            ctx.attach_net(net)
+            dqs, used = [], []
+            for step in range(nsteps):
+                if mode == "network":
+                    ctx.set_guess("network")
+                else:
+                    ctx.set_guess("quadratic" if runs["network"][2][step] else "extrapolate")
+                ctx.channel_step(dt, tol=1e-8, max_inner=40)
+                u = ctx.get_many(nv.U, 5)
+                uq = ctx.get_many(nv.UQ, 5)
+                dqs.append(np.stack([interior(u[q] - uq[q], nx, ny, nz) for q in range(5)], -1).astype(np.float32))
+                used.append(ctx.guess_info()[0])
+            runs[mode] = (ctx.get_many(nv.U, 5), dqs, used, net)
+            ctx.attach_net(None)
+    un, dqs, used, net = runs["network"]
+    up = runs["polynomial"][0]
+    print("network guess taken per step:", used)
+    assert used[:3] == [False, False, False] and any(used[3:])
+    for q in range(5):
+        assert np.array_equal(un[q], up[q])
+    x, _ = net.predict()
+    last = nsteps - 1
+    for t in range(last - 2, last + 1):
+        got = net.record_scales(t)
+        ref = oc.record_scales(dqs[t], dqs[t - 1], dqs[t - 2])
+        assert all(np.allclose(a, b, rtol=1e-6) for a, b in zip(got, ref))
+    ref_x = oc.features(dqs[last], dqs[last - 1], dqs[last - 2], *net.record_scales(last))
+    assert np.array_equal(x.view(np.uint32), ref_x.view(np.uint32))
+    for m in runs.values():
+        m[3].close()
