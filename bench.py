@@ -7,6 +7,16 @@ Re_tau=180, Mach 0.5, acoustic CFL 0.02, laminar Poiseuille + seeded +-10%
 noise.  Every physical step iterates the dual-time LU-SGS inner loop until
 max_v er_v/er_v(first) <= 1e-6 (or --max-inner), after one evaluation of
 the filter sensor.  State is resident in HBM before the timed region starts.
+`value` is this path's physical steps/s.
+
+`variants` then compares, from one saved state, plain FP64 (each step
+starting from U^n, as the reference does) with FP32 residuals plus the
+quadratic-extrapolation and learned initial guesses, under the reference's
+stopping rule (max_v er_v / E_v <= 1e-6 with E_v the running maximum of er_v
+since the rule was set) and under the per-step rule above.  The network is
+first trained on the GPU (Adam, batches of random crops) on increments the
+solver records while stepping; that training and its snapshot run are set-up,
+outside the timed regions, and their costs are reported.
 
 Prints ONE JSON line.  `roofline` is the dominant kernel class measured
 with HIP events on the library's stream over the timed region;
@@ -55,6 +65,12 @@ def parse():
     ap.add_argument("--no-filter", action="store_true", help="skip the per-step filter sensor")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="mixed: FP32 residual kernels until the FP32 floor, FP32 off-diagonal sweep")
+    ap.add_argument("--no-variants", action="store_true", help="skip the initial-guess / precision comparison")
+    ap.add_argument("--variant-steps", type=int, default=20, help="timed steps of each variant")
+    ap.add_argument("--gen", type=int, default=200, help="recorded steps the network trains on")
+    ap.add_argument("--train", type=int, default=1000, help="Adam steps")
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--crop", type=int, default=32)
     return ap.parse_args()
 
 
@@ -120,6 +136,107 @@ def cpu_baseline(ctx, nv, cfg, dt, inner_per_step, budget_s, with_filter):
                       "on %d OpenMP threads, %.3f s each, scaled by the GPU run's %.2f inner iterations per "
                       "physical step, plus %.3f s for one filter evaluation"
                       % (iters, cfg.nx, cfg.ny, cfg.nz, threads, t_inner, inner_per_step, t_filter)}
+
+
+# forward flop per voxel of the network: 2 x 27 x (15x32 + 32x32 + 32x32 + 32x5)
+NET_FLOP_PER_VOXEL = 2 * 27 * (15 * 32 + 32 * 32 + 32 * 32 + 32 * 5)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def state_of(ctx, nv):
+    return [ctx.get_many(f, 5) for f in (nv.U_, nv.U, nv.UQ)]
+
+
+def restore(ctx, nv, state, E):
+    for f, arrs in zip((nv.U_, nv.U, nv.UQ), state):
+        ctx.set_many(f, arrs)
+    ctx.rule_state(E)
+
+
+def timed_steps(ctx, dt, a, steps, warm, with_filter, label):
+    for s in range(warm):
+        if with_filter:
+            ctx.filter()
+        ctx.channel_step(dt, a.tol, a.max_inner)
+    ctx.sync()
+    t0 = time.perf_counter()
+    inner = []
+    for s in range(steps):
+        if with_filter:
+            ctx.filter()
+        it, _, _ = ctx.channel_step(dt, a.tol, a.max_inner)
+        inner.append(it)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    print("%s: %d steps, %.2f inner/step, %.2f ms/step" % (label, steps, np.mean(inner), 1e3 * el / steps),
+          file=sys.stderr, flush=True)
+    return {"steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "inner_iterations_per_step": float(np.mean(inner))}
+
+
+def variants(a, nv, ch, ctx64, dt, ncell):
+    """plain FP64 vs FP32 residuals + quadratic / learned guesses from one state"""
+    cfg32 = nv.channel_config(ch, a.nx, a.ny, a.nz)
+    cfg32.precision = 1
+    ctx32 = nv.Context(cfg32, device=0)
+    ctx32.channel_init(ch)
+    restore(ctx32, nv, state_of(ctx64, nv), np.zeros(5))
+    net = nv.Net(seed=20261203)
+    net.ring(a.gen + 8, a.nx, a.ny, a.nz)
+    ctx32.attach_net(net)
+    ctx32.set_rule(1)
+    ctx32.set_guess("quadratic")
+    with_filter = not a.no_filter
+    t0 = time.perf_counter()
+    for s in range(a.gen):
+        if with_filter:
+            ctx32.filter()
+        ctx32.channel_step(dt, a.tol, a.max_inner)
+    ctx32.sync()
+    t_gen = time.perf_counter() - t0
+    print("recorded %d steps in %.1f s" % (a.gen, t_gen), file=sys.stderr, flush=True)
+    loss = net.train(a.train, batch=a.batch, crop=(a.crop, min(a.crop, a.ny), a.crop), lr=1e-3, seed=7)
+    tm = net.timing()
+    print("trained %d steps in %.2f s: loss %.3e -> %.3e" % (a.train, tm["train_ms"] / 1e3, float(np.mean(loss[:20])),
+                                                           float(np.mean(loss[-20:]))), file=sys.stderr, flush=True)
+    state = state_of(ctx32, nv)
+    E = ctx32.rule_state()
+    nrec = net.records()
+    out = {"rule_running_max": {}, "rule_per_step": {}}
+    runs = [("mixed_network", ctx32, "network", 1), ("mixed_quadratic", ctx32, "quadratic", 1),
+            ("mixed_extrapolate", ctx32, "extrapolate", 1), ("plain_fp64", ctx64, "previous", 1),
+            ("mixed_network", ctx32, "network", 0), ("plain_fp64", ctx64, "previous", 0)]
+    for name, ctx, guess, rule in runs:
+        ctx.set_rule(rule)
+        restore(ctx, nv, state, E)
+        net.rewind(nrec)
+        ctx.attach_net(net if ctx is ctx32 else None)
+        ctx.set_guess(guess)
+        p0 = net.timing()
+        r = timed_steps(ctx, dt, a, a.variant_steps, 1, with_filter, "%s (%s rule)" % (name, "running-max" if rule else "per-step"))
+        p1 = net.timing()
+        if guess == "network":
+            npred = p1["predicts"] - p0["predicts"]
+            if npred:
+                ms = (p1["predict_ms"] - p0["predict_ms"]) / npred
+                r["predict_ms"] = ms
+                r["predict_tflops"] = NET_FLOP_PER_VOXEL * ncell / (ms * 1e-3) / 1e12
+            used = ctx.guess_info()[0]
+            r["network_guess_taken_last_step"] = bool(used)
+        out["rule_running_max" if rule else "rule_per_step"][name] = r
+    rm = out["rule_running_max"]
+    out["speedup_mixed_network_vs_plain"] = rm["mixed_network"]["steps_per_s"] / rm["plain_fp64"]["steps_per_s"]
+    out["speedup_mixed_network_vs_extrapolate"] = (rm["mixed_network"]["steps_per_s"] /
+                                                   rm["mixed_extrapolate"]["steps_per_s"])
+    out["network"] = {"recorded_steps": a.gen, "record_seconds": t_gen, "train_steps": a.train,
+                      "train_ms_per_step": tm["train_ms"] / max(1, tm["train_steps"]),
+                      "batch": a.batch, "crop": a.crop, "loss_first20": float(np.mean(loss[:20])),
+                      "loss_last20": float(np.mean(loss[-20:])),
+                      "train_tflops": 3 * NET_FLOP_PER_VOXEL * a.batch * a.crop * min(a.crop, a.ny) * a.crop /
+                                      (tm["train_ms"] / max(1, tm["train_steps"]) * 1e-3) / 1e12}
+    ctx32.attach_net(None)
+    ctx32.close()
+    net.close()
+    return out
 
 
 def main():
@@ -197,6 +314,8 @@ def main():
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ctx, nv, cfg, dt, float(np.mean(inner)), a.cpu_seconds, not a.no_filter)
+    if not a.no_variants and a.precision == "fp64":
+        out["variants"] = variants(a, nv, ch, ctx, dt, ncell)
     ctx.close()
     print(json.dumps(out))
 

@@ -114,18 +114,21 @@ void net_to_bf16(const float *x, void *y, long n, hipStream_t s);
 void net_transpose_w(const void *W, void *WT, int cin_pad, int cout_pad, hipStream_t s);
 int net_wgrad(int cin_pad, int cout_pad, const void *dZ, const void *X, const NetDims &m, int nblk, float *part,
               int cout, int cin, float *dW, hipStream_t s);
-void net_bias_grad(const float *dZ, long nvox, int cpad, int cout, float *db, hipStream_t s);
+constexpr int NET_BIAS_BLOCKS = 256;
+// cpad 16 or 32; part holds NET_BIAS_BLOCKS x cpad doubles
+void net_bias_grad(const float *dZ, long nvox, int cpad, int cout, float *db, double *part, hipStream_t s);
 void net_mse(const float *Y, const float *T, const unsigned char *mask, long nvox, float inv, float *dY,
              double *partial, int nblk, double *loss, double loss_scale, hipStream_t s);
 void net_adam(float *w, float *m1, float *m2, const float *g, void *wb, long n, float lr, float b1, float b2,
               float eps, float c1, float c2, hipStream_t s);
-void net_store(const Geo &g, C5 U, C5 Uq, float *S, hipStream_t s);
+void net_store(const Geo &g, C5 U, C5 Uq, const double *J, float *S, hipStream_t s);
 void net_gather(const float *S, const float *rs, int cap, long vol, int nx, int ny, int nz, const int *org,
-                const NetDims &m, int halo, void *X, float *T, unsigned char *mask, hipStream_t s);
+                int whole_rec, const NetDims &m, int halo, void *X, float *T, unsigned char *mask, hipStream_t s);
 void net_pack_input(const float *x, void *X, long nv, hipStream_t s);
 void net_record_scales(const float *Sa, const float *Sb, const float *Sc, long vol, double *partial, int nblk,
                        float *rs, hipStream_t s);
-void net_guess(const Geo &g, C5 U, C5 Uq, const float *prev, const float *Y, const float *rs, F5 U_, hipStream_t s);
+void net_guess(const Geo &g, C5 U, C5 Uq, const double *J, const float *prev, const float *Y, const float *rs, F5 U_,
+               hipStream_t s);
 // sum over the interior cells of f_q^2 for q < 5 into out_dev[5] (stats.hip)
 void launch_sumsq5(const Geo &g, C5 f, double *partials, double *out_dev, hipStream_t s);
 

@@ -269,6 +269,8 @@ int initial_guess(cgli_ctx *ctx, double deltaT, ChannelState &cs, bool *ready)
         if (ctx_field(ctx, CGLI_U_ + q, &u_[q]) || ctx_field(ctx, CGLI_U + q, &u[q]) ||
             ctx_field(ctx, CGLI_UQ + q, &uq[q]))
             return CGLI_E_NOMEM;
+    double *J = nullptr;
+    if (ctx_field(ctx, CGLI_J, &J)) return CGLI_E_NOMEM;
     const F5 U_{{u_[0], u_[1], u_[2], u_[3], u_[4]}};
     const C5 U{{u[0], u[1], u[2], u[3], u[4]}}, Uq{{uq[0], uq[1], uq[2], uq[3], uq[4]}};
     const float *prev = cs.guess >= 2 ? net_prev_record(cs.net, ctx) : nullptr;
@@ -278,24 +280,24 @@ int initial_guess(cgli_ctx *ctx, double deltaT, ChannelState &cs, bool *ready)
         if (r) return r;
         // the extrapolated guess first, the network's last: when the
         // network's is taken its right-hand side is still in place
-        net_guess(g, U, Uq, nullptr, nullptr, nullptr, U_, s);
+        net_guess(g, U, Uq, J, nullptr, nullptr, nullptr, U_, s);
         r = guess_residual(ctx, deltaT, cs.r_ext);
         if (r) return r;
-        net_guess(g, U, Uq, prev, Y, rs, U_, s);
+        net_guess(g, U, Uq, J, prev, Y, rs, U_, s);
         r = guess_residual(ctx, deltaT, cs.r_net);
         if (r) return r;
         bool worse = false;
         for (int q = 0; q < 5; q++) worse = worse || cs.r_net[q] > cs.r_ext[q];
         if (worse) {
-            net_guess(g, U, Uq, nullptr, nullptr, nullptr, U_, s);
+            net_guess(g, U, Uq, J, nullptr, nullptr, nullptr, U_, s);
         } else {
             cs.used_net = 1;
             *ready = true;
         }
     } else if (cs.guess == 3 && prev) {
-        net_guess(g, U, Uq, prev, nullptr, nullptr, U_, s);
+        net_guess(g, U, Uq, J, prev, nullptr, nullptr, U_, s);
     } else {
-        net_guess(g, U, Uq, nullptr, nullptr, nullptr, U_, s);
+        net_guess(g, U, Uq, J, nullptr, nullptr, nullptr, U_, s);
     }
     return hipGetLastError() == hipSuccess ? CGLI_OK : CGLI_E_HIP;
 }

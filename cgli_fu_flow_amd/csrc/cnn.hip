@@ -24,6 +24,8 @@
 //                    per tap a GEMM over voxels with K = voxels, partial sums
 //                    per block, then a fixed-order FP64 reduction, so the
 //                    bits do not depend on scheduling
+#include <algorithm>
+
 #include "cgli_internal.hpp"
 
 namespace cgli {
@@ -65,57 +67,76 @@ __device__ __forceinline__ bf16x8 frag(const u16 *__restrict__ X, int C, const N
     return *reinterpret_cast<const bf16x8 *>(X + v * C + c0);
 }
 
-// y = conv(x, W) + bias for all voxels.  One wave per 16-voxel row tile and
-// all COUT = 16*MT outputs.  Outputs (either may be null): Z FP32
-// [v][COUT] (the pre-activation), A BF16 [v][COUT] = GELU(Z) when ACT, else
-// Z; channels co >= cout are written as 0.
-template <int CIN, int MT, bool ACT>
+// y = conv(x, W) + bias for all voxels.  Persistent blocks: the layer's
+// weights (COUT x 27 x CIN BF16, at most 54 KiB) are staged once per block in
+// LDS; then each wave takes items of RW rows (h .. h+RW-1) x 16 voxels (w0 ..
+// w0+15) at one (b, d), striding over the items.  Per K step a wave reads
+// MT weight fragments from LDS and RW input fragments, and issues RW x MT
+// MFMAs.  Outputs (either may be null): Z FP32 [v][COUT] (the pre-
+// activation), A BF16 [v][COUT] = GELU(Z) when ACT, else Z; channels
+// co >= cout are written as 0.
+template <int CIN, int MT, bool ACT, int RW>
 __global__ void __launch_bounds__(256) k_conv(const u16 *__restrict__ X, const u16 *__restrict__ Wt,
                                               const float *__restrict__ bias, float *__restrict__ Z,
-                                              u16 *__restrict__ A, NetDims m, int cout, long ntiles, int per)
+                                              u16 *__restrict__ A, NetDims m, int cout, long nitems, int per)
 {
     constexpr int COUT = 16 * MT;
     constexpr int TPS = 32 / CIN;                 // taps per K step of 32
     constexpr int NK = (27 + TPS - 1) / TPS;
-    const int lane = threadIdx.x & 63;
-    const long tile = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (tile >= ntiles) return;
-    const int rowsW = m.W / 16;
-    const long row = tile / rowsW;                // (b, d, h)
-    const int w0 = (int)(tile - row * rowsW) * 16;
-    const int h = (int)(row % m.H), d = (int)((row / m.H) % m.D), b = (int)(row / ((long)m.H * m.D));
+    constexpr int NWV = COUT * 27 * CIN / 8;      // 16-byte weight vectors
+    __shared__ __align__(16) u16 sw[COUT * 27 * CIN];
+    for (int t = threadIdx.x; t < NWV; t += blockDim.x)
+        reinterpret_cast<uint4 *>(sw)[t] = reinterpret_cast<const uint4 *>(Wt)[t];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n = lane & 15, kq = lane >> 4;
-    f32x4 acc[MT];
+    const int kk = 8 * kq, tsub = kk / CIN, c0 = kk % CIN;
+    const int rowsW = m.W / 16, hgroups = (m.H + RW - 1) / RW;
+    for (long item = (long)blockIdx.x * 4 + wave; item < nitems; item += (long)gridDim.x * 4) {
+        const int w0 = (int)(item % rowsW) * 16;
+        long r = item / rowsW;
+        const int hb = (int)(r % hgroups) * RW;
+        r /= hgroups;
+        const int d = (int)(r % m.D), b = (int)(r / m.D);
+        f32x4 acc[RW][MT];
 #pragma unroll
-    for (int t = 0; t < MT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int kk = 8 * kq;
-    const int tsub = kk / CIN, c0 = kk % CIN;
-#pragma unroll 2
-    for (int ks = 0; ks < NK; ks++) {
-        const int tap = ks * TPS + tsub;
-        bf16x8 bfr = zero8();
-        if (tap < 27) {
+        for (int q = 0; q < RW; q++)
+#pragma unroll
+            for (int t = 0; t < MT; t++) acc[q][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int ks = 0; ks < NK; ks++) {
+            const int tap = ks * TPS + tsub;
+            const bool live = tap < 27;
             const int dd = tap / 9 - 1, dh = (tap / 3) % 3 - 1, dw = tap % 3 - 1;
-            bfr = frag(X, CIN, m, b, d + dd, h + dh, w0 + n + dw, c0, per);
+            bf16x8 afr[MT], bfr[RW];
+#pragma unroll
+            for (int t = 0; t < MT; t++)
+                afr[t] = live ? *reinterpret_cast<const bf16x8 *>(&sw[((16 * t + n) * 27 + tap) * CIN + c0]) : zero8();
+#pragma unroll
+            for (int q = 0; q < RW; q++)
+                bfr[q] = live ? frag(X, CIN, m, b, d + dd, hb + q + dh, w0 + n + dw, c0, per) : zero8();
+#pragma unroll
+            for (int q = 0; q < RW; q++)
+#pragma unroll
+                for (int t = 0; t < MT; t++)
+                    acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[t], bfr[q], acc[q][t], 0, 0, 0);
         }
 #pragma unroll
-        for (int t = 0; t < MT; t++) {
-            bf16x8 afr = zero8();
-            if (tap < 27) afr = *reinterpret_cast<const bf16x8 *>(Wt + ((long)(16 * t + n) * 27 + tap) * CIN + c0);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr, acc[t], 0, 0, 0);
+        for (int q = 0; q < RW; q++) {
+            if (hb + q >= m.H) break;
+            const long v = (((long)b * m.D + d) * m.H + hb + q) * m.W + w0 + n;
+#pragma unroll
+            for (int t = 0; t < MT; t++)
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++) {
+                    const int co = 16 * t + 4 * kq + rr;
+                    const bool on = co < cout;
+                    const float z = on ? acc[q][t][rr] + (bias ? bias[co] : 0.0f) : 0.0f;
+                    if (Z) Z[v * COUT + co] = z;
+                    if (A) A[v * COUT + co] = on ? f2bf(ACT ? gelu(z) : z) : (u16)0;
+                }
         }
     }
-    const long v = (((long)b * m.D + d) * m.H + h) * m.W + w0 + n;
-#pragma unroll
-    for (int t = 0; t < MT; t++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int co = 16 * t + 4 * kq + r;
-            const bool live = co < cout;
-            const float z = live ? acc[t][r] + (bias ? bias[co] : 0.0f) : 0.0f;
-            if (Z) Z[v * COUT + co] = z;
-            if (A) A[v * COUT + co] = live ? f2bf(ACT ? gelu(z) : z) : (u16)0;
-        }
 }
 
 // dz = da * GELU'(z): BF16 for the next convolutions, FP32 for the bias
@@ -232,21 +253,36 @@ __global__ void k_wgrad_reduce(const float *__restrict__ part, int nblk, int cou
     dW[t] = (co < cout && ci < cin) ? (float)s : 0.0f;
 }
 
-// db[co] = sum_v dz[v][co]: one block per channel, fixed reduction tree
-__global__ void __launch_bounds__(256) k_bias_grad(const float *__restrict__ dZ, long nvox, int cpad, int cout,
-                                                   float *__restrict__ db)
+// db[co] = sum_v dz[v][co] (dz FP32 [v][CP]) in two fixed-order stages:
+// block b sums its voxel range per channel (thread t: channel t % CP,
+// voxels strided by 256 / CP, then a tree over those lanes), then one block
+// sums the blocks in index order.
+template <int CP>
+__global__ void __launch_bounds__(256) k_bias_part(const float *__restrict__ dZ, long nvox, long per,
+                                                   double *__restrict__ part)
 {
+    constexpr int VL = 256 / CP;
     __shared__ double sh[256];
-    const int co = blockIdx.x;
+    const int c = threadIdx.x % CP, vl = threadIdx.x / CP;
+    const long v0 = (long)blockIdx.x * per, v1 = v0 + per < nvox ? v0 + per : nvox;
     double s = 0.0;
-    for (long v = threadIdx.x; v < nvox; v += 256) s += dZ[v * cpad + co];
+    for (long v = v0 + vl; v < v1; v += VL) s += dZ[v * CP + c];
     sh[threadIdx.x] = s;
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+    for (int h = VL / 2; h > 0; h >>= 1) {
+        if (vl < h) sh[threadIdx.x] += sh[threadIdx.x + h * CP];
         __syncthreads();
     }
-    if (threadIdx.x == 0) db[co] = co < cout ? (float)sh[0] : 0.0f;
+    if (vl == 0) part[(long)blockIdx.x * CP + c] = sh[threadIdx.x];
+}
+
+__global__ void k_bias_final(const double *__restrict__ part, int nblk, int cp, int cout, float *__restrict__ db)
+{
+    const int c = threadIdx.x;
+    if (c >= cp) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += part[(long)b * cp + c];
+    db[c] = c < cout ? (float)s : 0.0f;
 }
 
 // Masked mean-square error over the first 5 of 16 output channels:
@@ -306,9 +342,10 @@ __global__ void k_adam(float *__restrict__ w, float *__restrict__ m1, float *__r
 }
 
 // ---------------------------------------------------------------- snapshots
-// Ring slot = the physical-step increment dq = U - Uq over the interior
-// cells, FP32, [cell][5] with cell = ((i-3)*ny + (j-2))*nz + (k-2).
-__global__ void k_store(Geo g, C5 U, C5 Uq, float *__restrict__ S)
+// Ring slot = the physical-step increment per unit volume, dq = J (U - Uq),
+// over the interior cells (U holds conserved variables over J), FP32,
+// [cell][5] with cell = ((i-3)*ny + (j-2))*nz + (k-2).
+__global__ void k_store(Geo g, C5 U, C5 Uq, const double *__restrict__ J, float *__restrict__ S)
 {
     const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -316,11 +353,13 @@ __global__ void k_store(Geo g, C5 U, C5 Uq, float *__restrict__ S)
     if (k > g.NZ || j > g.NY) return;
     const long c = ix(g, i, j, k);
     const long v = ((long)(i - 3) * g.ny + (j - 2)) * g.nz + (k - 2);
+    const double Jc = J[c];
 #pragma unroll
-    for (int q = 0; q < 5; q++) S[v * 5 + q] = (float)(U.v[q][c] - Uq.v[q][c]);
+    for (int q = 0; q < 5; q++) S[v * 5 + q] = (float)((U.v[q][c] - Uq.v[q][c]) * Jc);
 }
 
 // Network input and target of sample b.  org[b] = (t, i0, j0, k0): record t
+// (or, when whole_rec >= 0, org = (whole_rec, 0, 0, 0) for the one sample)
 // of the ring (slot t % cap), crop origin (i0, j0, k0), wrapping in i and k.
 // rs[slot][15] holds each record's inverse scales 1/s1, 1/sd, 1/s2 (five
 // each): the field RMS of dq_t, of d1_t = dq_t - dq_{t-1} and of
@@ -333,15 +372,17 @@ __global__ void k_store(Geo g, C5 U, C5 Uq, float *__restrict__ S)
 // mask[v] (when not null): 1 where the voxel's `halo`-voxel neighbourhood
 // lies in the crop or beyond a wall (j outside 0..ny-1), else 0.
 __global__ void k_gather(const float *__restrict__ S, const float *__restrict__ rs, int cap, long vol, int nx,
-                         int ny, int nz, const int *__restrict__ org, NetDims m, int halo, u16 *__restrict__ X,
-                         float *__restrict__ T, unsigned char *__restrict__ mask)
+                         int ny, int nz, const int *__restrict__ org, int whole_rec, NetDims m, int halo,
+                         u16 *__restrict__ X, float *__restrict__ T, unsigned char *__restrict__ mask)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nv = (long)m.B * m.D * m.H * m.W;
     if (t >= nv) return;
     const int w = (int)(t % m.W), h = (int)((t / m.W) % m.H), d = (int)((t / ((long)m.W * m.H)) % m.D);
     const int b = (int)(t / ((long)m.W * m.H * m.D));
-    const int *o = org + 4 * b;
+    // whole_rec >= 0: one sample, the whole grid of record whole_rec
+    const int o[4] = {whole_rec >= 0 ? whole_rec : org[4 * b], whole_rec >= 0 ? 0 : org[4 * b + 1],
+                      whole_rec >= 0 ? 0 : org[4 * b + 2], whole_rec >= 0 ? 0 : org[4 * b + 3]};
     const int i = (o[1] + d) % nx, k = (o[3] + w) % nz, j = o[2] + h;
     const long cell = ((long)i * ny + j) * nz + k;
     const int rec = o[0];
@@ -427,13 +468,13 @@ __global__ void k_record_scales(const double *__restrict__ partial, int nblk, lo
     rs[q] = (rms > 0.0 && nprev >= q / 5) ? (float)(1.0 / rms) : 1.0f;
 }
 
-// Initial guess U_ = U + dq (+ (dq - dq_prev) + s2 Y when Y is given), with
-// dq = U - Uq in FP64 and dq_prev the ring's previous record (FP32): the
-// linear extrapolation, or the quadratic one plus the network's correction
-// of the second difference (Y [cell][16], first 5 channels; rs = the
-// latest record's inverse scales).  Y without rs: quadratic only (Y unused).
-__global__ void k_guess(Geo g, C5 U, C5 Uq, const float *__restrict__ prev, const float *__restrict__ Y,
-                        const float *__restrict__ rs, F5 U_)
+// Initial guess U_ = U + dq (+ (dq - dq_prev) + s2 Y / J when prev and rs
+// are given), with dq = U - Uq in FP64 and dq_prev = (the ring's previous
+// record) / J: the linear extrapolation, or the quadratic one plus the
+// network's correction of the second difference (Y [cell][16], first 5
+// channels, per unit volume; rs = the latest record's inverse scales).
+__global__ void k_guess(Geo g, C5 U, C5 Uq, const double *__restrict__ J, const float *__restrict__ prev,
+                        const float *__restrict__ Y, const float *__restrict__ rs, F5 U_)
 {
     const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -441,14 +482,15 @@ __global__ void k_guess(Geo g, C5 U, C5 Uq, const float *__restrict__ prev, cons
     if (k > g.NZ || j > g.NY) return;
     const long c = ix(g, i, j, k);
     const long v = ((long)(i - 3) * g.ny + (j - 2)) * g.nz + (k - 2);
+    const double Jc = J[c];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
         const double u = U.v[q][c];
         const double dq = u - Uq.v[q][c];
         double next = dq;
         if (prev) {
-            next += dq - (double)prev[v * 5 + q];
-            if (rs) next += (double)Y[v * 16 + q] / (double)rs[10 + q];
+            next += dq - (double)prev[v * 5 + q] / Jc;
+            if (rs) next += (double)Y[v * 16 + q] / ((double)rs[10 + q] * Jc);
         }
         U_.v[q][c] = u + next;
     }
@@ -457,24 +499,39 @@ __global__ void k_guess(Geo g, C5 U, C5 Uq, const float *__restrict__ prev, cons
 // ---------------------------------------------------------------- launchers
 static unsigned blocks_of(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
+constexpr int CONV_RW = 4;
+
+static int conv_blocks(long nitems)
+{
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+            ncu = 256;
+    }
+    const long want = (nitems + 3) / 4;
+    return (int)std::max(1L, std::min<long>(want, 2L * ncu));
+}
+
 int net_conv(int cin_pad, int cout_pad, bool act, const void *X, const void *W, const float *bias, float *Z, void *A,
              const NetDims &m, int cout, int per, hipStream_t s)
 {
     if (m.W % 16) return -1;
-    const long nt = (long)m.B * m.D * m.H * (m.W / 16);
-    const dim3 gr(blocks_of(nt, 4)), bl(256);
+    const long nitems = (long)m.B * m.D * ((m.H + CONV_RW - 1) / CONV_RW) * (m.W / 16);
+    const dim3 gr(conv_blocks(nitems)), bl(256);
     const u16 *x = static_cast<const u16 *>(X), *w = static_cast<const u16 *>(W);
     u16 *a = static_cast<u16 *>(A);
     if (cin_pad == 16 && cout_pad == 32 && act)
-        hipLaunchKernelGGL((k_conv<16, 2, true>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nt, per);
+        hipLaunchKernelGGL((k_conv<16, 2, true, CONV_RW>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems, per);
     else if (cin_pad == 32 && cout_pad == 32 && act)
-        hipLaunchKernelGGL((k_conv<32, 2, true>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nt, per);
+        hipLaunchKernelGGL((k_conv<32, 2, true, CONV_RW>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems, per);
     else if (cin_pad == 32 && cout_pad == 16 && !act)
-        hipLaunchKernelGGL((k_conv<32, 1, false>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nt, per);
+        hipLaunchKernelGGL((k_conv<32, 1, false, CONV_RW>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems, per);
     else if (cin_pad == 16 && cout_pad == 32 && !act)
-        hipLaunchKernelGGL((k_conv<16, 2, false>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nt, per);
+        hipLaunchKernelGGL((k_conv<16, 2, false, CONV_RW>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems, per);
     else if (cin_pad == 32 && cout_pad == 32 && !act)
-        hipLaunchKernelGGL((k_conv<32, 2, false>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nt, per);
+        hipLaunchKernelGGL((k_conv<32, 2, false, CONV_RW>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems, per);
     else
         return -1;
     return 0;
@@ -518,9 +575,16 @@ int net_wgrad(int cin_pad, int cout_pad, const void *dZ, const void *X, const Ne
     return 0;
 }
 
-void net_bias_grad(const float *dZ, long nvox, int cpad, int cout, float *db, hipStream_t s)
+// part: NET_BIAS_BLOCKS x cpad doubles
+void net_bias_grad(const float *dZ, long nvox, int cpad, int cout, float *db, double *part, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_bias_grad, dim3(cpad), dim3(256), 0, s, dZ, nvox, cpad, cout, db);
+    const int nblk = (int)std::max(1L, std::min<long>(NET_BIAS_BLOCKS, (nvox + 255) / 256));
+    const long per = (nvox + nblk - 1) / nblk;
+    if (cpad == 32)
+        hipLaunchKernelGGL((k_bias_part<32>), dim3(nblk), dim3(256), 0, s, dZ, nvox, per, part);
+    else
+        hipLaunchKernelGGL((k_bias_part<16>), dim3(nblk), dim3(256), 0, s, dZ, nvox, per, part);
+    hipLaunchKernelGGL(k_bias_final, dim3(1), dim3(64), 0, s, part, nblk, cpad, cout, db);
 }
 
 void net_mse(const float *Y, const float *T, const unsigned char *mask, long nvox, float inv, float *dY,
@@ -539,17 +603,17 @@ void net_adam(float *w, float *m1, float *m2, const float *g, void *wb, long n, 
 
 static dim3 interior_grid(const Geo &g) { return dim3((g.nz + 63) / 64, (g.ny + 3) / 4, g.nx); }
 
-void net_store(const Geo &g, C5 U, C5 Uq, float *S, hipStream_t s)
+void net_store(const Geo &g, C5 U, C5 Uq, const double *J, float *S, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_store, interior_grid(g), dim3(64, 4), 0, s, g, U, Uq, S);
+    hipLaunchKernelGGL(k_store, interior_grid(g), dim3(64, 4), 0, s, g, U, Uq, J, S);
 }
 
 void net_gather(const float *S, const float *rs, int cap, long vol, int nx, int ny, int nz, const int *org,
-                const NetDims &m, int halo, void *X, float *T, unsigned char *mask, hipStream_t s)
+                int whole_rec, const NetDims &m, int halo, void *X, float *T, unsigned char *mask, hipStream_t s)
 {
     const long nv = (long)m.B * m.D * m.H * m.W;
-    hipLaunchKernelGGL(k_gather, dim3(blocks_of(nv, 256)), dim3(256), 0, s, S, rs, cap, vol, nx, ny, nz, org, m,
-                       halo, static_cast<u16 *>(X), T, mask);
+    hipLaunchKernelGGL(k_gather, dim3(blocks_of(nv, 256)), dim3(256), 0, s, S, rs, cap, vol, nx, ny, nz, org,
+                       whole_rec, m, halo, static_cast<u16 *>(X), T, mask);
 }
 
 void net_pack_input(const float *x, void *X, long nv, hipStream_t s)
@@ -568,9 +632,10 @@ void net_record_scales(const float *Sa, const float *Sb, const float *Sc, long v
     hipLaunchKernelGGL(k_record_scales, dim3(1), dim3(64), 0, s, partial, nblk, vol, nprev, rs);
 }
 
-void net_guess(const Geo &g, C5 U, C5 Uq, const float *prev, const float *Y, const float *rs, F5 U_, hipStream_t s)
+void net_guess(const Geo &g, C5 U, C5 Uq, const double *J, const float *prev, const float *Y, const float *rs, F5 U_,
+               hipStream_t s)
 {
-    hipLaunchKernelGGL(k_guess, interior_grid(g), dim3(64, 4), 0, s, g, U, Uq, prev, Y, rs, U_);
+    hipLaunchKernelGGL(k_guess, interior_grid(g), dim3(64, 4), 0, s, g, U, Uq, J, prev, Y, rs, U_);
 }
 
 }  // namespace cgli

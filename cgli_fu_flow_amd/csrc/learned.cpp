@@ -67,6 +67,7 @@ struct Work {
     float *dZf = nullptr; long cdZf = 0;      // [nv][32]
     float *part = nullptr; long cpart = 0;    // weight-gradient partials
     double *lpart = nullptr; long clpart = 0; // loss partials
+    double *bpart = nullptr; long cbpart = 0; // bias-gradient partials
     float *xin = nullptr; long cxin = 0;      // host input staging [nv][15]
 };
 
@@ -74,6 +75,8 @@ int wgrad_blocks(long nv) { return (int)std::max(1L, std::min(256L, nv / 2048));
 int mse_blocks(long nv) { return (int)std::max(1L, std::min(512L, (nv * 16 + 2047) / 2048)); }
 
 }  // namespace
+
+struct EventPair { hipEvent_t a = nullptr, b = nullptr; };
 
 struct cgli_net {
     int device = 0;
@@ -96,11 +99,37 @@ struct cgli_net {
     double *part10 = nullptr;    // partial sums of the record scales
     Work wk;
     hipEvent_t ev[4] = {};
+    std::vector<EventPair> pred_ev;   // one pair per prediction since the last timing read
+    size_t pred_used = 0;
     double predict_ms = 0, train_ms = 0;
     long n_predict = 0, n_train = 0;
 };
 
 namespace {
+
+EventPair &timing_pair(std::vector<EventPair> &pool, size_t &used)
+{
+    if (used == pool.size()) {
+        EventPair e;
+        hipEventCreate(&e.a);
+        hipEventCreate(&e.b);
+        pool.push_back(e);
+    }
+    return pool[used++];
+}
+
+// fold the recorded prediction times into the totals
+void collect_predict_times(cgli_net *n)
+{
+    for (size_t q = 0; q < n->pred_used; q++) {
+        float ms = 0;
+        if (hipEventSynchronize(n->pred_ev[q].b) == hipSuccess &&
+            hipEventElapsedTime(&ms, n->pred_ev[q].a, n->pred_ev[q].b) == hipSuccess)
+            n->predict_ms += ms;
+        n->n_predict++;
+    }
+    n->pred_used = 0;
+}
 
 // BF16 copy and transposed weights from the FP32 master
 void refresh_bf16(cgli_net *n, hipStream_t s)
@@ -132,6 +161,7 @@ int ensure_work(cgli_net *n, long nv, bool train)
         if (!r) r = grow(k.dZf, k.cdZf, nv * 32);
         if (!r) r = grow(k.part, k.cpart, (long)wgrad_blocks(nv) * 27 * 32 * 32);
         if (!r) r = grow(k.lpart, k.clpart, (long)mse_blocks(nv));
+        if (!r) r = grow(k.bpart, k.cbpart, (long)NET_BIAS_BLOCKS * 32);
     }
     return r;
 }
@@ -170,7 +200,7 @@ int backward(cgli_net *n, const NetDims &m, float inv, double *loss_dev, hipStre
         const void *xin = l == 0 ? (const void *)k.X0 : (const void *)k.A[l - 1];
         if (net_wgrad(L.cin_pad, L.cout_pad, k.dZb, xin, m, wb, k.part, L.cout, L.cin, n->g + L.w, s))
             return CGLI_E_ARG;
-        net_bias_grad(l == 3 ? k.dY : k.dZf, nv, L.cout_pad, L.cout, n->g + L.b, s);
+        net_bias_grad(l == 3 ? k.dY : k.dZf, nv, L.cout_pad, L.cout, n->g + L.b, k.bpart, s);
         if (l == 0) break;
         // dA_{l-1} = conv(dz_l, W_l^T), then dz_{l-1} = dA * GELU'(z_{l-1})
         if (net_conv(L.cout_pad, L.cin_pad, false, k.dZb, n->wt[l], nullptr, k.dA, nullptr, m, L.cin, 0, s))
@@ -219,11 +249,12 @@ int net_record(cgli_net *net, cgli_ctx *c)
 {
     const Geo &g = ctx_geo(c);
     if (!net->ring || g.nx != net->nx || g.ny != net->ny || g.nz != net->nz) return CGLI_E_STATE;
-    double *u[5], *uq[5];
+    double *u[5], *uq[5], *J = nullptr;
     for (int q = 0; q < 5; q++) {
         if (ctx_field(c, CGLI_U + q, &u[q]) || ctx_field(c, CGLI_UQ + q, &uq[q])) return CGLI_E_NOMEM;
     }
-    net_store(g, C5{{u[0], u[1], u[2], u[3], u[4]}}, C5{{uq[0], uq[1], uq[2], uq[3], uq[4]}},
+    if (ctx_field(c, CGLI_J, &J)) return CGLI_E_NOMEM;
+    net_store(g, C5{{u[0], u[1], u[2], u[3], u[4]}}, C5{{uq[0], uq[1], uq[2], uq[3], uq[4]}}, J,
               net->ring + (net->nrec % net->cap) * net->vol * 5, ctx_stream(c));
     const int r = record_scales(net, net->nrec, ctx_stream(c));
     net->nrec++;
@@ -249,23 +280,16 @@ int net_ready(cgli_net *net, cgli_ctx *c)
 int net_predict(cgli_net *net, hipStream_t s, const float **Y, const float **rs)
 {
     const NetDims m{1, net->nx, net->ny, net->nz};
-    const long nv = net->vol;
-    int r = ensure_work(net, nv, false);
-    if (!r) r = grow(net->org, net->corg, 4L);
+    int r = ensure_work(net, net->vol, false);
     if (r) return r;
-    const int o[4] = {(int)(net->nrec - 1), 0, 0, 0};
-    if (hipMemcpy(net->org, o, sizeof(o), hipMemcpyHostToDevice) != hipSuccess) return CGLI_E_HIP;
-    hipEventRecord(net->ev[0], s);
-    net_gather(net->ring, net->rscale, net->cap, net->vol, net->nx, net->ny, net->nz, net->org, m, 0, net->wk.X0,
-               nullptr, nullptr, s);
+    if (net->pred_used >= 256) collect_predict_times(net);
+    EventPair &e = timing_pair(net->pred_ev, net->pred_used);
+    hipEventRecord(e.a, s);
+    net_gather(net->ring, net->rscale, net->cap, net->vol, net->nx, net->ny, net->nz, nullptr, (int)(net->nrec - 1),
+               m, 0, net->wk.X0, nullptr, nullptr, s);
     r = forward(net, m, 1, false, s);
-    hipEventRecord(net->ev[1], s);
+    hipEventRecord(e.b, s);
     if (r) return r;
-    if (hipEventSynchronize(net->ev[1]) != hipSuccess) return CGLI_E_HIP;
-    float ms = 0;
-    hipEventElapsedTime(&ms, net->ev[0], net->ev[1]);
-    net->predict_ms += ms;
-    net->n_predict++;
     *Y = net->wk.Y;
     *rs = net->rscale + ((net->nrec - 1) % net->cap) * NET_NSCALE;
     return hipGetLastError() == hipSuccess ? CGLI_OK : CGLI_E_HIP;
@@ -339,10 +363,14 @@ int cgli_net_destroy(cgli_net *n)
     Work &k = n->wk;
     for (void *p : {(void *)k.X0, (void *)k.A[0], (void *)k.A[1], (void *)k.A[2], (void *)k.Z[0], (void *)k.Z[1],
                     (void *)k.Z[2], (void *)k.Y, (void *)k.T, (void *)k.mask, (void *)k.dY, (void *)k.dZb,
-                    (void *)k.dA, (void *)k.dZf, (void *)k.part, (void *)k.lpart, (void *)k.xin})
+                    (void *)k.dA, (void *)k.dZf, (void *)k.part, (void *)k.lpart, (void *)k.bpart, (void *)k.xin})
         if (p) hipFree(p);
     for (int e = 0; e < 4; e++)
         if (n->ev[e]) hipEventDestroy(n->ev[e]);
+    for (EventPair &e : n->pred_ev) {
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
     if (n->stream) hipStreamDestroy(n->stream);
     delete n;
     return CGLI_OK;
@@ -512,7 +540,7 @@ int cgli_net_train(cgli_net *n, int steps, int batch, int cd, int ch, int cw, fl
     if (hipMemcpy(n->org, o.data(), sizeof(int) * o.size(), hipMemcpyHostToDevice) != hipSuccess) return CGLI_E_HIP;
     hipEventRecord(n->ev[2], s);
     for (int it = 0; it < steps && !r; it++) {
-        net_gather(n->ring, n->rscale, n->cap, n->vol, n->nx, n->ny, n->nz, n->org + 4L * it * batch, m, halo,
+        net_gather(n->ring, n->rscale, n->cap, n->vol, n->nx, n->ny, n->nz, n->org + 4L * it * batch, -1, m, halo,
                    n->wk.X0, n->wk.T, n->wk.mask, s);
         r = forward(n, m, 0, true, s);
         if (!r) r = backward(n, m, inv[it], n->loss_dev + it, s);
@@ -541,6 +569,7 @@ int cgli_net_predict(cgli_net *n, float *x, float *y)
     const float *Y = nullptr, *rs = nullptr;
     int r = net_predict(n, n->stream, &Y, &rs);
     if (r) return r;
+    if (hipStreamSynchronize(n->stream) != hipSuccess) return CGLI_E_HIP;
     const long nv = n->vol;
     if (x) {
         std::vector<uint16_t> h(nv * 16);
@@ -564,6 +593,8 @@ int cgli_net_predict(cgli_net *n, float *x, float *y)
 int cgli_net_timing(cgli_net *n, double *predict_ms, long *n_predict, double *train_ms, long *n_train)
 {
     if (!n) return CGLI_E_ARG;
+    if (hipSetDevice(n->device) != hipSuccess) return CGLI_E_HIP;
+    collect_predict_times(n);
     if (predict_ms) *predict_ms = n->predict_ms;
     if (n_predict) *n_predict = n->n_predict;
     if (train_ms) *train_ms = n->train_ms;

@@ -78,13 +78,25 @@ __global__ void __launch_bounds__(256) k_sumsq5_rows(Geo g, C5 f, double *__rest
     if (threadIdx.x < 5) partial[(long)blockIdx.x * 5 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-__global__ void k_sumsq5_final(const double *__restrict__ partial, long nblk, double *__restrict__ out)
+// one block: thread t sums blocks t, t+256, ... in order, then a fixed tree
+__global__ void __launch_bounds__(256) k_sumsq5_final(const double *__restrict__ partial, long nblk,
+                                                      double *__restrict__ out)
 {
-    const int q = threadIdx.x;
-    if (q >= 5) return;
-    double s = 0.0;
-    for (long b = 0; b < nblk; b++) s += partial[b * 5 + q];
-    out[q] = s;
+    __shared__ double sh[5][256];
+    double s[5] = {0, 0, 0, 0, 0};
+    for (long b = threadIdx.x; b < nblk; b += 256)
+#pragma unroll
+        for (int q = 0; q < 5; q++) s[q] += partial[b * 5 + q];
+#pragma unroll
+    for (int q = 0; q < 5; q++) sh[q][threadIdx.x] = s[q];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+#pragma unroll
+            for (int q = 0; q < 5; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // partials: 5 x nx x ny doubles
@@ -92,7 +104,7 @@ void launch_sumsq5(const Geo &g, C5 f, double *partials, double *out_dev, hipStr
 {
     const long nblk = (long)g.nx * (g.NY - 1);
     hipLaunchKernelGGL(k_sumsq5_rows, dim3((unsigned)nblk), dim3(256), 0, s, g, f, partials);
-    hipLaunchKernelGGL(k_sumsq5_final, dim3(1), dim3(64), 0, s, partials, nblk, out_dev);
+    hipLaunchKernelGGL(k_sumsq5_final, dim3(1), dim3(256), 0, s, partials, nblk, out_dev);
 }
 
 }  // namespace cgli

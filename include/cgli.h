@@ -195,14 +195,14 @@ int cgli_channel_rule_state(cgli_ctx *ctx, double E[5], int set);
 /* guess 0: U_ = U^n; 1: U_ = U^n + dq^n (linear extrapolation), with
  * dq^n = U^n - U^{n-1}; 3: U_ = U^n + 2 dq^n - dq^{n-1} (quadratic
  * extrapolation, dq^{n-1} from the attached network's ring); 2: the
- * quadratic extrapolation plus s2 y, y the attached network's output on its
- * latest records and s2 their scale (cgli_net_record_scales), unless the
+ * quadratic extrapolation plus s2 y / J, y the attached network's output on
+ * its latest records and s2 their scale (cgli_net_record_scales), unless the
  * residual at that guess is larger (any variable) than at guess 1, which is
  * then used.  Modes 2 and 3 fall back to guess 1 while the network is not
  * ready (trained, three records of this grid) or the ring lacks dq^{n-1}. */
 int cgli_channel_set_guess(cgli_ctx *ctx, int guess);
-/* record every step's dq = U^{n+1} - U^n into the network's ring (net may
- * be NULL to detach) */
+/* record every step's increment per unit volume, J (U^{n+1} - U^n), into
+ * the network's ring (net may be NULL to detach) */
 int cgli_channel_attach_net(cgli_ctx *ctx, cgli_net *net);
 /* last step: *used_net = 1 if the network's guess was taken; residual sums
  * of squares per variable at the network's and the extrapolated guess */

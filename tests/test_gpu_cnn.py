@@ -154,7 +154,7 @@ def test_network_guess_with_zero_output_is_the_quadratic_extrapolation():
     check against the linear extrapolation, first iteration on the check's
     right-hand side when the network's guess is taken) is bit-identical to a
     step started from the guess it chose, and the ring holds exactly the
-    steps' increments."""
+    steps' increments per unit volume, J (U^{n+1} - U^n)."""
     nx, ny, nz = 12, 16, 16
     nsteps = 7
     ch = nv.channel(180.0, beta=2.2, skew=0.05)
@@ -179,7 +179,9 @@ def test_network_guess_with_zero_output_is_the_quadratic_extrapolation():
                 ctx.channel_step(dt, tol=1e-8, max_inner=40)
                 u = ctx.get_many(nv.U, 5)
                 uq = ctx.get_many(nv.UQ, 5)
-                dqs.append(np.stack([interior(u[q] - uq[q], nx, ny, nz) for q in range(5)], -1).astype(np.float32))
+                J = ctx.get(nv.J)
+                dqs.append(np.stack([interior((u[q] - uq[q]) * J, nx, ny, nz) for q in range(5)],
+                                    -1).astype(np.float32))
                 used.append(ctx.guess_info()[0])
             runs[mode] = (ctx.get_many(nv.U, 5), dqs, used, net)
             ctx.attach_net(None)
