@@ -36,7 +36,7 @@ struct cgli_ctx {
     int *ijk_dev = nullptr;
     std::vector<int> Nijk;       // (gend+NY+NZ+8) x 2
     std::vector<int> ijk;        // (cells+1) x 3
-    int lusgs_mode = 1;          // 0: one launch per hyperplane, 1: tiled wavefront
+    int lusgs_mode = 1;          // 0: one launch per hyperplane, 1: tiled wavefront, 2: with loader waves
     double *coef = nullptr;      // split-coefficient blocks of the wavefront (float in the mixed mode)
     double *dgn = nullptr;       // diagonal blocks of the wavefront (FP64)
     double *xbuf = nullptr;      // 5 fields: tile-edge exchange of the wavefront
@@ -506,7 +506,7 @@ static int wave_setup(cgli_ctx *c)
 
 int cgli_set_lusgs_mode(cgli_ctx *c, int mode)
 {
-    if (!c || mode < 0 || mode > 1) return CGLI_E_ARG;
+    if (!c || mode < 0 || mode > 2) return CGLI_E_ARG;
     c->lusgs_mode = mode;
     return CGLI_OK;
 }
@@ -559,7 +559,7 @@ int ctx_dts(cgli_ctx *c, double deltaT, double e, double er[5], int sweeps, int 
         tend(c, T_RHS);
     }
     const int mlast = g.gend + g.NY + g.NZ;
-    if (c->lusgs_mode == 1) {
+    if (c->lusgs_mode >= 1) {
         r = wave_setup(c);
         if (r) return r;
         launch_reset_ints(c->counters, 2, s);
@@ -569,16 +569,16 @@ int ctx_dts(cgli_ctx *c, double deltaT, double e, double er[5], int sweeps, int 
         tend(c, T_COEF);
     }
     for (int sw = 0; sw < sweeps; sw++) {
-        if (c->lusgs_mode == 1) {
+        if (c->lusgs_mode >= 1) {
             launch_lusgs_wave_prep(g, c->xbuf, c->counters, s);
             tbegin(c, T_FWD);
             launch_lusgs_wave(g, 1, c->coef, c->dgn, c->cfg.precision == 1, c5(c, CGLI_MR), f5(c, CGLI_NL), c->xbuf, c->trash, c->order_dev,
-                              c->ntiles, c->counters, c->max_spin, s);
+                              c->ntiles, c->counters, c->max_spin, c->lusgs_mode == 2, s);
             tend(c, T_FWD);
             launch_lusgs_wave_prep(g, c->xbuf, c->counters, s);
             tbegin(c, T_BWD);
             launch_lusgs_wave(g, 0, c->coef, c->dgn, c->cfg.precision == 1, c5(c, CGLI_NL), f5(c, CGLI_NR), c->xbuf, c->trash, c->order_dev,
-                              c->ntiles, c->counters, c->max_spin, s);
+                              c->ntiles, c->counters, c->max_spin, c->lusgs_mode == 2, s);
             tend(c, T_BWD);
         } else {
             tbegin(c, T_FWD);
@@ -608,7 +608,7 @@ int ctx_dts(cgli_ctx *c, double deltaT, double e, double er[5], int sweeps, int 
     tend(c, T_UPDATE);
     HIPCHK(hipGetLastError());
     mark(c, CGLI_MR, 15);
-    return finish_norm(c, nblk, er, c->lusgs_mode == 1);
+    return finish_norm(c, nblk, er, c->lusgs_mode >= 1);
 }
 
 }  // namespace cgli

@@ -1050,6 +1050,327 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_wave(Geo g, const CT *__restri
     }
 }
 
+// ---------------------------------------------------------------- loader waves
+// The same march with its coefficient stream moved out of the computing
+// wave.  A workgroup is one computing wave (the TJ x TK columns, as in
+// k_lusgs_wave) and two loader waves.  Loader lane l serves column l:
+// loader 1 fetches the x and y split records, loader 2 the z record, the
+// diagonal and R.  Each keeps LDEPTH steps of loads in flight in register
+// sets used in rotation; one step before the computing wave needs a step's
+// data, its loader waits for that set and copies it into one of two LDS
+// slots.  The per-step barrier the march already has orders the slot
+// hand-over, so the computing wave reads its coefficients from LDS and only
+// its tile-edge exchange still waits on memory.
+constexpr int LDEPTH = 3;   // steps of coefficient loads in flight per loader
+
+template <typename CT>
+struct LdSlot {
+    typedef typename Pair2<CT>::t V2;
+    V2 ab[12][64];      // x then y direction: 6 pairs each
+    V2 cz[6][64];       // z direction
+    double2 dg[2][64];  // (d11, d22), (d51, pad)
+    double r[5][64];
+};
+
+// one loader's half of a step: loader 1 holds the x record (6 pairs), the
+// first 3 pairs of the y record and R[0..1]; loader 2 the last 3 pairs of the
+// y record, the z record (6 pairs), the diagonal and R[2..4]
+template <typename CT>
+struct LdSet { CT p[18]; double d[4]; double r[3]; };   // scalars: 9 pairs, 2 diagonal pairs
+
+template <bool FWD, typename CT>
+__device__ __forceinline__ void ld_dir6(typename Pair2<CT>::t *o, const CT *__restrict__ r)
+{
+    typedef typename Pair2<CT>::t V2;
+    const V2 *b = reinterpret_cast<const V2 *>(r);
+#pragma unroll
+    for (int q = 0; q < 5; q++) o[q] = b[q * CB];
+    o[5] = b[(FWD ? 5 : 6) * CB];
+}
+
+// the computing wave's step: as wave_step, with the coefficients of this
+// step read from LDS slot `sl`
+template <int TJ, int TK, bool FWD, int PAR, typename CT>
+__device__ __forceinline__ void wave_step_ld(const WaveArgs<CT> &w, const Lane &L, int s, double prev[5],
+                                             double (*sx)[5][TJ * TK], const LdSlot<CT> &sl,
+                                             unsigned long long *nbc, unsigned long long *nbn,
+                                             unsigned long long *ncc, unsigned long long *ncn)
+{
+    const int il = s - L.jl - L.kl;
+    const bool act = L.col && il >= 0 && il < w.g.nx;
+    const long c = cell_at<FWD>(w, L, il);
+    double xb[5], xc[5];
+    const int tb = L.tid >= TK ? L.tid - TK : L.tid, tc = L.tid >= 1 ? L.tid - 1 : L.tid;
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        xb[v] = sx[PAR ^ 1][v][tb];
+        xc[v] = sx[PAR ^ 1][v][tc];
+    }
+    settle5(w.xbuf, w.g.N, c + w.d * w.g.Zm, act && L.jl == 0, nbc, xb, w.max_spin, w.err);
+    settle5(w.xbuf, w.g.N, c + w.d, act && L.kl == 0, ncc, xc, w.max_spin, w.err);
+    {   // the next step's tile-edge values (only the lanes that consume them)
+        const long cn = cell_at<FWD>(w, L, il + 1);
+        if (L.jl == 0) issue5(w.xbuf, w.g.N, cn + w.d * w.g.Zm, nbn);
+        if (L.kl == 0) issue5(w.xbuf, w.g.N, cn + w.d, ncn);
+    }
+    CT A[12], B[12], Cc[12];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const typename Pair2<CT>::t a = sl.ab[q][L.tid], b = sl.ab[6 + q][L.tid], cz = sl.cz[q][L.tid];
+        const int o = q < 5 ? 2 * q : 10;
+        A[o] = a.x; A[o + 1] = a.y;
+        B[o] = b.x; B[o + 1] = b.y;
+        Cc[o] = cz.x; Cc[o + 1] = cz.y;
+    }
+    const double2 d0 = sl.dg[0][L.tid], d1 = sl.dg[1][L.tid];
+    const double d11 = d0.x, d22 = d0.y, d51 = d1.x;
+    double r[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) r[v] = sl.r[v][L.tid];
+    CT xa[5], xbc[5], xcc[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        xa[v] = (CT)prev[v];
+        xbc[v] = (CT)xb[v];
+        xcc[v] = (CT)xc[v];
+    }
+    double out[5];
+    if (FWD) {
+        const CT LL0 = (-A[10] * xa[0] - A[0] * xa[1] - A[1] * xa[2] - A[2] * xa[3]) +
+                           (-B[10] * xbc[0] - B[0] * xbc[1] - B[1] * xbc[2] - B[2] * xbc[3]) +
+                           (-Cc[10] * xcc[0] - Cc[0] * xcc[1] - Cc[1] * xcc[2] - Cc[2] * xcc[3]);
+        const CT LL1 = (-A[3] * xa[0] - A[11] * xa[1]) + (-B[3] * xbc[0] - B[11] * xbc[1]) +
+                           (-Cc[3] * xcc[0] - Cc[11] * xcc[1]);
+        const CT LL2 = (-A[4] * xa[0] - A[11] * xa[2]) + (-B[4] * xbc[0] - B[11] * xbc[2]) +
+                           (-Cc[4] * xcc[0] - Cc[11] * xcc[2]);
+        const CT LL3 = (-A[5] * xa[0] - A[11] * xa[3]) + (-B[5] * xbc[0] - B[11] * xbc[3]) +
+                           (-Cc[5] * xcc[0] - Cc[11] * xcc[3]);
+        const CT LL4 = (-A[6] * xa[0] - A[7] * xa[1] - A[8] * xa[2] - A[9] * xa[3] - A[11] * xa[4]) +
+                           (-B[6] * xbc[0] - B[7] * xbc[1] - B[8] * xbc[2] - B[9] * xbc[3] - B[11] * xbc[4]) +
+                           (-Cc[6] * xcc[0] - Cc[7] * xcc[1] - Cc[8] * xcc[2] - Cc[9] * xcc[3] - Cc[11] * xcc[4]);
+        out[0] = d11 * (r[0] - (double)LL0);
+        out[1] = d22 * (r[1] - (double)LL1);
+        out[2] = d22 * (r[2] - (double)LL2);
+        out[3] = d22 * (r[3] - (double)LL3);
+        out[4] = d51 * (r[0] - (double)LL0) + d22 * (r[4] - (double)LL4);
+    } else {
+        const CT UU0 = (+A[10] * xa[0] + A[0] * xa[1] + A[1] * xa[2] + A[2] * xa[3]) +
+                           (+B[10] * xbc[0] + B[0] * xbc[1] + B[1] * xbc[2] + B[2] * xbc[3]) +
+                           (+Cc[10] * xcc[0] + Cc[0] * xcc[1] + Cc[1] * xcc[2] + Cc[2] * xcc[3]);
+        const CT UU1 = (+A[3] * xa[0] + A[11] * xa[1]) + (+B[3] * xbc[0] + B[11] * xbc[1]) +
+                           (+Cc[3] * xcc[0] + Cc[11] * xcc[1]);
+        const CT UU2 = (+A[4] * xa[0] + A[11] * xa[2]) + (+B[4] * xbc[0] + B[11] * xbc[2]) +
+                           (+Cc[4] * xcc[0] + Cc[11] * xcc[2]);
+        const CT UU3 = (+A[5] * xa[0] + A[11] * xa[3]) + (+B[5] * xbc[0] + B[11] * xbc[3]) +
+                           (+Cc[5] * xcc[0] + Cc[11] * xcc[3]);
+        const CT UU4 = (+A[6] * xa[0] + A[7] * xa[1] + A[8] * xa[2] + A[9] * xa[3] + A[11] * xa[4]) +
+                           (+B[6] * xbc[0] + B[7] * xbc[1] + B[8] * xbc[2] + B[9] * xbc[3] + B[11] * xbc[4]) +
+                           (+Cc[6] * xcc[0] + Cc[7] * xcc[1] + Cc[8] * xcc[2] + Cc[9] * xcc[3] + Cc[11] * xcc[4]);
+        out[0] = r[0] - d11 * (double)UU0;
+        out[1] = r[1] - d22 * (double)UU1;
+        out[2] = r[2] - d22 * (double)UU2;
+        out[3] = r[3] - d22 * (double)UU3;
+        out[4] = r[4] - (d51 * (double)UU0 + d22 * (double)UU4);
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        double *px = act ? w.X.v[v] + c : w.trash + L.tline + v;
+        *px = out[v];
+        sx[PAR][v][L.tid] = out[v];
+        prev[v] = act ? out[v] : prev[v];
+    }
+    if (act && L.edge_out) {
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            __hip_atomic_store(reinterpret_cast<unsigned long long *>(w.xbuf + v * w.g.N + c),
+                               (unsigned long long)__double_as_longlong(out[v]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// issue one loader's loads of local step il into a set (pairs are split into
+// scalars at once: the register sets never hold vector structs)
+template <typename CT>
+__device__ __forceinline__ void put_pair(CT *o, typename Pair2<CT>::t v)
+{
+    o[0] = v.x;
+    o[1] = v.y;
+}
+
+template <bool FWD, typename CT>
+__device__ __forceinline__ void ld_issue(LdSet<CT> &S, int role, const WaveArgs<CT> &w, const Lane &L, int il)
+{
+    typedef typename Pair2<CT>::t V2;
+    const long ic = icl(w, il);
+    const CT *rec = w.coef + L.r0 + ic * w.dcol;
+    const long c = cell_at<FWD>(w, L, il);
+    const V2 *b = reinterpret_cast<const V2 *>(rec + w.oB + 7 * CPAIR);
+    if (role == 1) {
+        const V2 *a = reinterpret_cast<const V2 *>(rec + w.oA);
+#pragma unroll
+        for (int q = 0; q < 5; q++) put_pair<CT>(S.p + 2 * q, a[q * CB]);
+        put_pair<CT>(S.p + 10, a[(FWD ? 5 : 6) * CB]);
+#pragma unroll
+        for (int q = 0; q < 3; q++) put_pair<CT>(S.p + 12 + 2 * q, b[q * CB]);
+        S.r[0] = w.R.v[0][c];
+        S.r[1] = w.R.v[1][c];
+    } else {
+        put_pair<CT>(S.p, b[3 * CB]);
+        put_pair<CT>(S.p + 2, b[4 * CB]);
+        put_pair<CT>(S.p + 4, b[(FWD ? 5 : 6) * CB]);
+        const V2 *cz = reinterpret_cast<const V2 *>(rec + w.oC + 14 * CPAIR);
+#pragma unroll
+        for (int q = 0; q < 5; q++) put_pair<CT>(S.p + 6 + 2 * q, cz[q * CB]);
+        put_pair<CT>(S.p + 16, cz[(FWD ? 5 : 6) * CB]);
+        const double2 *db = reinterpret_cast<const double2 *>(w.dgn + L.d0 + ic * w.ddcol);
+        put_pair<double>(S.d, db[0]);
+        put_pair<double>(S.d + 2, db[CB]);
+#pragma unroll
+        for (int v = 0; v < 3; v++) S.r[v] = w.R.v[2 + v][c];
+    }
+}
+
+template <typename CT>
+__device__ __forceinline__ void ld_put(LdSlot<CT> &sl, const LdSet<CT> &S, int role, int lane)
+{
+    if (role == 1) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) sl.ab[q][lane] = Pair2<CT>::make(S.p[2 * q], S.p[2 * q + 1]);
+        sl.r[0][lane] = S.r[0];
+        sl.r[1][lane] = S.r[1];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 3; q++) sl.ab[9 + q][lane] = Pair2<CT>::make(S.p[2 * q], S.p[2 * q + 1]);
+#pragma unroll
+        for (int q = 0; q < 6; q++) sl.cz[q][lane] = Pair2<CT>::make(S.p[6 + 2 * q], S.p[7 + 2 * q]);
+        sl.dg[0][lane] = make_double2(S.d[0], S.d[1]);
+        sl.dg[1][lane] = make_double2(S.d[2], S.d[3]);
+#pragma unroll
+        for (int v = 0; v < 3; v++) sl.r[2 + v][lane] = S.r[v];
+    }
+}
+
+// step s0+u of a group of LCM(2, LDEPTH) = 6 steps: every set index and
+// LDS parity is a compile-time constant
+template <int q, typename CT>
+__device__ __forceinline__ LdSet<CT> &pick_set(LdSet<CT> &a, LdSet<CT> &b, LdSet<CT> &c)
+{
+    if constexpr (q == 0) return a;
+    else if constexpr (q == 1) return b;
+    else return c;
+}
+
+template <int u, int TJ, int TK, bool FWD, typename CT>
+__device__ __forceinline__ void ld_group_step(const WaveArgs<CT> &w, const Lane &L, int s0, int S, int wave, int lane,
+                                              int il0, double prev[5], double (*sx)[5][TJ * TK], LdSlot<CT> *slot,
+                                              LdSet<CT> &set0, LdSet<CT> &set1, LdSet<CT> &set2,
+                                              unsigned long long *nb0, unsigned long long *nb1,
+                                              unsigned long long *nc0, unsigned long long *nc1)
+{
+    static_assert(LDEPTH == 3, "three named register sets");
+    const int s = s0 + u;
+    if (s >= S) return;
+    __syncthreads();
+    if (wave == 0) {
+        if (u & 1)
+            wave_step_ld<TJ, TK, FWD, 1, CT>(w, L, s, prev, sx, slot[1], nb1, nb0, nc1, nc0);
+        else
+            wave_step_ld<TJ, TK, FWD, 0, CT>(w, L, s, prev, sx, slot[0], nb0, nb1, nc0, nc1);
+    } else if (s + 1 < S) {
+        constexpr int q = (u + 1) % LDEPTH;
+        LdSet<CT> &st = pick_set<q, CT>(set0, set1, set2);
+        ld_put(slot[(u + 1) & 1], st, wave, lane);
+        ld_issue<FWD, CT>(st, wave, w, L, il0 + s + 1 + LDEPTH);
+    }
+}
+
+template <int TJ, int TK, bool FWD, typename CT>
+__global__ void __launch_bounds__(TJ *TK + 128) k_lusgs_wave_ld(Geo g, const CT *__restrict__ coef,
+                                                               const double *__restrict__ dgn, C5 R, F5 X,
+                                                               double *xbuf, double *trash,
+                                                               const int *__restrict__ order, int *ticket, int *err,
+                                                               int max_spin)
+{
+    static_assert(TJ * TK == 64, "one computing wave per tile");
+    constexpr int U = 6;   // steps per unrolled group (ld_group_step<0..5>): LCM(2 LDS parities, LDEPTH sets)
+    static_assert(U % 2 == 0 && U % LDEPTH == 0, "group length");
+    __shared__ int s_t;
+    __shared__ double sx[2][5][TJ * TK];
+    __shared__ LdSlot<CT> slot[2];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    Lane L;
+    L.tid = lane;   // every wave maps its lane to the same column
+    L.kl = lane % TK;
+    L.jl = lane / TK;
+    if (threadIdx.x == 0) s_t = atomicAdd(ticket, 1);
+    __syncthreads();
+    const int packed = order[s_t];
+    const int TJi = packed >> 16, TKi = packed & 0xffff;
+    const int jo = TJi * TJ + L.jl, ko = TKi * TK + L.kl;
+    L.col = (jo < g.ny) && (ko < g.nz);
+    const int joc = jo < g.ny ? jo : g.ny - 1, koc = ko < g.nz ? ko : g.nz - 1;
+    const int j = FWD ? 2 + joc : g.NY - joc;
+    const int k = FWD ? 2 + koc : g.NZ - koc;
+    L.c0 = ix(g, 0, j, k);
+    L.edge_out = (L.jl == TJ - 1) || (L.kl == TK - 1);
+    L.tline = ((long)blockIdx.x * 16) * 16;
+    WaveArgs<CT> w;
+    w.g = g;
+    w.coef = coef;
+    w.dgn = dgn;
+    w.R = R;
+    w.X = X;
+    w.xbuf = xbuf;
+    w.trash = trash;
+    w.err = err;
+    w.max_spin = max_spin;
+    w.sxs = (long)g.Ym * g.Zm;
+    w.d = FWD ? -1 : 1;
+    {
+        const long nkb = coef_nkb(g);
+        const int i0 = FWD ? 3 : g.gend;
+        w.dcol = -w.d * g.Ym * nkb * CBLK;
+        w.ddcol = -w.d * g.Ym * nkb * DBLK;
+        w.oA = -w.dcol;
+        w.oB = w.d * nkb * CBLK;
+        L.r0 = block_at(g, nkb, CBLK, i0, j, k);
+        L.d0 = block_at(g, nkb, DBLK, i0, j, k);
+        w.oC = block_at(g, nkb, CBLK, i0, j, k + (int)w.d) - L.r0;
+    }
+    const int S = g.nx + TJ + TK - 2;
+    const int il0 = -(L.jl + L.kl);   // local step of march step 0
+    // computing wave state
+    double prev[5];
+    unsigned long long nb0[5], nb1[5], nc0[5], nc1[5];
+    // loader state: LDEPTH register sets
+    LdSet<CT> set0, set1, set2;
+    if (wave == 0) {
+        const long c = cell_at<FWD>(w, L, 0);
+#pragma unroll
+        for (int v = 0; v < 5; v++) prev[v] = X.v[v][c + w.d * w.sxs];   // ghost plane (zero)
+        const long cs = cell_at<FWD>(w, L, il0);
+        if (L.jl == 0) issue5(xbuf, g.N, cs + w.d * g.Zm, nb0);
+        if (L.kl == 0) issue5(xbuf, g.N, cs + w.d, nc0);
+    } else {
+        ld_issue<FWD, CT>(set0, wave, w, L, il0);
+        ld_issue<FWD, CT>(set1, wave, w, L, il0 + 1);
+        ld_issue<FWD, CT>(set2, wave, w, L, il0 + 2);
+        ld_put(slot[0], set0, wave, lane);                       // step 0
+        ld_issue<FWD, CT>(set0, wave, w, L, il0 + LDEPTH);       // set 0 now holds step LDEPTH
+    }
+    // step s: one barrier, then the computing wave runs step s from slot
+    // s&1 while the loaders move step s+1 (register set (s+1) % LDEPTH)
+    // into slot (s+1)&1 and reuse that set for step s+1+LDEPTH
+    for (int s0 = 0; s0 < S; s0 += U) {
+        ld_group_step<0, TJ, TK, FWD, CT>(w, L, s0, S, wave, lane, il0, prev, sx, slot, set0, set1, set2, nb0, nb1, nc0, nc1);
+        ld_group_step<1, TJ, TK, FWD, CT>(w, L, s0, S, wave, lane, il0, prev, sx, slot, set0, set1, set2, nb0, nb1, nc0, nc1);
+        ld_group_step<2, TJ, TK, FWD, CT>(w, L, s0, S, wave, lane, il0, prev, sx, slot, set0, set1, set2, nb0, nb1, nc0, nc1);
+        ld_group_step<3, TJ, TK, FWD, CT>(w, L, s0, S, wave, lane, il0, prev, sx, slot, set0, set1, set2, nb0, nb1, nc0, nc1);
+        ld_group_step<4, TJ, TK, FWD, CT>(w, L, s0, S, wave, lane, il0, prev, sx, slot, set0, set1, set2, nb0, nb1, nc0, nc1);
+        ld_group_step<5, TJ, TK, FWD, CT>(w, L, s0, S, wave, lane, il0, prev, sx, slot, set0, set1, set2, nb0, nb1, nc0, nc1);
+    }
+}
+
 // tile order: anti-diagonals J+K ascending, J ascending inside
 static std::vector<int> tile_order(int ntj, int ntk)
 {
@@ -1098,12 +1419,28 @@ void launch_lusgs_wave_prep(const Geo &g, double *xbuf, int *counters, hipStream
 
 void launch_lusgs_wave(const Geo &g, int forward, const void *coef, const double *dgn, int fp32, C5 R, F5 X,
                        double *xbuf, double *trash, const int *order, int ntiles, int *counters, int max_spin,
-                       hipStream_t s)
+                       int loaders, hipStream_t s)
 {
     // counters[0]: ticket, counters[1]: error flag (cleared by the caller)
     const dim3 gr(ntiles), bl(WAVE_TJ * WAVE_TK);
     const float *cf = static_cast<const float *>(coef);
     const double *cd = static_cast<const double *>(coef);
+    if (loaders) {
+        const dim3 bl3(WAVE_TJ * WAVE_TK + 128);
+        if (forward && fp32)
+            hipLaunchKernelGGL((k_lusgs_wave_ld<WAVE_TJ, WAVE_TK, true, float>), gr, bl3, 0, s, g, cf, dgn, R, X, xbuf,
+                               trash, order, counters, counters + 1, max_spin);
+        else if (forward)
+            hipLaunchKernelGGL((k_lusgs_wave_ld<WAVE_TJ, WAVE_TK, true, double>), gr, bl3, 0, s, g, cd, dgn, R, X,
+                               xbuf, trash, order, counters, counters + 1, max_spin);
+        else if (fp32)
+            hipLaunchKernelGGL((k_lusgs_wave_ld<WAVE_TJ, WAVE_TK, false, float>), gr, bl3, 0, s, g, cf, dgn, R, X,
+                               xbuf, trash, order, counters, counters + 1, max_spin);
+        else
+            hipLaunchKernelGGL((k_lusgs_wave_ld<WAVE_TJ, WAVE_TK, false, double>), gr, bl3, 0, s, g, cd, dgn, R, X,
+                               xbuf, trash, order, counters, counters + 1, max_spin);
+        return;
+    }
     if (forward && fp32)
         hipLaunchKernelGGL((k_lusgs_wave<WAVE_TJ, WAVE_TK, true, float>), gr, bl, 0, s, g, cf, dgn, R, X, xbuf, trash,
                            order, counters, counters + 1, max_spin);

@@ -121,8 +121,10 @@ int cgli_hyperplane(cgli_ctx *ctx, int *Nijk, int *ijk);
 int cgli_dual_time_stepping(cgli_ctx *ctx, double deltaT, double e, double er[5], int sweeps,
                             int write_residual);
 /* LU-SGS schedule: 0 = one launch per i+j+k hyperplane (the reference's
- * order), 1 = tiled wavefront in one launch per sweep (default).  Both
- * give identical bits.  The wavefront's bounded waits give up after
+ * order), 1 = tiled wavefront in one launch per sweep (default), 2 = the
+ * same wavefront with two loader waves per tile streaming the coefficients
+ * through LDS, three steps ahead.  The wavefronts agree with each other bit
+ * for bit and with the plane order to rounding.  The wavefront's bounded waits give up after
  * max_spin polls each (then CGLI_E_TIMEOUT). */
 int cgli_set_lusgs_mode(cgli_ctx *ctx, int mode);
 int cgli_set_spin_limit(cgli_ctx *ctx, int max_spin);

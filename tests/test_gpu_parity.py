@@ -224,12 +224,14 @@ def test_missing_input_is_an_error(nv, orc):
 def test_wavefront_and_plane_schedules_agree(nv, orc, name):
     """The tiled wavefront visits cells in another topological order of the
     same dependencies as the reference's plane order, with the grid-spacing
-    division folded into its coefficients: results agree to rounding."""
+    division folded into its coefficients: results agree to rounding.  The
+    wavefront without loader waves (mode 1) and with them (mode 2) do the same
+    arithmetic on the same values: identical bits."""
     print("case", name, flush=True)
     case = make_case(name)
     f = make_inputs(orc, case.slab, SEED + 6)
     outs = []
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         with open_ctx(nv, orc, case) as ctx:
             ctx.set_lusgs_mode(mode)
             upload_inputs(ctx, f)
@@ -242,10 +244,12 @@ def test_wavefront_and_plane_schedules_agree(nv, orc, name):
     reg = regions(case.slab)["cell"]
     for v in range(5):
         assert abs(outs[0][0][v] - outs[1][0][v]) <= 1e-13 * abs(outs[0][0][v])
+        assert outs[1][0][v] == outs[2][0][v]
         for q in (1, 2, 3):
             a, b = outs[0][q][v][reg], outs[1][q][v][reg]
             assert np.all(np.isfinite(b))
             assert rel(b, a) <= 1e-13, (q, v, rel(b, a))
+            assert np.array_equal(outs[1][q][v], outs[2][q][v])
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
