@@ -292,6 +292,17 @@ int cgli_field_device_ptr(cgli_ctx *c, int id, void **dptr)
     return CGLI_OK;
 }
 
+int cgli_device_memory(int device, size_t *free_bytes, size_t *total_bytes)
+{
+    if (!free_bytes || !total_bytes) return CGLI_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CGLI_E_NODEV;
+    if (device < 0 || device >= ndev) return CGLI_E_ARG;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipMemGetInfo(free_bytes, total_bytes));
+    return CGLI_OK;
+}
+
 int cgli_synchronize(cgli_ctx *c)
 {
     if (!c) return CGLI_E_ARG;

@@ -149,7 +149,8 @@ int ensure_work(cgli_net *n, long nv, bool train)
 {
     Work &k = n->wk;
     int r = grow(k.X0, k.cX0, nv * 16);
-    for (int l = 0; l < 3 && !r; l++) r = grow(k.A[l], k.cA[l], nv * 32);
+    // inference ping-pongs A[0] and A[1]; training keeps all three
+    for (int l = 0; l < (train ? 3 : 2) && !r; l++) r = grow(k.A[l], k.cA[l], nv * 32);
     if (!r) r = grow(k.Y, k.cY, nv * 16);
     if (train) {
         for (int l = 0; l < 3 && !r; l++) r = grow(k.Z[l], k.cZ[l], nv * 32);

@@ -89,6 +89,7 @@ def lib():
         L.cgli_timing_reset.argtypes = [vp]
         L.cgli_timing_get.argtypes = [vp, C.c_int, dp, C.POINTER(C.c_long)]
         L.cgli_channel_set_rule.argtypes = [vp, C.c_int]
+        L.cgli_device_memory.argtypes = [C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
         L.cgli_channel_rule_state.argtypes = [vp, dp, C.c_int]
         L.cgli_channel_set_guess.argtypes = [vp, C.c_int]
         L.cgli_channel_attach_net.argtypes = [vp, vp]
@@ -110,6 +111,13 @@ def lib():
         L.cgli_net_timing.argtypes = [vp, dp, C.POINTER(C.c_long), dp, C.POINTER(C.c_long)]
         _lib = L
     return _lib
+
+
+def device_memory(device=0):
+    """(free, total) bytes of one device."""
+    f, t = C.c_size_t(), C.c_size_t()
+    _check(lib().cgli_device_memory(device, C.byref(f), C.byref(t)), "device_memory")
+    return f.value, t.value
 
 
 def reference_config(nx, ny, nz):

@@ -90,6 +90,8 @@ int cgli_get_field(cgli_ctx *ctx, int field, double *host);
 /* Device address of a field (allocating it), for zero-copy interop. */
 int cgli_field_device_ptr(cgli_ctx *ctx, int field, void **dptr);
 int cgli_synchronize(cgli_ctx *ctx);
+/* device memory of one device in bytes (free, total), for footprint reports */
+int cgli_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
 
 /* ---- hot path: each replaces the reference routine named ------------- */
 /* Flux_X.cpp:16-360 / Flux_Y.cpp:15-415 / Flux_Z.cpp:15-360:
