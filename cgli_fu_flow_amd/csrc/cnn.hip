@@ -67,30 +67,92 @@ __device__ __forceinline__ bf16x8 frag(const u16 *__restrict__ X, int C, const N
     return *reinterpret_cast<const bf16x8 *>(X + v * C + c0);
 }
 
+// Per-lane K fragment of one tap: CIN = 32 -> 8 channels (16 bytes) for
+// v_mfma_f32_16x16x32_bf16; CIN = 16 -> 4 channels (8 bytes) for
+// v_mfma_f32_16x16x16_bf16.  Fragments travel as 32-bit words.
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+template <int CIN> struct KF;
+template <> struct KF<32> {
+    typedef uint4 raw;
+    static __device__ __forceinline__ raw zero() { return make_uint4(0u, 0u, 0u, 0u); }
+    static __device__ __forceinline__ raw sel(bool ok, raw v)
+    {
+        return make_uint4(ok ? v.x : 0u, ok ? v.y : 0u, ok ? v.z : 0u, ok ? v.w : 0u);
+    }
+    static __device__ __forceinline__ f32x4 mfma(raw a, raw b, f32x4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                       0, 0, 0);
+    }
+};
+template <> struct KF<16> {
+    typedef uint2 raw;
+    static __device__ __forceinline__ raw zero() { return make_uint2(0u, 0u); }
+    static __device__ __forceinline__ raw sel(bool ok, raw v) { return make_uint2(ok ? v.x : 0u, ok ? v.y : 0u); }
+    static __device__ __forceinline__ f32x4 mfma(raw a, raw b, f32x4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, a), __builtin_bit_cast(s16x4, b), c,
+                                                         0, 0, 0);
+    }
+};
+
+// voxel of (b, d+dd, h, w+dw) for tap offsets, and whether it lies in the
+// sample (periodic in d and w when `per`); an outside voxel is replaced by
+// `safe` so that every load stays in bounds
+template <bool PER>
+__device__ __forceinline__ int tap_voxel(const NetDims &m, int b, int d, int dd, int h, int w, int dw, int safe,
+                                         bool &ok)
+{
+    int d2 = d + dd, w2 = w + dw;
+    if constexpr (PER) {
+        d2 = d2 < 0 ? d2 + m.D : (d2 >= m.D ? d2 - m.D : d2);
+        w2 = w2 < 0 ? w2 + m.W : (w2 >= m.W ? w2 - m.W : w2);
+    }
+    ok = d2 >= 0 && d2 < m.D && h >= 0 && h < m.H && w2 >= 0 && w2 < m.W;
+    const int v = ((b * m.D + d2) * m.H + h) * m.W + w2;
+    return ok ? v : safe;
+}
+
+// one tap of an item: the RW rows' fragments of tap t into F (in flight)
+template <int CIN, int RW, bool PER>
+__device__ __forceinline__ void tap_load(typename KF<CIN>::raw *F, bool *ok, const u16 *__restrict__ X,
+                                         const NetDims &m, int t, int b, int d, int hb, int w, int c0, int safe)
+{
+    const int dd = t / 9 - 1, dh = (t / 3) % 3 - 1, dw = t % 3 - 1;
+#pragma unroll
+    for (int q = 0; q < RW; q++) {
+        const int v = tap_voxel<PER>(m, b, d, dd, hb + q + dh, w, dw, safe, ok[q]);
+        F[q] = *reinterpret_cast<const typename KF<CIN>::raw *>(X + (long)v * CIN + c0);
+    }
+}
+
 // y = conv(x, W) + bias for all voxels.  Persistent blocks: the layer's
-// weights (COUT x 27 x CIN BF16, at most 54 KiB) are staged once per block in
-// LDS; then each wave takes items of RW rows (h .. h+RW-1) x 16 voxels (w0 ..
-// w0+15) at one (b, d), striding over the items.  Per K step a wave reads
-// MT weight fragments from LDS and RW input fragments, and issues RW x MT
-// MFMAs.  Outputs (either may be null): Z FP32 [v][COUT] (the pre-
-// activation), A BF16 [v][COUT] = GELU(Z) when ACT, else Z; channels
-// co >= cout are written as 0.
-template <int CIN, int MT, bool ACT, int RW>
+// weights (COUT x 27 x CIN BF16, rows padded by 8 elements against LDS bank
+// conflicts) are staged once per block in LDS; then each wave takes items of
+// RW rows (h .. h+RW-1) x 16 voxels (w0 .. w0+15) at one (b, d), striding
+// over the items.  The K loop runs over the 27 taps: per tap MT weight
+// fragments from LDS, RW input fragments (loaded one tap ahead into the
+// other of two register buffers), RW x MT MFMAs.  Outputs (either may be
+// null): Z FP32 [v][COUT] (the pre-activation), A BF16 [v][COUT] = GELU(Z)
+// when ACT, else Z; channels co >= cout are written as 0.
+template <int CIN, int MT, bool ACT, int RW, bool PER>
 __global__ void __launch_bounds__(256) k_conv(const u16 *__restrict__ X, const u16 *__restrict__ Wt,
                                               const float *__restrict__ bias, float *__restrict__ Z,
-                                              u16 *__restrict__ A, NetDims m, int cout, long nitems, int per)
+                                              u16 *__restrict__ A, NetDims m, int cout, long nitems)
 {
     constexpr int COUT = 16 * MT;
-    constexpr int TPS = 32 / CIN;                 // taps per K step of 32
-    constexpr int NK = (27 + TPS - 1) / TPS;
-    constexpr int NWV = COUT * 27 * CIN / 8;      // 16-byte weight vectors
-    __shared__ __align__(16) u16 sw[COUT * 27 * CIN];
-    for (int t = threadIdx.x; t < NWV; t += blockDim.x)
-        reinterpret_cast<uint4 *>(sw)[t] = reinterpret_cast<const uint4 *>(Wt)[t];
+    constexpr int CPL = CIN / 4;                  // channels per lane and tap
+    constexpr int WROW = 27 * CIN + 8;            // padded LDS weight row
+    typedef KF<CIN> K;
+    typedef typename K::raw R;
+    __shared__ __align__(16) u16 sw[COUT * WROW];
+    for (int t = threadIdx.x; t < COUT * 27 * CIN / 8; t += blockDim.x) {
+        const int e = 8 * t, co = e / (27 * CIN), r = e % (27 * CIN);
+        *reinterpret_cast<uint4 *>(&sw[co * WROW + r]) = reinterpret_cast<const uint4 *>(Wt)[t];
+    }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int n = lane & 15, kq = lane >> 4;
-    const int kk = 8 * kq, tsub = kk / CIN, c0 = kk % CIN;
+    const int n = lane & 15, kq = lane >> 4, c0 = CPL * kq;
     const int rowsW = m.W / 16, hgroups = (m.H + RW - 1) / RW;
     for (long item = (long)blockIdx.x * 4 + wave; item < nitems; item += (long)gridDim.x * 4) {
         const int w0 = (int)(item % rowsW) * 16;
@@ -98,43 +160,67 @@ __global__ void __launch_bounds__(256) k_conv(const u16 *__restrict__ X, const u
         const int hb = (int)(r % hgroups) * RW;
         r /= hgroups;
         const int d = (int)(r % m.D), b = (int)(r / m.D);
+        const int w = w0 + n;
+        const int safe = ((b * m.D + d) * m.H + hb) * m.W + w;   // (b, d, hb, w): always inside
         f32x4 acc[RW][MT];
 #pragma unroll
         for (int q = 0; q < RW; q++)
 #pragma unroll
             for (int t = 0; t < MT; t++) acc[q][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
-        for (int ks = 0; ks < NK; ks++) {
-            const int tap = ks * TPS + tsub;
-            const bool live = tap < 27;
-            const int dd = tap / 9 - 1, dh = (tap / 3) % 3 - 1, dw = tap % 3 - 1;
-            bf16x8 afr[MT], bfr[RW];
+        R F0[RW], F1[RW];
+        bool ok0[RW], ok1[RW];
+        tap_load<CIN, RW, PER>(F0, ok0, X, m, 0, b, d, hb, w, c0, safe);
+        // taps in pairs: tap t from F0 while t+1 loads into F1, then t+1
+        // from F1 while t+2 loads into F0; fully unrolled, so that every wait
+        // for a tap's loads leaves the next tap's loads in flight
 #pragma unroll
-            for (int t = 0; t < MT; t++)
-                afr[t] = live ? *reinterpret_cast<const bf16x8 *>(&sw[((16 * t + n) * 27 + tap) * CIN + c0]) : zero8();
+        for (int t = 0; t < 27; t += 2) {
+            if (t + 1 < 27) tap_load<CIN, RW, PER>(F1, ok1, X, m, t + 1, b, d, hb, w, c0, safe);
+            {
+                R af[MT];
 #pragma unroll
-            for (int q = 0; q < RW; q++)
-                bfr[q] = live ? frag(X, CIN, m, b, d + dd, hb + q + dh, w0 + n + dw, c0, per) : zero8();
+                for (int u = 0; u < MT; u++) af[u] = *reinterpret_cast<const R *>(&sw[(16 * u + n) * WROW + t * CIN + c0]);
 #pragma unroll
-            for (int q = 0; q < RW; q++)
+                for (int q = 0; q < RW; q++) {
+                    const R bq = K::sel(ok0[q], F0[q]);
 #pragma unroll
-                for (int t = 0; t < MT; t++)
-                    acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[t], bfr[q], acc[q][t], 0, 0, 0);
+                    for (int u = 0; u < MT; u++) acc[q][u] = K::mfma(af[u], bq, acc[q][u]);
+                }
+            }
+            if (t + 1 < 27) {
+                if (t + 2 < 27) tap_load<CIN, RW, PER>(F0, ok0, X, m, t + 2, b, d, hb, w, c0, safe);
+                R af[MT];
+#pragma unroll
+                for (int u = 0; u < MT; u++)
+                    af[u] = *reinterpret_cast<const R *>(&sw[(16 * u + n) * WROW + (t + 1) * CIN + c0]);
+#pragma unroll
+                for (int q = 0; q < RW; q++) {
+                    const R bq = K::sel(ok1[q], F1[q]);
+#pragma unroll
+                    for (int u = 0; u < MT; u++) acc[q][u] = K::mfma(af[u], bq, acc[q][u]);
+                }
+            }
         }
 #pragma unroll
         for (int q = 0; q < RW; q++) {
             if (hb + q >= m.H) break;
-            const long v = (((long)b * m.D + d) * m.H + hb + q) * m.W + w0 + n;
+            const long v = (((long)b * m.D + d) * m.H + hb + q) * m.W + w;
 #pragma unroll
-            for (int t = 0; t < MT; t++)
+            for (int u = 0; u < MT; u++) {
+                const int cb = 16 * u + 4 * kq;   // channels cb .. cb+3 of this lane
+                float z[4];
+                unsigned short a16[4];
 #pragma unroll
                 for (int rr = 0; rr < 4; rr++) {
-                    const int co = 16 * t + 4 * kq + rr;
-                    const bool on = co < cout;
-                    const float z = on ? acc[q][t][rr] + (bias ? bias[co] : 0.0f) : 0.0f;
-                    if (Z) Z[v * COUT + co] = z;
-                    if (A) A[v * COUT + co] = on ? f2bf(ACT ? gelu(z) : z) : (u16)0;
+                    const bool on = cb + rr < cout;
+                    z[rr] = on ? acc[q][u][rr] + (bias ? bias[cb + rr] : 0.0f) : 0.0f;
+                    a16[rr] = on ? f2bf(ACT ? gelu(z[rr]) : z[rr]) : (u16)0;
                 }
+                if (Z) *reinterpret_cast<float4 *>(Z + v * COUT + cb) = make_float4(z[0], z[1], z[2], z[3]);
+                if (A)
+                    *reinterpret_cast<uint2 *>(A + v * COUT + cb) =
+                        make_uint2((unsigned)a16[0] | ((unsigned)a16[1] << 16), (unsigned)a16[2] | ((unsigned)a16[3] << 16));
+            }
         }
     }
 }
@@ -499,7 +585,7 @@ __global__ void k_guess(Geo g, C5 U, C5 Uq, const double *__restrict__ J, const 
 // ---------------------------------------------------------------- launchers
 static unsigned blocks_of(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
-constexpr int CONV_RW = 4;
+constexpr int CONV_RW = 4;   // rows per wave item
 
 static int conv_blocks(long nitems)
 {
@@ -514,27 +600,35 @@ static int conv_blocks(long nitems)
     return (int)std::max(1L, std::min<long>(want, 2L * ncu));
 }
 
+template <bool PER>
+static int conv_launch(int cin_pad, int cout_pad, bool act, const u16 *x, const u16 *w, const float *bias, float *Z,
+                       u16 *a, const NetDims &m, int cout, long nitems, hipStream_t s)
+{
+    const dim3 gr(conv_blocks(nitems)), bl(256);
+    if (cin_pad == 16 && cout_pad == 32 && act)
+        hipLaunchKernelGGL((k_conv<16, 2, true, CONV_RW, PER>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems);
+    else if (cin_pad == 32 && cout_pad == 32 && act)
+        hipLaunchKernelGGL((k_conv<32, 2, true, CONV_RW, PER>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems);
+    else if (cin_pad == 32 && cout_pad == 16 && !act)
+        hipLaunchKernelGGL((k_conv<32, 1, false, CONV_RW, PER>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems);
+    else if (cin_pad == 16 && cout_pad == 32 && !act)
+        hipLaunchKernelGGL((k_conv<16, 2, false, CONV_RW, PER>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems);
+    else if (cin_pad == 32 && cout_pad == 32 && !act)
+        hipLaunchKernelGGL((k_conv<32, 2, false, CONV_RW, PER>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems);
+    else
+        return -1;
+    return 0;
+}
+
 int net_conv(int cin_pad, int cout_pad, bool act, const void *X, const void *W, const float *bias, float *Z, void *A,
              const NetDims &m, int cout, int per, hipStream_t s)
 {
     if (m.W % 16) return -1;
     const long nitems = (long)m.B * m.D * ((m.H + CONV_RW - 1) / CONV_RW) * (m.W / 16);
-    const dim3 gr(conv_blocks(nitems)), bl(256);
     const u16 *x = static_cast<const u16 *>(X), *w = static_cast<const u16 *>(W);
     u16 *a = static_cast<u16 *>(A);
-    if (cin_pad == 16 && cout_pad == 32 && act)
-        hipLaunchKernelGGL((k_conv<16, 2, true, CONV_RW>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems, per);
-    else if (cin_pad == 32 && cout_pad == 32 && act)
-        hipLaunchKernelGGL((k_conv<32, 2, true, CONV_RW>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems, per);
-    else if (cin_pad == 32 && cout_pad == 16 && !act)
-        hipLaunchKernelGGL((k_conv<32, 1, false, CONV_RW>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems, per);
-    else if (cin_pad == 16 && cout_pad == 32 && !act)
-        hipLaunchKernelGGL((k_conv<16, 2, false, CONV_RW>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems, per);
-    else if (cin_pad == 32 && cout_pad == 32 && !act)
-        hipLaunchKernelGGL((k_conv<32, 2, false, CONV_RW>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems, per);
-    else
-        return -1;
-    return 0;
+    return per ? conv_launch<true>(cin_pad, cout_pad, act, x, w, bias, Z, a, m, cout, nitems, s)
+               : conv_launch<false>(cin_pad, cout_pad, act, x, w, bias, Z, a, m, cout, nitems, s);
 }
 
 void net_gelu_back(const float *dA, const float *Z, void *dZ, float *dZf, long n, hipStream_t s)
