@@ -386,7 +386,7 @@ int cgli_filter(cgli_ctx *c, double Roe_criterion, double E_high, double E_low)
     }
     tbegin(c, T_FILTER);
     launch_filter(c->g, Roe_criterion, E_high, E_low, c5(c, CGLI_U_), c->f[CGLI_J], c->f[CGLI_YPOINT], c->filt,
-                  c->f[CGLI_EPX], c->f[CGLI_EPY], c->f[CGLI_EPZ], c->stream);
+                  c->cfg.precision == 1, c->f[CGLI_EPX], c->f[CGLI_EPY], c->f[CGLI_EPZ], c->stream);
     tend(c, T_FILTER);
     HIPCHK(hipGetLastError());
     mark(c, CGLI_EPX, 3);

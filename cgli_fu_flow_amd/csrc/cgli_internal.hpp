@@ -80,8 +80,9 @@ void launch_copy_field(double *dst, const double *src, long n, hipStream_t s);
 
 // filter sensor (filter.hip); work holds filter_scratch_doubles(g) doubles
 long filter_scratch_doubles(const Geo &g);
+// fp32: the passes, energies and sensor ratio in FP32 (work then holds floats)
 void launch_filter(const Geo &g, double crit, double E_high, double E_low, C5 U_, const double *J,
-                   const double *Y, double *work, double *EpX, double *EpY, double *EpZ, hipStream_t s);
+                   const double *Y, double *work, int fp32, double *EpX, double *EpY, double *EpZ, hipStream_t s);
 
 // wall-normal profiles (stats.hip): out = 11 x ny doubles on the device
 void launch_profiles(const Geo &g, const Phys &p, C5 U, const double *J, double *out, hipStream_t s);

@@ -15,12 +15,18 @@
 // bit-identical with the reference built by g++ on x86 (no FMA
 // contraction: the library is compiled with -ffp-contract=off).
 // Bytes per pass: 4 fields in, 4 out (the y pass adds J and Y_point).
+// Every kernel is templated on the arithmetic type Real: FP64, or FP32 for
+// the mixed-precision path (work arrays, passes, energies and the sensor
+// ratio in float; U_, J and Y_point read as FP64 and rounded once; the
+// E_high/E_low outputs are FP64 fields either way).
 #include "cgli_internal.hpp"
 
 namespace cgli {
 
-struct L4 { double *v[4]; };
-struct R4 { const double *v[4]; };
+template <typename T> struct L4T { T *v[4]; };
+template <typename T> struct R4T { const T *v[4]; };
+typedef L4T<double> L4;
+typedef R4T<double> R4;
 
 // x range of level l (0..5): Filter.cpp:149-153 (levels 0, 3), 253-257
 // (levels 1, 4), 348-349 (levels 2, 5)
@@ -34,7 +40,8 @@ static void level_range(const Geo &g, int lvl, int &lo, int &hi)
 // every work array starts as U_ on i=0..gend+3, j=1..NYY, k=1..NZZ
 // (Filter.cpp:117-134); only their ghost shells are read before being
 // overwritten, but the whole box is copied like the reference does
-__global__ void k_filt_init(Geo g, R4 U, double *__restrict__ work, long N)
+template <typename Real>
+__global__ void k_filt_init(Geo g, R4 U, Real *__restrict__ work, long N)
 {
     const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -43,14 +50,15 @@ __global__ void k_filt_init(Geo g, R4 U, double *__restrict__ work, long N)
     const long c = ix(g, i, j, k);
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const double u = U.v[v][c];
+        const Real u = (Real)U.v[v][c];
 #pragma unroll
         for (int a = 0; a < 8; a++) work[(a * 4 + v) * N + c] = u;
     }
 }
 
-template <bool TWOH>
-__global__ void k_filt_x(Geo g, int lo, R4 in, L4 out)
+// In: the level's input type (U_ itself, FP64, at level 0)
+template <bool TWOH, typename Real, typename In>
+__global__ void k_filt_x(Geo g, int lo, R4T<In> in, L4T<Real> out)
 {
     const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -59,38 +67,40 @@ __global__ void k_filt_x(Geo g, int lo, R4 in, L4 out)
     const long c = ix(g, i, j, k), sx = g.Ym * g.Zm;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const double *u = in.v[v];
+        const In *u = in.v[v];
+        const Real um = (Real)u[c - sx], u0 = (Real)u[c], up = (Real)u[c + sx];
         if (!TWOH)
-            out.v[v][c] = 0.125 * (u[c - sx] + 6 * u[c] + u[c + sx]);
+            out.v[v][c] = Real(0.125) * (um + 6 * u0 + up);
         else
-            out.v[v][c] = 0.25 * (u[c - sx] + 2 * u[c] + u[c + sx]);
+            out.v[v][c] = Real(0.25) * (um + 2 * u0 + up);
     }
 }
 
-template <bool TWOH>
-__global__ void k_filt_y(Geo g, int lo, R4 gx, L4 gy, const double *__restrict__ J, const double *__restrict__ Y)
+template <bool TWOH, typename Real>
+__global__ void k_filt_y(Geo g, int lo, R4T<Real> gx, L4T<Real> gy, const double *__restrict__ J,
+                         const double *__restrict__ Y)
 {
     const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
     const int i = lo + blockIdx.z;
     if (k > g.NZ || j > g.NY) return;
     const long c = ix(g, i, j, k), jm = c - g.Zm, jp = c + g.Zm;
-    const double a = Y[jp] - Y[jm];
-    const double b = Y[jp] - Y[c];
-    const double cc = Y[c] - Y[jm];
-    const double Jm = J[jm], Jc = J[c], Jp = J[jp];
+    const Real a = (Real)(Y[jp] - Y[jm]);
+    const Real b = (Real)(Y[jp] - Y[c]);
+    const Real cc = (Real)(Y[c] - Y[jm]);
+    const Real Jm = (Real)J[jm], Jc = (Real)J[c], Jp = (Real)J[jp];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const double *x = gx.v[v];
+        const Real *x = gx.v[v];
         if (!TWOH)
-            gy.v[v][c] = 0.125 * (2 * b / a * x[jm] * Jm + 6 * x[c] * Jc + 2 * cc / a * x[jp] * Jp) / Jc;
+            gy.v[v][c] = Real(0.125) * (2 * b / a * x[jm] * Jm + 6 * x[c] * Jc + 2 * cc / a * x[jp] * Jp) / Jc;
         else
-            gy.v[v][c] = 0.5 * (b / a * x[jm] * Jm + x[c] * Jc + cc / a * x[jp] * Jp) / Jc;
+            gy.v[v][c] = Real(0.5) * (b / a * x[jm] * Jm + x[c] * Jc + cc / a * x[jp] * Jp) / Jc;
     }
 }
 
-template <bool TWOH>
-__global__ void k_filt_z(Geo g, int lo, R4 gy, L4 out)
+template <bool TWOH, typename Real>
+__global__ void k_filt_z(Geo g, int lo, R4T<Real> gy, L4T<Real> out)
 {
     const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -99,16 +109,17 @@ __global__ void k_filt_z(Geo g, int lo, R4 gy, L4 out)
     const long c = ix(g, i, j, k);
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const double *y = gy.v[v];
+        const Real *y = gy.v[v];
         if (!TWOH)
-            out.v[v][c] = 0.125 * (y[c - 1] + 6 * y[c] + y[c + 1]);
+            out.v[v][c] = Real(0.125) * (y[c - 1] + 6 * y[c] + y[c + 1]);
         else
-            out.v[v][c] = 0.25 * (y[c - 1] + 2 * y[c] + y[c + 1]);
+            out.v[v][c] = Real(0.25) * (y[c - 1] + 2 * y[c] + y[c + 1]);
     }
 }
 
 // periodic image of the three planes at each x end
-__global__ void k_filt_exch(Geo g, L4 G)
+template <typename Real>
+__global__ void k_filt_exch(Geo g, L4T<Real> G)
 {
     const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -116,36 +127,38 @@ __global__ void k_filt_exch(Geo g, L4 G)
     if (k > g.NZZ || j > g.NYY) return;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        double *a = G.v[v];
+        Real *a = G.v[v];
         a[ix(g, g.gend + 1 + q, j, k)] = a[ix(g, 3 + q, j, k)];
         a[ix(g, q, j, k)] = a[ix(g, g.gend - 2 + q, j, k)];
     }
 }
 
 // work: level l, variable v at (l*4 + v)*N; E over i=3..gend
-__global__ void k_filt_energy(Geo g, R4 U, const double *__restrict__ work, long N, double *__restrict__ E)
+template <typename Real>
+__global__ void k_filt_energy(Geo g, R4 U, const Real *__restrict__ work, long N, Real *__restrict__ E)
 {
     const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
     const int i = 3 + blockIdx.z;
     if (k > g.NZ || j > g.NY) return;
     const long c = ix(g, i, j, k);
-    double f[4];
+    Real f[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const double *w = work + v * N + c;
+        const Real *w = work + v * N + c;
         f[v] = 6 * w[0] - 15 * w[4 * N] + 20 * w[8 * N] - 15 * w[12 * N] + 6 * w[16 * N] - w[20 * N];
     }
-    const double U1 = U.v[0][c];
-    const double Ue = U.v[1][c] / U1 - f[1] / f[0];
-    const double Ve = U.v[2][c] / U1 - f[2] / f[0];
-    const double We = U.v[3][c] / U1 - f[3] / f[0];
+    const Real U1 = (Real)U.v[0][c];
+    const Real Ue = (Real)U.v[1][c] / U1 - f[1] / f[0];
+    const Real Ve = (Real)U.v[2][c] / U1 - f[2] / f[0];
+    const Real We = (Real)U.v[3][c] / U1 - f[3] / f[0];
     E[c] = Ue * Ue + Ve * Ve + We * We;
 }
 
 // ghost cells of E (both scales): x ends when first/last (periodic images
 // for a periodic slab), then the j and k ghost layers of the interior i
-__global__ void k_filt_eghost(Geo g, double *__restrict__ Eh, double *__restrict__ E2h)
+template <typename Real>
+__global__ void k_filt_eghost(Geo g, Real *__restrict__ Eh, Real *__restrict__ E2h)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int ny = g.NY - 1, nz = g.NZ - 1, nxi = g.nx;
@@ -165,16 +178,16 @@ __global__ void k_filt_eghost(Geo g, double *__restrict__ Eh, double *__restrict
             Eh[lo] = Eh[src];
             E2h[lo] = E2h[src];
         } else {
-            Eh[lo] = 0.0;
-            E2h[lo] = 0.0;
+            Eh[lo] = Real(0);
+            E2h[lo] = Real(0);
         }
         if (g.last || g.xper) {
             const long src = ix(g, (g.xper && !g.last) ? 3 : g.gend, j, k);
             Eh[hi] = Eh[src];
             E2h[hi] = E2h[src];
         } else {
-            Eh[hi] = 0.0;
-            E2h[hi] = 0.0;
+            Eh[hi] = Real(0);
+            E2h[hi] = Real(0);
         }
     } else if (t < nx_part + ny_part) {
         const int u = t - nx_part;
@@ -195,7 +208,8 @@ __global__ void k_filt_eghost(Geo g, double *__restrict__ Eh, double *__restrict
 
 // one thread per (i,j,k) of the box i=2..gend, j=1..NY, k=1..NZ; each face
 // family writes where its reference loop does
-__global__ void k_filt_sensor(Geo g, const double *__restrict__ Eh, const double *__restrict__ E2h, double crit,
+template <typename Real>
+__global__ void k_filt_sensor(Geo g, const Real *__restrict__ Eh, const Real *__restrict__ E2h, Real crit,
                               double E_high, double E_low, double *__restrict__ EpX, double *__restrict__ EpY,
                               double *__restrict__ EpZ)
 {
@@ -206,62 +220,80 @@ __global__ void k_filt_sensor(Geo g, const double *__restrict__ Eh, const double
     const long c = ix(g, i, j, k), sx = g.Ym * g.Zm, sy = g.Zm;
     if (j < g.NY && k < g.NZ) {   // Filter.cpp:1564-1583
         const long a = c + sy + 1, b = a + sx;
-        const double Emax = (Eh[a] + Eh[b]) / (E2h[a] + E2h[b]);
+        const Real Emax = (Eh[a] + Eh[b]) / (E2h[a] + E2h[b]);
         EpX[c] = (Emax >= crit) ? E_high : E_low;
     }
     if (i <= g.gend - 1 && k < g.NZ) {   // Filter.cpp:1593-1614
         const long a = c + sx + 1, b = a + sy;
-        const double Emax = (Eh[a] + Eh[b]) / (E2h[a] + E2h[b]);
+        const Real Emax = (Eh[a] + Eh[b]) / (E2h[a] + E2h[b]);
         EpY[c] = (Emax >= crit) ? E_high : E_low;
     }
     if (i <= g.gend - 1 && j < g.NY) {   // Filter.cpp:1624-1643
         const long a = c + sx + sy, b = a + 1;
-        const double Emax = (Eh[a] + Eh[b]) / (E2h[a] + E2h[b]);
+        const Real Emax = (Eh[a] + Eh[b]) / (E2h[a] + E2h[b]);
         EpZ[c] = (Emax >= crit) ? E_high : E_low;
     }
 }
 
 long filter_scratch_doubles(const Geo &g) { return 34 * g.N; }
 
-// work (34 N doubles): levels 0..5 x 4 variables, Gx (4), Gy (4), E_h, E_2h
-void launch_filter(const Geo &g, double crit, double E_high, double E_low, C5 U_, const double *J,
-                   const double *Y, double *work, double *EpX, double *EpY, double *EpZ, hipStream_t s)
+// work (34 N doubles; the FP32 path uses the first half as 34 N floats):
+// levels 0..5 x 4 variables, Gx (4), Gy (4), E_h, E_2h
+template <typename Real>
+static void filter_run(const Geo &g, double crit, double E_high, double E_low, C5 U_, const double *J,
+                       const double *Y, Real *work, double *EpX, double *EpY, double *EpZ, hipStream_t s)
 {
     const long N = g.N;
     const R4 U{{U_.v[0], U_.v[1], U_.v[2], U_.v[3]}};
     auto lvl = [&](int l) {
-        return L4{{work + (l * 4 + 0) * N, work + (l * 4 + 1) * N, work + (l * 4 + 2) * N, work + (l * 4 + 3) * N}};
+        return L4T<Real>{{work + (l * 4 + 0) * N, work + (l * 4 + 1) * N, work + (l * 4 + 2) * N,
+                          work + (l * 4 + 3) * N}};
     };
-    auto cst = [](const L4 &a) { return R4{{a.v[0], a.v[1], a.v[2], a.v[3]}}; };
-    const L4 Gx = lvl(6), Gy = lvl(7);
-    double *Eh = work + 32 * N, *E2h = work + 33 * N;
+    auto cst = [](const L4T<Real> &a) { return R4T<Real>{{a.v[0], a.v[1], a.v[2], a.v[3]}}; };
+    const L4T<Real> Gx = lvl(6), Gy = lvl(7);
+    Real *Eh = work + 32 * N, *E2h = work + 33 * N;
     dim3 b(64, 4);
-    hipLaunchKernelGGL(k_filt_init, dim3((g.NZZ + 63) / 64, (g.NYY + 3) / 4, g.gend + 4), b, 0, s, g, U, work, N);
+    hipLaunchKernelGGL(k_filt_init<Real>, dim3((g.NZZ + 63) / 64, (g.NYY + 3) / 4, g.gend + 4), b, 0, s, g, U, work,
+                       N);
     for (int sc = 0; sc < 2; sc++) {
         for (int l = 0; l < 6; l++) {
             int lo, hi;
             level_range(g, l, lo, hi);
             dim3 gr((g.NZ - 1 + 63) / 64, (g.NY - 1 + 3) / 4, hi - lo + 1);
-            const R4 in = l == 0 ? U : cst(lvl(l - 1));
             if (sc == 0) {
-                hipLaunchKernelGGL(k_filt_x<false>, gr, b, 0, s, g, lo, in, Gx);
-                hipLaunchKernelGGL(k_filt_y<false>, gr, b, 0, s, g, lo, cst(Gx), Gy, J, Y);
-                hipLaunchKernelGGL(k_filt_z<false>, gr, b, 0, s, g, lo, cst(Gy), lvl(l));
+                if (l == 0)
+                    hipLaunchKernelGGL((k_filt_x<false, Real, double>), gr, b, 0, s, g, lo, U, Gx);
+                else
+                    hipLaunchKernelGGL((k_filt_x<false, Real, Real>), gr, b, 0, s, g, lo, cst(lvl(l - 1)), Gx);
+                hipLaunchKernelGGL((k_filt_y<false, Real>), gr, b, 0, s, g, lo, cst(Gx), Gy, J, Y);
+                hipLaunchKernelGGL((k_filt_z<false, Real>), gr, b, 0, s, g, lo, cst(Gy), lvl(l));
             } else {
-                hipLaunchKernelGGL(k_filt_x<true>, gr, b, 0, s, g, lo, in, Gx);
-                hipLaunchKernelGGL(k_filt_y<true>, gr, b, 0, s, g, lo, cst(Gx), Gy, J, Y);
-                hipLaunchKernelGGL(k_filt_z<true>, gr, b, 0, s, g, lo, cst(Gy), lvl(l));
+                if (l == 0)
+                    hipLaunchKernelGGL((k_filt_x<true, Real, double>), gr, b, 0, s, g, lo, U, Gx);
+                else
+                    hipLaunchKernelGGL((k_filt_x<true, Real, Real>), gr, b, 0, s, g, lo, cst(lvl(l - 1)), Gx);
+                hipLaunchKernelGGL((k_filt_y<true, Real>), gr, b, 0, s, g, lo, cst(Gx), Gy, J, Y);
+                hipLaunchKernelGGL((k_filt_z<true, Real>), gr, b, 0, s, g, lo, cst(Gy), lvl(l));
             }
             if (l == 2 && g.xper)
-                hipLaunchKernelGGL(k_filt_exch, dim3((g.NZZ + 63) / 64, (g.NYY + 3) / 4, 3), b, 0, s, g, lvl(2));
+                hipLaunchKernelGGL(k_filt_exch<Real>, dim3((g.NZZ + 63) / 64, (g.NYY + 3) / 4, 3), b, 0, s, g, lvl(2));
         }
-        hipLaunchKernelGGL(k_filt_energy, dim3((g.NZ - 1 + 63) / 64, (g.NY - 1 + 3) / 4, g.nx), b, 0, s, g, U,
+        hipLaunchKernelGGL(k_filt_energy<Real>, dim3((g.NZ - 1 + 63) / 64, (g.NY - 1 + 3) / 4, g.nx), b, 0, s, g, U,
                            work, N, sc == 0 ? Eh : E2h);
     }
     const int nt = (g.NY - 1) * (g.NZ - 1) + g.nx * (g.NZ - 1) + g.nx * (g.NY - 1);
-    hipLaunchKernelGGL(k_filt_eghost, dim3((nt + 255) / 256), dim3(256), 0, s, g, Eh, E2h);
-    hipLaunchKernelGGL(k_filt_sensor, dim3((g.NZ + 63) / 64, (g.NY + 3) / 4, g.gend - 1), b, 0, s, g, Eh, E2h, crit,
-                       E_high, E_low, EpX, EpY, EpZ);
+    hipLaunchKernelGGL(k_filt_eghost<Real>, dim3((nt + 255) / 256), dim3(256), 0, s, g, Eh, E2h);
+    hipLaunchKernelGGL(k_filt_sensor<Real>, dim3((g.NZ + 63) / 64, (g.NY + 3) / 4, g.gend - 1), b, 0, s, g, Eh, E2h,
+                       (Real)crit, E_high, E_low, EpX, EpY, EpZ);
+}
+
+void launch_filter(const Geo &g, double crit, double E_high, double E_low, C5 U_, const double *J,
+                   const double *Y, double *work, int fp32, double *EpX, double *EpY, double *EpZ, hipStream_t s)
+{
+    if (fp32)
+        filter_run<float>(g, crit, E_high, E_low, U_, J, Y, reinterpret_cast<float *>(work), EpX, EpY, EpZ, s);
+    else
+        filter_run<double>(g, crit, E_high, E_low, U_, J, Y, work, EpX, EpY, EpZ, s);
 }
 
 }  // namespace cgli

@@ -43,7 +43,7 @@ typedef struct {
     int sponge;              /* 1: absorbing-zone sources of Dual_time_stepping.cpp:532-664 */
     int xi_mode;             /* Flux_Y/Z pressure-jump normal: 0 = as the reference (none), 1 = face normal */
     int x_periodic;          /* 1: one periodic slab: x ghosts filled periodically, zero implicit x ghosts */
-    int precision;           /* 0: FP64 residual kernels, 1: FP32 residual kernels (FP64 update) */
+    int precision;           /* 0: FP64 residual kernels, 1: FP32 residual and filter kernels (FP64 update) */
     double dxi, deta, dzeta; /* deltaXI, deltaET, deltaZT */
     double high;             /* prm.h:16 */
     double K, R;             /* ratio of specific heats, gas constant */
@@ -107,6 +107,9 @@ int cgli_viscous(cgli_ctx *ctx);
  * kinetic-energy ratio is >= Roe_criterion, else E_low (the reference's
  * defaults are 0.007, 1, 0.1, main.cpp:60-62).  Needs nx >= 3. */
 int cgli_filter(cgli_ctx *ctx, double Roe_criterion, double E_high, double E_low);
+/* (with cfg.precision = 1 the filter passes, energies and sensor ratio run
+ * in FP32; faces whose ratio lies within FP32 rounding of the criterion may
+ * then take the other value) */
 /* Statistic.cpp:130-164: the mean and raw second-moment profiles U V W T
  * UU VV WW TT UV UT VT of U_, averaged over x and z (the channel's
  * homogeneous directions) for every interior j; out[11*ny] on the host,

@@ -130,6 +130,55 @@ def test_channel_filter_sensor_matches_oracle():
             print("crit", crit, "fraction high", [float(np.mean(got[d][wins[d]] == 1.0)) for d in range(3)])
 
 
+def test_fp32_filter_decisions():
+    """With FP32 filter kernels the sensor decisions equal the FP64 oracle's
+    except on faces whose ratio lies within FP32 rounding of the criterion.
+    The filtered field is a six-level combination with coefficients up to
+    20 whose result is compared with u itself, so FP32 rounding reaches the
+    ratio amplified (observed 1.2e-4 relative): every differing face has
+    |ratio/criterion - 1| <= 1e-3, and they are few."""
+    from cgli_fu_flow_amd import native as nv
+    from oracle_harness import Slab, run_filter, ptrs, dp
+    orc = load_oracle()
+    ch = nv.channel(180.0, beta=2.2, skew=0.05)
+    cfg = nv.channel_config(ch, 16, 24, 16)
+    cfg.precision = 1
+    with nv.Context(cfg) as ctx:
+        dt = ctx.channel_init(ch)
+        for _ in range(2):
+            ctx.channel_step(dt, 1e-6, 40)
+        ctx.channel_ghosts()
+        f = {"U_": ctx.get_many(nv.U_, 5), "J": ctx.get(nv.J), "Y_point": ctx.get(nv.YPOINT)}
+        oc = OrcCfg()
+        for name, _ in OrcCfg._fields_:
+            setattr(oc, name, getattr(cfg, name))
+        case = SimpleNamespace(slab=Slab(cfg.nx, cfg.ny, cfg.nz), cfg=lambda lib: oc)
+        Eh = np.zeros(ctx.shape)
+        E2h = np.zeros(ctx.shape)
+        orc.orc_filter_energy(C.byref(oc), ptrs(f["U_"]), dp(f["Y_point"]), dp(f["J"]), dp(Eh), dp(E2h))
+        g = cfg.nx + 2
+        NY, NZ = cfg.ny + 1, cfg.nz + 1
+        wins = [(slice(2, g + 1), slice(1, NY), slice(1, NZ)), (slice(2, g), slice(1, NY + 1), slice(1, NZ)),
+                (slice(2, g), slice(1, NY), slice(1, NZ + 1))]
+        # the two cells each face family compares (Filter.cpp:1564-1643)
+        pairs = [((slice(2, g + 1), slice(2, NY + 1), slice(2, NZ + 1)), (slice(3, g + 2), slice(2, NY + 1), slice(2, NZ + 1))),
+                 ((slice(3, g + 1), slice(1, NY + 1), slice(2, NZ + 1)), (slice(3, g + 1), slice(2, NY + 2), slice(2, NZ + 1))),
+                 ((slice(3, g + 1), slice(2, NY + 1), slice(1, NZ + 1)), (slice(3, g + 1), slice(2, NY + 1), slice(2, NZ + 2)))]
+        for crit in (0.007, 0.05):
+            want = run_filter("orc", orc, case, f, crit)
+            ctx.filter(crit, 1.0, 0.1)
+            got = ctx.get_many(nv.EPX, 3)
+            for d in range(3):
+                a, b = pairs[d]
+                ratio = (Eh[a] + Eh[b]) / (E2h[a] + E2h[b])
+                diff = got[d][wins[d]] != want[d][wins[d]]
+                near = np.abs(ratio / crit - 1.0) <= 1e-3
+                print("crit", crit, "family", d, "differing faces", int(diff.sum()), "of", diff.size,
+                      "ratio/crit there", (ratio[diff] / crit).tolist())
+                assert np.all(near[diff])
+                assert diff.sum() <= max(2, 1e-3 * diff.size)
+
+
 def test_fp32_residuals_keep_the_mean_profiles():
     """The channel in the mixed mode (FP32 residual kernels until the drop
     nears the FP32 floor, then FP64; FP32 off-diagonal sweep; FP64 update,
