@@ -14,7 +14,7 @@ using namespace cgli;
 
 namespace {
 
-enum { T_FLUX_X, T_FLUX_Y, T_FLUX_Z, T_VISCOUS, T_RHS, T_FWD, T_BWD, T_UPDATE, T_COEF, T_FILTER, T_NCLASS };
+enum { T_FLUX_X, T_FLUX_Y, T_FLUX_Z, T_VISCOUS, T_RHS, T_FWD, T_BWD, T_UPDATE, T_COEF, T_FILTER, T_SCATTER, T_NCLASS };
 
 struct EventPair { hipEvent_t a, b; };
 
@@ -36,10 +36,16 @@ struct cgli_ctx {
     int *ijk_dev = nullptr;
     std::vector<int> Nijk;       // (gend+NY+NZ+8) x 2
     std::vector<int> ijk;        // (cells+1) x 3
-    int lusgs_mode = 1;          // 0: one launch per hyperplane, 1: tiled wavefront, 2: with loader waves
+    int lusgs_mode = 3;          // 0: one launch per hyperplane, 1: tiled wavefront, 2: with loader waves,
+                                 // 3: tiled wavefront on march-ordered records (default)
     double *coef = nullptr;      // split-coefficient blocks of the wavefront (float in the mixed mode)
     double *dgn = nullptr;       // diagonal blocks of the wavefront (FP64)
-    double *xbuf = nullptr;      // 5 fields: tile-edge exchange of the wavefront
+    void *coefM = nullptr;       // the same records in march order (mode 3)
+    double *dgnM = nullptr;
+    double *marchF = nullptr;    // mode 3: MR, NL and NR in march order
+    int *morder_dev = nullptr;   // mode 3: tile order of its tiles
+    int mtiles = 0;
+    double *xbuf = nullptr;      // tile-edge exchange of the wavefront: 5 values per cell
     double *trash = nullptr;     // write-only scratch of the wavefront
     int *order_dev = nullptr;    // wavefront tile order
     int ntiles = 0;
@@ -240,6 +246,10 @@ int cgli_destroy(cgli_ctx *c)
     if (c->ijk_dev) hipFree(c->ijk_dev);
     if (c->coef) hipFree(c->coef);
     if (c->dgn) hipFree(c->dgn);
+    if (c->coefM) hipFree(c->coefM);
+    if (c->dgnM) hipFree(c->dgnM);
+    if (c->marchF) hipFree(c->marchF);
+    if (c->morder_dev) hipFree(c->morder_dev);
     if (c->xbuf) hipFree(c->xbuf);
     if (c->trash) hipFree(c->trash);
     if (c->filt) hipFree(c->filt);
@@ -490,20 +500,44 @@ static int finish_norm(cgli_ctx *c, int nb, double er[5], bool check_wave = fals
     return finite ? CGLI_OK : CGLI_E_NONFINITE;
 }
 
+// the wavefront's records (in the layout of the current schedule) and its
+// exchange buffers, allocated on first use
 static int wave_setup(cgli_ctx *c)
 {
-    if (c->coef) return CGLI_OK;
     void *p = nullptr;
-    if (hipMalloc(&p, sizeof(double) * coef_doubles(c->g)) != hipSuccess) return CGLI_E_NOMEM;
-    c->coef = static_cast<double *>(p);
-    if (hipMalloc(&p, sizeof(double) * diag_doubles(c->g)) != hipSuccess) return CGLI_E_NOMEM;
-    c->dgn = static_cast<double *>(p);
+    if (c->lusgs_mode == 3 && !march_fits(c->g, c->cfg.precision == 1)) return CGLI_E_ARG;
+    if (c->lusgs_mode == 3 && !c->coefM) {
+        // FP64 pairs are 16 bytes: the FP32 records use half
+        if (hipMalloc(&p, 16 * march_coef_pairs(c->g)) != hipSuccess) return CGLI_E_NOMEM;
+        c->coefM = p;
+        if (hipMalloc(&p, 16 * march_diag_pairs(c->g)) != hipSuccess) return CGLI_E_NOMEM;
+        c->dgnM = static_cast<double *>(p);
+        if (hipMalloc(&p, sizeof(double) * 3 * march_field_doubles(c->g)) != hipSuccess) return CGLI_E_NOMEM;
+        c->marchF = static_cast<double *>(p);
+        std::vector<int> order;
+        c->mtiles = wave_tiles(c->g, MARCH_TJ, MARCH_TK, order);
+        if (hipMalloc(&p, sizeof(int) * order.size()) != hipSuccess) return CGLI_E_NOMEM;
+        c->morder_dev = static_cast<int *>(p);
+        HIPCHK(hipMemcpy(c->morder_dev, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+    }
+    if (c->lusgs_mode != 3 && !c->coef) {
+        if (hipMalloc(&p, sizeof(double) * coef_doubles(c->g)) != hipSuccess) return CGLI_E_NOMEM;
+        c->coef = static_cast<double *>(p);
+        if (hipMalloc(&p, sizeof(double) * diag_doubles(c->g)) != hipSuccess) return CGLI_E_NOMEM;
+        c->dgn = static_cast<double *>(p);
+    }
+    if (c->xbuf) return CGLI_OK;
     if (hipMalloc(&p, sizeof(double) * 5 * c->g.N) != hipSuccess) return CGLI_E_NOMEM;
     c->xbuf = static_cast<double *>(p);
     HIPCHK(hipMemset(c->xbuf, 0, sizeof(double) * 5 * c->g.N));   // ghost entries stay zero
     std::vector<int> order;
     c->ntiles = wave_tiles(c->g, WAVE_TJ, WAVE_TK, order);
-    if (hipMalloc(&p, sizeof(double) * 256 * c->ntiles) != hipSuccess) return CGLI_E_NOMEM;
+    {
+        // one 256-double scratch line per workgroup of either wavefront
+        std::vector<int> morder;
+        const int mt = wave_tiles(c->g, MARCH_TJ, MARCH_TK, morder);
+        if (hipMalloc(&p, sizeof(double) * 256 * std::max(c->ntiles, mt)) != hipSuccess) return CGLI_E_NOMEM;
+    }
     c->trash = static_cast<double *>(p);
     if (hipMalloc(&p, sizeof(int) * order.size()) != hipSuccess) return CGLI_E_NOMEM;
     c->order_dev = static_cast<int *>(p);
@@ -517,7 +551,7 @@ static int wave_setup(cgli_ctx *c)
 
 int cgli_set_lusgs_mode(cgli_ctx *c, int mode)
 {
-    if (!c || mode < 0 || mode > 2) return CGLI_E_ARG;
+    if (!c || mode < 0 || mode > 3) return CGLI_E_ARG;
     c->lusgs_mode = mode;
     return CGLI_OK;
 }
@@ -575,12 +609,35 @@ int ctx_dts(cgli_ctx *c, double deltaT, double e, double er[5], int sweeps, int 
         if (r) return r;
         launch_reset_ints(c->counters, 2, s);
         tbegin(c, T_COEF);
-        launch_split_coef(g, c->p, deltaT, e, c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c->coef, c->dgn,
-                          c->cfg.precision == 1, s);
+        if (c->lusgs_mode == 3)
+            launch_split_march(g, c->p, deltaT, e, c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_MR),
+                               c->coefM, c->dgnM, c->marchF, c->cfg.precision == 1, s);
+        else
+            launch_split_coef(g, c->p, deltaT, e, c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c->coef, c->dgn,
+                              c->cfg.precision == 1, s);
         tend(c, T_COEF);
     }
     for (int sw = 0; sw < sweeps; sw++) {
-        if (c->lusgs_mode >= 1) {
+        if (c->lusgs_mode == 3) {
+            launch_lusgs_wave_prep(g, c->xbuf, c->counters, s);
+            tbegin(c, T_FWD);
+            const long mf = march_field_doubles(g);
+            launch_lusgs_march(g, 1, c->coefM, c->dgnM, c->cfg.precision == 1, c->marchF, c->marchF + mf,
+                               c5(c, CGLI_NL), c->xbuf, c->trash, c->morder_dev, c->mtiles, c->counters, c->max_spin,
+                               s);
+            tend(c, T_FWD);
+            launch_lusgs_wave_prep(g, c->xbuf, c->counters, s);
+            tbegin(c, T_BWD);
+            launch_lusgs_march(g, 0, c->coefM, c->dgnM, c->cfg.precision == 1, c->marchF + mf,
+                               c->marchF + 2 * mf, c5(c, CGLI_NR), c->xbuf, c->trash, c->morder_dev, c->mtiles,
+                               c->counters, c->max_spin, s);
+            tend(c, T_BWD);
+            if (sw == sweeps - 1) {
+                tbegin(c, T_SCATTER);
+                launch_march_scatter(g, c->marchF + mf, c->marchF + 2 * mf, f5(c, CGLI_NL), f5(c, CGLI_NR), s);
+                tend(c, T_SCATTER);
+            }
+        } else if (c->lusgs_mode >= 1) {
             launch_lusgs_wave_prep(g, c->xbuf, c->counters, s);
             tbegin(c, T_FWD);
             launch_lusgs_wave(g, 1, c->coef, c->dgn, c->cfg.precision == 1, c5(c, CGLI_MR), f5(c, CGLI_NL), c->xbuf, c->trash, c->order_dev,

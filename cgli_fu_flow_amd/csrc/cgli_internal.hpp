@@ -89,6 +89,12 @@ void launch_profiles(const Geo &g, const Phys &p, C5 U, const double *J, double 
 
 // wavefront LU-SGS (implicit.hip)
 constexpr int WAVE_TJ = 4, WAVE_TK = 16;
+// tile of the wavefront on march-ordered records (one wave: TJ*TK = 64)
+#ifndef CGLI_MARCH_TJ
+#define CGLI_MARCH_TJ 8
+#define CGLI_MARCH_TK 8
+#endif
+constexpr int MARCH_TJ = CGLI_MARCH_TJ, MARCH_TK = CGLI_MARCH_TK;
 int wave_tiles(const Geo &g, int tj, int tk, std::vector<int> &order);
 void launch_split_coef(const Geo &g, const Phys &p, double deltaT, double e, C5 U_, const double *J, M9 met,
                        void *coef, double *dgn, int fp32, hipStream_t s);
@@ -106,6 +112,20 @@ void launch_lusgs_wave(const Geo &g, int forward, const void *coef, const double
                        double *xbuf, double *trash, const int *order, int ntiles, int *counters, int max_spin,
                        int loaders, hipStream_t s);
 void launch_reset_ints(int *p, int n, hipStream_t s);
+// march layout (implicit.hip): coefficient records, right-hand sides and
+// increments in the order the wavefront reads them; sizes in CT pairs, FP64
+// pairs and doubles
+long march_coef_pairs(const Geo &g);
+long march_diag_pairs(const Geo &g);
+long march_field_doubles(const Geo &g);
+bool march_fits(const Geo &g, int fp32);
+void launch_split_march(const Geo &g, const Phys &p, double deltaT, double e, C5 U_, const double *J, M9 met,
+                        C5 MR, void *coefM, double *dgnM, double *mrM, int fp32, hipStream_t s);
+void launch_lusgs_march(const Geo &g, int forward, const void *coefM, const double *dgnM, int fp32,
+                        const double *Rm, double *Xm, C5 Xg, double *xbuf, double *trash, const int *order,
+                        int ntiles, int *counters, int max_spin, hipStream_t s);
+void launch_march_scatter(const Geo &g, const double *nlM, const double *nrM, F5 NL, F5 NR, hipStream_t s);
+
 
 // learned initial guess (cnn.hip): see that file for layouts
 struct NetDims { int B, D, H, W; };

@@ -761,7 +761,7 @@ __global__ void k_fill_sentinel(Geo g, double *__restrict__ xbuf)
     if (k > g.NZ || j > g.NY) return;
     const long c = ix(g, i, j, k);
 #pragma unroll
-    for (int v = 0; v < 5; v++) xbuf[v * g.N + c] = __longlong_as_double((long long)SENTINEL);
+    for (int v = 0; v < 5; v++) xbuf[5 * c + v] = __longlong_as_double((long long)SENTINEL);
 }
 
 __global__ void k_reset_ints(int *p, int n)
@@ -818,11 +818,11 @@ __device__ __forceinline__ unsigned long long aload(const double *p)
 }
 
 // issue the five exchange loads of one neighbour without waiting (the
-// exchange buffer is five fields of N cells)
+// exchange buffer holds the five values of a cell together: xbuf[5c + v])
 __device__ __forceinline__ void issue5(const double *xbuf, long N, long c, unsigned long long b[5])
 {
 #pragma unroll
-    for (int v = 0; v < 5; v++) b[v] = aload(xbuf + v * N + c);
+    for (int v = 0; v < 5; v++) b[v] = aload(xbuf + 5 * c + v);
 }
 
 // where `need`, re-poll the entries that still hold the sentinel (bounded)
@@ -835,7 +835,7 @@ __device__ __forceinline__ void settle5(const double *xbuf, long N, long c, bool
         int n = 0;
         while (need && b[v] == SENTINEL && n < max_spin) {
             __builtin_amdgcn_s_sleep(1);
-            b[v] = aload(xbuf + v * N + c);
+            b[v] = aload(xbuf + 5 * c + v);
             n++;
         }
         if (need && b[v] == SENTINEL) atomicOr(err, 1);
@@ -973,7 +973,7 @@ __device__ __forceinline__ void wave_step(const WaveArgs<CT> &w, const Lane &L, 
     if (act && L.edge_out) {
 #pragma unroll
         for (int v = 0; v < 5; v++)
-            __hip_atomic_store(reinterpret_cast<unsigned long long *>(w.xbuf + v * w.g.N + c),
+            __hip_atomic_store(reinterpret_cast<unsigned long long *>(w.xbuf + 5 * c + v),
                                (unsigned long long)__double_as_longlong(out[v]), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -1182,7 +1182,7 @@ __device__ __forceinline__ void wave_step_ld(const WaveArgs<CT> &w, const Lane &
     if (act && L.edge_out) {
 #pragma unroll
         for (int v = 0; v < 5; v++)
-            __hip_atomic_store(reinterpret_cast<unsigned long long *>(w.xbuf + v * w.g.N + c),
+            __hip_atomic_store(reinterpret_cast<unsigned long long *>(w.xbuf + 5 * c + v),
                                (unsigned long long)__double_as_longlong(out[v]), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -1369,6 +1369,498 @@ __global__ void __launch_bounds__(TJ *TK + 128) k_lusgs_wave_ld(Geo g, const CT 
         ld_group_step<4, TJ, TK, FWD, CT>(w, L, s0, S, wave, lane, il0, prev, sx, slot, set0, set1, set2, nb0, nb1, nc0, nc1);
         ld_group_step<5, TJ, TK, FWD, CT>(w, L, s0, S, wave, lane, il0, prev, sx, slot, set0, set1, set2, nb0, nb1, nc0, nc1);
     }
+}
+
+// ---------------------------------------------------------------- march layout
+// The tiled wavefront with its coefficients stored in the order the march
+// reads them.  Lane (jl, kl) of tile T works at step s on the cell whose
+// forward march index t = (i-3) + jl + kl equals s, so the 64 cells of a
+// tile at one step lie on a diagonal of (i, j, k): in any i-major layout
+// their records are 64 scattered pieces.  Here the record of the cell at
+// march position (T, t, lane) is stored in row T*S + t of the records, pair
+// p of it at [row][p][lane], so a tile reads one contiguous block per step.
+// The backward sweep runs the same schedule in reverse (t = S-1-s, tiles in
+// descending anti-diagonals), which visits the cells in a reverse
+// topological order, so it reads the same records.  Records are written for
+// interior cells only: where a neighbour is a ghost cell its increment is
+// zero and the cell's own record stands in for the ghost's coefficients.
+// The sweep's right-hand sides and increments are held in the same order
+// ([row][variable][lane]): the split kernel gathers MR into it, the forward
+// sweep writes NL there for the backward sweep, and a parallel kernel
+// scatters NL and NR back into the field layout afterwards.  A wave's loads
+// and stores are then contiguous blocks; the scattered accesses happen in
+// kernels that run every cell at once, where they cost bandwidth and not
+// latency on the dependent march.
+//
+// Every array has MPAD rows of padding before and after, so the rows a lane
+// reads at steps where it has no cell (its neighbours' rows shifted by up
+// to TJ+TK steps) are inside the allocation without clamping.  Addresses
+// are a wave-uniform row base plus a per-lane 32-bit offset fixed for the
+// whole sweep.
+struct March { int ntj, ntk, S; };
+
+constexpr int MPAD = MARCH_TJ + MARCH_TK + 2;
+
+// padded row of march position (T, t)
+__host__ __device__ inline long march_row(const March &M, int T, int t) { return (long)T * M.S + t + MPAD; }
+
+static March march_of(const Geo &g)
+{
+    March M;
+    M.ntj = (g.ny + MARCH_TJ - 1) / MARCH_TJ;
+    M.ntk = (g.nz + MARCH_TK - 1) / MARCH_TK;
+    M.S = g.nx + MARCH_TJ + MARCH_TK - 2;
+    return M;
+}
+
+static long march_rows(const Geo &g)
+{
+    const March M = march_of(g);
+    return (long)M.ntj * M.ntk * M.S + 2 * MPAD;
+}
+long march_coef_pairs(const Geo &g) { return march_rows(g) * NSPLIT * 64; }
+long march_diag_pairs(const Geo &g) { return march_rows(g) * 2 * 64; }
+long march_field_doubles(const Geo &g) { return march_rows(g) * 5 * 64; }
+
+// the per-lane record offsets of a sweep span ntk tiles of rows: they must
+// fit 32 bits
+bool march_fits(const Geo &g, int fp32)
+{
+    const March M = march_of(g);
+    const double rowb = (double)NSPLIT * 64 * 2 * (fp32 ? 4 : 8);
+    return ((double)(M.ntk + 1) * M.S + 4.0 * MPAD) * rowb < 4.0e9;
+}
+
+// one thread per march position (T, t, lane); blocks of 4 steps x 64 lanes
+template <typename CT>
+__global__ void __launch_bounds__(256) k_split_march(Geo g, Phys p, double deltaT, double e, C5 U_,
+                                                     const double *__restrict__ J, M9 met, C5 MR, March M,
+                                                     CT *__restrict__ coefM, double *__restrict__ dgnM,
+                                                     double *__restrict__ mrM)
+{
+    constexpr int TJ = MARCH_TJ, TK = MARCH_TK;
+    const int l = threadIdx.x & 63, t = blockIdx.x * 4 + (threadIdx.x >> 6), T = blockIdx.y;
+    if (t >= M.S) return;
+    const int jl = l / TK, kl = l % TK;
+    const int TJi = T / M.ntk, TKi = T % M.ntk;
+    const int jo = TJi * TJ + jl, ko = TKi * TK + kl, il = t - jl - kl;
+    if (il < 0 || il >= g.nx || jo >= g.ny || ko >= g.nz) return;
+    const long c = ix(g, 3 + il, 2 + jo, 2 + ko);
+    const long row = march_row(M, T, t);
+    typedef typename Pair2<CT>::t V2;
+    V2 *o = reinterpret_cast<V2 *>(coefM) + row * NSPLIT * 64 + l;
+    const CP q = cell_prim(U_, J, c, p.K, e);
+    const double dd[3] = {p.dxi, p.deta, p.dzeta};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const SC sp = split(q, met.m[3 * d][c], met.m[3 * d + 1][c], met.m[3 * d + 2][c], p.K, d != 0);
+        const double h = dd[d];
+        V2 *b = o + 7 * d * 64;
+        b[0 * 64] = Pair2<CT>::make(sp.p12 / h, sp.p13 / h);
+        b[1 * 64] = Pair2<CT>::make(sp.p14 / h, sp.p21 / h);
+        b[2 * 64] = Pair2<CT>::make(sp.p31 / h, sp.p41 / h);
+        b[3 * 64] = Pair2<CT>::make(sp.p51 / h, sp.p52 / h);
+        b[4 * 64] = Pair2<CT>::make(sp.p53 / h, sp.p54 / h);
+        b[5 * 64] = Pair2<CT>::make(sp.p11 / h, sp.p22 / h);
+        b[6 * 64] = Pair2<CT>::make(sp.m11 / h, sp.m22 / h);
+    }
+    double m[9];
+    load9(met, c, m);
+    const DG D = diag(q, m, p, deltaT);
+    double2 *od = reinterpret_cast<double2 *>(dgnM) + row * 2 * 64 + l;
+    od[0] = make_double2(D.d11, D.d22);
+    od[64] = make_double2(D.d51, 0.0);
+    double *orr = mrM + row * 5 * 64 + l;
+#pragma unroll
+    for (int v = 0; v < 5; v++) orr[v * 64] = MR.v[v][c];
+}
+
+// march-ordered NL and NR back into their fields, interior cells only
+__global__ void __launch_bounds__(256) k_march_scatter(Geo g, March M, const double *__restrict__ nlM,
+                                                       const double *__restrict__ nrM, F5 NL, F5 NR)
+{
+    constexpr int TJ = MARCH_TJ, TK = MARCH_TK;
+    const int l = threadIdx.x & 63, t = blockIdx.x * 4 + (threadIdx.x >> 6), T = blockIdx.y;
+    if (t >= M.S) return;
+    const int jl = l / TK, kl = l % TK;
+    const int TJi = T / M.ntk, TKi = T % M.ntk;
+    const int jo = TJi * TJ + jl, ko = TKi * TK + kl, il = t - jl - kl;
+    if (il < 0 || il >= g.nx || jo >= g.ny || ko >= g.nz) return;
+    const long c = ix(g, 3 + il, 2 + jo, 2 + ko);
+    const long o = march_row(M, T, t) * 5 * 64 + l;
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        NL.v[v][c] = nlM[o + v * 64];
+        NR.v[v][c] = nrM[o + v * 64];
+    }
+}
+
+// per-lane constants of a march lane
+struct MLane {
+    int tid, jl, kl;
+    bool col, edge_out, need_b, need_c;   // need_*: neighbour across a tile edge (exchange)
+    int ib, ic;                           // LDS slots of the in-tile j and k neighbours
+    unsigned oA, oB, oC;                  // record offsets (bytes) of the three upstream neighbours at t = 0
+    long x0;                              // exchange-buffer index of (i = 3 - jl - kl, j, k), times 5
+};
+
+template <typename CT>
+struct MArgs {
+    Geo g;
+    int S;
+    __amdgpu_buffer_rsrc_t creg;   // coefficient rows from the wave's region base
+    __amdgpu_buffer_rsrc_t drow;   // diagonal rows from (T, t = 0)
+    __amdgpu_buffer_rsrc_t rrow;   // right-hand-side rows from (T, t = 0)
+    char *xrow0;                   // increment row of (T, t = 0)
+    double *xbuf;
+    double *trash;                 // this wave's scratch line: exchange stores of lanes with nothing to publish
+    int max_spin;
+    long sxs5, dnb5, dnc5;   // exchange-buffer strides (times 5): one i-plane, the j and k neighbours
+    int d;                   // -1 forward, +1 backward: towards the upstream neighbours
+};
+
+template <typename CT> struct MRowB { static constexpr long v = (long)NSPLIT * 64 * 2 * sizeof(CT); };
+constexpr long DROWB = 2 * 64 * 16, FROWB = 5 * 64 * 8;
+
+template <bool FWD, typename CT>
+__device__ __forceinline__ int m_t(const MArgs<CT> &w, int s) { return FWD ? s : w.S - 1 - s; }
+
+// buffer loads: a wave-uniform descriptor and scalar offset, a per-lane
+// offset
+template <typename CT> struct BufPair;
+template <> struct BufPair<double> {
+    static __device__ __forceinline__ double2 get(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
+    {
+        return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
+    }
+};
+template <> struct BufPair<float> {
+    static __device__ __forceinline__ float2 get(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
+    {
+        return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+    }
+};
+
+__device__ __forceinline__ double buf_f64(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+}
+
+// one direction's 12 values (pairs 0..4 and the sweep's p- or m-pair) of
+// the record at per-lane offset `vo`, row offset `so`, direction DIR
+template <bool FWD, int DIR, typename CT>
+__device__ __forceinline__ void load_dir_m(Dir12<CT> &s, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
+{
+    constexpr unsigned P = 2 * sizeof(CT) * 64;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const int p = 7 * DIR + (q < 5 ? q : (FWD ? 5 : 6));
+        const typename Pair2<CT>::t t = BufPair<CT>::get(r, vo, so + p * P);
+        s.v[2 * q] = t.x;
+        s.v[2 * q + 1] = t.y;
+    }
+}
+
+// the loads of step s (the last step's again for s = S, the prefetch past
+// the end): the three upstream records, the own diagonal and R
+template <bool FWD, typename CT>
+__device__ __forceinline__ void m_load(Pref<CT> &P, const MArgs<CT> &w, const MLane &L, int s)
+{
+    const int t = m_t<FWD>(w, s < w.S ? s : w.S - 1);
+    const unsigned so = (unsigned)(t * MRowB<CT>::v);   // wave-uniform
+    load_dir_m<FWD, 0>(P.A, w.creg, L.oA, so);
+    load_dir_m<FWD, 1>(P.B, w.creg, L.oB, so);
+    load_dir_m<FWD, 2>(P.C, w.creg, L.oC, so);
+    const unsigned sd = (unsigned)(t * DROWB), sr = (unsigned)(t * FROWB);
+    const double2 t0 = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(w.drow, L.tid * 16, sd, 0));
+    P.d11 = t0.x;
+    P.d22 = t0.y;
+    P.d51 = buf_f64(w.drow, L.tid * 16, sd + 1024);
+#pragma unroll
+    for (int v = 0; v < 5; v++) P.r[v] = buf_f64(w.rrow, L.tid * 8, sr + v * 512);
+}
+
+__device__ __forceinline__ void issue5m(const double *p, unsigned long long b[5])
+{
+#pragma unroll
+    for (int v = 0; v < 5; v++) b[v] = aload(p + v);
+}
+
+// where `need`, re-poll the five entries of one neighbour until none holds
+// the sentinel (bounded: `late` records an expired wait) and take them.
+// The polled values are examined on every lane and path, so the wait for
+// them happens here for all of them and a later poll into the same
+// registers needs none.
+__device__ __forceinline__ void settle5m(const double *p, bool need, unsigned long long b[5], double out[5],
+                                         int max_spin, bool &late)
+{
+    bool pend = need & ((b[0] == SENTINEL) | (b[1] == SENTINEL) | (b[2] == SENTINEL) | (b[3] == SENTINEL) |
+                        (b[4] == SENTINEL));
+    if (pend) {
+        for (int n = 0; pend && n < max_spin; n++) {
+            __builtin_amdgcn_s_sleep(1);
+            issue5m(p, b);
+            pend = (b[0] == SENTINEL) | (b[1] == SENTINEL) | (b[2] == SENTINEL) | (b[3] == SENTINEL) |
+                   (b[4] == SENTINEL);
+        }
+        late |= pend;
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) out[v] = need ? __longlong_as_double((long long)b[v]) : out[v];
+}
+
+// one step: the records of step s are in register set Pc; step s+1's
+// loads go into Pn
+template <int TJ, int TK, bool FWD, int PAR, typename CT>
+__device__ __forceinline__ void march_step(const MArgs<CT> &w, const MLane &L, int s, double prev[5],
+                                           double (*sx)[5][TJ * TK], Pref<CT> &Pc, Pref<CT> &Pn,
+                                           unsigned long long *nb, unsigned long long *nc, bool &late)
+{
+    const int t = m_t<FWD>(w, s);
+    const int il = t - L.jl - L.kl;
+    const bool act = L.col && il >= 0 && il < w.g.nx;
+    __syncthreads();
+    // exchange-buffer entry of this step's cell (only dereferenced where act)
+    const long xc5 = L.x0 + (long)t * w.sxs5;
+    double xb[5], xc[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        xb[v] = sx[PAR ^ 1][v][L.ib];
+        xc[v] = sx[PAR ^ 1][v][L.ic];
+    }
+    settle5m(w.xbuf + xc5 + w.dnb5, act && L.need_b, nb, xb, w.max_spin, late);
+    settle5m(w.xbuf + xc5 + w.dnc5, act && L.need_c, nc, xc, w.max_spin, late);
+    {   // prefetch for the next step: the tile-edge values (consuming lanes
+        // with a cell there), then every lane's records
+        const int tn = m_t<FWD>(w, s + 1);
+        const int iln = tn - L.jl - L.kl;
+        const bool actn = L.col && iln >= 0 && iln < w.g.nx;
+        const long xn5 = L.x0 + (long)tn * w.sxs5;
+        if (actn && L.need_b) issue5m(w.xbuf + xn5 + w.dnb5, nb);
+        if (actn && L.need_c) issue5m(w.xbuf + xn5 + w.dnc5, nc);
+        m_load<FWD>(Pn, w, L, s + 1);
+    }
+    // keep the prefetch ahead of the arithmetic that waits for this step's data
+    __builtin_amdgcn_sched_barrier(0);
+    const CT *A = Pc.A.v, *B = Pc.B.v, *Cc = Pc.C.v;
+    CT xa[5], xbc[5], xcc[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        xa[v] = (CT)prev[v];
+        xbc[v] = (CT)xb[v];
+        xcc[v] = (CT)xc[v];
+    }
+    double out[5];
+    if (FWD) {
+        const CT LL0 = (-A[10] * xa[0] - A[0] * xa[1] - A[1] * xa[2] - A[2] * xa[3]) +
+                           (-B[10] * xbc[0] - B[0] * xbc[1] - B[1] * xbc[2] - B[2] * xbc[3]) +
+                           (-Cc[10] * xcc[0] - Cc[0] * xcc[1] - Cc[1] * xcc[2] - Cc[2] * xcc[3]);
+        const CT LL1 = (-A[3] * xa[0] - A[11] * xa[1]) + (-B[3] * xbc[0] - B[11] * xbc[1]) +
+                           (-Cc[3] * xcc[0] - Cc[11] * xcc[1]);
+        const CT LL2 = (-A[4] * xa[0] - A[11] * xa[2]) + (-B[4] * xbc[0] - B[11] * xbc[2]) +
+                           (-Cc[4] * xcc[0] - Cc[11] * xcc[2]);
+        const CT LL3 = (-A[5] * xa[0] - A[11] * xa[3]) + (-B[5] * xbc[0] - B[11] * xbc[3]) +
+                           (-Cc[5] * xcc[0] - Cc[11] * xcc[3]);
+        const CT LL4 = (-A[6] * xa[0] - A[7] * xa[1] - A[8] * xa[2] - A[9] * xa[3] - A[11] * xa[4]) +
+                           (-B[6] * xbc[0] - B[7] * xbc[1] - B[8] * xbc[2] - B[9] * xbc[3] - B[11] * xbc[4]) +
+                           (-Cc[6] * xcc[0] - Cc[7] * xcc[1] - Cc[8] * xcc[2] - Cc[9] * xcc[3] - Cc[11] * xcc[4]);
+        out[0] = Pc.d11 * (Pc.r[0] - (double)LL0);
+        out[1] = Pc.d22 * (Pc.r[1] - (double)LL1);
+        out[2] = Pc.d22 * (Pc.r[2] - (double)LL2);
+        out[3] = Pc.d22 * (Pc.r[3] - (double)LL3);
+        out[4] = Pc.d51 * (Pc.r[0] - (double)LL0) + Pc.d22 * (Pc.r[4] - (double)LL4);
+    } else {
+        const CT UU0 = (+A[10] * xa[0] + A[0] * xa[1] + A[1] * xa[2] + A[2] * xa[3]) +
+                           (+B[10] * xbc[0] + B[0] * xbc[1] + B[1] * xbc[2] + B[2] * xbc[3]) +
+                           (+Cc[10] * xcc[0] + Cc[0] * xcc[1] + Cc[1] * xcc[2] + Cc[2] * xcc[3]);
+        const CT UU1 = (+A[3] * xa[0] + A[11] * xa[1]) + (+B[3] * xbc[0] + B[11] * xbc[1]) +
+                           (+Cc[3] * xcc[0] + Cc[11] * xcc[1]);
+        const CT UU2 = (+A[4] * xa[0] + A[11] * xa[2]) + (+B[4] * xbc[0] + B[11] * xbc[2]) +
+                           (+Cc[4] * xcc[0] + Cc[11] * xcc[2]);
+        const CT UU3 = (+A[5] * xa[0] + A[11] * xa[3]) + (+B[5] * xbc[0] + B[11] * xbc[3]) +
+                           (+Cc[5] * xcc[0] + Cc[11] * xcc[3]);
+        const CT UU4 = (+A[6] * xa[0] + A[7] * xa[1] + A[8] * xa[2] + A[9] * xa[3] + A[11] * xa[4]) +
+                           (+B[6] * xbc[0] + B[7] * xbc[1] + B[8] * xbc[2] + B[9] * xbc[3] + B[11] * xbc[4]) +
+                           (+Cc[6] * xcc[0] + Cc[7] * xcc[1] + Cc[8] * xcc[2] + Cc[9] * xcc[3] + Cc[11] * xcc[4]);
+        out[0] = Pc.r[0] - Pc.d11 * (double)UU0;
+        out[1] = Pc.r[1] - Pc.d22 * (double)UU1;
+        out[2] = Pc.r[2] - Pc.d22 * (double)UU2;
+        out[3] = Pc.r[3] - Pc.d22 * (double)UU3;
+        out[4] = Pc.r[4] - (Pc.d51 * (double)UU0 + Pc.d22 * (double)UU4);
+    }
+    // every lane stores into its own march slot; a slot with no cell is
+    // never read as a value
+    double *xo = reinterpret_cast<double *>(w.xrow0 + (long)t * FROWB) + L.tid;
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        xo[v * 64] = out[v];
+        // a column outside the slab holds the ghost value (zero) for its
+        // neighbours: in the backward march it lies upstream of real ones
+        sx[PAR][v][L.tid] = L.col ? out[v] : 0.0;
+        prev[v] = act ? out[v] : prev[v];
+    }
+    // every lane stores: the edge lanes with a cell publish, the others
+    // write the wave's scratch line, so no branch splits the stores and the
+    // next step's wait for its records counts them exactly
+    double *xe = act && L.edge_out ? w.xbuf + xc5 : w.trash;
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        __hip_atomic_store(reinterpret_cast<unsigned long long *>(xe + v),
+                           (unsigned long long)__double_as_longlong(out[v]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// coefA/dgnA/RmA/XmA: the padded march arrays; Xg: the increment field,
+// whose ghost plane upstream in i starts each column
+template <int TJ, int TK, bool FWD, typename CT>
+__global__ void __launch_bounds__(TJ *TK) k_lusgs_march(Geo g, March M, const CT *__restrict__ coefA,
+                                                      const double *__restrict__ dgnA, const double *RmA,
+                                                      double *XmA, C5 Xg, double *xbuf, double *trash,
+                                                      const int *__restrict__ order, int ntiles, int *ticket,
+                                                      int *err, int max_spin)
+{
+    static_assert(TJ * TK == 64, "one wave per tile");
+    __shared__ int s_t;
+    __shared__ double sx[2][5][TJ * TK];
+    MLane L;
+    L.tid = threadIdx.x;
+    L.kl = L.tid % TK;
+    L.jl = L.tid / TK;
+    if (L.tid == 0) s_t = atomicAdd(ticket, 1);
+    __syncthreads();
+    // forward: anti-diagonals ascending; backward: the same list reversed
+    const int tk = __builtin_amdgcn_readfirstlane(s_t);
+    const int packed = order[FWD ? tk : ntiles - 1 - tk];
+    const int TJi = packed >> 16, TKi = packed & 0xffff;
+    const int T = TJi * M.ntk + TKi;
+    const int jo = TJi * TJ + L.jl, ko = TKi * TK + L.kl;
+    L.col = (jo < g.ny) && (ko < g.nz);
+    const int joc = jo < g.ny ? jo : g.ny - 1, koc = ko < g.nz ? ko : g.nz - 1;
+    const long c0 = ix(g, 0, 2 + joc, 2 + koc);
+    const int d = FWD ? -1 : 1;
+    // the neighbour records: tile, lane and march-index offset of the j and
+    // k neighbours (the own record where there is none)
+    int Tb, lb, tbo, Tc, lc, tco;
+    if (FWD) {
+        L.need_b = L.jl == 0;
+        L.need_c = L.kl == 0;
+        L.edge_out = (L.jl == TJ - 1) || (L.kl == TK - 1);
+        // j-1: the lane above in this tile, or the last row of the tile above
+        if (L.jl >= 1) { Tb = T; lb = L.tid - TK; tbo = -1; }
+        else if (TJi >= 1) { Tb = T - M.ntk; lb = (TJ - 1) * TK + L.kl; tbo = TJ - 1; }
+        else { Tb = T; lb = L.tid; tbo = 0; }
+        if (L.kl >= 1) { Tc = T; lc = L.tid - 1; tco = -1; }
+        else if (TKi >= 1) { Tc = T - 1; lc = L.jl * TK + TK - 1; tco = TK - 1; }
+        else { Tc = T; lc = L.tid; tco = 0; }
+        L.ib = L.jl >= 1 ? L.tid - TK : L.tid;
+        L.ic = L.kl >= 1 ? L.tid - 1 : L.tid;
+    } else {
+        L.need_b = L.jl == TJ - 1;
+        L.need_c = L.kl == TK - 1;
+        L.edge_out = (L.jl == 0) || (L.kl == 0);
+        // j+1: the lane below in this tile, or the first row of the tile below
+        if (L.jl <= TJ - 2 && jo + 1 < g.ny) { Tb = T; lb = L.tid + TK; tbo = 1; }
+        else if (L.jl == TJ - 1 && TJi + 1 < M.ntj) { Tb = T + M.ntk; lb = L.kl; tbo = -(TJ - 1); }
+        else { Tb = T; lb = L.tid; tbo = 0; }
+        if (L.kl <= TK - 2 && ko + 1 < g.nz) { Tc = T; lc = L.tid + 1; tco = 1; }
+        else if (L.kl == TK - 1 && TKi + 1 < M.ntk) { Tc = T + 1; lc = L.jl * TK; tco = -(TK - 1); }
+        else { Tc = T; lc = L.tid; tco = 0; }
+        L.ib = L.jl <= TJ - 2 ? L.tid + TK : L.tid;
+        L.ic = L.kl <= TK - 2 ? L.tid + 1 : L.tid;
+    }
+    // records: a wave-uniform region base (the lowest row the sweep reads
+    // from) and per-lane byte offsets of the rows at t = 0
+    typedef typename Pair2<CT>::t V2;
+    constexpr long RB = MRowB<CT>::v;
+    const int Tlo = FWD ? (T >= M.ntk ? T - M.ntk : 0) : T;
+    const long rlo = (long)Tlo * M.S;   // padded row of (Tlo, t = -MPAD)
+    MArgs<CT> w;
+    w.g = g;
+    w.S = M.S;
+    constexpr int F = 0x00020000;   // raw buffer, 32-bit data format
+    w.creg = __builtin_amdgcn_make_buffer_rsrc((void *)(reinterpret_cast<const char *>(coefA) + rlo * RB), (short)0, -1, F);
+    L.oA = (unsigned)((march_row(M, T, d) - rlo) * RB + L.tid * (long)sizeof(V2));
+    L.oB = (unsigned)((march_row(M, Tb, tbo) - rlo) * RB + lb * (long)sizeof(V2));
+    L.oC = (unsigned)((march_row(M, Tc, tco) - rlo) * RB + lc * (long)sizeof(V2));
+    w.drow = __builtin_amdgcn_make_buffer_rsrc((void *)(reinterpret_cast<const char *>(dgnA) + march_row(M, T, 0) * DROWB),
+                                               (short)0, -1, F);
+    w.rrow = __builtin_amdgcn_make_buffer_rsrc((void *)(reinterpret_cast<const char *>(RmA) + march_row(M, T, 0) * FROWB),
+                                               (short)0, -1, F);
+    w.xrow0 = reinterpret_cast<char *>(XmA) + march_row(M, T, 0) * FROWB;
+    w.xbuf = xbuf;
+    w.trash = trash + (long)blockIdx.x * 256;
+    w.max_spin = max_spin;
+    const long sxs = (long)g.Ym * g.Zm;
+    w.sxs5 = 5 * sxs;
+    w.dnb5 = 5L * d * g.Zm;
+    w.dnc5 = 5L * d;
+    w.d = d;
+    L.x0 = 5 * (c0 + (long)(3 - L.jl - L.kl) * sxs);
+    const int S = M.S;
+    double prev[5];
+    bool late = false;
+    Pref<CT> P0, P1;
+    unsigned long long nb[5], nc[5];
+    {
+        // the ghost plane upstream in i (zero increments)
+        const long cg = c0 + (long)(FWD ? 2 : g.gend + 1) * sxs;
+#pragma unroll
+        for (int v = 0; v < 5; v++) prev[v] = Xg.v[v][cg];
+        const int t0 = m_t<FWD>(w, 0);
+        const int il0 = t0 - L.jl - L.kl;
+        const bool act0 = L.col && il0 >= 0 && il0 < g.nx;
+        const long x5 = L.x0 + (long)t0 * w.sxs5;
+        if (act0 && L.need_b) issue5m(xbuf + x5 + w.dnb5, nb);
+        if (act0 && L.need_c) issue5m(xbuf + x5 + w.dnc5, nc);
+        m_load<FWD>(P0, w, L, 0);
+    }
+    for (int s = 0; s < S; s += 2) {
+        march_step<TJ, TK, FWD, 0>(w, L, s, prev, sx, P0, P1, nb, nc, late);
+        if (s + 1 < S) march_step<TJ, TK, FWD, 1>(w, L, s + 1, prev, sx, P1, P0, nb, nc, late);
+    }
+    if (late) atomicOr(err, 1);
+}
+
+// the launchers take the allocations (march_*_pairs / march_field_doubles)
+void launch_split_march(const Geo &g, const Phys &p, double deltaT, double e, C5 U_, const double *J, M9 met,
+                        C5 MR, void *coefM, double *dgnM, double *mrM, int fp32, hipStream_t s)
+{
+    const March M = march_of(g);
+    const dim3 gr((M.S + 3) / 4, M.ntj * M.ntk), bl(256);
+    if (fp32)
+        hipLaunchKernelGGL(k_split_march<float>, gr, bl, 0, s, g, p, deltaT, e, U_, J, met, MR, M,
+                           static_cast<float *>(coefM), dgnM, mrM);
+    else
+        hipLaunchKernelGGL(k_split_march<double>, gr, bl, 0, s, g, p, deltaT, e, U_, J, met, MR, M,
+                           static_cast<double *>(coefM), dgnM, mrM);
+}
+
+void launch_march_scatter(const Geo &g, const double *nlM, const double *nrM, F5 NL, F5 NR, hipStream_t s)
+{
+    const March M = march_of(g);
+    hipLaunchKernelGGL(k_march_scatter, dim3((M.S + 3) / 4, M.ntj * M.ntk), dim3(256), 0, s, g, M, nlM, nrM, NL, NR);
+}
+
+void launch_lusgs_march(const Geo &g, int forward, const void *coefM, const double *dgnM, int fp32,
+                        const double *Rm, double *Xm, C5 Xg, double *xbuf, double *trash, const int *order,
+                        int ntiles, int *counters, int max_spin, hipStream_t s)
+{
+    const March M = march_of(g);
+    const dim3 gr(ntiles), bl(MARCH_TJ * MARCH_TK);
+    const float *cf = static_cast<const float *>(coefM);
+    const double *cd = static_cast<const double *>(coefM);
+    if (forward && fp32)
+        hipLaunchKernelGGL((k_lusgs_march<MARCH_TJ, MARCH_TK, true, float>), gr, bl, 0, s, g, M, cf, dgnM, Rm, Xm, Xg,
+                           xbuf, trash, order, ntiles, counters, counters + 1, max_spin);
+    else if (forward)
+        hipLaunchKernelGGL((k_lusgs_march<MARCH_TJ, MARCH_TK, true, double>), gr, bl, 0, s, g, M, cd, dgnM, Rm, Xm, Xg,
+                           xbuf, trash, order, ntiles, counters, counters + 1, max_spin);
+    else if (fp32)
+        hipLaunchKernelGGL((k_lusgs_march<MARCH_TJ, MARCH_TK, false, float>), gr, bl, 0, s, g, M, cf, dgnM, Rm, Xm, Xg,
+                           xbuf, trash, order, ntiles, counters, counters + 1, max_spin);
+    else
+        hipLaunchKernelGGL((k_lusgs_march<MARCH_TJ, MARCH_TK, false, double>), gr, bl, 0, s, g, M, cd, dgnM, Rm, Xm, Xg,
+                           xbuf, trash, order, ntiles, counters, counters + 1, max_spin);
 }
 
 // tile order: anti-diagonals J+K ascending, J ascending inside

@@ -126,10 +126,12 @@ int cgli_hyperplane(cgli_ctx *ctx, int *Nijk, int *ijk);
 int cgli_dual_time_stepping(cgli_ctx *ctx, double deltaT, double e, double er[5], int sweeps,
                             int write_residual);
 /* LU-SGS schedule: 0 = one launch per i+j+k hyperplane (the reference's
- * order), 1 = tiled wavefront in one launch per sweep (default), 2 = the
+ * order), 1 = tiled wavefront in one launch per sweep, 2 = the
  * same wavefront with two loader waves per tile streaming the coefficients
- * through LDS, three steps ahead.  The wavefronts agree with each other bit
- * for bit and with the plane order to rounding.  The wavefront's bounded waits give up after
+ * through LDS, three steps ahead, 3 = the wavefront with its coefficient
+ * records, right-hand sides and increments stored in the order the march
+ * reads them (one contiguous block per tile and step; the default).  The wavefronts agree with each other bit for bit and
+ * with the plane order to rounding.  The wavefront's bounded waits give up after
  * max_spin polls each (then CGLI_E_TIMEOUT). */
 int cgli_set_lusgs_mode(cgli_ctx *ctx, int mode);
 int cgli_set_spin_limit(cgli_ctx *ctx, int max_spin);
@@ -274,7 +276,8 @@ int cgli_net_timing(cgli_net *net, double *predict_ms, long *n_predict, double *
 /* Average device time (ms) of the last `n` launches of a kernel class
  * recorded since cgli_timing_reset; classes: 0 flux_x 1 flux_y 2 flux_z
  * 3 viscous 4 rhs_diag 5 lusgs_fwd 6 lusgs_bwd 7 update_norm 8 split_coef
- * 9 filter. */
+ * 9 filter 10 march_scatter (LU-SGS schedule 3: increments back to their
+ * fields). */
 int cgli_timing_enable(cgli_ctx *ctx, int on);
 int cgli_timing_reset(cgli_ctx *ctx);
 int cgli_timing_get(cgli_ctx *ctx, int cls, double *total_ms, long *count);

@@ -225,13 +225,14 @@ def test_wavefront_and_plane_schedules_agree(nv, orc, name):
     """The tiled wavefront visits cells in another topological order of the
     same dependencies as the reference's plane order, with the grid-spacing
     division folded into its coefficients: results agree to rounding.  The
-    wavefront without loader waves (mode 1) and with them (mode 2) do the same
-    arithmetic on the same values: identical bits."""
+    wavefront without loader waves (mode 1), with them (mode 2) and on
+    march-ordered records (mode 3) do the same arithmetic on the same
+    values: identical bits."""
     print("case", name, flush=True)
     case = make_case(name)
     f = make_inputs(orc, case.slab, SEED + 6)
     outs = []
-    for mode in (0, 1, 2):
+    for mode in (0, 1, 2, 3):
         with open_ctx(nv, orc, case) as ctx:
             ctx.set_lusgs_mode(mode)
             upload_inputs(ctx, f)
@@ -244,12 +245,13 @@ def test_wavefront_and_plane_schedules_agree(nv, orc, name):
     reg = regions(case.slab)["cell"]
     for v in range(5):
         assert abs(outs[0][0][v] - outs[1][0][v]) <= 1e-13 * abs(outs[0][0][v])
-        assert outs[1][0][v] == outs[2][0][v]
+        assert outs[1][0][v] == outs[2][0][v] == outs[3][0][v]
         for q in (1, 2, 3):
             a, b = outs[0][q][v][reg], outs[1][q][v][reg]
             assert np.all(np.isfinite(b))
             assert rel(b, a) <= 1e-13, (q, v, rel(b, a))
             assert np.array_equal(outs[1][q][v], outs[2][q][v])
+            assert np.array_equal(outs[1][q][v], outs[3][q][v]), (q, v, rel(outs[3][q][v][reg], b))
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -344,20 +346,27 @@ def test_fp32_residuals_track_the_fp64_oracle(nv, orc, name):
 def test_mixed_precision_iteration_tracks_the_fp64_oracle(nv, orc, name):
     """One inner iteration with cfg.precision = 1 (FP32 residual kernels and
     an FP32 off-diagonal sweep around the FP64 diagonal, update and norm):
-    the increments agree with the FP64 oracle to single-precision rounding."""
+    the increments agree with the FP64 oracle to single-precision rounding.
+    The wavefront on march-ordered records gives the same bits."""
     print("case", name, flush=True)
     case = make_case(name)
     f = make_inputs(orc, case.slab, SEED + 10)
     case.overrides = {"sponge": 0}
     want = run_chain("orc", orc, case, f, dts=True, sweeps=1)
-    with open_ctx(nv, orc, case, sponge=0, precision=1) as ctx:
-        upload_inputs(ctx, f)
-        ctx.flux_x(1.0, 1e-4)
-        ctx.flux_y(1.0, 1e-4)
-        ctx.flux_z(1.0, 1e-4)
-        ctx.viscous()
-        er = ctx.dual_time_stepping(2e-4, 1e-4, sweeps=1)
-        NR = ctx.get_many(nv.NR, 5)
+    got = []
+    for mode in (1, 3):
+        with open_ctx(nv, orc, case, sponge=0, precision=1) as ctx:
+            ctx.set_lusgs_mode(mode)
+            upload_inputs(ctx, f)
+            ctx.flux_x(1.0, 1e-4)
+            ctx.flux_y(1.0, 1e-4)
+            ctx.flux_z(1.0, 1e-4)
+            ctx.viscous()
+            got.append((ctx.dual_time_stepping(2e-4, 1e-4, sweeps=1), ctx.get_many(nv.NR, 5)))
+    (er, NR), (er3, NR3) = got
+    assert list(er) == list(er3)
+    for v in range(5):
+        assert np.array_equal(NR[v], NR3[v])
     reg = regions(case.slab)["cell"]
     worst = 0.0
     for v in range(5):
