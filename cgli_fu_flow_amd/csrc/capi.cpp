@@ -514,6 +514,11 @@ static int wave_setup(cgli_ctx *c)
         c->dgnM = static_cast<double *>(p);
         if (hipMalloc(&p, sizeof(double) * 3 * march_field_doubles(c->g)) != hipSuccess) return CGLI_E_NOMEM;
         c->marchF = static_cast<double *>(p);
+        // slots with no cell are never written: NaN in them, so that a read
+        // of one where a cell's value belongs spoils every result at once
+        HIPCHK(hipMemset(c->coefM, 0xFF, 16 * march_coef_pairs(c->g)));
+        HIPCHK(hipMemset(c->dgnM, 0xFF, 16 * march_diag_pairs(c->g)));
+        HIPCHK(hipMemset(c->marchF, 0xFF, sizeof(double) * 3 * march_field_doubles(c->g)));
         std::vector<int> order;
         c->mtiles = wave_tiles(c->g, MARCH_TJ, MARCH_TK, order);
         if (hipMalloc(&p, sizeof(int) * order.size()) != hipSuccess) return CGLI_E_NOMEM;

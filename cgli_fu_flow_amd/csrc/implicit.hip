@@ -1431,20 +1431,72 @@ bool march_fits(const Geo &g, int fp32)
     return ((double)(M.ntk + 1) * M.S + 4.0 * MPAD) * rowb < 4.0e9;
 }
 
-// one thread per march position (T, t, lane); blocks of 4 steps x 64 lanes
+// Gather and scatter between the field layout and march order: one wave
+// per tile column row jl and block of MR_ROWS = 64/TK march steps; its 64
+// threads cover the positions (step t0+r, lane jl*TK+kl), ordered by the
+// cell's plane a = r - kl (then kl), so that the field-layout side of each
+// memory instruction is a few runs of consecutive k (one per plane) and the
+// march side is MR_ROWS runs of TK consecutive lanes (one per step).
+constexpr int MR_ROWS = 64 / MARCH_TK;
+static_assert(MR_ROWS * MARCH_TK == 64, "a wave covers whole steps of one tile row");
+
+struct MPos { int r, kl; };
+__device__ __forceinline__ MPos mpos_of_lane(int q)
+{
+    constexpr int R = MR_ROWS, TK = MARCH_TK;
+    // planes a = r - kl + TK - 1 in ascending order, kl ascending inside
+    for (int a = 0; a < R + TK - 1; a++) {
+        // kl range of plane a: r = a - (TK - 1) + kl in [0, R)
+        const int lo = (TK - 1) - a > 0 ? (TK - 1) - a : 0;
+        const int hi = (R - 1) + (TK - 1) - a < TK - 1 ? (R - 1) + (TK - 1) - a : TK - 1;
+        const int n = hi - lo + 1;
+        if (q < n) return MPos{a - (TK - 1) + lo + q, lo + q};
+        q -= n;
+    }
+    return MPos{0, 0};
+}
+
+// grid: (steps / MR_ROWS rounded up, tiles), blocks of 4 waves = 4 tile rows
+// (times ceil(TJ / 4) in x)
+struct MGather {
+    int T, t, l, jl, kl, jo, ko, il;
+    bool ok;
+};
+__device__ __forceinline__ MGather mgather_at(const Geo &g, const March &M)
+{
+    constexpr int TJ = MARCH_TJ, TK = MARCH_TK, WPB = 4;
+    constexpr int JB = (TJ + WPB - 1) / WPB;   // blocks per tile row set
+    MGather q;
+    const int wv = threadIdx.x >> 6;
+    q.jl = (blockIdx.x % JB) * WPB + wv;
+    const int tb = blockIdx.x / JB;
+    const MPos P = mpos_of_lane(threadIdx.x & 63);
+    q.T = blockIdx.y;
+    q.t = tb * MR_ROWS + P.r;
+    q.kl = P.kl;
+    q.l = q.jl * TK + q.kl;
+    const int TJi = q.T / M.ntk, TKi = q.T % M.ntk;
+    q.jo = TJi * TJ + q.jl;
+    q.ko = TKi * TK + q.kl;
+    q.il = q.t - q.jl - q.kl;
+    q.ok = q.jl < TJ && q.t < M.S && q.il >= 0 && q.il < g.nx && q.jo < g.ny && q.ko < g.nz;
+    return q;
+}
+static dim3 mgather_grid(const March &M)
+{
+    constexpr int JB = (MARCH_TJ + 3) / 4;
+    return dim3(((M.S + MR_ROWS - 1) / MR_ROWS) * JB, M.ntj * M.ntk);
+}
+
 template <typename CT>
 __global__ void __launch_bounds__(256) k_split_march(Geo g, Phys p, double deltaT, double e, C5 U_,
                                                      const double *__restrict__ J, M9 met, C5 MR, March M,
                                                      CT *__restrict__ coefM, double *__restrict__ dgnM,
                                                      double *__restrict__ mrM)
 {
-    constexpr int TJ = MARCH_TJ, TK = MARCH_TK;
-    const int l = threadIdx.x & 63, t = blockIdx.x * 4 + (threadIdx.x >> 6), T = blockIdx.y;
-    if (t >= M.S) return;
-    const int jl = l / TK, kl = l % TK;
-    const int TJi = T / M.ntk, TKi = T % M.ntk;
-    const int jo = TJi * TJ + jl, ko = TKi * TK + kl, il = t - jl - kl;
-    if (il < 0 || il >= g.nx || jo >= g.ny || ko >= g.nz) return;
+    const MGather q0 = mgather_at(g, M);
+    if (!q0.ok) return;
+    const int T = q0.T, t = q0.t, l = q0.l, jo = q0.jo, ko = q0.ko, il = q0.il;
     const long c = ix(g, 3 + il, 2 + jo, 2 + ko);
     const long row = march_row(M, T, t);
     typedef typename Pair2<CT>::t V2;
@@ -1479,15 +1531,10 @@ __global__ void __launch_bounds__(256) k_split_march(Geo g, Phys p, double delta
 __global__ void __launch_bounds__(256) k_march_scatter(Geo g, March M, const double *__restrict__ nlM,
                                                        const double *__restrict__ nrM, F5 NL, F5 NR)
 {
-    constexpr int TJ = MARCH_TJ, TK = MARCH_TK;
-    const int l = threadIdx.x & 63, t = blockIdx.x * 4 + (threadIdx.x >> 6), T = blockIdx.y;
-    if (t >= M.S) return;
-    const int jl = l / TK, kl = l % TK;
-    const int TJi = T / M.ntk, TKi = T % M.ntk;
-    const int jo = TJi * TJ + jl, ko = TKi * TK + kl, il = t - jl - kl;
-    if (il < 0 || il >= g.nx || jo >= g.ny || ko >= g.nz) return;
-    const long c = ix(g, 3 + il, 2 + jo, 2 + ko);
-    const long o = march_row(M, T, t) * 5 * 64 + l;
+    const MGather q = mgather_at(g, M);
+    if (!q.ok) return;
+    const long c = ix(g, 3 + q.il, 2 + q.jo, 2 + q.ko);
+    const long o = march_row(M, q.T, q.t) * 5 * 64 + q.l;
 #pragma unroll
     for (int v = 0; v < 5; v++) {
         NL.v[v][c] = nlM[o + v * 64];
@@ -1568,7 +1615,13 @@ __device__ __forceinline__ void m_load(Pref<CT> &P, const MArgs<CT> &w, const ML
 {
     const int t = m_t<FWD>(w, s < w.S ? s : w.S - 1);
     const unsigned so = (unsigned)(t * MRowB<CT>::v);   // wave-uniform
-    load_dir_m<FWD, 0>(P.A, w.creg, L.oA, so);
+    // the i neighbour's record is the previous march slot of the lane;
+    // at the first interior plane that neighbour is a ghost cell (zero
+    // increment) and its slot holds no record: the own record stands in
+    const int il = t - L.jl - L.kl;
+    const bool inA = FWD ? il >= 1 : il <= w.g.nx - 2;
+    const unsigned oa = inA ? L.oA : (unsigned)((long)L.oA - (long)w.d * MRowB<CT>::v);
+    load_dir_m<FWD, 0>(P.A, w.creg, oa, so);
     load_dir_m<FWD, 1>(P.B, w.creg, L.oB, so);
     load_dir_m<FWD, 2>(P.C, w.creg, L.oC, so);
     const unsigned sd = (unsigned)(t * DROWB), sr = (unsigned)(t * FROWB);
@@ -1826,7 +1879,7 @@ void launch_split_march(const Geo &g, const Phys &p, double deltaT, double e, C5
                         C5 MR, void *coefM, double *dgnM, double *mrM, int fp32, hipStream_t s)
 {
     const March M = march_of(g);
-    const dim3 gr((M.S + 3) / 4, M.ntj * M.ntk), bl(256);
+    const dim3 gr = mgather_grid(M), bl(256);
     if (fp32)
         hipLaunchKernelGGL(k_split_march<float>, gr, bl, 0, s, g, p, deltaT, e, U_, J, met, MR, M,
                            static_cast<float *>(coefM), dgnM, mrM);
@@ -1838,7 +1891,7 @@ void launch_split_march(const Geo &g, const Phys &p, double deltaT, double e, C5
 void launch_march_scatter(const Geo &g, const double *nlM, const double *nrM, F5 NL, F5 NR, hipStream_t s)
 {
     const March M = march_of(g);
-    hipLaunchKernelGGL(k_march_scatter, dim3((M.S + 3) / 4, M.ntj * M.ntk), dim3(256), 0, s, g, M, nlM, nrM, NL, NR);
+    hipLaunchKernelGGL(k_march_scatter, mgather_grid(M), dim3(256), 0, s, g, M, nlM, nrM, NL, NR);
 }
 
 void launch_lusgs_march(const Geo &g, int forward, const void *coefM, const double *dgnM, int fp32,
