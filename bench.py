@@ -42,7 +42,8 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic FP64 bytes per interior cell per launch of each kernel class
 # (each input element read once, each output written once)
 ALGO_BYTES = {"flux_x": 120, "flux_y": 120, "flux_z": 120, "viscous": 320, "rhs_diag": 472,
-              "lusgs_fwd": 240, "lusgs_bwd": 240, "update_norm": 128, "split_coef": 480, "filter": 72}
+              "lusgs_fwd": 240, "lusgs_bwd": 240, "update_norm": 128, "split_coef": 560, "filter": 72,
+              "march_scatter": 120}
 TRAFFIC = os.path.join(ROOT, "profiles", "channel_128_traffic.json")
 
 
@@ -66,6 +67,9 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="mixed: FP32 residual kernels until the FP32 floor, FP32 off-diagonal sweep")
     ap.add_argument("--no-variants", action="store_true", help="skip the initial-guess / precision comparison")
+    ap.add_argument("--lusgs-mode", type=int, default=None,
+                    help="LU-SGS schedule (0 plane order, 1 tiled wavefront, 2 with loader waves, "
+                         "3 march-ordered records; default: the library's, 3)")
     ap.add_argument("--variant-steps", type=int, default=20, help="timed steps of each variant")
     ap.add_argument("--gen", type=int, default=200, help="recorded steps the network trains on")
     ap.add_argument("--train", type=int, default=1000, help="Adam steps")
@@ -178,6 +182,8 @@ def variants(a, nv, ch, ctx64, dt, ncell):
     cfg32 = nv.channel_config(ch, a.nx, a.ny, a.nz)
     cfg32.precision = 1
     ctx32 = nv.Context(cfg32, device=0)
+    if a.lusgs_mode is not None:
+        ctx32.set_lusgs_mode(a.lusgs_mode)
     ctx32.channel_init(ch)
     restore(ctx32, nv, state_of(ctx64, nv), np.zeros(5))
     net = nv.Net(seed=20261203)
@@ -250,6 +256,8 @@ def main():
     cfg = nv.channel_config(ch, a.nx, a.ny, a.nz)
     cfg.precision = 1 if a.precision == "mixed" else 0
     ctx = nv.Context(cfg, device=0)
+    if a.lusgs_mode is not None:
+        ctx.set_lusgs_mode(a.lusgs_mode)
     dt = ctx.channel_init(ch)
     print("setup done: dt=%.4e" % dt, file=sys.stderr, flush=True)
     for s in range(a.warmup):

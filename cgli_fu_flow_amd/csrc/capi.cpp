@@ -639,7 +639,7 @@ int ctx_dts(cgli_ctx *c, double deltaT, double e, double er[5], int sweeps, int 
             tend(c, T_BWD);
             if (sw == sweeps - 1) {
                 tbegin(c, T_SCATTER);
-                launch_march_scatter(g, c->marchF + mf, c->marchF + 2 * mf, f5(c, CGLI_NL), f5(c, CGLI_NR), s);
+                launch_march_scatter(g, c->marchF + 2 * mf, f5(c, CGLI_NL), f5(c, CGLI_NR), s);
                 tend(c, T_SCATTER);
             }
         } else if (c->lusgs_mode >= 1) {
@@ -673,7 +673,8 @@ int ctx_dts(cgli_ctx *c, double deltaT, double e, double er[5], int sweeps, int 
             }
             tend(c, T_BWD);
         }
-        launch_copy_interior(g, c5(c, CGLI_NR), f5(c, CGLI_NL), s);
+        // NL = NR at the end of a sweep pair (the march scatter writes both)
+        if (c->lusgs_mode != 3) launch_copy_interior(g, c5(c, CGLI_NR), f5(c, CGLI_NL), s);
     }
     tbegin(c, T_UPDATE);
     const int nblk = launch_update_norm(g, c->p, f5(c, CGLI_U_), c->f[CGLI_J], c5(c, CGLI_NR), f5(c, CGLI_RES),

@@ -124,7 +124,8 @@ void launch_split_march(const Geo &g, const Phys &p, double deltaT, double e, C5
 void launch_lusgs_march(const Geo &g, int forward, const void *coefM, const double *dgnM, int fp32,
                         const double *Rm, double *Xm, C5 Xg, double *xbuf, double *trash, const int *order,
                         int ntiles, int *counters, int max_spin, hipStream_t s);
-void launch_march_scatter(const Geo &g, const double *nlM, const double *nrM, F5 NL, F5 NR, hipStream_t s);
+// NR (march order) into the fields NR and NL: the end of a sweep pair
+void launch_march_scatter(const Geo &g, const double *nrM, F5 NL, F5 NR, hipStream_t s);
 
 
 // learned initial guess (cnn.hip): see that file for layouts
@@ -177,6 +178,8 @@ struct ChannelState {
     cgli_net *net = nullptr;
     int used_net = 0;        // last step: 1 if the network's guess was taken
     double r_net[5] = {0, 0, 0, 0, 0}, r_ext[5] = {0, 0, 0, 0, 0};
+    int since_check = 0;     // steps since the network's guess was last checked
+    bool verify = true;      // check at the next step (first use, after a rejection)
 };
 ChannelState &ctx_channel(cgli_ctx *c);
 // network side of the channel driver (learned.cpp)
@@ -185,6 +188,13 @@ int net_ready(cgli_net *net, cgli_ctx *c);                  // trained, 3 record
 // whole-grid prediction on the latest records: Y [cell][16] and the
 // latest record's inverse scales rs[NET_NSCALE], both on the device
 int net_predict(cgli_net *net, hipStream_t s, const float **Y, const float **rs);
+// start the prediction for the next step on the network's stream, after the
+// context stream's work so far (the latest record); it runs while the host
+// does other work (the filter) and the next guess takes it
+int net_predict_ahead(cgli_net *net, cgli_ctx *c);
+// the prediction started ahead for the current records, if any: the
+// context stream waits for it; false when there is none
+bool net_take_ahead(cgli_net *net, cgli_ctx *c, const float **Y, const float **rs);
 // the ring's record before the latest ([cell][5], device), or null when
 // the ring does not hold two records of this grid
 const float *net_prev_record(cgli_net *net, cgli_ctx *c);

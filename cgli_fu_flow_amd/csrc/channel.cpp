@@ -254,6 +254,11 @@ int guess_residual(cgli_ctx *ctx, double deltaT, double out[5])
     return r;
 }
 
+// The network's guess is checked against the linear extrapolation (one
+// more residual evaluation) on its first use, on every NET_CHECK_EVERY-th
+// step and on every step after a rejection; in between it is taken as it is.
+constexpr int NET_CHECK_EVERY = 8;
+
 // U_ at the start of a step (after the time shift U_ = U = U^n).  *ready
 // is set when the last evaluation left fluxes and right-hand side of the
 // chosen guess in place, so that the first inner iteration can reuse them.
@@ -276,23 +281,32 @@ int initial_guess(cgli_ctx *ctx, double deltaT, ChannelState &cs, bool *ready)
     const float *prev = cs.guess >= 2 ? net_prev_record(cs.net, ctx) : nullptr;
     if (cs.guess == 2 && prev && net_ready(cs.net, ctx)) {
         const float *Y = nullptr, *rs = nullptr;
-        int r = net_predict(cs.net, s, &Y, &rs);
+        int r = CGLI_OK;
+        if (!net_take_ahead(cs.net, ctx, &Y, &rs)) r = net_predict(cs.net, s, &Y, &rs);
         if (r) return r;
-        // the extrapolated guess first, the network's last: when the
-        // network's is taken its right-hand side is still in place
-        net_guess(g, U, Uq, J, nullptr, nullptr, nullptr, U_, s);
-        r = guess_residual(ctx, deltaT, cs.r_ext);
-        if (r) return r;
-        net_guess(g, U, Uq, J, prev, Y, rs, U_, s);
-        r = guess_residual(ctx, deltaT, cs.r_net);
-        if (r) return r;
-        bool worse = false;
-        for (int q = 0; q < 5; q++) worse = worse || cs.r_net[q] > cs.r_ext[q];
-        if (worse) {
+        if (cs.verify || cs.since_check + 1 >= NET_CHECK_EVERY) {
+            // the extrapolated guess first, the network's last: when the
+            // network's is taken its right-hand side is still in place
             net_guess(g, U, Uq, J, nullptr, nullptr, nullptr, U_, s);
+            r = guess_residual(ctx, deltaT, cs.r_ext);
+            if (r) return r;
+            net_guess(g, U, Uq, J, prev, Y, rs, U_, s);
+            r = guess_residual(ctx, deltaT, cs.r_net);
+            if (r) return r;
+            bool worse = false;
+            for (int q = 0; q < 5; q++) worse = worse || cs.r_net[q] > cs.r_ext[q];
+            if (worse) {
+                net_guess(g, U, Uq, J, nullptr, nullptr, nullptr, U_, s);
+            } else {
+                cs.used_net = 1;
+                *ready = true;
+            }
+            cs.verify = worse;
+            cs.since_check = 0;
         } else {
+            net_guess(g, U, Uq, J, prev, Y, rs, U_, s);
             cs.used_net = 1;
-            *ready = true;
+            cs.since_check++;
         }
     } else if (cs.guess == 3 && prev) {
         net_guess(g, U, Uq, J, prev, nullptr, nullptr, U_, s);
@@ -329,14 +343,20 @@ int cgli_channel_rule_state(cgli_ctx *ctx, double E[5], int set)
 int cgli_channel_set_guess(cgli_ctx *ctx, int guess)
 {
     if (!ctx || guess < 0 || guess > 3) return CGLI_E_ARG;
-    ctx_channel(ctx).guess = guess;
+    ChannelState &cs = ctx_channel(ctx);
+    cs.guess = guess;
+    cs.verify = true;   // the network's guess is checked on its next use
+    cs.since_check = 0;
     return CGLI_OK;
 }
 
 int cgli_channel_attach_net(cgli_ctx *ctx, cgli_net *net)
 {
     if (!ctx) return CGLI_E_ARG;
-    ctx_channel(ctx).net = net;
+    ChannelState &cs = ctx_channel(ctx);
+    cs.net = net;
+    cs.verify = true;
+    cs.since_check = 0;
     return CGLI_OK;
 }
 
@@ -419,7 +439,9 @@ int cgli_channel_step(cgli_ctx *ctx, double deltaT, double tol, int max_inner, i
         launch_copy_field(u, u_, N, s);
     }
     if (cs.net) {
-        const int r = net_record(cs.net, ctx);
+        int r = net_record(cs.net, ctx);
+        // the next step's prediction runs on the network's stream from here
+        if (!r && cs.guess == 2) r = net_predict_ahead(cs.net, ctx);
         if (r) return r;
     }
     if (hipStreamSynchronize(s) != hipSuccess) return CGLI_E_HIP;

@@ -1527,9 +1527,11 @@ __global__ void __launch_bounds__(256) k_split_march(Geo g, Phys p, double delta
     for (int v = 0; v < 5; v++) orr[v * 64] = MR.v[v][c];
 }
 
-// march-ordered NL and NR back into their fields, interior cells only
-__global__ void __launch_bounds__(256) k_march_scatter(Geo g, March M, const double *__restrict__ nlM,
-                                                       const double *__restrict__ nrM, F5 NL, F5 NR)
+// the march-ordered increments NR back into their field, and into NL
+// (a sweep pair ends with NL = NR, Dual_time_stepping.cpp), interior cells
+// only
+__global__ void __launch_bounds__(256) k_march_scatter(Geo g, March M, const double *__restrict__ nrM, F5 NL,
+                                                       F5 NR)
 {
     const MGather q = mgather_at(g, M);
     if (!q.ok) return;
@@ -1537,8 +1539,9 @@ __global__ void __launch_bounds__(256) k_march_scatter(Geo g, March M, const dou
     const long o = march_row(M, q.T, q.t) * 5 * 64 + q.l;
 #pragma unroll
     for (int v = 0; v < 5; v++) {
-        NL.v[v][c] = nlM[o + v * 64];
-        NR.v[v][c] = nrM[o + v * 64];
+        const double x = nrM[o + v * 64];
+        NL.v[v][c] = x;
+        NR.v[v][c] = x;
     }
 }
 
@@ -1888,10 +1891,10 @@ void launch_split_march(const Geo &g, const Phys &p, double deltaT, double e, C5
                            static_cast<double *>(coefM), dgnM, mrM);
 }
 
-void launch_march_scatter(const Geo &g, const double *nlM, const double *nrM, F5 NL, F5 NR, hipStream_t s)
+void launch_march_scatter(const Geo &g, const double *nrM, F5 NL, F5 NR, hipStream_t s)
 {
     const March M = march_of(g);
-    hipLaunchKernelGGL(k_march_scatter, mgather_grid(M), dim3(256), 0, s, g, M, nlM, nrM, NL, NR);
+    hipLaunchKernelGGL(k_march_scatter, mgather_grid(M), dim3(256), 0, s, g, M, nrM, NL, NR);
 }
 
 void launch_lusgs_march(const Geo &g, int forward, const void *coefM, const double *dgnM, int fp32,

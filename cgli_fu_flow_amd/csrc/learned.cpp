@@ -99,6 +99,11 @@ struct cgli_net {
     double *part10 = nullptr;    // partial sums of the record scales
     Work wk;
     hipEvent_t ev[4] = {};
+    // a prediction started at the end of a step on this stream, for the
+    // next step's guess: valid while ahead_rec equals nrec
+    hipEvent_t ahead_in = nullptr, ahead_out = nullptr;
+    long ahead_rec = -1;
+    const float *ahead_Y = nullptr, *ahead_rs = nullptr;
     std::vector<EventPair> pred_ev;   // one pair per prediction since the last timing read
     size_t pred_used = 0;
     double predict_ms = 0, train_ms = 0;
@@ -250,6 +255,9 @@ int net_record(cgli_net *net, cgli_ctx *c)
 {
     const Geo &g = ctx_geo(c);
     if (!net->ring || g.nx != net->nx || g.ny != net->ny || g.nz != net->nz) return CGLI_E_STATE;
+    // a prediction still running reads the ring: the new record waits
+    if (net->ahead_rec >= 0 && hipStreamWaitEvent(ctx_stream(c), net->ahead_out, 0) != hipSuccess)
+        return CGLI_E_HIP;
     double *u[5], *uq[5], *J = nullptr;
     for (int q = 0; q < 5; q++) {
         if (ctx_field(c, CGLI_U + q, &u[q]) || ctx_field(c, CGLI_UQ + q, &uq[q])) return CGLI_E_NOMEM;
@@ -296,6 +304,30 @@ int net_predict(cgli_net *net, hipStream_t s, const float **Y, const float **rs)
     return hipGetLastError() == hipSuccess ? CGLI_OK : CGLI_E_HIP;
 }
 
+int net_predict_ahead(cgli_net *net, cgli_ctx *c)
+{
+    net->ahead_rec = -1;
+    if (!net_ready(net, c)) return CGLI_OK;
+    if (hipEventRecord(net->ahead_in, ctx_stream(c)) != hipSuccess ||
+        hipStreamWaitEvent(net->stream, net->ahead_in, 0) != hipSuccess)
+        return CGLI_E_HIP;
+    const int r = net_predict(net, net->stream, &net->ahead_Y, &net->ahead_rs);
+    if (r) return r;
+    if (hipEventRecord(net->ahead_out, net->stream) != hipSuccess) return CGLI_E_HIP;
+    net->ahead_rec = net->nrec;
+    return CGLI_OK;
+}
+
+bool net_take_ahead(cgli_net *net, cgli_ctx *c, const float **Y, const float **rs)
+{
+    if (!net || net->ahead_rec < 0 || net->ahead_rec != net->nrec) return false;
+    if (hipStreamWaitEvent(ctx_stream(c), net->ahead_out, 0) != hipSuccess) return false;
+    *Y = net->ahead_Y;
+    *rs = net->ahead_rs;
+    net->ahead_rec = -1;
+    return true;
+}
+
 }  // namespace cgli
 
 extern "C" {
@@ -319,6 +351,8 @@ int cgli_net_create(int device, unsigned long long seed, cgli_net **out)
     ok = ok && hipMalloc(&n->wb, sizeof(uint16_t) * NPARAM) == hipSuccess;
     for (int l = 1; l < 4 && ok; l++) ok = hipMalloc(&n->wt[l], sizeof(uint16_t) * 27 * 32 * 32) == hipSuccess;
     for (int e = 0; e < 4 && ok; e++) ok = hipEventCreate(&n->ev[e]) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&n->ahead_in, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&n->ahead_out, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         cgli_net_destroy(n);
         return CGLI_E_NOMEM;
@@ -368,6 +402,8 @@ int cgli_net_destroy(cgli_net *n)
         if (p) hipFree(p);
     for (int e = 0; e < 4; e++)
         if (n->ev[e]) hipEventDestroy(n->ev[e]);
+    if (n->ahead_in) hipEventDestroy(n->ahead_in);
+    if (n->ahead_out) hipEventDestroy(n->ahead_out);
     for (EventPair &e : n->pred_ev) {
         hipEventDestroy(e.a);
         hipEventDestroy(e.b);
@@ -396,6 +432,7 @@ int cgli_net_params(cgli_net *n, float *params, int set)
 int cgli_net_eval(cgli_net *n, int B, int D, int H, int W, int periodic, const float *x, const float *t,
                   const unsigned char *mask, float *y, double *loss, float *grad)
 {
+    if (n) n->ahead_rec = -1;   // its work buffers, weights or ring change
     if (!n || !x || B < 1 || D < 1 || H < 1 || W < 16 || W % 16) return CGLI_E_ARG;
     if ((loss || grad) && (!t || !mask)) return CGLI_E_ARG;
     if (hipSetDevice(n->device) != hipSuccess) return CGLI_E_HIP;
@@ -437,6 +474,7 @@ int cgli_net_eval(cgli_net *n, int B, int D, int H, int W, int periodic, const f
 
 int cgli_net_adam(cgli_net *n, float lr)
 {
+    if (n) n->ahead_rec = -1;   // its work buffers, weights or ring change
     if (!n || !(lr > 0)) return CGLI_E_ARG;
     if (hipSetDevice(n->device) != hipSuccess) return CGLI_E_HIP;
     adam_step(n, lr, n->stream);
@@ -445,6 +483,7 @@ int cgli_net_adam(cgli_net *n, float lr)
 
 int cgli_net_ring(cgli_net *n, int capacity, int nx, int ny, int nz)
 {
+    if (n) n->ahead_rec = -1;   // its work buffers, weights or ring change
     if (!n || capacity < 4 || nx < 1 || ny < 1 || nz < 16 || nz % 16) return CGLI_E_ARG;
     if (hipSetDevice(n->device) != hipSuccess) return CGLI_E_HIP;
     hipStreamSynchronize(n->stream);
@@ -466,6 +505,7 @@ int cgli_net_ring(cgli_net *n, int capacity, int nx, int ny, int nz)
 
 int cgli_net_push(cgli_net *n, const float *dq)
 {
+    if (n) n->ahead_rec = -1;   // its work buffers, weights or ring change
     if (!n || !dq || !n->ring) return CGLI_E_ARG;
     if (hipSetDevice(n->device) != hipSuccess) return CGLI_E_HIP;
     hipStreamSynchronize(n->stream);
@@ -480,6 +520,7 @@ int cgli_net_push(cgli_net *n, const float *dq)
 
 int cgli_net_rewind(cgli_net *n, long nrec)
 {
+    if (n) n->ahead_rec = -1;   // its work buffers, weights or ring change
     // the records nrec-3 .. nrec-1 must still be in the ring
     if (!n || nrec < 0 || nrec > n->nrec || n->nrec - nrec > n->cap - 3) return CGLI_E_ARG;
     n->nrec = nrec;
@@ -507,6 +548,7 @@ int cgli_net_record_scales(cgli_net *n, long rec, float inv[15])
 int cgli_net_train(cgli_net *n, int steps, int batch, int cd, int ch, int cw, float lr, unsigned long long seed,
                    float *loss)
 {
+    if (n) n->ahead_rec = -1;   // its work buffers, weights or ring change
     const int halo = 4;
     if (!n || steps < 1 || batch < 1 || !(lr > 0) || !n->ring) return CGLI_E_ARG;
     if (cd > n->nx || ch > n->ny || cw > n->nz || cw % 16 || cd <= 2 * halo || cw <= 2 * halo || ch < 1)
@@ -565,6 +607,7 @@ int cgli_net_train(cgli_net *n, int steps, int batch, int cd, int ch, int cw, fl
 
 int cgli_net_predict(cgli_net *n, float *x, float *y)
 {
+    if (n) n->ahead_rec = -1;   // its work buffers, weights or ring change
     if (!n || !n->ring || n->nrec < 3) return CGLI_E_STATE;
     if (hipSetDevice(n->device) != hipSuccess) return CGLI_E_HIP;
     const float *Y = nullptr, *rs = nullptr;

@@ -207,14 +207,19 @@ int cgli_channel_rule_state(cgli_ctx *ctx, double E[5], int set);
  * quadratic extrapolation plus s2 y / J, y the attached network's output on
  * its latest records and s2 their scale (cgli_net_record_scales), unless the
  * residual at that guess is larger (any variable) than at guess 1, which is
- * then used.  Modes 2 and 3 fall back to guess 1 while the network is not
+ * then used.  That check costs one residual evaluation; it is made on the
+ * network's first step, on every 8th step and on every step after a
+ * rejection.  The prediction itself starts at the end of the previous step
+ * on the network's stream, so it overlaps what the host does between
+ * steps (the filter).  Modes 2 and 3 fall back to guess 1 while the network is not
  * ready (trained, three records of this grid) or the ring lacks dq^{n-1}. */
 int cgli_channel_set_guess(cgli_ctx *ctx, int guess);
 /* record every step's increment per unit volume, J (U^{n+1} - U^n), into
  * the network's ring (net may be NULL to detach) */
 int cgli_channel_attach_net(cgli_ctx *ctx, cgli_net *net);
 /* last step: *used_net = 1 if the network's guess was taken; residual sums
- * of squares per variable at the network's and the extrapolated guess */
+ * of squares per variable at the network's and the extrapolated guess, as
+ * of the last step that checked it */
 int cgli_channel_guess_info(cgli_ctx *ctx, int *used_net, double r_net[5], double r_ext[5]);
 
 /* ---- learned initial guess: 3-D CNN (no counterpart in the reference) --
