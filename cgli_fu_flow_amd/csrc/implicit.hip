@@ -1456,8 +1456,11 @@ __device__ __forceinline__ MPos mpos_of_lane(int q)
     return MPos{0, 0};
 }
 
-// grid: (steps / MR_ROWS rounded up, tiles), blocks of 4 waves = 4 tile rows
-// (times ceil(TJ / 4) in x)
+// grid: x = the tile's column TKi (fastest) times ceil(TJ / 4) row groups,
+// y = block of MR_ROWS steps, z = the tile's row TJi; blocks of 4 waves =
+// 4 tile rows.  Tiles next to each other in k come one after the other, so
+// a cache line shared by two of them (the field layout's rows of k) is
+// still in L2 for the second.
 struct MGather {
     int T, t, l, jl, kl, jo, ko, il;
     bool ok;
@@ -1469,13 +1472,12 @@ __device__ __forceinline__ MGather mgather_at(const Geo &g, const March &M)
     MGather q;
     const int wv = threadIdx.x >> 6;
     q.jl = (blockIdx.x % JB) * WPB + wv;
-    const int tb = blockIdx.x / JB;
+    const int TKi = blockIdx.x / JB, TJi = blockIdx.z;
     const MPos P = mpos_of_lane(threadIdx.x & 63);
-    q.T = blockIdx.y;
-    q.t = tb * MR_ROWS + P.r;
+    q.T = TJi * M.ntk + TKi;
+    q.t = blockIdx.y * MR_ROWS + P.r;
     q.kl = P.kl;
     q.l = q.jl * TK + q.kl;
-    const int TJi = q.T / M.ntk, TKi = q.T % M.ntk;
     q.jo = TJi * TJ + q.jl;
     q.ko = TKi * TK + q.kl;
     q.il = q.t - q.jl - q.kl;
@@ -1485,7 +1487,7 @@ __device__ __forceinline__ MGather mgather_at(const Geo &g, const March &M)
 static dim3 mgather_grid(const March &M)
 {
     constexpr int JB = (MARCH_TJ + 3) / 4;
-    return dim3(((M.S + MR_ROWS - 1) / MR_ROWS) * JB, M.ntj * M.ntk);
+    return dim3(M.ntk * JB, (M.S + MR_ROWS - 1) / MR_ROWS, M.ntj);
 }
 
 template <typename CT>
