@@ -1456,11 +1456,14 @@ __device__ __forceinline__ MPos mpos_of_lane(int q)
     return MPos{0, 0};
 }
 
-// grid: x = the tile's column TKi (fastest) times ceil(TJ / 4) row groups,
-// y = block of MR_ROWS steps, z = the tile's row TJi; blocks of 4 waves =
-// 4 tile rows.  Tiles next to each other in k come one after the other, so
-// a cache line shared by two of them (the field layout's rows of k) is
-// still in L2 for the second.
+// Work units: (tile column TKi, group of 4 tile rows, block of MR_ROWS
+// steps, tile row TJi); a block of 4 waves takes one.  The grid is
+// XCD-aware: workgroup b runs on XCD b % 8, and XCD x takes the units of a
+// contiguous group of tile columns, k-neighbours first, then step blocks.
+// A cache line of a field-layout row of k shared by two tiles, and the
+// planes two step blocks share, are then read through one XCD's L2.  (With
+// another placement of workgroups the result is the same, only slower.)
+constexpr int MG_XCD = 8;
 struct MGather {
     int T, t, l, jl, kl, jo, ko, il;
     bool ok;
@@ -1471,11 +1474,20 @@ __device__ __forceinline__ MGather mgather_at(const Geo &g, const March &M)
     constexpr int JB = (TJ + WPB - 1) / WPB;   // blocks per tile row set
     MGather q;
     const int wv = threadIdx.x >> 6;
-    q.jl = (blockIdx.x % JB) * WPB + wv;
-    const int TKi = blockIdx.x / JB, TJi = blockIdx.z;
+    const int x = blockIdx.x % MG_XCD, u = blockIdx.x / MG_XCD;
+    const int k0 = x * M.ntk / MG_XCD, nk = (x + 1) * M.ntk / MG_XCD - k0;
+    const int ntb = (M.S + MR_ROWS - 1) / MR_ROWS;
+    q.ok = false;
+    if (nk == 0 || u >= nk * JB * ntb * M.ntj) return q;
+    const int TKi = k0 + u % nk;
+    int r = u / nk;
+    const int jb = r % JB;
+    r /= JB;
+    const int tb = r % ntb, TJi = r / ntb;
+    q.jl = jb * WPB + wv;
     const MPos P = mpos_of_lane(threadIdx.x & 63);
     q.T = TJi * M.ntk + TKi;
-    q.t = blockIdx.y * MR_ROWS + P.r;
+    q.t = tb * MR_ROWS + P.r;
     q.kl = P.kl;
     q.l = q.jl * TK + q.kl;
     q.jo = TJi * TJ + q.jl;
@@ -1487,7 +1499,8 @@ __device__ __forceinline__ MGather mgather_at(const Geo &g, const March &M)
 static dim3 mgather_grid(const March &M)
 {
     constexpr int JB = (MARCH_TJ + 3) / 4;
-    return dim3(M.ntk * JB, (M.S + MR_ROWS - 1) / MR_ROWS, M.ntj);
+    const long per_xcd = (long)((M.ntk + MG_XCD - 1) / MG_XCD) * JB * ((M.S + MR_ROWS - 1) / MR_ROWS) * M.ntj;
+    return dim3((unsigned)(MG_XCD * per_xcd));
 }
 
 template <typename CT>
