@@ -2,6 +2,8 @@
 //   k_bc_y          Y_boundary_condition.cpp:98-182 (isothermal no-slip walls)
 //   k_bc_z          Z_boundary_condition.cpp:48-66 (one periodic layer)
 //   k_bc_x_periodic three periodic planes at each x end of a single slab
+//   k_pack_q / k_unpack_q  one block of the restart file (Output.cpp:183-201,
+//                   Initial_condition.cpp:196-212)
 #include "cgli_internal.hpp"
 
 namespace cgli {
@@ -112,6 +114,50 @@ __global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src,
 void launch_copy_field(double *dst, const double *src, long n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, n);
+}
+
+// One block of the restart file: the interior cells of one field in (i, j, k)
+// order with k fastest, Q = U*J (Output.cpp:183-201).
+__global__ void k_pack_q(Geo g, const double *__restrict__ U, const double *__restrict__ J,
+                         double *__restrict__ out)
+{
+    const long n = (long)g.nx * g.ny * g.nz;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int k = (int)(t % g.nz);
+    const long r = t / g.nz;
+    const int j = (int)(r % g.ny), i = (int)(r / g.ny);
+    const long c = ix(g, i + 3, j + 2, k + 2);
+    out[t] = U[c] * J[c];
+}
+
+// The inverse (Initial_condition.cpp:196-212): U = Q/J, and U2 = U where the
+// reference also starts the new level from the read one (U_ = U).
+__global__ void k_unpack_q(Geo g, const double *__restrict__ in, const double *__restrict__ J,
+                           double *__restrict__ U, double *__restrict__ U2)
+{
+    const long n = (long)g.nx * g.ny * g.nz;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int k = (int)(t % g.nz);
+    const long r = t / g.nz;
+    const int j = (int)(r % g.ny), i = (int)(r / g.ny);
+    const long c = ix(g, i + 3, j + 2, k + 2);
+    const double v = in[t] / J[c];
+    U[c] = v;
+    if (U2) U2[c] = v;
+}
+
+void launch_pack_q(const Geo &g, const double *U, const double *J, double *out, hipStream_t s)
+{
+    const long n = (long)g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL(k_pack_q, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, U, J, out);
+}
+
+void launch_unpack_q(const Geo &g, const double *in, const double *J, double *U, double *U2, hipStream_t s)
+{
+    const long n = (long)g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL(k_unpack_q, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, in, J, U, U2);
 }
 
 }  // namespace cgli

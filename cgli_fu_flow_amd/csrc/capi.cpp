@@ -3,6 +3,9 @@
 // computed by the kernels in flux.hip, viscous.hip, implicit.hip, bc.hip.
 #include <hip/hip_runtime.h>
 
+#include <fcntl.h>
+#include <unistd.h>
+
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -36,6 +39,7 @@ struct cgli_ctx {
     int *ijk_dev = nullptr;
     std::vector<int> Nijk;       // (gend+NY+NZ+8) x 2
     std::vector<int> ijk;        // (cells+1) x 3
+    int diss_scale = 0;          // Roe dissipation scaling: 0 none, 1 scalar Ep, 2 filter sensor
     int lusgs_mode = 3;          // 0: one launch per hyperplane, 1: tiled wavefront, 2: with loader waves,
                                  // 3: tiled wavefront on march-ordered records (default)
     double *coef = nullptr;      // split-coefficient blocks of the wavefront (float in the mixed mode)
@@ -187,6 +191,7 @@ const char *cgli_error_string(int code)
     case CGLI_E_STATE: return "a required input field was never set";
     case CGLI_E_NODEV: return "no HIP device";
     case CGLI_E_TIMEOUT: return "bounded device wait expired";
+    case CGLI_E_IO: return "restart file could not be opened, read or written";
     default: return "unknown error";
     }
 }
@@ -333,15 +338,20 @@ static int flux_common(cgli_ctx *c, int dir, double Ep, double e)
     if (r) return r;
     r = ensure_range(c, mf, 9);   // unset metric components are never read
     if (r) return r;
+    const int ep_id = CGLI_EPX + dir;
+    if (c->diss_scale == 2 && !have(c, ep_id, 1)) return CGLI_E_STATE;
+    const double epS = c->diss_scale == 1 ? Ep : 1.0;
+    const double *EpF = c->diss_scale == 2 ? c->f[ep_id] : nullptr;
     tbegin(c, T_FLUX_X + dir);
     if (dir == 0)
-        launch_flux_x(c->g, c->p, e, c5(c, CGLI_U_), c->f[CGLI_J], c->f[jf], m9(c, mf), f5(c, out), c->stream);
+        launch_flux_x(c->g, c->p, e, c5(c, CGLI_U_), c->f[CGLI_J], c->f[jf], m9(c, mf), f5(c, out), epS, EpF,
+                      c->stream);
     else if (dir == 1)
         launch_flux_y(c->g, c->p, Ep, e, c->cfg.xi_mode, c5(c, CGLI_U_), c->f[CGLI_J], c->f[jf], m9(c, mf),
-                      f5(c, out), c->stream);
+                      f5(c, out), epS, EpF, c->stream);
     else
         launch_flux_z(c->g, c->p, e, c->cfg.xi_mode, c5(c, CGLI_U_), c->f[CGLI_J], c->f[jf], m9(c, mf),
-                      f5(c, out), c->stream);
+                      f5(c, out), epS, EpF, c->stream);
     tend(c, T_FLUX_X + dir);
     HIPCHK(hipGetLastError());
     mark(c, out, 5);
@@ -351,6 +361,13 @@ static int flux_common(cgli_ctx *c, int dir, double Ep, double e)
 int cgli_flux_x(cgli_ctx *c, double Ep, double e) { return flux_common(c, 0, Ep, e); }
 int cgli_flux_y(cgli_ctx *c, double Ep, double e) { return flux_common(c, 1, Ep, e); }
 int cgli_flux_z(cgli_ctx *c, double Ep, double e) { return flux_common(c, 2, Ep, e); }
+
+int cgli_set_dissipation_scaling(cgli_ctx *c, int mode)
+{
+    if (!c || mode < 0 || mode > 2) return CGLI_E_ARG;
+    c->diss_scale = mode;
+    return CGLI_OK;
+}
 
 int cgli_viscous(cgli_ctx *c)
 {
@@ -752,6 +769,91 @@ int cgli_bc_x_periodic(cgli_ctx *c)
     HIPCHK(hipGetLastError());
     return CGLI_OK;
 }
+
+}  // extern "C"
+
+// Whole-buffer positional transfers (short counts from pread/pwrite repeat).
+static bool file_xfer(int fd, void *buf, size_t bytes, off_t off, bool write)
+{
+    char *b = static_cast<char *>(buf);
+    while (bytes) {
+        const ssize_t n = write ? pwrite(fd, b, bytes, off) : pread(fd, b, bytes, off);
+        if (n <= 0) return false;   // error, or end of file before the block ends
+        b += n;
+        off += n;
+        bytes -= (size_t)n;
+    }
+    return true;
+}
+
+// The restart file of Output.cpp:167-284 / Initial_condition.cpp:112-219:
+// block b (U1..U5, then U1q..U5q) holds nx_global*ny*nz doubles, and this
+// slab's cells start gstart planes into it (the MPI_File_set_view offsets,
+// Output.cpp:232-279).  One block at a time goes through a device staging
+// buffer and a pinned host buffer.
+static int restart_io(cgli_ctx *c, const char *path, bool write)
+{
+    if (!c || !path) return CGLI_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    const cgli_config &cf = c->cfg;
+    if (cf.gstart < 0 || cf.nx_global < cf.gstart + cf.nx) return CGLI_E_ARG;
+    if (!have(c, CGLI_J, 1)) return CGLI_E_STATE;
+    if (write && (!have(c, CGLI_U, 5) || !have(c, CGLI_UQ, 5))) return CGLI_E_STATE;
+    if (!write) {
+        int r = ensure_range(c, CGLI_U_, 15);   // U_, U, Uq
+        if (r) return r;
+    }
+    const long plane = (long)cf.ny * cf.nz;
+    const size_t bytes = sizeof(double) * (size_t)cf.nx * plane;
+    void *dbuf = nullptr, *hbuf = nullptr;
+    if (hipMalloc(&dbuf, bytes) != hipSuccess) return CGLI_E_NOMEM;
+    if (hipHostMalloc(&hbuf, bytes) != hipSuccess) {
+        hipFree(dbuf);
+        return CGLI_E_NOMEM;
+    }
+    const int fd = write ? open(path, O_RDWR | O_CREAT, 0644) : open(path, O_RDONLY);
+    int rc = fd < 0 ? CGLI_E_IO : CGLI_OK;
+    for (int b = 0; b < 10 && rc == CGLI_OK; b++) {
+        const int fid = (b < 5 ? CGLI_U : CGLI_UQ) + b % 5;
+        const off_t off = (off_t)sizeof(double) * ((off_t)b * cf.nx_global * plane + (off_t)cf.gstart * plane);
+        double *d = static_cast<double *>(dbuf);
+        if (write) {
+            launch_pack_q(c->g, c->f[fid], c->f[CGLI_J], d, c->stream);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(hbuf, d, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess)
+                rc = CGLI_E_HIP;
+            else if (!file_xfer(fd, hbuf, bytes, off, true))
+                rc = CGLI_E_IO;
+        } else {
+            if (!file_xfer(fd, hbuf, bytes, off, false)) {
+                rc = CGLI_E_IO;
+                break;
+            }
+            if (hipMemcpyAsync(d, hbuf, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+                rc = CGLI_E_HIP;
+                break;
+            }
+            launch_unpack_q(c->g, d, c->f[CGLI_J], c->f[fid], b < 5 ? c->f[CGLI_U_ + b] : nullptr, c->stream);
+            // the host buffer is refilled by the next block: wait for this one
+            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) rc = CGLI_E_HIP;
+        }
+    }
+    if (fd >= 0 && close(fd) != 0 && rc == CGLI_OK) rc = CGLI_E_IO;
+    if (hipStreamSynchronize(c->stream) != hipSuccess && rc == CGLI_OK) rc = CGLI_E_HIP;
+    hipHostFree(hbuf);
+    hipFree(dbuf);
+    if (rc == CGLI_OK && !write) {
+        mark(c, CGLI_U_, 15);
+        ctx_channel(c).verify = true;
+    }
+    return rc;
+}
+
+extern "C" {
+
+int cgli_write_restart(cgli_ctx *c, const char *path) { return restart_io(c, path, true); }
+int cgli_read_restart(cgli_ctx *c, const char *path) { return restart_io(c, path, false); }
 
 int cgli_timing_enable(cgli_ctx *c, int on)
 {

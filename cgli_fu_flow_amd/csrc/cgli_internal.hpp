@@ -41,12 +41,14 @@ __host__ __device__ inline long ix(const Geo &g, int i, int j, int k)
 }
 
 // launchers (kernels.hip)
+// epS scales every face's Roe dissipation, or EpF[face] does when EpF is
+// not null (1 and null: the live reference flux)
 void launch_flux_x(const Geo &g, const Phys &p, double e, C5 U, const double *J, const double *Jf, M9 mf,
-                   F5 F, hipStream_t s);
+                   F5 F, double epS, const double *EpF, hipStream_t s);
 void launch_flux_y(const Geo &g, const Phys &p, double Ep, double e, int xi_mode, C5 U, const double *J,
-                   const double *Jf, M9 mf, F5 F, hipStream_t s);
+                   const double *Jf, M9 mf, F5 F, double epS, const double *EpF, hipStream_t s);
 void launch_flux_z(const Geo &g, const Phys &p, double e, int xi_mode, C5 U, const double *J, const double *Jf,
-                   M9 mf, F5 F, hipStream_t s);
+                   M9 mf, F5 F, double epS, const double *EpF, hipStream_t s);
 void launch_viscous(const Geo &g, const Phys &p, C5 U, const double *J, const double *Ju, M9 mu,
                     const double *Jv, M9 mv, const double *Jw, M9 mw, double *prim4, double *faces12,
                     F4 vF, hipStream_t s);
@@ -72,6 +74,8 @@ void launch_bc_y(const Geo &g, const Phys &p, int gstart, F5 U, const double *J,
                  double T_high, int heated_lo, int heated_hi, hipStream_t s);
 void launch_bc_z(const Geo &g, F5 U, hipStream_t s);
 void launch_bc_x_periodic(const Geo &g, F5 U, hipStream_t s);
+void launch_pack_q(const Geo &g, const double *U, const double *J, double *out, hipStream_t s);
+void launch_unpack_q(const Geo &g, const double *in, const double *J, double *U, double *U2, hipStream_t s);
 // zero fill by a kernel (safe inside a captured graph, unlike a memset node)
 void launch_zero(double *p, long n, hipStream_t s);
 // copy plane 0 of dst to planes 1..nplanes-1

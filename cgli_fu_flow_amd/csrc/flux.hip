@@ -11,6 +11,10 @@
 // reference expression by expression and the library is compiled without
 // FMA contraction, so results match the CPU build to the last bit except
 // where libm and the device math library round sqrt/pow differently.
+// Optional dissipation scaling (the older flux files, Flux_X~~.cpp:552-556):
+// each face's Roe dissipation Fav is multiplied by epS, or by EpF[face] (the
+// filter sensor, face-indexed like the fluxes) when EpF is given.  The live
+// reference has epS = 1 and no EpF; multiplying by 1 is exact.
 #include "cgli_internal.hpp"
 
 namespace cgli {
@@ -89,7 +93,7 @@ __device__ __forceinline__ T apply(const Rec<T> &r, const double *__restrict__ U
 // jump in the momentum dissipation.
 template <typename T>
 __device__ __forceinline__ void roe_face(const T L[5], const T Rq[5], T mx, T my, T mz, T Jf, T e, T K, T nx_,
-                                         T ny_, T nz_, T F[5], T &pL_out, T &pR_out, T &Fav3_out)
+                                         T ny_, T nz_, T ep, T F[5], T &pL_out, T &pR_out, T &Fav3_out)
 {
     const T insqr = T(1.0) / sqrt(mx * mx + my * my + mz * mz);
 
@@ -148,11 +152,11 @@ __device__ __forceinline__ void roe_face(const T L[5], const T Rq[5], T mx, T my
     const T Fav4 = aU * dU4 + dUc * rho * w + dPc * nz_;
     const T Fav5 = aU * dU5 + dUc * rho * H + dPc * Un * insqr;
 
-    F[0] = T(0.5) * ((rL * UnL + rR * UnR) - Fav1 / insqr) / Jf;
-    F[1] = T(0.5) * ((rL * uL * UnL + rR * uR * UnR + mx * (pL + pR)) - Fav2 / insqr) / Jf;
-    F[2] = T(0.5) * ((rL * vL * UnL + rR * vR * UnR + my * (pL + pR)) - Fav3 / insqr) / Jf;
-    F[3] = T(0.5) * ((rL * wL * UnL + rR * wR * UnR + mz * (pL + pR)) - Fav4 / insqr) / Jf;
-    F[4] = T(0.5) * ((UnL * (T(3.5) * pL + T(0.5) * rL * qL) + UnR * (T(3.5) * pR + T(0.5) * rR * qR)) - Fav5 / insqr) / Jf;
+    F[0] = T(0.5) * ((rL * UnL + rR * UnR) - ep * Fav1 / insqr) / Jf;
+    F[1] = T(0.5) * ((rL * uL * UnL + rR * uR * UnR + mx * (pL + pR)) - ep * Fav2 / insqr) / Jf;
+    F[2] = T(0.5) * ((rL * vL * UnL + rR * vR * UnR + my * (pL + pR)) - ep * Fav3 / insqr) / Jf;
+    F[3] = T(0.5) * ((rL * wL * UnL + rR * wR * UnR + mz * (pL + pR)) - ep * Fav4 / insqr) / Jf;
+    F[4] = T(0.5) * ((UnL * (T(3.5) * pL + T(0.5) * rL * qL) + UnR * (T(3.5) * pR + T(0.5) * rR * qR)) - ep * Fav5 / insqr) / Jf;
     pL_out = pL;
     pR_out = pR;
     Fav3_out = Fav3;
@@ -163,7 +167,8 @@ __device__ __forceinline__ void roe_face(const T L[5], const T Rq[5], T mx, T my
 // [i][j][k] for i=2..gend, j=1..NY-1, k=1..NZ-1.
 template <typename T>
 __global__ void __launch_bounds__(256) k_flux_x(Geo g, double K, double e, C5 U, const double *__restrict__ J,
-                                                const double *__restrict__ Jf, M9 mf, F5 F)
+                                                const double *__restrict__ Jf, M9 mf, F5 F, double epS,
+                                                const double *__restrict__ EpF)
 {
     const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -196,8 +201,9 @@ __global__ void __launch_bounds__(256) k_flux_x(Geo g, double K, double e, C5 U,
     const T xix = ld<T>(mf.m[0], f), xiy = ld<T>(mf.m[1], f), xiz = ld<T>(mf.m[2], f);
     const T insqr = T(1.0) / sqrt(xix * xix + xiy * xiy + xiz * xiz);
     T out[5], pl, pr, fav3;
-    roe_face<T>(Lq, Rq, xix, xiy, xiz, ld<T>(Jf, f), T(e), T(K), xix * insqr, xiy * insqr, xiz * insqr, out, pl, pr,
-                fav3);
+    const T ep = EpF ? ld<T>(EpF, f) : T(epS);
+    roe_face<T>(Lq, Rq, xix, xiy, xiz, ld<T>(Jf, f), T(e), T(K), xix * insqr, xiy * insqr, xiz * insqr, ep, out, pl,
+                pr, fav3);
 #pragma unroll
     for (int v = 0; v < 5; v++) F.v[v][f] = (double)out[v];
 }
@@ -209,7 +215,7 @@ __global__ void __launch_bounds__(256) k_flux_x(Geo g, double K, double e, C5 U,
 template <typename T>
 __global__ void __launch_bounds__(256) k_flux_y(Geo g, double K, double Ep, double e, int xi_mode, C5 U,
                                                 const double *__restrict__ J, const double *__restrict__ Jf,
-                                                M9 mf, F5 F)
+                                                M9 mf, F5 F, double epS, const double *__restrict__ EpF)
 {
     const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -246,7 +252,8 @@ __global__ void __launch_bounds__(256) k_flux_y(Geo g, double K, double Ep, doub
     if (xi_mode) { nx_ = etx * insqr; ny_ = ety * insqr; nz_ = etz * insqr; }
     T out[5], pl, pr, fav3;
     const T jf = ld<T>(Jf, f);
-    roe_face<T>(Lq, Rq, etx, ety, etz, jf, T(e), T(K), nx_, ny_, nz_, out, pl, pr, fav3);
+    const T ep = EpF ? ld<T>(EpF, f) : T(epS);
+    roe_face<T>(Lq, Rq, etx, ety, etz, jf, T(e), T(K), nx_, ny_, nz_, ep, out, pl, pr, fav3);
     if (j == 1 || j == NY) {
         out[0] = 0;
         out[1] = 0;
@@ -264,7 +271,7 @@ __global__ void __launch_bounds__(256) k_flux_y(Geo g, double K, double Ep, doub
 template <typename T>
 __global__ void __launch_bounds__(256) k_flux_z(Geo g, double K, double e, int xi_mode, C5 U,
                                                 const double *__restrict__ J, const double *__restrict__ Jf,
-                                                M9 mf, F5 F)
+                                                M9 mf, F5 F, double epS, const double *__restrict__ EpF)
 {
     const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -295,7 +302,8 @@ __global__ void __launch_bounds__(256) k_flux_z(Geo g, double K, double e, int x
     T nx_ = 0, ny_ = 0, nz_ = 0;
     if (xi_mode) { nx_ = ztx * insqr; ny_ = zty * insqr; nz_ = ztz * insqr; }
     T out[5], pl, pr, fav3;
-    roe_face<T>(Lq, Rq, ztx, zty, ztz, ld<T>(Jf, f), T(e), T(K), nx_, ny_, nz_, out, pl, pr, fav3);
+    const T ep = EpF ? ld<T>(EpF, f) : T(epS);
+    roe_face<T>(Lq, Rq, ztx, zty, ztz, ld<T>(Jf, f), T(e), T(K), nx_, ny_, nz_, ep, out, pl, pr, fav3);
 #pragma unroll
     for (int v = 0; v < 5; v++) F.v[v][f] = (double)out[v];
 }
@@ -306,39 +314,39 @@ static dim3 grid_for(int nk, int nj, int ni, dim3 b)
 }
 
 void launch_flux_x(const Geo &g, const Phys &p, double e, C5 U, const double *J, const double *Jf, M9 mf,
-                   F5 F, hipStream_t s)
+                   F5 F, double epS, const double *EpF, hipStream_t s)
 {
     dim3 b(64, 4);
     if (p.fp32)
         hipLaunchKernelGGL(k_flux_x<float>, grid_for(g.NZ - 1, g.NY - 1, g.gend - 1, b), b, 0, s, g, p.K, e, U, J,
-                           Jf, mf, F);
+                           Jf, mf, F, epS, EpF);
     else
         hipLaunchKernelGGL(k_flux_x<double>, grid_for(g.NZ - 1, g.NY - 1, g.gend - 1, b), b, 0, s, g, p.K, e, U, J,
-                           Jf, mf, F);
+                           Jf, mf, F, epS, EpF);
 }
 
 void launch_flux_y(const Geo &g, const Phys &p, double Ep, double e, int xi_mode, C5 U, const double *J,
-                   const double *Jf, M9 mf, F5 F, hipStream_t s)
+                   const double *Jf, M9 mf, F5 F, double epS, const double *EpF, hipStream_t s)
 {
     dim3 b(64, 4);
     if (p.fp32)
         hipLaunchKernelGGL(k_flux_y<float>, grid_for(g.NZ - 1, g.NY, g.gend - 2, b), b, 0, s, g, p.K, Ep, e, xi_mode,
-                           U, J, Jf, mf, F);
+                           U, J, Jf, mf, F, epS, EpF);
     else
         hipLaunchKernelGGL(k_flux_y<double>, grid_for(g.NZ - 1, g.NY, g.gend - 2, b), b, 0, s, g, p.K, Ep, e, xi_mode,
-                           U, J, Jf, mf, F);
+                           U, J, Jf, mf, F, epS, EpF);
 }
 
 void launch_flux_z(const Geo &g, const Phys &p, double e, int xi_mode, C5 U, const double *J, const double *Jf,
-                   M9 mf, F5 F, hipStream_t s)
+                   M9 mf, F5 F, double epS, const double *EpF, hipStream_t s)
 {
     dim3 b(64, 4);
     if (p.fp32)
         hipLaunchKernelGGL(k_flux_z<float>, grid_for(g.NZ, g.NY - 1, g.gend - 2, b), b, 0, s, g, p.K, e, xi_mode, U,
-                           J, Jf, mf, F);
+                           J, Jf, mf, F, epS, EpF);
     else
         hipLaunchKernelGGL(k_flux_z<double>, grid_for(g.NZ, g.NY - 1, g.gend - 2, b), b, 0, s, g, p.K, e, xi_mode, U,
-                           J, Jf, mf, F);
+                           J, Jf, mf, F, epS, EpF);
 }
 
 }  // namespace cgli

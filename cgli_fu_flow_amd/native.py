@@ -21,7 +21,7 @@ METRIC_NAMES = ["xidx", "xidy", "xidz", "etdx", "etdy", "etdz", "ztdx", "ztdy", 
 
 ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "device allocation failed",
           -4: "non-finite residual norm", -5: "a required input field was never set", -6: "no HIP device",
-          -7: "bounded device wait expired"}
+          -7: "bounded device wait expired", -8: "restart file could not be opened, read or written"}
 
 
 class CgliError(RuntimeError):
@@ -68,6 +68,7 @@ def lib():
         for n in ("cgli_flux_x", "cgli_flux_y", "cgli_flux_z"):
             getattr(L, n).argtypes = [vp, C.c_double, C.c_double]
         L.cgli_viscous.argtypes = [vp]
+        L.cgli_set_dissipation_scaling.argtypes = [vp, C.c_int]
         L.cgli_filter.argtypes = [vp, C.c_double, C.c_double, C.c_double]
         L.cgli_profiles.argtypes = [vp, dp]
         L.cgli_channel_last_step.argtypes = [vp, C.POINTER(C.c_int)]
@@ -77,6 +78,8 @@ def lib():
         L.cgli_bc_y.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int]
         L.cgli_bc_z.argtypes = [vp]
         L.cgli_bc_x_periodic.argtypes = [vp]
+        L.cgli_write_restart.argtypes = [vp, C.c_char_p]
+        L.cgli_read_restart.argtypes = [vp, C.c_char_p]
         L.cgli_channel_defaults.argtypes = [C.POINTER(Channel), C.c_double]
         L.cgli_channel_defaults.restype = None
         L.cgli_channel_config.argtypes = [C.POINTER(Channel), C.c_int, C.c_int, C.c_int, C.POINTER(Config)]
@@ -206,6 +209,14 @@ class Context:
     def flux_z(self, Ep, e):
         _check(lib().cgli_flux_z(self.h, Ep, e), "flux_z")
 
+    DISSIPATION = {"none": 0, "ep": 1, "sensor": 2}
+
+    def set_dissipation_scaling(self, mode):
+        """Roe dissipation scaling: 'none' (the live reference), 'ep' (the flux
+        calls' Ep argument, Flux_X~~.cpp:552-556) or 'sensor' (EpX/EpY/EpZ)."""
+        _check(lib().cgli_set_dissipation_scaling(self.h, self.DISSIPATION.get(mode, mode)),
+               "set_dissipation_scaling")
+
     def viscous(self):
         _check(lib().cgli_viscous(self.h), "viscous")
 
@@ -257,6 +268,15 @@ class Context:
 
     def bc_x_periodic(self):
         _check(lib().cgli_bc_x_periodic(self.h), "bc_x_periodic")
+
+    # -- restart file (Output.cpp:167-284, Initial_condition.cpp:112-219)
+    def write_restart(self, path):
+        """Write U*J and Uq*J of the interior in the reference's LES%05d.bin layout."""
+        _check(lib().cgli_write_restart(self.h, os.fsencode(path)), "write_restart")
+
+    def read_restart(self, path):
+        """Read such a file: U = Q/J, Uq = Q_old/J, U_ = U on the interior."""
+        _check(lib().cgli_read_restart(self.h, os.fsencode(path)), "read_restart")
 
     # -- channel workload
     def channel_init(self, ch):

@@ -28,6 +28,7 @@ extern "C" {
 #define CGLI_E_STATE (-5)      /* a required field was never set */
 #define CGLI_E_NODEV (-6)      /* no HIP device */
 #define CGLI_E_TIMEOUT (-7)    /* a bounded device wait gave up */
+#define CGLI_E_IO (-8)         /* a restart file could not be opened, read or written */
 
 typedef struct cgli_ctx cgli_ctx;
 typedef struct cgli_net cgli_net;
@@ -100,6 +101,15 @@ int cgli_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
 int cgli_flux_x(cgli_ctx *ctx, double Ep, double e);
 int cgli_flux_y(cgli_ctx *ctx, double Ep, double e);
 int cgli_flux_z(cgli_ctx *ctx, double Ep, double e);
+/* Roe dissipation scaling of the three fluxes (off by default).  The live
+ * reference fluxes ignore their Ep argument and EpX/EpY/EpZ; the older flux
+ * files scale the dissipation Fav of every face by Ep
+ * (Flux_X~~.cpp:552-556).  mode 0: unscaled (the live reference); 1: the
+ * Ep argument of each flux call scales it; 2: the filter sensor's face
+ * value EpX/EpY/EpZ (from the last cgli_filter) scales it, which is how the
+ * sensor was meant to act.  Flux_Y's wall-face pressure flux keeps its own
+ * Ep term (Flux_Y.cpp:400-408) in every mode. */
+int cgli_set_dissipation_scaling(cgli_ctx *ctx, int mode);
 /* Viscous_terms.cpp:15-908: reads U_, J and the face metrics; writes vF. */
 int cgli_viscous(cgli_ctx *ctx);
 /* Filter.cpp:12-1720: the multi-scale filter sensor.  Reads U_ (U1..U4),
@@ -147,6 +157,21 @@ int cgli_bc_y(cgli_ctx *ctx, double T_low, double T_high, int heated_lo, int hea
 int cgli_bc_z(cgli_ctx *ctx);
 /* three periodic ghost planes at each x end of a single periodic slab */
 int cgli_bc_x_periodic(cgli_ctx *ctx);
+
+/* ---- restart file ---------------------------------------------------- */
+/* Output.cpp:167-284: writes the reference's restart file (LES%05d.bin;
+ * the caller names it): ten blocks of nx_global*ny*nz doubles, Q = U*J of
+ * U1..U5 (level n) then of U1q..U5q (level n-1), interior cells in (i, j, k)
+ * order with k fastest, native byte order.  This slab writes its cells of
+ * every block gstart planes in; the file is created if missing and never
+ * truncated, so the slabs of one problem can write it one after another. */
+int cgli_write_restart(cgli_ctx *ctx, const char *path);
+/* Initial_condition.cpp:112-219 (switch_initial = 1): reads such a file into
+ * U = Q/J, Uq = Q_old/J and U_ = U on the interior cells (ghosts are left to
+ * the boundary fills).  Needs J.  A file shorter than the blocks is
+ * CGLI_E_IO.  Neither the running maxima of stopping rule 1 nor a network's
+ * ring are in the file (the reference has neither). */
+int cgli_read_restart(cgli_ctx *ctx, const char *path);
 
 /* ---- compressible turbulent channel (the benchmark workload) ---------- */
 /* Periodic x and z, isothermal no-slip walls at y = +-1 with tanh

@@ -69,6 +69,17 @@ void orc_flux_y(const orc_cfg *c, double Ep, double e, double *const U_[5],
 void orc_flux_z(const orc_cfg *c, double Ep, double e, double *const U_[5],
                 double *MR[5], double *ML[5], const double *J, const double *J_w,
                 const double *const met_w[9], double *inFz[5]);
+/* the same with the Roe dissipation of every face scaled by 1 (mode 0),
+ * Ep (mode 1) or EpF[face] (mode 2) */
+void orc_flux_x_scaled(const orc_cfg *c, int mode, double Ep, double e, double *const U_[5],
+                       double *MR[5], double *ML[5], const double *J, const double *J_u,
+                       const double *const met_u[9], double *inFx[5], const double *EpF);
+void orc_flux_y_scaled(const orc_cfg *c, int mode, double Ep, double e, double *const U_[5],
+                       double *MR[5], double *ML[5], const double *J, const double *J_v,
+                       const double *const met_v[9], double *inFy[5], const double *EpF);
+void orc_flux_z_scaled(const orc_cfg *c, int mode, double Ep, double e, double *const U_[5],
+                       double *MR[5], double *ML[5], const double *J, const double *J_w,
+                       const double *const met_w[9], double *inFz[5], const double *EpF);
 /* vF[0..3] = vF2..vF5; scratch LL/ML/NL hold the face fluxes 2..5 */
 void orc_viscous(const orc_cfg *c, double *const U_[5], double *vF[4],
                  const double *J, const double *J_u, const double *const met_u[9],

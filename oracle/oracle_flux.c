@@ -7,6 +7,13 @@
  *   Flux_Z.cpp:91-359   (order drop at both z ends)
  * Arithmetic is kept in the reference's evaluation order so that results
  * agree to the last bit on an x86-64 build without FMA contraction.
+ *
+ * The *_scaled variants add the dissipation scaling of the older flux
+ * files (Flux_X~~.cpp:552-556, Flux_Y~~.cpp, Flux_Z~~.cpp): the Roe
+ * dissipation Fav of every face is multiplied by a factor, 1 (mode 0, the
+ * live reference), the scalar Ep (mode 1) or the face's filter-sensor value
+ * EpF (mode 2, face-indexed like the fluxes).  Multiplying by 1 is exact,
+ * so mode 0 is the unscaled flux bit for bit.
  */
 #include "oracle_common.h"
 
@@ -15,7 +22,7 @@
  * metric components, (nx_,ny_,nz_) the vector multiplying deltaP in
  * Fav2..4.  Returns the flux and, for the wall override, iP, P_ and Fav3. */
 static void roe_face(const double Lq[5], const double Rq[5], double mx, double my, double mz,
-                     double Jf, double e, double K, double nx_, double ny_, double nz_,
+                     double Jf, double e, double K, double nx_, double ny_, double nz_, double ep,
                      double F[5], double *iP_out, double *P_out, double *Fav3_out)
 {
     double insqr = 1.0 / sqrt(mx * mx + my * my + mz * mz);
@@ -73,11 +80,11 @@ static void roe_face(const double Lq[5], const double Rq[5], double mx, double m
     double Fav4 = fabs(Un * insqr) * dU4 + dUc * rho * w + dPc * nz_;
     double Fav5 = fabs(Un * insqr) * dU5 + dUc * rho * H + dPc * Un * insqr;
 
-    F[0] = 0.5 * ((rL * UnL + rR * UnR) - Fav1 / insqr) / Jf;
-    F[1] = 0.5 * ((rL * uL * UnL + rR * uR * UnR + mx * (pL + pR)) - Fav2 / insqr) / Jf;
-    F[2] = 0.5 * ((rL * vL * UnL + rR * vR * UnR + my * (pL + pR)) - Fav3 / insqr) / Jf;
-    F[3] = 0.5 * ((rL * wL * UnL + rR * wR * UnR + mz * (pL + pR)) - Fav4 / insqr) / Jf;
-    F[4] = 0.5 * ((UnL * (3.5 * pL + 0.5 * rL * qL) + UnR * (3.5 * pR + 0.5 * rR * qR)) - Fav5 / insqr) / Jf;
+    F[0] = 0.5 * ((rL * UnL + rR * UnR) - ep * Fav1 / insqr) / Jf;
+    F[1] = 0.5 * ((rL * uL * UnL + rR * uR * UnR + mx * (pL + pR)) - ep * Fav2 / insqr) / Jf;
+    F[2] = 0.5 * ((rL * vL * UnL + rR * vR * UnR + my * (pL + pR)) - ep * Fav3 / insqr) / Jf;
+    F[3] = 0.5 * ((rL * wL * UnL + rR * wR * UnR + mz * (pL + pR)) - ep * Fav4 / insqr) / Jf;
+    F[4] = 0.5 * ((UnL * (3.5 * pL + 0.5 * rL * qL) + UnR * (3.5 * pR + 0.5 * rR * qR)) - ep * Fav5 / insqr) / Jf;
     if (iP_out) *iP_out = pL;
     if (P_out) *P_out = pR;
     if (Fav3_out) *Fav3_out = Fav3;
@@ -116,12 +123,16 @@ static double rec2(const double *U, const double *J, long a, long b)
     return temp * (U[a] + U[b]);
 }
 
-void orc_flux_x(const orc_cfg *c, double Ep, double e, double *const U_[5],
-                double *MR[5], double *ML[5], const double *J, const double *J_u,
-                const double *const met_u[9], double *inFx[5])
+static double face_scale(int mode, double Ep, const double *EpF, long id)
+{
+    return mode == 2 ? EpF[id] : (mode == 1 ? Ep : 1.0);
+}
+
+void orc_flux_x_scaled(const orc_cfg *c, int mode, double Ep, double e, double *const U_[5],
+                       double *MR[5], double *ML[5], const double *J, const double *J_u,
+                       const double *const met_u[9], double *inFx[5], const double *EpF)
 {
     DIMS(c);
-    (void)Ep;
     int istart = c->first ? 4 : 2;
     int iend = c->last ? gend : gend + 1;
     #pragma omp parallel for schedule(static)
@@ -162,14 +173,14 @@ void orc_flux_x(const orc_cfg *c, double Ep, double e, double *const U_[5],
                 double xix = met_u[0][id], xiy = met_u[1][id], xiz = met_u[2][id];
                 double insqr = 1.0 / sqrt(xix * xix + xiy * xiy + xiz * xiz);
                 roe_face(L, R, xix, xiy, xiz, J_u[id], e, c->K, xix * insqr, xiy * insqr, xiz * insqr,
-                         F, 0, 0, 0);
+                         face_scale(mode, Ep, EpF, id), F, 0, 0, 0);
                 for (int v = 0; v < 5; v++) inFx[v][id] = F[v];
             }
 }
 
-void orc_flux_y(const orc_cfg *c, double Ep, double e, double *const U_[5],
-                double *MR[5], double *ML[5], const double *J, const double *J_v,
-                const double *const met_v[9], double *inFy[5])
+void orc_flux_y_scaled(const orc_cfg *c, int mode, double Ep, double e, double *const U_[5],
+                       double *MR[5], double *ML[5], const double *J, const double *J_v,
+                       const double *const met_v[9], double *inFy[5], const double *EpF)
 {
     DIMS(c);
     int istart = c->first ? 2 : 3;
@@ -217,7 +228,8 @@ void orc_flux_y(const orc_cfg *c, double Ep, double e, double *const U_[5],
                 double insqr = 1.0 / sqrt(etx * etx + ety * ety + etz * etz);
                 double nx_ = 0, ny_ = 0, nz_ = 0;
                 if (c->xi_mode) { nx_ = etx * insqr; ny_ = ety * insqr; nz_ = etz * insqr; }
-                roe_face(L, R, etx, ety, etz, J_v[id], e, c->K, nx_, ny_, nz_, F, &iP, &P_, &Fav3);
+                roe_face(L, R, etx, ety, etz, J_v[id], e, c->K, nx_, ny_, nz_, face_scale(mode, Ep, EpF, id), F,
+                         &iP, &P_, &Fav3);
                 for (int v = 0; v < 5; v++) inFy[v][id] = F[v];
                 if (j == 1 || j == NY) {
                     inFy[0][id] = 0;
@@ -229,12 +241,11 @@ void orc_flux_y(const orc_cfg *c, double Ep, double e, double *const U_[5],
             }
 }
 
-void orc_flux_z(const orc_cfg *c, double Ep, double e, double *const U_[5],
-                double *MR[5], double *ML[5], const double *J, const double *J_w,
-                const double *const met_w[9], double *inFz[5])
+void orc_flux_z_scaled(const orc_cfg *c, int mode, double Ep, double e, double *const U_[5],
+                       double *MR[5], double *ML[5], const double *J, const double *J_w,
+                       const double *const met_w[9], double *inFz[5], const double *EpF)
 {
     DIMS(c);
-    (void)Ep;
     int istart = c->first ? 2 : 3;
     int iend = gend;
     #pragma omp parallel for schedule(static)
@@ -274,7 +285,27 @@ void orc_flux_z(const orc_cfg *c, double Ep, double e, double *const U_[5],
                 double insqr = 1.0 / (sqrt(ztx * ztx + zty * zty + ztz * ztz));
                 double nx_ = 0, ny_ = 0, nz_ = 0;
                 if (c->xi_mode) { nx_ = ztx * insqr; ny_ = zty * insqr; nz_ = ztz * insqr; }
-                roe_face(L, R, ztx, zty, ztz, J_w[id], e, c->K, nx_, ny_, nz_, F, 0, 0, 0);
+                roe_face(L, R, ztx, zty, ztz, J_w[id], e, c->K, nx_, ny_, nz_, face_scale(mode, Ep, EpF, id), F, 0,
+                         0, 0);
                 for (int v = 0; v < 5; v++) inFz[v][id] = F[v];
             }
+}
+
+/* the live reference fluxes: dissipation unscaled */
+void orc_flux_x(const orc_cfg *c, double Ep, double e, double *const U_[5], double *MR[5], double *ML[5],
+                const double *J, const double *J_u, const double *const met_u[9], double *inFx[5])
+{
+    orc_flux_x_scaled(c, 0, Ep, e, U_, MR, ML, J, J_u, met_u, inFx, 0);
+}
+
+void orc_flux_y(const orc_cfg *c, double Ep, double e, double *const U_[5], double *MR[5], double *ML[5],
+                const double *J, const double *J_v, const double *const met_v[9], double *inFy[5])
+{
+    orc_flux_y_scaled(c, 0, Ep, e, U_, MR, ML, J, J_v, met_v, inFy, 0);
+}
+
+void orc_flux_z(const orc_cfg *c, double Ep, double e, double *const U_[5], double *MR[5], double *ML[5],
+                const double *J, const double *J_w, const double *const met_w[9], double *inFz[5])
+{
+    orc_flux_z_scaled(c, 0, Ep, e, U_, MR, ML, J, J_w, met_w, inFz, 0);
 }

@@ -35,7 +35,7 @@ sed -e "s/^#define X_out [0-9]*/#define X_out $X/" \
 sed -e "s/^#define np [0-9]*/#define np $NP/" "$REF/MPI_prm.h" > "$TMP/MPI_prm.h"
 grep -q "#define X_out $X\b" "$TMP/Resolution.h" && grep -q "#define np $NP\b" "$TMP/MPI_prm.h" || { echo "header resize failed" >&2; exit 4; }
 INC="-I$TMP -I- -I$REF -I$MPIINC"
-TUS="Flux_X Flux_Y Flux_Z Viscous_terms Dual_time_stepping Hyperplane Jacobian Jacobian_u Jacobian_v Jacobian_w Y_boundary_condition Z_boundary_condition Filter"
+TUS="Flux_X Flux_Y Flux_Z Viscous_terms Dual_time_stepping Hyperplane Jacobian Jacobian_u Jacobian_v Jacobian_w Y_boundary_condition Z_boundary_condition Filter Output Initial_condition"
 for t in $TUS; do
   g++ -O3 -fPIC -w $INC -c "$REF/$t.cpp" -o "$TMP/$t.o"
 done

@@ -3,8 +3,8 @@
 // C entry points around the reference solver's own routines, compiled from
 // the reference sources where they lie (see build_ref.sh).  It lets the
 // golden-vector script call Flux_X/Y/Z, Viscous_terms, Dual_time_stepping,
-// Hyperplane, Jacobian*, Y_/Z_boundary_condition and Filter on arrays it
-// owns.  The
+// Hyperplane, Jacobian*, Y_/Z_boundary_condition, Filter, and the restart
+// file's writer and reader (Output, Initial_condition) on arrays it owns.  The
 // reference's grid sizes are compile-time macros (Resolution.h), so one
 // library is built per size; ref_dims() reports them.
 #include <mpi.h>
@@ -133,6 +133,24 @@ void ref_filter(int xnp, int myid, double Roe_criterion, double E_high, double E
     X_np = xnp;
     Filter(myid, Roe_criterion, E_high, E_low, a(U_[0]), a(U_[1]), a(U_[2]), a(U_[3]), a(U_[4]), a(Y_point),
            a(EpX), a(EpY), a(EpZ), a(J));
+}
+
+// Output.cpp with switch_output = 0: only the restart file LES%05d.bin,
+// written into the current directory (the grid arrays are not read then).
+void ref_output(int xnp, int myid, int step, double **U, double **Uq, double *J)
+{
+    X_np = xnp;
+    Output(step, myid, 0, a(U[0]), a(U[1]), a(U[2]), a(U[3]), a(U[4]), a(Uq[0]), a(Uq[1]), a(Uq[2]), a(Uq[3]),
+           a(Uq[4]), a(J), a(J), a(J), a(J));
+}
+
+// Initial_condition.cpp with switch_initial = 1: the uniform state, then
+// the interior from initial.bin in the current directory.
+void ref_initial_condition(int xnp, int myid, double **U_, double **U, double **Uq, double *J)
+{
+    X_np = xnp;
+    Initial_condition(myid, 1, a(U_[0]), a(U_[1]), a(U_[2]), a(U_[3]), a(U_[4]), a(U[0]), a(U[1]), a(U[2]),
+                      a(U[3]), a(U[4]), a(Uq[0]), a(Uq[1]), a(Uq[2]), a(Uq[3]), a(Uq[4]), a(J));
 }
 
 // X_out Y_out Z_out Y_m Z_m np nx_inlet nx_outlet.  MPI_prm.h is a

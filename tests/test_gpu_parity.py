@@ -87,6 +87,52 @@ def test_fluxes_match_oracle(nv, orc, name, xi_mode):
                 assert rel(g, w) <= TOL, (key, v, rel(g, w))
 
 
+@pytest.mark.parametrize("name", ["ref_24x10x6_interior", "ragged_20x24x17"])
+@pytest.mark.parametrize("mode", ["ep", "sensor"])
+def test_scaled_dissipation_matches_oracle(nv, orc, name, mode):
+    """The optional Roe dissipation scaling (Flux_X~~.cpp:552-556 semantics)
+    by the call's Ep or by the filter sensor's face values."""
+    print("case", name, "dissipation scaling", mode, flush=True)
+    case = make_case(name)
+    sl = case.slab
+    f = make_inputs(orc, sl, SEED + 5)
+    mi = {"ep": 1, "sensor": 2}[mode]
+    Ep = 0.37
+    rng = np.random.default_rng(13)
+    EpF = [np.where(rng.random(sl.shape) < 0.5, 1.0, 0.1) for _ in range(3)]
+    from oracle_harness import met9
+    want = {}
+    for d, tag in enumerate(("_u", "_v", "_w")):
+        U_ = [a.copy() for a in f["U_"]]
+        MR, ML, F = sl.zeros(5), sl.zeros(5), sl.zeros(5)
+        fn = (orc.orc_flux_x_scaled, orc.orc_flux_y_scaled, orc.orc_flux_z_scaled)[d]
+        fn(C.byref(case.cfg(orc, xi_mode=1)), mi, C.c_double(Ep), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML),
+           dp(f["J"]), dp(f["J" + tag]), ptrs(met9(f, tag)), ptrs(F), dp(EpF[d]))
+        want["inF" + "xyz"[d]] = F
+    with open_ctx(nv, orc, case, xi_mode=1) as ctx:
+        upload_inputs(ctx, f, with_levels=False)
+        for d in range(3):
+            ctx.set(nv.EPX + d, EpF[d])
+        ctx.flux_x(Ep, 1e-4)   # unscaled first: the scaled fluxes must differ
+        plain = ctx.get_many(nv.INFX, 5)
+        ctx.set_dissipation_scaling(mode)
+        ctx.flux_x(Ep, 1e-4)
+        ctx.flux_y(Ep, 1e-4)
+        ctx.flux_z(Ep, 1e-4)
+        reg = regions(sl)
+        for d, fid in enumerate((nv.INFX, nv.INFY, nv.INFZ)):
+            key = "inF" + "xyz"[d]
+            got = ctx.get_many(fid, 5)
+            for v in range(5):
+                g, w = got[v][reg[key]], want[key][v][reg[key]]
+                assert np.all(np.isfinite(g))
+                assert rel(g, w) <= TOL, (key, v, rel(g, w))
+        assert not np.array_equal(plain[0][reg["inFx"]], ctx.get(nv.INFX)[reg["inFx"]])
+        ctx.set_dissipation_scaling("none")
+        ctx.flux_x(Ep, 1e-4)
+        assert np.array_equal(plain[1], ctx.get(nv.INFX + 1))
+
+
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_viscous_matches_oracle(nv, orc, name):
     print("case", name, flush=True)
@@ -218,6 +264,15 @@ def test_missing_input_is_an_error(nv, orc):
         with pytest.raises(nv.CgliError) as ei:
             ctx.flux_x(1.0, 1e-4)
         assert ei.value.code == -5
+    with open_ctx(nv, orc, case) as ctx:
+        upload_inputs(ctx, make_inputs(orc, case.slab, SEED), with_levels=False)
+        ctx.set_dissipation_scaling("sensor")   # no sensor values yet
+        with pytest.raises(nv.CgliError) as ei:
+            ctx.flux_z(1.0, 1e-4)
+        assert ei.value.code == -5
+        with pytest.raises(nv.CgliError) as ei:
+            ctx.set_dissipation_scaling(3)
+        assert ei.value.code == -1
 
 
 @pytest.mark.parametrize("name", [n for n in sorted(CASES) if CASES[n][3] == 1])

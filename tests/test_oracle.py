@@ -367,3 +367,84 @@ def test_profiles_second_moments_bound_the_means(orc):
     np.testing.assert_array_equal(P2[0], -P[0])
     np.testing.assert_array_equal(P2[4], P[4])
     np.testing.assert_array_equal(P2[8], -P[8])
+
+
+# ------------------------------------------------- scaled Roe dissipation
+
+def _scaled_flux(orc, case, f, d, mode, Ep, EpF=None):
+    sl = case.slab
+    tag = ("_u", "_v", "_w")[d]
+    U_ = [a.copy() for a in f["U_"]]
+    MR, ML, F = sl.zeros(5), sl.zeros(5), sl.zeros(5)
+    fn = (orc.orc_flux_x_scaled, orc.orc_flux_y_scaled, orc.orc_flux_z_scaled)[d]
+    fn(C.byref(case.cfg(orc, xi_mode=1)), mode, C.c_double(Ep), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML),
+       dp(f["J"]), dp(f["J" + tag]), ptrs(met9(f, tag)), ptrs(F), dp(EpF) if EpF is not None else None)
+    return F, ML, MR
+
+
+def _central(f, d, ML, MR, K=1.4):
+    """0.5 (rho_L Un_L + rho_R Un_R, ..., pressure terms) / J_f from the face
+    states: the flux with its dissipation dropped (Flux_X.cpp:344-350)."""
+    tag = ("_u", "_v", "_w")[d]
+    names = [("xidx", "xidy", "xidz"), ("etdx", "etdy", "etdz"), ("ztdx", "ztdy", "ztdz")][d]
+    mx, my, mz = (f[n + tag] for n in names)
+    out = []
+    side = []
+    for S in (ML, MR):
+        r = np.where(S[0] != 0, S[0], 1.0)   # zero outside the faces
+        u, v, w = S[1] / r, S[2] / r, S[3] / r
+        q = u * u + v * v + w * w
+        p = (S[4] - 0.5 * r * q) * (K - 1)
+        side.append((r, u, v, w, q, p, mx * u + my * v + mz * w))
+    (rL, uL, vL, wL, qL, pL, UL), (rR, uR, vR, wR, qR, pR, UR) = side
+    Jf = f["J" + tag]
+    out.append(0.5 * (rL * UL + rR * UR) / Jf)
+    out.append(0.5 * (rL * uL * UL + rR * uR * UR + mx * (pL + pR)) / Jf)
+    out.append(0.5 * (rL * vL * UL + rR * vR * UR + my * (pL + pR)) / Jf)
+    out.append(0.5 * (rL * wL * UL + rR * wR * UR + mz * (pL + pR)) / Jf)
+    out.append(0.5 * (UL * (3.5 * pL + 0.5 * rL * qL) + UR * (3.5 * pR + 0.5 * rR * qR)) / Jf)
+    return out
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_scaled_dissipation_pins(orc, d):
+    """Mode 0 is the live flux bit for bit; Ep = 0 leaves the central flux of
+    the face states; the flux is affine in Ep; the sensor mode takes each
+    face's own factor (Flux_Y's wall faces keep the call's scalar Ep)."""
+    case = Case(12, 16, 10, 1, 4, 4, 0, (0.001, 0.005, 0.002))
+    sl = case.slab
+    f = make_inputs(orc, sl, SEED + 41)
+    key = ("inFx", "inFy", "inFz")[d]
+    reg = regions(sl)[key]
+    live = _scaled_flux(orc, case, f, d, 0, 1.0)[0]
+    tag = ("_u", "_v", "_w")[d]
+    U_ = [a.copy() for a in f["U_"]]
+    Fl, MRl, MLl = sl.zeros(5), sl.zeros(5), sl.zeros(5)
+    (orc.orc_flux_x, orc.orc_flux_y, orc.orc_flux_z)[d](
+        C.byref(case.cfg(orc, xi_mode=1)), C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MRl), ptrs(MLl),
+        dp(f["J"]), dp(f["J" + tag]), ptrs(met9(f, tag)), ptrs(Fl))
+    F1, _, _ = _scaled_flux(orc, case, f, d, 1, 1.0)
+    for v in range(5):
+        assert np.array_equal(live[v], Fl[v]) and np.array_equal(F1[v], Fl[v])
+
+    F0, ML, MR = _scaled_flux(orc, case, f, d, 1, 0.0)
+    cen = _central(f, d, ML, MR)
+    inner = reg if d != 1 else (reg[0], slice(2, sl.NY), reg[2])   # Flux_Y's wall faces are overridden
+    for v in range(5):
+        assert np.abs(F0[v][inner] - cen[v][inner]).max() <= 1e-13 * np.abs(cen[v][inner]).max()
+        assert not np.array_equal(F0[v][inner], F1[v][inner])     # the dissipation is not zero
+
+    a = 0.3
+    Fa, _, _ = _scaled_flux(orc, case, f, d, 1, a)
+    for v in range(5):
+        lin = F0[v][reg] + a * (F1[v][reg] - F0[v][reg])
+        assert np.abs(Fa[v][reg] - lin).max() <= 1e-12 * np.abs(F1[v][reg]).max()
+
+    rng = np.random.default_rng(5 + d)
+    EpF = np.where(rng.random(sl.shape) < 0.5, 1.0, 0.1)   # the sensor's E_high / E_low
+    Fs, _, _ = _scaled_flux(orc, case, f, d, 2, 1.0, EpF)
+    Flo, _, _ = _scaled_flux(orc, case, f, d, 1, 0.1)
+    hi, lo = EpF[inner] == 1.0, EpF[inner] == 0.1
+    for v in range(5):
+        assert np.array_equal(Fs[v][inner][hi], F1[v][inner][hi])
+        assert np.array_equal(Fs[v][inner][lo], Flo[v][inner][lo])
