@@ -18,8 +18,13 @@ first trained on the GPU (Adam, batches of random crops) on increments the
 solver records while stepping; that training and its snapshot run are set-up,
 outside the timed regions, and their costs are reported.
 
-Prints ONE JSON line.  `roofline` is the dominant kernel class measured
-with HIP events on the library's stream over the timed region;
+Prints ONE JSON line.  `kernels` times every kernel class with HIP events
+over one step before the timed region; that names the dominant class, and
+the timed region records events around that class only (two event records
+per timed launch).  `roofline` is the dominant class measured so, on the
+library's stream over the timed region;
+`graph_replay` runs the same steps from one saved state eagerly and with
+each inner iteration replayed from a captured HIP graph;
 `traffic` is the HBM bytes per launch of that kernel measured by two
 rocprofv3 counter passes (profiles/channel_128_traffic.md) for the default
 workload; `cpu_baseline` is the oracle (C restatement, OpenMP) on this host
@@ -67,6 +72,7 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="mixed: FP32 residual kernels until the FP32 floor, FP32 off-diagonal sweep")
     ap.add_argument("--no-variants", action="store_true", help="skip the initial-guess / precision comparison")
+    ap.add_argument("--no-graph", action="store_true", help="skip the eager / graph-replay comparison")
     ap.add_argument("--lusgs-mode", type=int, default=None,
                     help="LU-SGS schedule (0 plane order, 1 tiled wavefront, 2 with loader waves, "
                          "3 march-ordered records; default: the library's, 3)")
@@ -245,6 +251,24 @@ def variants(a, nv, ch, ctx64, dt, ncell):
     return out
 
 
+def graph_replay(a, nv, ctx, dt):
+    """The same steps from one saved state, eager and with each inner
+    iteration replayed from a captured HIP graph (kernel timing off for both:
+    events do not see graph launches); the results are the same bits."""
+    saved = state_of(ctx, nv)
+    eager = timed_steps(ctx, dt, a, a.steps, 0, not a.no_filter, "eager, untimed")
+    restore(ctx, nv, saved, ctx.rule_state())
+    ctx.set_graph(True)
+    ctx.channel_step(dt, a.tol, 3)   # captures the iteration (its first occurrence ran above)
+    restore(ctx, nv, saved, ctx.rule_state())
+    graph = timed_steps(ctx, dt, a, a.steps, 0, not a.no_filter, "graph replay")
+    ctx.set_graph(False)
+    return {"eager_ms_per_step": eager["ms_per_step"], "graph_ms_per_step": graph["ms_per_step"],
+            "inner_iterations_per_step": graph["inner_iterations_per_step"],
+            "same_inner_counts": eager["inner_iterations_per_step"] == graph["inner_iterations_per_step"],
+            "speedup": eager["ms_per_step"] / graph["ms_per_step"]}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -265,7 +289,23 @@ def main():
             ctx.filter()
         it, _, _ = ctx.channel_step(dt, a.tol, a.max_inner)
         print("warmup step %d: %d inner iterations" % (s + 1, it), file=sys.stderr, flush=True)
+    # one step with every kernel class timed: the per-class table and the
+    # dominant class; the timed region then records events around that class
+    # only (each timed launch adds two event records to the stream)
     ctx.timing(True)
+    if not a.no_filter:
+        ctx.filter()
+    ctx.channel_step(dt, a.tol, a.max_inner)
+    ctx.sync()
+    ncell = a.nx * a.ny * a.nz
+    kernels = {}
+    for name, (tot, cnt) in ctx.timing_report().items():
+        if cnt:
+            avg = tot / cnt
+            kernels[name] = {"avg_ms": avg, "launches": cnt, "total_ms": tot,
+                             "GBps": ALGO_BYTES[name] * ncell / (avg * 1e-3) / 1e9}
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    ctx.timing(True, classes=[dom])
     ctx.sync()
     t0 = time.perf_counter()
     inner = []
@@ -278,19 +318,12 @@ def main():
               flush=True)
     ctx.sync()
     t1 = time.perf_counter()
-    rep = ctx.timing_report()
+    tot, cnt = ctx.timing_report()[dom]
     ctx.timing(False)
     elapsed = t1 - t0
-    ncell = a.nx * a.ny * a.nz
     steps_per_s = a.steps / elapsed
-    kernels = {}
-    for name, (tot, cnt) in rep.items():
-        if cnt:
-            avg = tot / cnt
-            kernels[name] = {"avg_ms": avg, "launches": cnt, "total_ms": tot,
-                             "GBps": ALGO_BYTES[name] * ncell / (avg * 1e-3) / 1e9}
-    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
-    achieved = kernels[dom]["GBps"]
+    dom_avg_ms = tot / cnt
+    achieved = ALGO_BYTES[dom] * ncell / (dom_avg_ms * 1e-3) / 1e9
     traffic = None
     if (a.nx, a.ny, a.nz) == (128, 128, 128) and os.path.exists(TRAFFIC):
         traffic = json.load(open(TRAFFIC))["bytes_per_launch"].get(dom)
@@ -314,12 +347,15 @@ def main():
         "mcell_updates_per_s": ncell * steps_per_s / 1e6,
         "inner_iterations_per_step": float(np.mean(inner)),
         "mcell_inner_iterations_per_s": ncell * float(np.sum(inner)) / elapsed / 1e6,
-        "kernels": kernels,
+        "kernels": kernels,   # one fully timed step before the timed region
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": ALGO_BYTES[dom] * ncell},
+                     "algorithmic_bytes_per_launch": ALGO_BYTES[dom] * ncell,
+                     "avg_ms_in_timed_region": dom_avg_ms, "launches_in_timed_region": cnt},
         "cpu_baseline": None,
     }
+    if not a.no_graph:
+        out["graph_replay"] = graph_replay(a, nv, ctx, dt)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ctx, nv, cfg, dt, float(np.mean(inner)), a.cpu_seconds, not a.no_filter)
     if not a.no_variants and a.precision == "fp64":

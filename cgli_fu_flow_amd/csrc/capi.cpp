@@ -23,6 +23,14 @@ struct EventPair { hipEvent_t a, b; };
 
 }  // namespace
 
+struct cgli_graph {
+    cgli_ctx *ctx = nullptr;
+    hipGraphExec_t exec = nullptr;
+    long generation = 0;         // of the context at capture
+    bool norm = false;           // ends with a deferred residual norm
+    bool norm_wave = false;
+};
+
 struct cgli_ctx {
     cgli_config cfg;
     int device = 0;
@@ -40,6 +48,10 @@ struct cgli_ctx {
     std::vector<int> Nijk;       // (gend+NY+NZ+8) x 2
     std::vector<int> ijk;        // (cells+1) x 3
     int diss_scale = 0;          // Roe dissipation scaling: 0 none, 1 scalar Ep, 2 filter sensor
+    bool capturing = false;      // between cgli_graph_begin and cgli_graph_end
+    long generation = 0;         // bumped whenever a captured graph would go stale
+    bool norm_pending = false;   // a deferred residual norm awaits cgli_dual_time_stepping_wait
+    bool norm_wave = false;      // ... with the wavefront's timeout flag
     int lusgs_mode = 3;          // 0: one launch per hyperplane, 1: tiled wavefront, 2: with loader waves,
                                  // 3: tiled wavefront on march-ordered records (default)
     double *coef = nullptr;      // split-coefficient blocks of the wavefront (float in the mixed mode)
@@ -61,6 +73,7 @@ struct cgli_ctx {
     int *counters_host = nullptr;
     int max_spin = 1000000;
     bool timing = false;
+    unsigned tmask = ~0u;        // kernel classes timed while timing is on
     std::vector<EventPair> ev[T_NCLASS];
     size_t ev_used[T_NCLASS] = {};
 };
@@ -74,6 +87,7 @@ struct cgli_ctx {
 static int ensure(cgli_ctx *c, int id)
 {
     if (c->f[id]) return CGLI_OK;
+    if (c->capturing) return CGLI_E_STATE;   // allocations happen on an eager call first
     void *p = nullptr;
     if (hipMalloc(&p, sizeof(double) * c->g.N) != hipSuccess) return CGLI_E_NOMEM;
     if (hipMemsetAsync(p, 0, sizeof(double) * c->g.N, c->stream) != hipSuccess) return CGLI_E_HIP;
@@ -114,6 +128,8 @@ static M9 m9(cgli_ctx *c, int id)
 static int scratch(cgli_ctx *c, long n)
 {
     if (c->scratch_len >= n) return CGLI_OK;
+    if (c->capturing) return CGLI_E_STATE;
+    c->generation++;   // graphs captured with the old buffer are stale
     if (c->scratch) hipFree(c->scratch);
     c->scratch = nullptr;
     void *p = nullptr;
@@ -123,9 +139,12 @@ static int scratch(cgli_ctx *c, long n)
     return CGLI_OK;
 }
 
+// Kernel timing covers eager launches only (no events inside a graph).
+static bool timed(const cgli_ctx *c, int cls) { return c->timing && !c->capturing && ((c->tmask >> cls) & 1u); }
+
 static void tbegin(cgli_ctx *c, int cls)
 {
-    if (!c->timing) return;
+    if (!timed(c, cls)) return;
     auto &v = c->ev[cls];
     if (c->ev_used[cls] == v.size()) {
         EventPair e;
@@ -138,7 +157,7 @@ static void tbegin(cgli_ctx *c, int cls)
 
 static void tend(cgli_ctx *c, int cls)
 {
-    if (!c->timing) return;
+    if (!timed(c, cls)) return;
     hipEventRecord(c->ev[cls][c->ev_used[cls]].b, c->stream);
     c->ev_used[cls]++;
 }
@@ -154,6 +173,7 @@ hipStream_t ctx_stream(cgli_ctx *c) { return c->stream; }
 const cgli_config *ctx_config(cgli_ctx *c) { return &c->cfg; }
 
 void ctx_set_fp32(cgli_ctx *c, bool on) { c->p.fp32 = on ? 1 : 0; }
+bool ctx_timing_on(cgli_ctx *c) { return c->timing; }
 
 void ctx_set_last_fp32(cgli_ctx *c, int n) { c->last_fp32 = n; }
 void ctx_mark(cgli_ctx *c, int id, int n) { mark(c, id, n); }
@@ -243,6 +263,7 @@ int cgli_destroy(cgli_ctx *c)
     if (!c) return CGLI_E_ARG;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
+    channel_drop_graphs(c->chan);
     for (int q = 0; q < CGLI_NFIELDS; q++)
         if (c->f[q]) hipFree(c->f[q]);
     if (c->prim4) hipFree(c->prim4);
@@ -273,9 +294,16 @@ int cgli_destroy(cgli_ctx *c)
     return CGLI_OK;
 }
 
+// Entry points that wait for the device cannot be captured into a graph.
+#define NO_CAPTURE(c)                            \
+    do {                                         \
+        if ((c)->capturing) return CGLI_E_STATE; \
+    } while (0)
+
 int cgli_set_field(cgli_ctx *c, int id, const double *host)
 {
     if (!c || !host || id < 0 || id >= CGLI_NFIELDS) return CGLI_E_ARG;
+    NO_CAPTURE(c);
     HIPCHK(hipSetDevice(c->device));
     int r = ensure(c, id);
     if (r) return r;
@@ -288,6 +316,7 @@ int cgli_set_field(cgli_ctx *c, int id, const double *host)
 int cgli_get_field(cgli_ctx *c, int id, double *host)
 {
     if (!c || !host || id < 0 || id >= CGLI_NFIELDS) return CGLI_E_ARG;
+    NO_CAPTURE(c);
     HIPCHK(hipSetDevice(c->device));
     int r = ensure(c, id);
     if (r) return r;
@@ -321,6 +350,7 @@ int cgli_device_memory(int device, size_t *free_bytes, size_t *total_bytes)
 int cgli_synchronize(cgli_ctx *c)
 {
     if (!c) return CGLI_E_ARG;
+    NO_CAPTURE(c);
     HIPCHK(hipStreamSynchronize(c->stream));
     return CGLI_OK;
 }
@@ -365,7 +395,9 @@ int cgli_flux_z(cgli_ctx *c, double Ep, double e) { return flux_common(c, 2, Ep,
 int cgli_set_dissipation_scaling(cgli_ctx *c, int mode)
 {
     if (!c || mode < 0 || mode > 2) return CGLI_E_ARG;
+    NO_CAPTURE(c);
     c->diss_scale = mode;
+    c->generation++;
     return CGLI_OK;
 }
 
@@ -378,6 +410,7 @@ int cgli_viscous(cgli_ctx *c)
         return CGLI_E_STATE;
     int r = ensure_range(c, CGLI_VF, 4);
     if (r) return r;
+    if ((!c->prim4 || !c->faces12) && c->capturing) return CGLI_E_STATE;
     if (!c->prim4) {
         void *p = nullptr;
         if (hipMalloc(&p, sizeof(double) * 4 * c->g.N) != hipSuccess) return CGLI_E_NOMEM;
@@ -406,6 +439,7 @@ int cgli_filter(cgli_ctx *c, double Roe_criterion, double E_high, double E_low)
     if (c->g.nx < 3) return CGLI_E_ARG;
     int r = ensure_range(c, CGLI_EPX, 3);
     if (r) return r;
+    if (!c->filt && c->capturing) return CGLI_E_STATE;
     if (!c->filt) {
         void *p = nullptr;
         if (hipMalloc(&p, sizeof(double) * filter_scratch_doubles(c->g)) != hipSuccess) return CGLI_E_NOMEM;
@@ -430,6 +464,7 @@ int cgli_channel_last_step(cgli_ctx *c, int *inner_fp32)
 int cgli_profiles(cgli_ctx *c, double *out)
 {
     if (!c || !out) return CGLI_E_ARG;
+    NO_CAPTURE(c);
     HIPCHK(hipSetDevice(c->device));
     if (!have(c, CGLI_U_, 5) || !have(c, CGLI_J, 1)) return CGLI_E_STATE;
     const size_t bytes = sizeof(double) * 11 * c->g.ny;
@@ -471,6 +506,7 @@ static int build_planes(cgli_ctx *c)
         c->Nijk[2 * m] = ii;
         c->Nijk[2 * m + 1] = mi;
     }
+    if (c->capturing) return CGLI_E_STATE;
     void *p = nullptr;
     if (hipMalloc(&p, sizeof(int) * c->ijk.size()) != hipSuccess) return CGLI_E_NOMEM;
     c->ijk_dev = static_cast<int *>(p);
@@ -499,7 +535,8 @@ static int implicit_inputs(cgli_ctx *c)
     return ensure_range(c, CGLI_RES, 5);
 }
 
-static int finish_norm(cgli_ctx *c, int nb, double er[5], bool check_wave = false)
+// the norm's last reduction and its copies to pinned host memory, queued
+static int enqueue_norm(cgli_ctx *c, int nb, bool check_wave)
 {
     const double DN = 1. / (double)((long)c->cfg.nx_global * c->cfg.ny * c->cfg.nz);
     double *er_dev = c->scratch + c->scratch_len - 8;
@@ -507,6 +544,12 @@ static int finish_norm(cgli_ctx *c, int nb, double er[5], bool check_wave = fals
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->er_host, er_dev, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
     if (check_wave) HIPCHK(hipMemcpyAsync(c->counters_host, c->counters, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
+    return CGLI_OK;
+}
+
+// wait for the queued norm and check it
+static int collect_norm(cgli_ctx *c, double er[5], bool check_wave)
+{
     HIPCHK(hipStreamSynchronize(c->stream));
     if (check_wave && c->counters_host[1] != 0) return CGLI_E_TIMEOUT;
     bool finite = true;
@@ -517,12 +560,20 @@ static int finish_norm(cgli_ctx *c, int nb, double er[5], bool check_wave = fals
     return finite ? CGLI_OK : CGLI_E_NONFINITE;
 }
 
+static int finish_norm(cgli_ctx *c, int nb, double er[5], bool check_wave = false)
+{
+    int r = enqueue_norm(c, nb, check_wave);
+    return r ? r : collect_norm(c, er, check_wave);
+}
+
 // the wavefront's records (in the layout of the current schedule) and its
 // exchange buffers, allocated on first use
 static int wave_setup(cgli_ctx *c)
 {
     void *p = nullptr;
     if (c->lusgs_mode == 3 && !march_fits(c->g, c->cfg.precision == 1)) return CGLI_E_ARG;
+    if (c->capturing && ((c->lusgs_mode == 3 && !c->coefM) || (c->lusgs_mode != 3 && !c->coef) || !c->xbuf))
+        return CGLI_E_STATE;
     if (c->lusgs_mode == 3 && !c->coefM) {
         // FP64 pairs are 16 bytes: the FP32 records use half
         if (hipMalloc(&p, 16 * march_coef_pairs(c->g)) != hipSuccess) return CGLI_E_NOMEM;
@@ -574,20 +625,99 @@ static int wave_setup(cgli_ctx *c)
 int cgli_set_lusgs_mode(cgli_ctx *c, int mode)
 {
     if (!c || mode < 0 || mode > 3) return CGLI_E_ARG;
+    NO_CAPTURE(c);
     c->lusgs_mode = mode;
+    c->generation++;
     return CGLI_OK;
 }
 
 int cgli_set_spin_limit(cgli_ctx *c, int max_spin)
 {
     if (!c || max_spin < 1) return CGLI_E_ARG;
+    NO_CAPTURE(c);
     c->max_spin = max_spin;
+    c->generation++;
     return CGLI_OK;
 }
 
 int cgli_dual_time_stepping(cgli_ctx *c, double deltaT, double e, double er[5], int sweeps, int write_residual)
 {
+    if (c) NO_CAPTURE(c);
     return ctx_dts(c, deltaT, e, er, sweeps, write_residual, false);
+}
+
+int cgli_dual_time_stepping_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_residual)
+{
+    return ctx_dts_async(c, deltaT, e, sweeps, write_residual, false);
+}
+
+int cgli_dual_time_stepping_wait(cgli_ctx *c, double er[5])
+{
+    if (!c || !er) return CGLI_E_ARG;
+    NO_CAPTURE(c);
+    return ctx_dts_wait(c, er);
+}
+
+int cgli_graph_begin(cgli_ctx *c)
+{
+    if (!c) return CGLI_E_ARG;
+    if (c->capturing) return CGLI_E_STATE;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    c->norm_pending = false;
+    return CGLI_OK;
+}
+
+int cgli_graph_end(cgli_ctx *c, cgli_graph **out)
+{
+    if (!c || !out) return CGLI_E_ARG;
+    *out = nullptr;
+    if (!c->capturing) return CGLI_E_STATE;
+    c->capturing = false;
+    const bool norm = c->norm_pending, wave = c->norm_wave;
+    c->norm_pending = false;   // captured, not queued
+    hipGraph_t gr = nullptr;
+    if (hipStreamEndCapture(c->stream, &gr) != hipSuccess || !gr) return CGLI_E_HIP;
+    cgli_graph *G = new (std::nothrow) cgli_graph();
+    if (!G) {
+        hipGraphDestroy(gr);
+        return CGLI_E_NOMEM;
+    }
+    const hipError_t e = hipGraphInstantiate(&G->exec, gr, nullptr, nullptr, 0);
+    hipGraphDestroy(gr);
+    if (e != hipSuccess) {
+        delete G;
+        return CGLI_E_HIP;
+    }
+    G->ctx = c;
+    G->generation = c->generation;
+    G->norm = norm;
+    G->norm_wave = wave;
+    *out = G;
+    return CGLI_OK;
+}
+
+int cgli_graph_launch(cgli_graph *G)
+{
+    if (!G || !G->ctx) return CGLI_E_ARG;
+    cgli_ctx *c = G->ctx;
+    if (c->capturing || G->generation != c->generation) return CGLI_E_STATE;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipGraphLaunch(G->exec, c->stream));
+    if (G->norm) {
+        c->norm_pending = true;
+        c->norm_wave = G->norm_wave;
+    }
+    return CGLI_OK;
+}
+
+int cgli_graph_destroy(cgli_graph *G)
+{
+    if (!G) return CGLI_E_ARG;
+    if (G->exec) hipGraphExecDestroy(G->exec);
+    delete G;
+    return CGLI_OK;
 }
 
 }  // extern "C"
@@ -598,7 +728,21 @@ namespace cgli {
 // (ctx_rhs_sumsq was the last call on it), so rhs_diag is not repeated
 int ctx_dts(cgli_ctx *c, double deltaT, double e, double er[5], int sweeps, int write_residual, bool rhs_ready)
 {
-    if (!c || !er || sweeps < 1) return CGLI_E_ARG;
+    if (!c || !er) return CGLI_E_ARG;
+    int r = ctx_dts_async(c, deltaT, e, sweeps, write_residual, rhs_ready);
+    return r ? r : ctx_dts_wait(c, er);
+}
+
+int ctx_dts_wait(cgli_ctx *c, double er[5])
+{
+    if (!c->norm_pending) return CGLI_E_STATE;
+    c->norm_pending = false;
+    return collect_norm(c, er, c->norm_wave);
+}
+
+int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_residual, bool rhs_ready)
+{
+    if (!c || sweeps < 1) return CGLI_E_ARG;
     HIPCHK(hipSetDevice(c->device));
     int r = implicit_inputs(c);
     if (r) return r;
@@ -699,7 +843,11 @@ int ctx_dts(cgli_ctx *c, double deltaT, double e, double er[5], int sweeps, int 
     tend(c, T_UPDATE);
     HIPCHK(hipGetLastError());
     mark(c, CGLI_MR, 15);
-    return finish_norm(c, nblk, er, c->lusgs_mode >= 1);
+    r = enqueue_norm(c, nblk, c->lusgs_mode >= 1);
+    if (r) return r;
+    c->norm_pending = true;
+    c->norm_wave = c->lusgs_mode >= 1;
+    return CGLI_OK;
 }
 
 }  // namespace cgli
@@ -709,6 +857,7 @@ extern "C" {
 int cgli_dplusgs(cgli_ctx *c, double deltaT, double er[5], double e, int sweeps)
 {
     if (!c || !er || sweeps < 1) return CGLI_E_ARG;
+    NO_CAPTURE(c);
     HIPCHK(hipSetDevice(c->device));
     int r = implicit_inputs(c);
     if (r) return r;
@@ -794,6 +943,7 @@ static bool file_xfer(int fd, void *buf, size_t bytes, off_t off, bool write)
 static int restart_io(cgli_ctx *c, const char *path, bool write)
 {
     if (!c || !path) return CGLI_E_ARG;
+    NO_CAPTURE(c);
     HIPCHK(hipSetDevice(c->device));
     const cgli_config &cf = c->cfg;
     if (cf.gstart < 0 || cf.nx_global < cf.gstart + cf.nx) return CGLI_E_ARG;
@@ -859,6 +1009,13 @@ int cgli_timing_enable(cgli_ctx *c, int on)
 {
     if (!c) return CGLI_E_ARG;
     c->timing = on != 0;
+    return CGLI_OK;
+}
+
+int cgli_timing_select(cgli_ctx *c, unsigned mask)
+{
+    if (!c) return CGLI_E_ARG;
+    c->tmask = mask;
     return CGLI_OK;
 }
 

@@ -174,6 +174,11 @@ int ctx_rhs_sumsq(cgli_ctx *c, double deltaT, double e, double out[5]);
 // one inner iteration (cgli_dual_time_stepping); rhs_ready: the last call
 // on U_ was ctx_rhs_sumsq, whose right-hand side is reused
 int ctx_dts(cgli_ctx *c, double deltaT, double e, double er[5], int sweeps, int write_residual, bool rhs_ready);
+// the same in two halves: queue everything up to the norm's copy to the
+// host, then wait for it and check it
+int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_residual, bool rhs_ready);
+int ctx_dts_wait(cgli_ctx *c, double er[5]);
+bool ctx_timing_on(cgli_ctx *c);
 // per-context state of the channel driver (channel.cpp)
 struct ChannelState {
     int rule = 0;            // 0: per-step drop, 1: running maximum
@@ -184,7 +189,18 @@ struct ChannelState {
     double r_net[5] = {0, 0, 0, 0, 0}, r_ext[5] = {0, 0, 0, 0, 0};
     int since_check = 0;     // steps since the network's guess was last checked
     bool verify = true;      // check at the next step (first use, after a rejection)
+    // inner iterations replayed from captured graphs, one per (residual
+    // precision, deltaT, sweeps); a key runs eagerly the first time
+    struct GraphSlot {
+        int fp32;
+        double dt;
+        int sweeps;
+        cgli_graph *g;
+    };
+    bool graph = false;
+    std::vector<GraphSlot> graphs;
 };
+void channel_drop_graphs(ChannelState &cs);
 ChannelState &ctx_channel(cgli_ctx *c);
 // network side of the channel driver (learned.cpp)
 int net_record(cgli_net *net, cgli_ctx *c);                 // append U - Uq to the ring

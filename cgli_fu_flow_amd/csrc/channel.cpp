@@ -19,6 +19,13 @@
 
 using namespace cgli;
 
+void cgli::channel_drop_graphs(ChannelState &cs)
+{
+    for (auto &q : cs.graphs)
+        if (q.g) cgli_graph_destroy(q.g);
+    cs.graphs.clear();
+}
+
 namespace {
 
 const double PI = 3.14159265358979323846;
@@ -241,6 +248,60 @@ int cgli_channel_ghosts(cgli_ctx *ctx)
 
 namespace {
 
+// one inner iteration queued on the context stream: ghosts, fluxes and
+// viscous terms (unless the right-hand side is already in place), then the
+// dual-time update up to the norm's copy to the host
+int queue_iteration(cgli_ctx *ctx, double deltaT, int sweeps, bool reuse)
+{
+    int r = CGLI_OK;
+    if (!reuse) {
+        r = cgli_channel_ghosts(ctx);
+        if (!r) r = cgli_flux_x(ctx, 1.0, 1e-4);
+        if (!r) r = cgli_flux_y(ctx, 1.0, 1e-4);
+        if (!r) r = cgli_flux_z(ctx, 1.0, 1e-4);
+        if (!r) r = cgli_viscous(ctx);
+    }
+    if (!r) r = ctx_dts_async(ctx, deltaT, 1e-4, sweeps, 0, reuse);
+    return r;
+}
+
+// queue one inner iteration, from the graph of its key when graphs are on:
+// a key's first occurrence runs eagerly (it allocates what the iteration
+// needs), its second is captured; a graph gone stale is captured again
+int run_iteration(cgli_ctx *ctx, ChannelState &cs, double deltaT, int sweeps, bool reuse, bool fp32)
+{
+    if (!cs.graph || reuse || ctx_timing_on(ctx)) return queue_iteration(ctx, deltaT, sweeps, reuse);
+    ChannelState::GraphSlot *slot = nullptr;
+    for (auto &q : cs.graphs)
+        if (q.fp32 == (int)fp32 && q.dt == deltaT && q.sweeps == sweeps) slot = &q;
+    if (!slot) {
+        cs.graphs.push_back({(int)fp32, deltaT, sweeps, nullptr});
+        return queue_iteration(ctx, deltaT, sweeps, reuse);
+    }
+    if (slot->g) {
+        const int r = cgli_graph_launch(slot->g);
+        if (r != CGLI_E_STATE) return r;
+        // stale (the context changed): eagerly now, captured next time
+        cgli_graph_destroy(slot->g);
+        slot->g = nullptr;
+        return queue_iteration(ctx, deltaT, sweeps, reuse);
+    }
+    int r = cgli_graph_begin(ctx);
+    if (r) return r;
+    r = queue_iteration(ctx, deltaT, sweeps, reuse);
+    cgli_graph *G = nullptr;
+    const int re = cgli_graph_end(ctx, &G);
+    if (!r) r = re;
+    if (r) {
+        if (G) cgli_graph_destroy(G);
+        // CGLI_E_STATE: the iteration needed an allocation (a new schedule's
+        // buffers); nothing ran, so run it eagerly and capture next time
+        return r == CGLI_E_STATE ? queue_iteration(ctx, deltaT, sweeps, reuse) : r;
+    }
+    slot->g = G;
+    return cgli_graph_launch(G);
+}
+
 // residual sums of squares of the current U_ (ghosts, fluxes, viscous
 // terms and the right-hand side of the inner iteration)
 int guess_residual(cgli_ctx *ctx, double deltaT, double out[5])
@@ -340,6 +401,15 @@ int cgli_channel_rule_state(cgli_ctx *ctx, double E[5], int set)
     return CGLI_OK;
 }
 
+int cgli_channel_set_graph(cgli_ctx *ctx, int on)
+{
+    if (!ctx) return CGLI_E_ARG;
+    ChannelState &cs = ctx_channel(ctx);
+    cs.graph = on != 0;
+    if (!cs.graph) channel_drop_graphs(cs);
+    return CGLI_OK;
+}
+
 int cgli_channel_set_guess(cgli_ctx *ctx, int guess)
 {
     if (!ctx || guess < 0 || guess > 3) return CGLI_E_ARG;
@@ -401,15 +471,8 @@ int cgli_channel_step(cgli_ctx *ctx, double deltaT, double tol, int max_inner, i
         // the first iteration reuses the residual evaluation of the guess
         // check when the network's guess was taken
         const bool reuse = ready && it == 1;
-        int r = CGLI_OK;
-        if (!reuse) {
-            r = cgli_channel_ghosts(ctx);
-            if (!r) r = cgli_flux_x(ctx, 1.0, 1e-4);
-            if (!r) r = cgli_flux_y(ctx, 1.0, 1e-4);
-            if (!r) r = cgli_flux_z(ctx, 1.0, 1e-4);
-            if (!r) r = cgli_viscous(ctx);
-        }
-        if (!r) r = ctx_dts(ctx, deltaT, 1e-4, er, sweeps, 0, reuse);
+        int r = run_iteration(ctx, cs, deltaT, sweeps, reuse, fp32);
+        if (!r) r = ctx_dts_wait(ctx, er);
         if (r) return r;
         if (it == 1)
             for (int v = 0; v < 5; v++) e1[v] = er[v];

@@ -165,7 +165,7 @@ __device__ __forceinline__ void roe_face(const T L[5], const T Rq[5], T mx, T my
 // ---------------------------------------------------------------- x faces
 // Face i+1/2 between cells (i, j+1, k+1) and (i+1, j+1, k+1), stored at
 // [i][j][k] for i=2..gend, j=1..NY-1, k=1..NZ-1.
-template <typename T>
+template <typename T, bool SCALE>
 __global__ void __launch_bounds__(256) k_flux_x(Geo g, double K, double e, C5 U, const double *__restrict__ J,
                                                 const double *__restrict__ Jf, M9 mf, F5 F, double epS,
                                                 const double *__restrict__ EpF)
@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(256) k_flux_x(Geo g, double K, double e, C5 U,
     const T xix = ld<T>(mf.m[0], f), xiy = ld<T>(mf.m[1], f), xiz = ld<T>(mf.m[2], f);
     const T insqr = T(1.0) / sqrt(xix * xix + xiy * xiy + xiz * xiz);
     T out[5], pl, pr, fav3;
-    const T ep = EpF ? ld<T>(EpF, f) : T(epS);
+    const T ep = SCALE ? (EpF ? ld<T>(EpF, f) : T(epS)) : T(1);
     roe_face<T>(Lq, Rq, xix, xiy, xiz, ld<T>(Jf, f), T(e), T(K), xix * insqr, xiy * insqr, xiz * insqr, ep, out, pl,
                 pr, fav3);
 #pragma unroll
@@ -212,7 +212,7 @@ __global__ void __launch_bounds__(256) k_flux_x(Geo g, double K, double e, C5 U,
 // Face j+1/2 between cells (i+1, j, k+1) and (i+1, j+1, k+1), stored at
 // [i][j][k] for i=2..gend-1, j=1..NY, k=1..NZ-1.  Near-wall order drop of
 // Flux_Y.cpp:135-249 and the wall-face override of Flux_Y.cpp:400-408.
-template <typename T>
+template <typename T, bool SCALE>
 __global__ void __launch_bounds__(256) k_flux_y(Geo g, double K, double Ep, double e, int xi_mode, C5 U,
                                                 const double *__restrict__ J, const double *__restrict__ Jf,
                                                 M9 mf, F5 F, double epS, const double *__restrict__ EpF)
@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(256) k_flux_y(Geo g, double K, double Ep, doub
     if (xi_mode) { nx_ = etx * insqr; ny_ = ety * insqr; nz_ = etz * insqr; }
     T out[5], pl, pr, fav3;
     const T jf = ld<T>(Jf, f);
-    const T ep = EpF ? ld<T>(EpF, f) : T(epS);
+    const T ep = SCALE ? (EpF ? ld<T>(EpF, f) : T(epS)) : T(1);
     roe_face<T>(Lq, Rq, etx, ety, etz, jf, T(e), T(K), nx_, ny_, nz_, ep, out, pl, pr, fav3);
     if (j == 1 || j == NY) {
         out[0] = 0;
@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(256) k_flux_y(Geo g, double K, double Ep, doub
 // ---------------------------------------------------------------- z faces
 // Face k+1/2 between cells (i+1, j+1, k) and (i+1, j+1, k+1), stored at
 // [i][j][k] for i=2..gend-1, j=1..NY-1, k=1..NZ (Flux_Z.cpp:105-216).
-template <typename T>
+template <typename T, bool SCALE>
 __global__ void __launch_bounds__(256) k_flux_z(Geo g, double K, double e, int xi_mode, C5 U,
                                                 const double *__restrict__ J, const double *__restrict__ Jf,
                                                 M9 mf, F5 F, double epS, const double *__restrict__ EpF)
@@ -302,11 +302,15 @@ __global__ void __launch_bounds__(256) k_flux_z(Geo g, double K, double e, int x
     T nx_ = 0, ny_ = 0, nz_ = 0;
     if (xi_mode) { nx_ = ztx * insqr; ny_ = zty * insqr; nz_ = ztz * insqr; }
     T out[5], pl, pr, fav3;
-    const T ep = EpF ? ld<T>(EpF, f) : T(epS);
+    const T ep = SCALE ? (EpF ? ld<T>(EpF, f) : T(epS)) : T(1);
     roe_face<T>(Lq, Rq, ztx, zty, ztz, ld<T>(Jf, f), T(e), T(K), nx_, ny_, nz_, ep, out, pl, pr, fav3);
 #pragma unroll
     for (int v = 0; v < 5; v++) F.v[v][f] = (double)out[v];
 }
+
+// the unscaled instantiation is the live reference flux, compiled without
+// the factor
+#define SCALED(K, T) ((EpF || epS != 1.0) ? K<T, true> : K<T, false>)
 
 static dim3 grid_for(int nk, int nj, int ni, dim3 b)
 {
@@ -318,10 +322,10 @@ void launch_flux_x(const Geo &g, const Phys &p, double e, C5 U, const double *J,
 {
     dim3 b(64, 4);
     if (p.fp32)
-        hipLaunchKernelGGL(k_flux_x<float>, grid_for(g.NZ - 1, g.NY - 1, g.gend - 1, b), b, 0, s, g, p.K, e, U, J,
+        hipLaunchKernelGGL(SCALED(k_flux_x, float), grid_for(g.NZ - 1, g.NY - 1, g.gend - 1, b), b, 0, s, g, p.K, e, U, J,
                            Jf, mf, F, epS, EpF);
     else
-        hipLaunchKernelGGL(k_flux_x<double>, grid_for(g.NZ - 1, g.NY - 1, g.gend - 1, b), b, 0, s, g, p.K, e, U, J,
+        hipLaunchKernelGGL(SCALED(k_flux_x, double), grid_for(g.NZ - 1, g.NY - 1, g.gend - 1, b), b, 0, s, g, p.K, e, U, J,
                            Jf, mf, F, epS, EpF);
 }
 
@@ -330,10 +334,10 @@ void launch_flux_y(const Geo &g, const Phys &p, double Ep, double e, int xi_mode
 {
     dim3 b(64, 4);
     if (p.fp32)
-        hipLaunchKernelGGL(k_flux_y<float>, grid_for(g.NZ - 1, g.NY, g.gend - 2, b), b, 0, s, g, p.K, Ep, e, xi_mode,
+        hipLaunchKernelGGL(SCALED(k_flux_y, float), grid_for(g.NZ - 1, g.NY, g.gend - 2, b), b, 0, s, g, p.K, Ep, e, xi_mode,
                            U, J, Jf, mf, F, epS, EpF);
     else
-        hipLaunchKernelGGL(k_flux_y<double>, grid_for(g.NZ - 1, g.NY, g.gend - 2, b), b, 0, s, g, p.K, Ep, e, xi_mode,
+        hipLaunchKernelGGL(SCALED(k_flux_y, double), grid_for(g.NZ - 1, g.NY, g.gend - 2, b), b, 0, s, g, p.K, Ep, e, xi_mode,
                            U, J, Jf, mf, F, epS, EpF);
 }
 
@@ -342,10 +346,10 @@ void launch_flux_z(const Geo &g, const Phys &p, double e, int xi_mode, C5 U, con
 {
     dim3 b(64, 4);
     if (p.fp32)
-        hipLaunchKernelGGL(k_flux_z<float>, grid_for(g.NZ, g.NY - 1, g.gend - 2, b), b, 0, s, g, p.K, e, xi_mode, U,
+        hipLaunchKernelGGL(SCALED(k_flux_z, float), grid_for(g.NZ, g.NY - 1, g.gend - 2, b), b, 0, s, g, p.K, e, xi_mode, U,
                            J, Jf, mf, F, epS, EpF);
     else
-        hipLaunchKernelGGL(k_flux_z<double>, grid_for(g.NZ, g.NY - 1, g.gend - 2, b), b, 0, s, g, p.K, e, xi_mode, U,
+        hipLaunchKernelGGL(SCALED(k_flux_z, double), grid_for(g.NZ, g.NY - 1, g.gend - 2, b), b, 0, s, g, p.K, e, xi_mode, U,
                            J, Jf, mf, F, epS, EpF);
 }
 

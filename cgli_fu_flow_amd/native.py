@@ -75,6 +75,13 @@ def lib():
         L.cgli_hyperplane.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.cgli_dual_time_stepping.argtypes = [vp, C.c_double, C.c_double, dp, C.c_int, C.c_int]
         L.cgli_dplusgs.argtypes = [vp, C.c_double, dp, C.c_double, C.c_int]
+        L.cgli_dual_time_stepping_async.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int]
+        L.cgli_dual_time_stepping_wait.argtypes = [vp, dp]
+        L.cgli_graph_begin.argtypes = [vp]
+        L.cgli_graph_end.argtypes = [vp, C.POINTER(vp)]
+        L.cgli_graph_launch.argtypes = [vp]
+        L.cgli_graph_destroy.argtypes = [vp]
+        L.cgli_channel_set_graph.argtypes = [vp, C.c_int]
         L.cgli_bc_y.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int]
         L.cgli_bc_z.argtypes = [vp]
         L.cgli_bc_x_periodic.argtypes = [vp]
@@ -90,6 +97,7 @@ def lib():
         L.cgli_set_spin_limit.argtypes = [vp, C.c_int]
         L.cgli_timing_enable.argtypes = [vp, C.c_int]
         L.cgli_timing_reset.argtypes = [vp]
+        L.cgli_timing_select.argtypes = [vp, C.c_uint]
         L.cgli_timing_get.argtypes = [vp, C.c_int, dp, C.POINTER(C.c_long)]
         L.cgli_channel_set_rule.argtypes = [vp, C.c_int]
         L.cgli_device_memory.argtypes = [C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
@@ -245,6 +253,29 @@ class Context:
                "dual_time_stepping")
         return np.array(list(er))
 
+    def dual_time_stepping_async(self, deltaT, e, sweeps=1, write_residual=False):
+        """Queue one dual-time iteration; dual_time_stepping_wait returns its norms."""
+        _check(lib().cgli_dual_time_stepping_async(self.h, deltaT, e, sweeps, 1 if write_residual else 0),
+               "dual_time_stepping_async")
+
+    def dual_time_stepping_wait(self):
+        er = (C.c_double * 5)()
+        _check(lib().cgli_dual_time_stepping_wait(self.h, er), "dual_time_stepping_wait")
+        return np.array(list(er))
+
+    # -- HIP graphs of a sequence of calls on this context
+    def graph_begin(self):
+        _check(lib().cgli_graph_begin(self.h), "graph_begin")
+
+    def graph_end(self):
+        g = C.c_void_p()
+        _check(lib().cgli_graph_end(self.h, C.byref(g)), "graph_end")
+        return Graph(g, self)
+
+    def set_graph(self, on=True):
+        """Replay channel_step's inner iterations from captured graphs."""
+        _check(lib().cgli_channel_set_graph(self.h, 1 if on else 0), "channel_set_graph")
+
     def dplusgs(self, deltaT, e, sweeps=4):
         er = (C.c_double * 5)()
         _check(lib().cgli_dplusgs(self.h, deltaT, er, e, sweeps), "dplusgs")
@@ -336,7 +367,10 @@ class Context:
     def set_spin_limit(self, n):
         _check(lib().cgli_set_spin_limit(self.h, n), "set_spin_limit")
 
-    def timing(self, on=True):
+    def timing(self, on=True, classes=None):
+        """Time kernel classes with HIP events (all, or the named ones), reset."""
+        mask = 0xFFFFFFFF if classes is None else sum(1 << self.TIMING_CLASSES.index(c) for c in classes)
+        _check(lib().cgli_timing_select(self.h, mask), "timing_select")
         _check(lib().cgli_timing_enable(self.h, int(on)), "timing_enable")
         _check(lib().cgli_timing_reset(self.h), "timing_reset")
 
@@ -348,6 +382,28 @@ class Context:
             _check(lib().cgli_timing_get(self.h, i, C.byref(t), C.byref(n)), "timing_get")
             out[name] = (t.value, n.value)
         return out
+
+
+class Graph:
+    """A captured sequence of calls on one context (cgli_graph)."""
+
+    def __init__(self, handle, ctx):
+        self.h = handle
+        self._ctx = ctx   # keep the context alive while the graph exists
+
+    def launch(self):
+        _check(lib().cgli_graph_launch(self.h), "graph_launch")
+
+    def close(self):
+        if self.h:
+            lib().cgli_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _fp(a):

@@ -32,6 +32,7 @@ extern "C" {
 
 typedef struct cgli_ctx cgli_ctx;
 typedef struct cgli_net cgli_net;
+typedef struct cgli_graph cgli_graph;
 
 /* Runtime form of the reference's compile-time configuration
  * (Resolution.h, prm.h, Viscous_terms.h, Mpi_division.h). */
@@ -135,6 +136,33 @@ int cgli_hyperplane(cgli_ctx *ctx, int *Nijk, int *ijk);
  * write_residual != 0 also stores the squared increments in Residual. */
 int cgli_dual_time_stepping(cgli_ctx *ctx, double deltaT, double e, double er[5], int sweeps,
                             int write_residual);
+/* The same in two halves, so that an inner iteration can be queued (or
+ * captured into a graph) without waiting for the device: _async queues
+ * every launch of cgli_dual_time_stepping up to the copy of the norms to
+ * the host; _wait waits for them and returns er[5] and the error checks
+ * (CGLI_E_NONFINITE, CGLI_E_TIMEOUT); CGLI_E_STATE when nothing is queued. */
+int cgli_dual_time_stepping_async(cgli_ctx *ctx, double deltaT, double e, int sweeps, int write_residual);
+int cgli_dual_time_stepping_wait(cgli_ctx *ctx, double er[5]);
+
+/* ---- HIP graphs (no counterpart in the reference) ---------------------
+ * cgli_graph_begin starts capturing the context's stream; the entry points
+ * called until cgli_graph_end are recorded into a graph instead of running
+ * (fluxes, viscous terms, filter, ghost fills, cgli_dual_time_stepping_async
+ * and the channel's ghost fill).  Entry points that wait for the device
+ * (field copies, cgli_dual_time_stepping, cgli_dplusgs, cgli_profiles,
+ * restart files, mode setters) return CGLI_E_STATE while capturing, as does
+ * any call that would allocate device memory: run the sequence once without
+ * capture first.  Kernel timing does not see launches made by a graph.
+ * cgli_graph_launch queues the graph on the context's stream (a deferred
+ * norm in it is then collected by cgli_dual_time_stepping_wait); it returns
+ * CGLI_E_STATE when the context changed since the capture (LU-SGS schedule,
+ * spin limit, dissipation scaling, a larger reduction buffer), and the
+ * sequence must be captured again.  Destroy graphs before their context. */
+int cgli_graph_begin(cgli_ctx *ctx);
+int cgli_graph_end(cgli_ctx *ctx, cgli_graph **out);
+int cgli_graph_launch(cgli_graph *graph);
+int cgli_graph_destroy(cgli_graph *graph);
+
 /* LU-SGS schedule: 0 = one launch per i+j+k hyperplane (the reference's
  * order), 1 = tiled wavefront in one launch per sweep, 2 = the
  * same wavefront with two loader waves per tile streaming the coefficients
@@ -214,6 +242,13 @@ int cgli_channel_step(cgli_ctx *ctx, double deltaT, double tol, int max_inner, i
  * in FP64 down to tol; *inner_fp32 receives how many of the last step's
  * iterations used FP32 residuals. */
 int cgli_channel_last_step(cgli_ctx *ctx, int *inner_fp32);
+
+/* on != 0: cgli_channel_step replays each inner iteration after a step's
+ * first from a HIP graph captured the second time its (residual precision,
+ * deltaT, sweeps) occurs, one launch instead of about twenty; results are
+ * the same bits.  Not while kernel timing is enabled (then every iteration
+ * runs eagerly).  Off by default. */
+int cgli_channel_set_graph(cgli_ctx *ctx, int on);
 
 /* ---- inner-iteration rule and initial guess of the channel step ------ */
 /* No counterpart in the reference's file set; the reference stops on
@@ -309,6 +344,10 @@ int cgli_net_timing(cgli_net *net, double *predict_ms, long *n_predict, double *
  * 9 filter 10 march_scatter (LU-SGS schedule 3: increments back to their
  * fields). */
 int cgli_timing_enable(cgli_ctx *ctx, int on);
+/* classes timed while timing is on: bit c of mask for class c (default all).
+ * Each timed launch costs two event records on the stream, so a run that
+ * needs one class's figure times only that class. */
+int cgli_timing_select(cgli_ctx *ctx, unsigned mask);
 int cgli_timing_reset(cgli_ctx *ctx);
 int cgli_timing_get(cgli_ctx *ctx, int cls, double *total_ms, long *count);
 
