@@ -109,6 +109,9 @@ long diag_doubles(const Geo &g);
 // trash: 256 doubles per tile
 // before each sweep: exchange sentinels and the tile ticket
 void launch_lusgs_wave_prep(const Geo &g, double *xbuf, int *counters, hipStream_t s);
+// before a march sweep: sentinels into the exchange entries it polls, ticket
+// counters[0] to zero
+void launch_march_prep(const Geo &g, int forward, double *xbuf, int *counters, hipStream_t s);
 // coef: split blocks (float when fp32), dgn: FP64 diagonal blocks
 // loaders: two loader waves per tile keep the coefficient stream in flight
 // (slower than the computing wave's own prefetch at 128^3: 2.17 vs 1.94 ms)

@@ -273,10 +273,14 @@ __global__ void __launch_bounds__(256) k_flux_z(Geo g, double K, double e, int x
                                                 const double *__restrict__ J, const double *__restrict__ Jf,
                                                 M9 mf, F5 F, double epS, const double *__restrict__ EpF)
 {
-    const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    // (j, k) = (1..NY-1, 1..NZ) of a plane flattened, k fastest: NZ faces
+    // per row would leave a third of the lanes idle in 64-wide rows
+    const int nk = g.NZ;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nk * (g.NY - 1)) return;
+    const int k = 1 + t % nk;
+    const int j = 1 + t / nk;
     const int i = 2 + blockIdx.z;
-    if (k > g.NZ || j > g.NY - 1) return;
     const long c0 = ix(g, i + 1, j + 1, k);   // cell k
     const long c1 = c0 + 1;                   // cell k+1
     const int NZ = g.NZ;
@@ -344,12 +348,13 @@ void launch_flux_y(const Geo &g, const Phys &p, double Ep, double e, int xi_mode
 void launch_flux_z(const Geo &g, const Phys &p, double e, int xi_mode, C5 U, const double *J, const double *Jf,
                    M9 mf, F5 F, double epS, const double *EpF, hipStream_t s)
 {
-    dim3 b(64, 4);
+    const dim3 b(256);
+    const dim3 gr((g.NZ * (g.NY - 1) + 255) / 256, 1, g.gend - 2);
     if (p.fp32)
-        hipLaunchKernelGGL(SCALED(k_flux_z, float), grid_for(g.NZ, g.NY - 1, g.gend - 2, b), b, 0, s, g, p.K, e, xi_mode, U,
+        hipLaunchKernelGGL(SCALED(k_flux_z, float), gr, b, 0, s, g, p.K, e, xi_mode, U,
                            J, Jf, mf, F, epS, EpF);
     else
-        hipLaunchKernelGGL(SCALED(k_flux_z, double), grid_for(g.NZ, g.NY - 1, g.gend - 2, b), b, 0, s, g, p.K, e, xi_mode, U,
+        hipLaunchKernelGGL(SCALED(k_flux_z, double), gr, b, 0, s, g, p.K, e, xi_mode, U,
                            J, Jf, mf, F, epS, EpF);
 }
 

@@ -764,6 +764,34 @@ __global__ void k_fill_sentinel(Geo g, double *__restrict__ xbuf)
     for (int v = 0; v < 5; v++) xbuf[5 * c + v] = __longlong_as_double((long long)SENTINEL);
 }
 
+// The sentinel into the exchange entries one march sweep polls, and the
+// sweep's tile ticket to zero.  A tile's consumers read the row of cells
+// just across its j edge and the column just across its k edge, which are
+// the rows (j-2) % TJ == rj and columns (k-2) % TK == rk of every x plane
+// (rj = TJ-1, rk = TK-1 forward; 0, 0 backward); no other entry is read.
+// blockIdx.y 0: the rows, 5 nz consecutive doubles each; 1: the columns.
+__global__ void k_fill_sentinel_edges(Geo g, double *__restrict__ xbuf, int TJ, int TK, int rj, int rk,
+                                      int *__restrict__ ticket)
+{
+    const int i = 3 + blockIdx.z;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ticket && t == 0 && blockIdx.y == 0 && blockIdx.z == 0) *ticket = 0;
+    const double S = __longlong_as_double((long long)SENTINEL);
+    const int nrow = rj < g.ny ? (g.ny - 1 - rj) / TJ + 1 : 0;
+    const int ncol = rk < g.nz ? (g.nz - 1 - rk) / TK + 1 : 0;
+    if (blockIdx.y == 0) {
+        const long per = 5L * g.nz;
+        if (t >= per * nrow) return;
+        const int j = 2 + rj + (int)(t / per) * TJ;
+        xbuf[5 * ix(g, i, j, 2) + t % per] = S;
+    } else {
+        if (t >= 5L * g.ny * ncol) return;
+        const long cell = t / 5;
+        const int j = 2 + (int)(cell / ncol), k = 2 + rk + (int)(cell % ncol) * TK;
+        xbuf[5 * ix(g, i, j, k) + t % 5] = S;
+    }
+}
+
 __global__ void k_reset_ints(int *p, int n)
 {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
@@ -1969,6 +1997,16 @@ void launch_split_coef(const Geo &g, const Phys &p, double deltaT, double e, C5 
     else
         hipLaunchKernelGGL(k_split_coef<double>, gr, b, 0, s, g, p, deltaT, e, U_, J, met,
                            static_cast<double *>(coef), dgn);
+}
+
+void launch_march_prep(const Geo &g, int forward, double *xbuf, int *counters, hipStream_t s)
+{
+    const int rj = forward ? MARCH_TJ - 1 : 0, rk = forward ? MARCH_TK - 1 : 0;
+    const long nrow = rj < g.ny ? (g.ny - 1 - rj) / MARCH_TJ + 1 : 0;
+    const long ncol = rk < g.nz ? (g.nz - 1 - rk) / MARCH_TK + 1 : 0;
+    const long n = std::max(std::max(5L * g.nz * nrow, 5L * g.ny * ncol), 1L);
+    const dim3 gr((unsigned)((n + 255) / 256), 2, g.nx);
+    hipLaunchKernelGGL(k_fill_sentinel_edges, gr, dim3(256), 0, s, g, xbuf, MARCH_TJ, MARCH_TK, rj, rk, counters);
 }
 
 void launch_lusgs_wave_prep(const Geo &g, double *xbuf, int *counters, hipStream_t s)

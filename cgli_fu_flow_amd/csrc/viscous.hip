@@ -18,11 +18,14 @@ namespace cgli {
 __global__ void __launch_bounds__(256) k_prim4(Geo g, double K, double R, C5 U, const double *__restrict__ J,
                                                double *__restrict__ P4)
 {
-    // box i=2..gend+1, j=1..NYY, k=1..NZZ
-    const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    // box i=2..gend+1, j=1..NYY, k=1..NZZ; (j, k) of a plane flattened, k
+    // fastest, so that rows of NZZ cells fill whole waves
+    const int nk = g.NZZ;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nk * g.NYY) return;
+    const int k = 1 + t % nk;
+    const int j = 1 + t / nk;
     const int i = 2 + blockIdx.z;
-    if (k > g.NZZ || j > g.NYY) return;
     const long c = ix(g, i, j, k);
     const double r = U.v[0][c];
     const double u = U.v[1][c] / r, v = U.v[2][c] / r, w = U.v[3][c] / r;
@@ -160,10 +163,13 @@ __global__ void __launch_bounds__(256) k_visc_faces(Geo g, Phys p, const double 
                                                     const double *__restrict__ Jw, M9 mw,
                                                     double *__restrict__ FF)
 {
-    const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+    // (j, k) = (2..NYY, 2..NZZ) of a plane flattened, k fastest
+    const int nk = g.NZZ - 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nk * (g.NYY - 1)) return;
+    const int k = 2 + t % nk;
+    const int j = 2 + t / nk;
     const int i = 3 + blockIdx.z;
-    if (k > g.NZZ || j > g.NYY) return;
     const bool xf = (j <= g.NY && k <= g.NZ);
     const bool yf = (i <= g.gend && k <= g.NZ);
     const bool zf = (i <= g.gend && j <= g.NY);
@@ -294,14 +300,16 @@ void launch_viscous(const Geo &g, const Phys &p, C5 U, const double *J, const do
                     const double *Jv, M9 mv, const double *Jw, M9 mw, double *prim4, double *faces12,
                     F4 vF, hipStream_t s)
 {
-    dim3 b(64, 4);
-    dim3 gp((g.NZZ + b.x - 1) / b.x, (g.NYY + b.y - 1) / b.y, g.gend);   // i=2..gend+1
-    hipLaunchKernelGGL(k_prim4, gp, b, 0, s, g, p.K, p.R, U, J, prim4);
-    dim3 gf((g.NZZ - 1 + b.x - 1) / b.x, (g.NYY - 1 + b.y - 1) / b.y, g.gend - 1);  // i=3..gend+1
+    const dim3 b1(256);
+    const int np = g.NZZ * g.NYY, nf = (g.NZZ - 1) * (g.NYY - 1);
+    dim3 gp((np + 255) / 256, 1, g.gend);   // i=2..gend+1
+    hipLaunchKernelGGL(k_prim4, gp, b1, 0, s, g, p.K, p.R, U, J, prim4);
+    dim3 gf((nf + 255) / 256, 1, g.gend - 1);   // i=3..gend+1
     if (p.fp32)
-        hipLaunchKernelGGL(k_visc_faces<float>, gf, b, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
+        hipLaunchKernelGGL(k_visc_faces<float>, gf, b1, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
     else
-        hipLaunchKernelGGL(k_visc_faces<double>, gf, b, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
+        hipLaunchKernelGGL(k_visc_faces<double>, gf, b1, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
+    dim3 b(64, 4);
     dim3 gd((g.NZ - 1 + b.x - 1) / b.x, (g.NY - 1 + b.y - 1) / b.y, g.gend - 2);   // i=3..gend
     hipLaunchKernelGGL(k_visc_div, gd, b, 0, s, g, p, faces12, vF);
 }
