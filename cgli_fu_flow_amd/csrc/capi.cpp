@@ -754,7 +754,12 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
     r = scratch(c, 5 * nb + (long)g.NYY * g.NZZ + 64);
     if (r) return r;
     const hipStream_t s = c->stream;
-    launch_zero_increment_ghosts(g, f5(c, CGLI_NL), f5(c, CGLI_NR), s);
+    if (c->lusgs_mode >= 1) {
+        r = wave_setup(c);
+        if (r) return r;
+    }
+    // also zeroes the wavefront's ticket and error flag
+    launch_zero_increment_ghosts(g, f5(c, CGLI_NL), f5(c, CGLI_NR), c->lusgs_mode >= 1 ? c->counters : nullptr, s);
     if (c->cfg.sponge || write_residual)
         for (int v = 0; v < 5; v++) launch_zero(c->f[CGLI_RES + v], g.N, s);
     if (c->cfg.sponge)
@@ -771,9 +776,6 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
     }
     const int mlast = g.gend + g.NY + g.NZ;
     if (c->lusgs_mode >= 1) {
-        r = wave_setup(c);
-        if (r) return r;
-        launch_reset_ints(c->counters, 2, s);
         tbegin(c, T_COEF);
         if (c->lusgs_mode == 3)
             launch_split_march(g, c->p, deltaT, e, c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_MR),
@@ -870,7 +872,7 @@ int cgli_dplusgs(cgli_ctx *c, double deltaT, double er[5], double e, int sweeps)
     Phys pd = c->p;
     pd.grav = 0.0;            // DPLUSGS.cpp:417,420 use the constant f of prm.h:11
     pd.f_const = c->cfg.f_dplusgs;
-    launch_zero_increment_ghosts(g, f5(c, CGLI_NL), f5(c, CGLI_NR), s);
+    launch_zero_increment_ghosts(g, f5(c, CGLI_NL), f5(c, CGLI_NR), nullptr, s);
     launch_rhs_diag(g, pd, deltaT, e, 0, c5(c, CGLI_U_), c5(c, CGLI_U), c5(c, CGLI_UQ), c->f[CGLI_J],
                     m9(c, CGLI_MET), c5(c, CGLI_INFX), c5(c, CGLI_INFY), c5(c, CGLI_INFZ),
                     C4{{c->f[CGLI_VF], c->f[CGLI_VF + 1], c->f[CGLI_VF + 2], c->f[CGLI_VF + 3]}},

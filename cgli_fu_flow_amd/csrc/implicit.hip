@@ -104,18 +104,38 @@ __device__ __forceinline__ void load9(const M9 &M, long c, double m[9])
 }
 
 // ------------------------------------------------------------------ ghosts
-__global__ void k_zero_ghosts(Geo g, F5 NL, F5 NR)
+__global__ void k_zero_ghosts(Geo g, F5 NL, F5 NR, int *__restrict__ counters)
 {
     // every (i,j,k) of i=2..gend+1, j=1..NYY, k=1..NZZ that is a ghost of
-    // the increment arrays: x planes (first/last), j=1/NYY, k=1/NZZ
-    const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = 2 + blockIdx.z;
-    if (k > g.NZZ || j > g.NYY) return;
-    const bool xg = ((g.first || g.xper) && i == 2) || ((g.last || g.xper) && i == g.gend + 1);
-    const bool yg = (j == 1 || j == g.NYY);
-    const bool zg = (k == 1 || k == g.NZZ);
-    if (!(xg || yg || zg)) return;
+    // the increment arrays, one shell per blockIdx.y: 0 the planes j=1 and
+    // j=NYY, 1 the planes k=1 and k=NZZ, 2 the x planes i=2 (first slab or
+    // periodic) and i=gend+1 (last slab or periodic).  counters (the
+    // wavefront's ticket and error flag) are zeroed on the way.
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (counters && t < 2 && blockIdx.y == 0) counters[t] = 0;
+    const int ni = g.gend, nj = g.NYY, nk = g.NZZ;
+    int i, j, k;
+    if (blockIdx.y == 0) {
+        if (t >= 2L * ni * nk) return;
+        k = 1 + (int)(t % nk);
+        const long r = t / nk;
+        i = 2 + (int)(r % ni);
+        j = (r / ni) ? g.NYY : 1;
+    } else if (blockIdx.y == 1) {
+        if (t >= 2L * ni * nj) return;
+        j = 1 + (int)(t % nj);
+        const long r = t / nj;
+        i = 2 + (int)(r % ni);
+        k = (r / ni) ? g.NZZ : 1;
+    } else {
+        if (t >= 2L * nj * nk) return;
+        k = 1 + (int)(t % nk);
+        const long r = t / nk;
+        j = 1 + (int)(r % nj);
+        const bool hi = (r / nj) != 0;
+        if (hi ? !(g.last || g.xper) : !(g.first || g.xper)) return;
+        i = hi ? g.gend + 1 : 2;
+    }
     const long c = ix(g, i, j, k);
 #pragma unroll
     for (int v = 0; v < 5; v++) {
@@ -124,11 +144,10 @@ __global__ void k_zero_ghosts(Geo g, F5 NL, F5 NR)
     }
 }
 
-void launch_zero_increment_ghosts(const Geo &g, F5 NL, F5 NR, hipStream_t s)
+void launch_zero_increment_ghosts(const Geo &g, F5 NL, F5 NR, int *counters, hipStream_t s)
 {
-    dim3 b(64, 4);
-    dim3 gr((g.NZZ + b.x - 1) / b.x, (g.NYY + b.y - 1) / b.y, g.gend);
-    hipLaunchKernelGGL(k_zero_ghosts, gr, b, 0, s, g, NL, NR);
+    const long n = std::max(2L * g.gend * std::max(g.NZZ, g.NYY), 2L * g.NYY * g.NZZ);
+    hipLaunchKernelGGL(k_zero_ghosts, dim3((unsigned)((n + 255) / 256), 3), dim3(256), 0, s, g, NL, NR, counters);
 }
 
 __global__ void k_zero(double *p, long n)
@@ -621,13 +640,13 @@ int launch_update_norm(const Geo &g, const Phys &p, F5 U_, const double *J, C5 N
 
 // one block: each thread sums a contiguous run of partials in order, then
 // the block combines the runs in thread order
-__global__ void __launch_bounds__(256) k_finish_norm(int nb, const double *__restrict__ partials, double DN,
-                                                     double *er)
+__global__ void __launch_bounds__(1024) k_finish_norm(int nb, const double *__restrict__ partials, double DN,
+                                                      double *er)
 {
+    // thread t sums partials t, t + 1024, ... in that order (coalesced),
+    // then the block sum in its fixed order
     double x[5] = {0, 0, 0, 0, 0};
-    const int per = (nb + blockDim.x - 1) / blockDim.x;
-    const int b0 = threadIdx.x * per;
-    for (int b = b0; b < b0 + per && b < nb; b++)
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
 #pragma unroll
         for (int v = 0; v < 5; v++) x[v] += partials[5 * (long)b + v];
     double o[5];
@@ -641,7 +660,7 @@ void launch_finish_norm(int nblocks, const double *partials, double DN, int squa
                         hipStream_t s)
 {
     (void)squared_twice;
-    hipLaunchKernelGGL(k_finish_norm, dim3(1), dim3(256), 0, s, nblocks, partials, DN, er_dev);
+    hipLaunchKernelGGL(k_finish_norm, dim3(1), dim3(1024), 0, s, nblocks, partials, DN, er_dev);
 }
 
 
