@@ -771,7 +771,7 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
         launch_rhs_diag(g, c->p, deltaT, e, c->cfg.sponge, c5(c, CGLI_U_), c5(c, CGLI_U), c5(c, CGLI_UQ),
                         c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_INFX), c5(c, CGLI_INFY), c5(c, CGLI_INFZ),
                         C4{{c->f[CGLI_VF], c->f[CGLI_VF + 1], c->f[CGLI_VF + 2], c->f[CGLI_VF + 3]}},
-                        c5(c, CGLI_RES), f5(c, CGLI_MR), f5(c, CGLI_NL), f5(c, CGLI_NR), s);
+                        c5(c, CGLI_RES), f5(c, CGLI_MR), f5(c, CGLI_NL), f5(c, CGLI_NR), s, c->lusgs_mode != 3);
         tend(c, T_RHS);
     }
     const int mlast = g.gend + g.NY + g.NZ;
@@ -1064,7 +1064,7 @@ int ctx_rhs_sumsq(cgli_ctx *c, double deltaT, double e, double out[5])
     launch_rhs_diag(g, c->p, deltaT, e, 0, c5(c, CGLI_U_), c5(c, CGLI_U), c5(c, CGLI_UQ),
                     c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_INFX), c5(c, CGLI_INFY), c5(c, CGLI_INFZ),
                     C4{{c->f[CGLI_VF], c->f[CGLI_VF + 1], c->f[CGLI_VF + 2], c->f[CGLI_VF + 3]}},
-                    c5(c, CGLI_RES), f5(c, CGLI_MR), f5(c, CGLI_NL), f5(c, CGLI_NR), s);
+                    c5(c, CGLI_RES), f5(c, CGLI_MR), f5(c, CGLI_NL), f5(c, CGLI_NR), s, c->lusgs_mode != 3);
     double *dev = c->scratch + c->scratch_len - 8;
     launch_sumsq5(g, c5(c, CGLI_MR), c->scratch, dev, s);
     HIPCHK(hipGetLastError());

@@ -57,9 +57,11 @@ void launch_zero_increment_ghosts(const Geo &g, F5 NL, F5 NR, int *counters, hip
 void launch_sponge(const Geo &g, const Phys &p, double e, int gstart, int gend0, int nx_global, int nx_inlet,
                    int nx_outlet, C5 U, const double *J, const double *etdy, F5 Res, double *scratch,
                    hipStream_t s);
+// write_inc = 0: MR only, not the starting increments NL = NR = D MR (the
+// march-ordered sweep keeps its own copies and its scatter writes both)
 void launch_rhs_diag(const Geo &g, const Phys &p, double deltaT, double e, int with_res, C5 U_, C5 U, C5 Uq,
                      const double *J, M9 met, C5 Fx, C5 Fy, C5 Fz, C4 vF, C5 Res, F5 MR, F5 NL, F5 NR,
-                     hipStream_t s);
+                     hipStream_t s, int write_inc = 1);
 void launch_lusgs_plane(const Geo &g, const Phys &p, double deltaT, double e, int forward, int m,
                         int first_cell, int count, const int *ijk, C5 U_, const double *J, M9 met, C5 MR,
                         F5 NL, F5 NR, hipStream_t s);

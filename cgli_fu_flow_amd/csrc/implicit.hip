@@ -305,7 +305,8 @@ void launch_sponge(const Geo &g, const Phys &p, double e, int gstart, int gend0,
 // --------------------------------------------------------------- RHS+diag
 __global__ void __launch_bounds__(256) k_rhs_diag(Geo g, Phys p, double deltaT, double e, int with_res, C5 U_,
                                                   C5 U, C5 Uq, const double *__restrict__ J, M9 met, C5 Fx,
-                                                  C5 Fy, C5 Fz, C4 vF, C5 Res, F5 MR, F5 NL, F5 NR)
+                                                  C5 Fy, C5 Fz, C4 vF, C5 Res, F5 MR, F5 NL, F5 NR,
+                                                  int write_inc)
 {
     const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -363,21 +364,23 @@ __global__ void __launch_bounds__(256) k_rhs_diag(Geo g, Phys p, double deltaT, 
     nl[3] = D.d22 * mr[3];
     nl[4] = D.d51 * mr[0] + D.d22 * mr[4];
 #pragma unroll
-    for (int a = 0; a < 5; a++) {
-        MR.v[a][c] = mr[a];
-        NL.v[a][c] = nl[a];
-        NR.v[a][c] = nl[a];
-    }
+    for (int a = 0; a < 5; a++) MR.v[a][c] = mr[a];
+    if (write_inc)   // the starting increments D MR (not needed by a sweep that keeps its own copies)
+#pragma unroll
+        for (int a = 0; a < 5; a++) {
+            NL.v[a][c] = nl[a];
+            NR.v[a][c] = nl[a];
+        }
 }
 
 void launch_rhs_diag(const Geo &g, const Phys &p, double deltaT, double e, int with_res, C5 U_, C5 U, C5 Uq,
                      const double *J, M9 met, C5 Fx, C5 Fy, C5 Fz, C4 vF, C5 Res, F5 MR, F5 NL, F5 NR,
-                     hipStream_t s)
+                     hipStream_t s, int write_inc)
 {
     dim3 b(64, 4);
     dim3 gr((g.NZ - 1 + b.x - 1) / b.x, (g.NY - 1 + b.y - 1) / b.y, g.nx);
     hipLaunchKernelGGL(k_rhs_diag, gr, b, 0, s, g, p, deltaT, e, with_res, U_, U, Uq, J, met, Fx, Fy, Fz, vF,
-                       Res, MR, NL, NR);
+                       Res, MR, NL, NR, write_inc);
 }
 
 // ------------------------------------------------------------- LU-SGS
