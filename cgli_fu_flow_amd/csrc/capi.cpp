@@ -792,13 +792,13 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
             const long mf = march_field_doubles(g);
             launch_lusgs_march(g, 1, c->coefM, c->dgnM, c->cfg.precision == 1, c->marchF, c->marchF + mf,
                                c5(c, CGLI_NL), c->xbuf, c->trash, c->morder_dev, c->mtiles, c->counters, c->max_spin,
-                               s);
+                               MARCH_SLACK, s);
             tend(c, T_FWD);
             launch_march_prep(g, 0, c->xbuf, c->counters, s);
             tbegin(c, T_BWD);
             launch_lusgs_march(g, 0, c->coefM, c->dgnM, c->cfg.precision == 1, c->marchF + mf,
                                c->marchF + 2 * mf, c5(c, CGLI_NR), c->xbuf, c->trash, c->morder_dev, c->mtiles,
-                               c->counters, c->max_spin, s);
+                               c->counters, c->max_spin, MARCH_SLACK, s);
             tend(c, T_BWD);
             if (sw == sweeps - 1) {
                 tbegin(c, T_SCATTER);

@@ -1838,7 +1838,7 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_march(Geo g, March M, const CT
                                                       const double *__restrict__ dgnA, const double *RmA,
                                                       double *XmA, C5 Xg, double *xbuf, double *trash,
                                                       const int *__restrict__ order, int ntiles, int *ticket,
-                                                      int *err, int max_spin)
+                                                      int *err, int max_spin, int slack)
 {
     static_assert(TJ * TK == 64, "one wave per tile");
     __shared__ int s_t;
@@ -1922,6 +1922,22 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_march(Geo g, March M, const CT
     bool late = false;
     Pref<CT> P0, P1;
     unsigned long long nb[5], nc[5];
+    // Start `slack` march steps later than the dependencies require: wait
+    // until the upstream tiles have published the exchange entries of step
+    // `slack`.  Tiles march at the same rate, so the lead persists and each
+    // step's poll then finds its entries written instead of re-polling.
+    if (slack > 0 && slack < S) {
+        const int ts = m_t<FWD>(w, slack);
+        const int ils = ts - L.jl - L.kl;
+        const bool acts = L.col && ils >= 0 && ils < g.nx;
+        const long xs5 = L.x0 + (long)ts * w.sxs5;
+        unsigned long long sb[5] = {0, 0, 0, 0, 0}, sc[5] = {0, 0, 0, 0, 0};
+        double tmp[5] = {0, 0, 0, 0, 0};
+        if (acts && L.need_b) issue5m(xbuf + xs5 + w.dnb5, sb);
+        if (acts && L.need_c) issue5m(xbuf + xs5 + w.dnc5, sc);
+        settle5m(xbuf + xs5 + w.dnb5, acts && L.need_b, sb, tmp, max_spin, late);
+        settle5m(xbuf + xs5 + w.dnc5, acts && L.need_c, sc, tmp, max_spin, late);
+    }
     {
         // the ghost plane upstream in i (zero increments)
         const long cg = c0 + (long)(FWD ? 2 : g.gend + 1) * sxs;
@@ -1964,7 +1980,7 @@ void launch_march_scatter(const Geo &g, const double *nrM, F5 NL, F5 NR, hipStre
 
 void launch_lusgs_march(const Geo &g, int forward, const void *coefM, const double *dgnM, int fp32,
                         const double *Rm, double *Xm, C5 Xg, double *xbuf, double *trash, const int *order,
-                        int ntiles, int *counters, int max_spin, hipStream_t s)
+                        int ntiles, int *counters, int max_spin, int slack, hipStream_t s)
 {
     const March M = march_of(g);
     const dim3 gr(ntiles), bl(MARCH_TJ * MARCH_TK);
@@ -1972,16 +1988,16 @@ void launch_lusgs_march(const Geo &g, int forward, const void *coefM, const doub
     const double *cd = static_cast<const double *>(coefM);
     if (forward && fp32)
         hipLaunchKernelGGL((k_lusgs_march<MARCH_TJ, MARCH_TK, true, float>), gr, bl, 0, s, g, M, cf, dgnM, Rm, Xm, Xg,
-                           xbuf, trash, order, ntiles, counters, counters + 1, max_spin);
+                           xbuf, trash, order, ntiles, counters, counters + 1, max_spin, slack);
     else if (forward)
         hipLaunchKernelGGL((k_lusgs_march<MARCH_TJ, MARCH_TK, true, double>), gr, bl, 0, s, g, M, cd, dgnM, Rm, Xm, Xg,
-                           xbuf, trash, order, ntiles, counters, counters + 1, max_spin);
+                           xbuf, trash, order, ntiles, counters, counters + 1, max_spin, slack);
     else if (fp32)
         hipLaunchKernelGGL((k_lusgs_march<MARCH_TJ, MARCH_TK, false, float>), gr, bl, 0, s, g, M, cf, dgnM, Rm, Xm, Xg,
-                           xbuf, trash, order, ntiles, counters, counters + 1, max_spin);
+                           xbuf, trash, order, ntiles, counters, counters + 1, max_spin, slack);
     else
         hipLaunchKernelGGL((k_lusgs_march<MARCH_TJ, MARCH_TK, false, double>), gr, bl, 0, s, g, M, cd, dgnM, Rm, Xm, Xg,
-                           xbuf, trash, order, ntiles, counters, counters + 1, max_spin);
+                           xbuf, trash, order, ntiles, counters, counters + 1, max_spin, slack);
 }
 
 // tile order: anti-diagonals J+K ascending, J ascending inside
