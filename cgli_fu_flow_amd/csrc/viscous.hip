@@ -156,7 +156,10 @@ __device__ __forceinline__ Real suth(const Phys &p, Real T)
 // Face fluxes of cell (i,j,k): x face i-1/2 for i<=gend+1 (j<=NY, k<=NZ),
 // y face j-1/2 for j<=NYY (i<=gend, k<=NZ), z face k-1/2 for k<=NZZ
 // (i<=gend, j<=NY).  Faces go to FF[var*3+dir] (vars 2..5 -> 0..3).
-template <typename Real>
+// DIR is the face direction (one launch each): a thread computes one face,
+// so it holds one direction's stencil and metric groups (the three
+// together needed about 250 VGPRs, two waves per SIMD).
+template <typename Real, int DIR>
 __global__ void __launch_bounds__(256) k_visc_faces(Geo g, Phys p, const double *__restrict__ P4,
                                                     const double *__restrict__ Ju, M9 mu,
                                                     const double *__restrict__ Jv, M9 mv,
@@ -170,9 +173,10 @@ __global__ void __launch_bounds__(256) k_visc_faces(Geo g, Phys p, const double 
     const int k = 2 + t % nk;
     const int j = 2 + t / nk;
     const int i = 3 + blockIdx.z;
-    const bool xf = (j <= g.NY && k <= g.NZ);
-    const bool yf = (i <= g.gend && k <= g.NZ);
-    const bool zf = (i <= g.gend && j <= g.NY);
+    constexpr int dir = DIR;
+    const bool xf = dir == 0 && (j <= g.NY && k <= g.NZ);
+    const bool yf = dir == 1 && (i <= g.gend && k <= g.NZ);
+    const bool zf = dir == 2 && (i <= g.gend && j <= g.NY);
     if (!(xf || yf || zf)) return;
     const long N = g.N, sx = g.Ym * g.Zm, sy = g.Zm;
     const long c = ix(g, i, j, k);
@@ -305,10 +309,15 @@ void launch_viscous(const Geo &g, const Phys &p, C5 U, const double *J, const do
     dim3 gp((np + 255) / 256, 1, g.gend);   // i=2..gend+1
     hipLaunchKernelGGL(k_prim4, gp, b1, 0, s, g, p.K, p.R, U, J, prim4);
     dim3 gf((nf + 255) / 256, 1, g.gend - 1);   // i=3..gend+1
-    if (p.fp32)
-        hipLaunchKernelGGL(k_visc_faces<float>, gf, b1, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
-    else
-        hipLaunchKernelGGL(k_visc_faces<double>, gf, b1, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
+    if (p.fp32) {
+        hipLaunchKernelGGL((k_visc_faces<float, 0>), gf, b1, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
+        hipLaunchKernelGGL((k_visc_faces<float, 1>), gf, b1, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
+        hipLaunchKernelGGL((k_visc_faces<float, 2>), gf, b1, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
+    } else {
+        hipLaunchKernelGGL((k_visc_faces<double, 0>), gf, b1, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
+        hipLaunchKernelGGL((k_visc_faces<double, 1>), gf, b1, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
+        hipLaunchKernelGGL((k_visc_faces<double, 2>), gf, b1, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
+    }
     dim3 b(64, 4);
     dim3 gd((g.NZ - 1 + b.x - 1) / b.x, (g.NY - 1 + b.y - 1) / b.y, g.gend - 2);   // i=3..gend
     hipLaunchKernelGGL(k_visc_div, gd, b, 0, s, g, p, faces12, vF);
