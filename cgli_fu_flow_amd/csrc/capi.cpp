@@ -51,6 +51,9 @@ struct cgli_ctx {
     bool capturing = false;      // between cgli_graph_begin and cgli_graph_end
     long generation = 0;         // bumped whenever a captured graph would go stale
     bool norm_pending = false;   // a deferred residual norm awaits cgli_dual_time_stepping_wait
+    hipEvent_t norm_ev = nullptr;    // recorded after the norm's copy to the host
+    double *er_mapped = nullptr;     // device address of er_host, where the runtime maps it
+    int *flags_mapped = nullptr;     // device address of counters_host
     bool norm_wave = false;      // ... with the wavefront's timeout flag
     int lusgs_mode = 3;          // 0: one launch per hyperplane, 1: tiled wavefront, 2: with loader waves,
                                  // 3: tiled wavefront on march-ordered records (default)
@@ -254,6 +257,17 @@ int cgli_create(const cgli_config *cfg, int device, cgli_ctx **out)
         return CGLI_E_HIP;
     }
     c->er_host = static_cast<double *>(h);
+    {
+        void *d = nullptr;   // without a mapping the norm is copied instead
+        if (hipHostGetDevicePointer(&d, h, 0) == hipSuccess) c->er_mapped = static_cast<double *>(d);
+        else (void)hipGetLastError();
+    }
+    if (hipEventCreateWithFlags(&c->norm_ev, hipEventDisableTiming) != hipSuccess) {
+        hipHostFree(h);
+        hipStreamDestroy(c->stream);
+        delete c;
+        return CGLI_E_HIP;
+    }
     *out = c;
     return CGLI_OK;
 }
@@ -284,6 +298,7 @@ int cgli_destroy(cgli_ctx *c)
     if (c->counters) hipFree(c->counters);
     if (c->counters_host) hipHostFree(c->counters_host);
     if (c->er_host) hipHostFree(c->er_host);
+    if (c->norm_ev) hipEventDestroy(c->norm_ev);
     for (int k = 0; k < T_NCLASS; k++)
         for (auto &e : c->ev[k]) {
             hipEventDestroy(e.a);
@@ -539,18 +554,29 @@ static int implicit_inputs(cgli_ctx *c)
 static int enqueue_norm(cgli_ctx *c, int nb, bool check_wave)
 {
     const double DN = 1. / (double)((long)c->cfg.nx_global * c->cfg.ny * c->cfg.nz);
-    double *er_dev = c->scratch + c->scratch_len - 8;
-    launch_finish_norm(nb, c->scratch, DN, 0, er_dev, c->stream);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(c->er_host, er_dev, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
-    if (check_wave) HIPCHK(hipMemcpyAsync(c->counters_host, c->counters, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
+    if (c->er_mapped && (!check_wave || c->flags_mapped)) {
+        // the last reduction writes the norms (and the wavefront's flags)
+        // into pinned host memory itself: no copy waits in the stream
+        launch_finish_norm(nb, c->scratch, DN, 0, c->er_mapped, c->stream, check_wave ? c->counters : nullptr,
+                           c->flags_mapped);
+        HIPCHK(hipGetLastError());
+    } else {
+        double *er_dev = c->scratch + c->scratch_len - 8;
+        launch_finish_norm(nb, c->scratch, DN, 0, er_dev, c->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(c->er_host, er_dev, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
+        if (check_wave)
+            HIPCHK(hipMemcpyAsync(c->counters_host, c->counters, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
+    }
+    // (a graph launch records the event after the graph instead)
+    if (!c->capturing) HIPCHK(hipEventRecord(c->norm_ev, c->stream));
     return CGLI_OK;
 }
 
-// wait for the queued norm and check it
+// wait for the queued norm (not for work queued after it) and check it
 static int collect_norm(cgli_ctx *c, double er[5], bool check_wave)
 {
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipEventSynchronize(c->norm_ev));
     if (check_wave && c->counters_host[1] != 0) return CGLI_E_TIMEOUT;
     bool finite = true;
     for (int v = 0; v < 5; v++) {
@@ -619,6 +645,11 @@ static int wave_setup(cgli_ctx *c)
     c->counters = static_cast<int *>(p);
     if (hipHostMalloc(&p, sizeof(int) * 4) != hipSuccess) return CGLI_E_NOMEM;
     c->counters_host = static_cast<int *>(p);
+    {
+        void *d = nullptr;
+        if (hipHostGetDevicePointer(&d, p, 0) == hipSuccess) c->flags_mapped = static_cast<int *>(d);
+        else (void)hipGetLastError();
+    }
     return CGLI_OK;
 }
 
@@ -706,6 +737,7 @@ int cgli_graph_launch(cgli_graph *G)
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipGraphLaunch(G->exec, c->stream));
     if (G->norm) {
+        HIPCHK(hipEventRecord(c->norm_ev, c->stream));
         c->norm_pending = true;
         c->norm_wave = G->norm_wave;
     }

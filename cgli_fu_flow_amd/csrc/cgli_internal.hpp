@@ -71,8 +71,10 @@ void launch_copy_interior(const Geo &g, C5 src, F5 dst, hipStream_t s);
 // returns the number of partial blocks written to `partials` (5 per block)
 int launch_update_norm(const Geo &g, const Phys &p, F5 U_, const double *J, C5 NR, F5 Res, int write_res,
                        int dplusgs, double *partials, hipStream_t s);
+// er_dev may be a device address of pinned host memory; with counters,
+// counters[0..1] are copied to flags
 void launch_finish_norm(int nblocks, const double *partials, double DN, int squared_twice, double *er_dev,
-                        hipStream_t s);
+                        hipStream_t s, const int *counters = nullptr, int *flags = nullptr);
 void launch_bc_y(const Geo &g, const Phys &p, int gstart, F5 U, const double *J, double T_low,
                  double T_high, int heated_lo, int heated_hi, hipStream_t s);
 void launch_bc_z(const Geo &g, F5 U, hipStream_t s);

@@ -248,15 +248,23 @@ int cgli_channel_ghosts(cgli_ctx *ctx)
 
 namespace {
 
+// the start of an inner iteration: ghost fill and x fluxes
+int queue_prefix(cgli_ctx *ctx)
+{
+    int r = cgli_channel_ghosts(ctx);
+    if (!r) r = cgli_flux_x(ctx, 1.0, 1e-4);
+    return r;
+}
+
 // one inner iteration queued on the context stream: ghosts, fluxes and
-// viscous terms (unless the right-hand side is already in place), then the
-// dual-time update up to the norm's copy to the host
-int queue_iteration(cgli_ctx *ctx, double deltaT, int sweeps, bool reuse)
+// viscous terms (unless the right-hand side is already in place; prefix:
+// the ghost fill and x fluxes are already queued), then the dual-time
+// update up to the norm's copy to the host
+int queue_iteration(cgli_ctx *ctx, double deltaT, int sweeps, bool reuse, bool prefix = false)
 {
     int r = CGLI_OK;
     if (!reuse) {
-        r = cgli_channel_ghosts(ctx);
-        if (!r) r = cgli_flux_x(ctx, 1.0, 1e-4);
+        if (!prefix) r = queue_prefix(ctx);
         if (!r) r = cgli_flux_y(ctx, 1.0, 1e-4);
         if (!r) r = cgli_flux_z(ctx, 1.0, 1e-4);
         if (!r) r = cgli_viscous(ctx);
@@ -268,9 +276,9 @@ int queue_iteration(cgli_ctx *ctx, double deltaT, int sweeps, bool reuse)
 // queue one inner iteration, from the graph of its key when graphs are on:
 // a key's first occurrence runs eagerly (it allocates what the iteration
 // needs), its second is captured; a graph gone stale is captured again
-int run_iteration(cgli_ctx *ctx, ChannelState &cs, double deltaT, int sweeps, bool reuse, bool fp32)
+int run_iteration(cgli_ctx *ctx, ChannelState &cs, double deltaT, int sweeps, bool reuse, bool fp32, bool prefix)
 {
-    if (!cs.graph || reuse || ctx_timing_on(ctx)) return queue_iteration(ctx, deltaT, sweeps, reuse);
+    if (!cs.graph || reuse || ctx_timing_on(ctx)) return queue_iteration(ctx, deltaT, sweeps, reuse, prefix);
     ChannelState::GraphSlot *slot = nullptr;
     for (auto &q : cs.graphs)
         if (q.fp32 == (int)fp32 && q.dt == deltaT && q.sweeps == sweeps) slot = &q;
@@ -465,13 +473,24 @@ int cgli_channel_step(cgli_ctx *ctx, double deltaT, double tol, int max_inner, i
     bool fp32 = cfg->precision == 1;
     int n32 = 0;
     double prev = 1.0;
+    // Eagerly launched iterations queue the next iteration's ghost fill and
+    // x fluxes before the host reads this iteration's norm, so that the
+    // device works while the host decides.  Should the step stop there, that
+    // work only refilled the ghosts of the final state and rewrote x fluxes
+    // that the next use recomputes.
+    bool prefix = false;
     for (it = 1; it <= max_inner; it++) {
         ctx_set_fp32(ctx, fp32);
         if (fp32) n32++;
         // the first iteration reuses the residual evaluation of the guess
         // check when the network's guess was taken
         const bool reuse = ready && it == 1;
-        int r = run_iteration(ctx, cs, deltaT, sweeps, reuse, fp32);
+        int r = run_iteration(ctx, cs, deltaT, sweeps, reuse, fp32, prefix);
+        prefix = false;
+        if (!r && it < max_inner && !(cs.graph && !ctx_timing_on(ctx))) {
+            r = queue_prefix(ctx);
+            prefix = r == CGLI_OK;
+        }
         if (!r) r = ctx_dts_wait(ctx, er);
         if (r) return r;
         if (it == 1)
@@ -485,8 +504,17 @@ int cgli_channel_step(cgli_ctx *ctx, double deltaT, double tol, int max_inner, i
         double drop = 0.0;
         for (int v = 0; v < 5; v++) drop = std::max(drop, ref[v] > 0 ? er[v] / ref[v] : 0.0);
         if (drop <= tol) break;
-        if (fp32 && (drop <= 100 * tol || (it > 2 && drop > 0.85 * prev))) fp32 = false;
+        if (fp32 && (drop <= 100 * tol || (it > 2 && drop > 0.85 * prev))) {
+            fp32 = false;
+            prefix = false;   // the queued x fluxes were FP32
+        }
         prev = drop;
+    }
+    // every step ends with the ghosts of its final state filled (a queued
+    // prefix has done it), whichever way its iterations were launched
+    if (!prefix) {
+        const int r = cgli_channel_ghosts(ctx);
+        if (r) return r;
     }
     ctx_set_fp32(ctx, cfg->precision == 1);
     ctx_set_last_fp32(ctx, n32);

@@ -643,8 +643,10 @@ int launch_update_norm(const Geo &g, const Phys &p, F5 U_, const double *J, C5 N
 
 // one block: each thread sums a contiguous run of partials in order, then
 // the block combines the runs in thread order
+// er may be pinned host memory (written through its device address); with
+// counters, the wavefront's ticket and error flag are copied to flags too
 __global__ void __launch_bounds__(1024) k_finish_norm(int nb, const double *__restrict__ partials, double DN,
-                                                      double *er)
+                                                      double *er, const int *counters, int *flags)
 {
     // thread t sums partials t, t + 1024, ... in that order (coalesced),
     // then the block sum in its fixed order
@@ -654,16 +656,26 @@ __global__ void __launch_bounds__(1024) k_finish_norm(int nb, const double *__re
         for (int v = 0; v < 5; v++) x[v] += partials[5 * (long)b + v];
     double o[5];
     block_sum<5>(x, o);
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
+        // system scope: the host reads these after the kernel's completion
+        // event, whatever caching the mapping of its memory allows
 #pragma unroll
-        for (int v = 0; v < 5; v++) er[v] = sqrt(o[v]) * DN;
+        for (int v = 0; v < 5; v++)
+            __hip_atomic_store(reinterpret_cast<unsigned long long *>(er + v),
+                               (unsigned long long)__double_as_longlong(sqrt(o[v]) * DN), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+        if (counters) {
+            __hip_atomic_store(flags, counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(flags + 1, counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
 }
 
 void launch_finish_norm(int nblocks, const double *partials, double DN, int squared_twice, double *er_dev,
-                        hipStream_t s)
+                        hipStream_t s, const int *counters, int *flags)
 {
     (void)squared_twice;
-    hipLaunchKernelGGL(k_finish_norm, dim3(1), dim3(1024), 0, s, nblocks, partials, DN, er_dev);
+    hipLaunchKernelGGL(k_finish_norm, dim3(1), dim3(1024), 0, s, nblocks, partials, DN, er_dev, counters, flags);
 }
 
 
