@@ -48,6 +48,7 @@ struct cgli_ctx {
     std::vector<int> Nijk;       // (gend+NY+NZ+8) x 2
     std::vector<int> ijk;        // (cells+1) x 3
     int diss_scale = 0;          // Roe dissipation scaling: 0 none, 1 scalar Ep, 2 filter sensor
+    bool nl_from_nr = false;     // NL's interior equals NR's and is not written yet (schedule 3)
     bool capturing = false;      // between cgli_graph_begin and cgli_graph_end
     long generation = 0;         // bumped whenever a captured graph would go stale
     bool norm_pending = false;   // a deferred residual norm awaits cgli_dual_time_stepping_wait
@@ -117,6 +118,17 @@ static bool have(cgli_ctx *c, int id, int n)
 static void mark(cgli_ctx *c, int id, int n)
 {
     for (int q = 0; q < n; q++) c->valid[id + q] = true;
+}
+
+// NL = NR on the interior, written where NL is read (schedule 3 leaves it
+// to this: its own sweeps never read the field)
+static void nl_sync(cgli_ctx *c)
+{
+    if (!c->nl_from_nr) return;
+    c->nl_from_nr = false;
+    launch_copy_interior(c->g, C5{{c->f[CGLI_NR], c->f[CGLI_NR + 1], c->f[CGLI_NR + 2], c->f[CGLI_NR + 3], c->f[CGLI_NR + 4]}},
+                         F5{{c->f[CGLI_NL], c->f[CGLI_NL + 1], c->f[CGLI_NL + 2], c->f[CGLI_NL + 3], c->f[CGLI_NL + 4]}},
+                         c->stream);
 }
 
 static F5 f5(cgli_ctx *c, int id) { return F5{{c->f[id], c->f[id + 1], c->f[id + 2], c->f[id + 3], c->f[id + 4]}}; }
@@ -322,6 +334,7 @@ int cgli_set_field(cgli_ctx *c, int id, const double *host)
     HIPCHK(hipSetDevice(c->device));
     int r = ensure(c, id);
     if (r) return r;
+    if (id >= CGLI_NL && id < CGLI_NL + 5) nl_sync(c);   // the other components stay consistent
     HIPCHK(hipMemcpyAsync(c->f[id], host, sizeof(double) * c->g.N, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->valid[id] = true;
@@ -335,6 +348,7 @@ int cgli_get_field(cgli_ctx *c, int id, double *host)
     HIPCHK(hipSetDevice(c->device));
     int r = ensure(c, id);
     if (r) return r;
+    if (id >= CGLI_NL && id < CGLI_NL + 5) nl_sync(c);
     HIPCHK(hipMemcpyAsync(host, c->f[id], sizeof(double) * c->g.N, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return CGLI_OK;
@@ -346,6 +360,7 @@ int cgli_field_device_ptr(cgli_ctx *c, int id, void **dptr)
     HIPCHK(hipSetDevice(c->device));
     int r = ensure(c, id);
     if (r) return r;
+    if (id >= CGLI_NL && id < CGLI_NL + 5) nl_sync(c);
     c->valid[id] = true;   // the caller writes it
     *dptr = c->f[id];
     return CGLI_OK;
@@ -834,7 +849,9 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
             tend(c, T_BWD);
             if (sw == sweeps - 1) {
                 tbegin(c, T_SCATTER);
-                launch_march_scatter(g, c->marchF + 2 * mf, f5(c, CGLI_NL), f5(c, CGLI_NR), s);
+                launch_march_scatter(g, c->marchF + 2 * mf, F5{{nullptr, nullptr, nullptr, nullptr, nullptr}},
+                                     f5(c, CGLI_NR), s);
+                c->nl_from_nr = true;
                 tend(c, T_SCATTER);
             }
         } else if (c->lusgs_mode >= 1) {
@@ -868,8 +885,11 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
             }
             tend(c, T_BWD);
         }
-        // NL = NR at the end of a sweep pair (the march scatter writes both)
-        if (c->lusgs_mode != 3) launch_copy_interior(g, c5(c, CGLI_NR), f5(c, CGLI_NL), s);
+        // NL = NR at the end of a sweep pair (schedule 3: where NL is read)
+        if (c->lusgs_mode != 3) {
+            launch_copy_interior(g, c5(c, CGLI_NR), f5(c, CGLI_NL), s);
+            c->nl_from_nr = false;
+        }
     }
     tbegin(c, T_UPDATE);
     const int nblk = launch_update_norm(g, c->p, f5(c, CGLI_U_), c->f[CGLI_J], c5(c, CGLI_NR), f5(c, CGLI_RES),
@@ -913,6 +933,7 @@ int cgli_dplusgs(cgli_ctx *c, double deltaT, double er[5], double e, int sweeps)
         launch_dplur_sweep(g, c->p, deltaT, e, c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_MR),
                            c5(c, CGLI_NL), f5(c, CGLI_NR), s);
         launch_copy_interior(g, c5(c, CGLI_NR), f5(c, CGLI_NL), s);
+        c->nl_from_nr = false;
     }
     for (int v = 0; v < 5; v++) launch_zero(c->f[CGLI_RES + v], g.N, s);
     const int nblk = launch_update_norm(g, c->p, f5(c, CGLI_U_), c->f[CGLI_J], c5(c, CGLI_NR), f5(c, CGLI_RES),

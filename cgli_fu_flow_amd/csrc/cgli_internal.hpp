@@ -145,6 +145,7 @@ void launch_lusgs_march(const Geo &g, int forward, const void *coefM, const doub
                         const double *Rm, double *Xm, C5 Xg, double *xbuf, double *trash, const int *order,
                         int ntiles, int *counters, int max_spin, int slack, hipStream_t s);
 // NR (march order) into the fields NR and NL: the end of a sweep pair
+// NL with null pointers: NR only
 void launch_march_scatter(const Geo &g, const double *nrM, F5 NL, F5 NR, hipStream_t s);
 
 

@@ -1605,11 +1605,12 @@ __global__ void __launch_bounds__(256) k_split_march(Geo g, Phys p, double delta
 }
 
 // the march-ordered increments NR back into their field, and into NL
-// (a sweep pair ends with NL = NR, Dual_time_stepping.cpp), interior cells
-// only
+// when NL.v[0] is not null (a sweep pair ends with NL = NR,
+// Dual_time_stepping.cpp), interior cells only
 __global__ void __launch_bounds__(256) k_march_scatter(Geo g, March M, const double *__restrict__ nrM, F5 NL,
                                                        F5 NR)
 {
+    const bool nl = NL.v[0] != nullptr;
     const MGather q = mgather_at(g, M);
     if (!q.ok) return;
     const long c = ix(g, 3 + q.il, 2 + q.jo, 2 + q.ko);
@@ -1617,7 +1618,7 @@ __global__ void __launch_bounds__(256) k_march_scatter(Geo g, March M, const dou
 #pragma unroll
     for (int v = 0; v < 5; v++) {
         const double x = nrM[o + v * 64];
-        NL.v[v][c] = x;
+        if (nl) NL.v[v][c] = x;
         NR.v[v][c] = x;
     }
 }
