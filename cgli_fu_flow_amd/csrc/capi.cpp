@@ -29,6 +29,7 @@ struct cgli_graph {
     long generation = 0;         // of the context at capture
     bool norm = false;           // ends with a deferred residual norm
     bool norm_wave = false;
+    bool scatters_nr = false;    // holds a schedule-3 scatter: NL is then to be copied from NR where read
 };
 
 struct cgli_ctx {
@@ -49,6 +50,8 @@ struct cgli_ctx {
     std::vector<int> ijk;        // (cells+1) x 3
     int diss_scale = 0;          // Roe dissipation scaling: 0 none, 1 scalar Ep, 2 filter sensor
     bool nl_from_nr = false;     // NL's interior equals NR's and is not written yet (schedule 3)
+    long nr_scatters = 0;        // schedule-3 scatters queued (a graph that holds one sets nl_from_nr when launched)
+    long nr_scatters_at_capture = 0;
     bool capturing = false;      // between cgli_graph_begin and cgli_graph_end
     long generation = 0;         // bumped whenever a captured graph would go stale
     bool norm_pending = false;   // a deferred residual norm awaits cgli_dual_time_stepping_wait
@@ -712,6 +715,7 @@ int cgli_graph_begin(cgli_ctx *c)
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
     c->norm_pending = false;
+    c->nr_scatters_at_capture = c->nr_scatters;
     return CGLI_OK;
 }
 
@@ -740,6 +744,7 @@ int cgli_graph_end(cgli_ctx *c, cgli_graph **out)
     G->generation = c->generation;
     G->norm = norm;
     G->norm_wave = wave;
+    G->scatters_nr = c->nr_scatters != c->nr_scatters_at_capture;
     *out = G;
     return CGLI_OK;
 }
@@ -751,6 +756,7 @@ int cgli_graph_launch(cgli_graph *G)
     if (c->capturing || G->generation != c->generation) return CGLI_E_STATE;
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipGraphLaunch(G->exec, c->stream));
+    if (G->scatters_nr) c->nl_from_nr = true;
     if (G->norm) {
         HIPCHK(hipEventRecord(c->norm_ev, c->stream));
         c->norm_pending = true;
@@ -852,6 +858,7 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
                 launch_march_scatter(g, c->marchF + 2 * mf, F5{{nullptr, nullptr, nullptr, nullptr, nullptr}},
                                      f5(c, CGLI_NR), s);
                 c->nl_from_nr = true;
+                c->nr_scatters++;
                 tend(c, T_SCATTER);
             }
         } else if (c->lusgs_mode >= 1) {

@@ -88,6 +88,13 @@ def test_graph_of_one_iteration_replays_the_eager_result():
                 assert np.array_equal(er, er0), rep
                 for q in range(5):
                     assert np.array_equal(ctx.get(nv.U_ + q), U0[q]), (rep, q)
+                # NL equals NR on the interior after every replay, also when NL
+                # was read (and so written) after the previous one
+                nl, nr = ctx.get_many(nv.NL, 5), ctx.get_many(nv.NR, 5)
+                inner = (slice(3, 23), slice(2, 26), slice(2, 19))
+                for q in range(5):
+                    assert np.array_equal(nl[q][inner], nr[q][inner]), (rep, q)
+                ctx.set(nv.NL, np.zeros(ctx.shape))   # clobbered: the next replay must restore it
             ctx.set_lusgs_mode(1)
             with pytest.raises(nv.CgliError) as e:   # stale after the schedule change
                 g.launch()
