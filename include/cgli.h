@@ -89,7 +89,11 @@ const char *cgli_error_string(int code);
 /* Whole-field copies in the reference layout (X_np*Y_m*Z_m doubles). */
 int cgli_set_field(cgli_ctx *ctx, int field, const double *host);
 int cgli_get_field(cgli_ctx *ctx, int field, double *host);
-/* Device address of a field (allocating it), for zero-copy interop. */
+/* Device address of a field (allocating it), for zero-copy interop.  Under
+ * LU-SGS schedule 3 the increments NL equal NR on the interior after every
+ * dual-time step but are written only when asked for: by cgli_get_field or
+ * by this call, so a caller that keeps NL's address across steps asks for it
+ * again before reading NL on the device. */
 int cgli_field_device_ptr(cgli_ctx *ctx, int field, void **dptr);
 int cgli_synchronize(cgli_ctx *ctx);
 /* device memory of one device in bytes (free, total), for footprint reports */
