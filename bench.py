@@ -338,11 +338,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if a.precision == "fp64" else "f64 (f32 residuals and sweep records)",
         "data": "synthetic: laminar Poiseuille + seeded +-10%% velocity noise (splitmix64 seed %d)" % ch.seed,
         "config": {"workload": "compressible channel %dx%dx%d, beta=%.1f, skew=%.2f, Re_tau=%g, Mach %.1f, "
-                               "acoustic CFL %.2f, plain FP64 dual-time LU-SGS to a %.0e inner drop"
-                               % (a.nx, a.ny, a.nz, a.beta, a.skew, a.re_tau, ch.mach, ch.cfl, a.tol),
+                               "acoustic CFL %.2f, %s dual-time LU-SGS to a %.0e inner drop"
+                               % (a.nx, a.ny, a.nz, a.beta, a.skew, a.re_tau, ch.mach, ch.cfl,
+                                  "plain FP64" if a.precision == "fp64" else "FP32-residual", a.tol),
                    "cells": ncell, "deltaT": dt, "max_inner": a.max_inner, "filter_every_step": not a.no_filter},
         "mcell_updates_per_s": ncell * steps_per_s / 1e6,
         "inner_iterations_per_step": float(np.mean(inner)),
