@@ -28,7 +28,7 @@ def oracle_state(ctx, nv):
     return f
 
 
-def oracle_step(orc, oc, f, shape, N, I, dt, inner):
+def oracle_step(orc, oc, f, shape, N, I, dt, inner, EpF=None):
     z = lambda n=5: [np.zeros(shape) for _ in range(n)]
     MR, ML, Fx, Fy, Fz, NL, NR, Res, LL, MLv, NLv = (z() for _ in range(11))
     vF = z(4)
@@ -39,10 +39,15 @@ def oracle_step(orc, oc, f, shape, N, I, dt, inner):
         orc.orc_bc_y(C.byref(oc), ptrs(U_), dp(f["J"]), C.c_double(1.0), C.c_double(1.0), -(2 ** 31), 2 ** 31 - 1)
         orc.orc_bc_z(C.byref(oc), ptrs(U_))
         orc.orc_bc_x_periodic(C.byref(oc), ptrs(U_))
-        for fn, jk, mk, F in ((orc.orc_flux_x, "J_u", "met_u", Fx), (orc.orc_flux_y, "J_v", "met_v", Fy),
-                              (orc.orc_flux_z, "J_w", "met_w", Fz)):
-            fn(C.byref(oc), C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]), dp(f[jk]),
-               ptrs(f[mk]), ptrs(F))
+        for d, (fn, jk, mk, F) in enumerate(((orc.orc_flux_x, "J_u", "met_u", Fx), (orc.orc_flux_y, "J_v", "met_v", Fy),
+                                             (orc.orc_flux_z, "J_w", "met_w", Fz))):
+            if EpF is None:
+                fn(C.byref(oc), C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]), dp(f[jk]),
+                   ptrs(f[mk]), ptrs(F))
+            else:   # the sensor-scaled Roe dissipation
+                fs = (orc.orc_flux_x_scaled, orc.orc_flux_y_scaled, orc.orc_flux_z_scaled)[d]
+                fs(C.byref(oc), 2, C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]),
+                   dp(f[jk]), ptrs(f[mk]), ptrs(F), dp(EpF[d]))
         orc.orc_viscous(C.byref(oc), ptrs(U_), ptrs(vF), dp(f["J"]), dp(f["J_u"]), ptrs(f["met_u"]), dp(f["J_v"]),
                         ptrs(f["met_v"]), dp(f["J_w"]), ptrs(f["met_w"]), ptrs(LL), ptrs(MLv), ptrs(NLv))
         orc.orc_dual_time_stepping(C.byref(oc), C.c_double(dt), C.c_double(1e-4), er,
@@ -82,6 +87,44 @@ def test_channel_steps_match_oracle(size, skew, xi_mode):
                 assert abs(en[v] - ers[-1][v]) <= 1e-10 * abs(ers[-1][v]), (step, v, en[v], ers[-1][v])
             got = ctx.get_many(nv.U, 5)
             sl = (slice(3, nx + 3), slice(2, ny + 2), slice(2, nz + 2))
+            for v in range(5):
+                r = rel(got[v][sl], f["U"][v][sl])
+                assert r <= TOL, (step, v, r)
+
+
+def test_channel_steps_with_sensor_scaled_dissipation_match_oracle():
+    """The channel driver with the Roe dissipation scaled by the filter
+    sensor: each step runs the filter, then the steps' fluxes use its face
+    values.  The oracle sequence takes the same sensor values (whose
+    decisions are tested equal to the oracle's elsewhere)."""
+    from cgli_fu_flow_amd import native as nv
+    nx, ny, nz = 16, 24, 12
+    print("channel with sensor-scaled dissipation", (nx, ny, nz), flush=True)
+    orc = load_oracle()
+    ch = nv.channel(180.0, beta=2.2, skew=0.05, xi_mode=1)
+    cfg = nv.channel_config(ch, nx, ny, nz)
+    with nv.Context(cfg) as ctx:
+        dt = ctx.channel_init(ch)
+        ctx.set_dissipation_scaling("sensor")
+        f = oracle_state(ctx, nv)
+        N, I = ctx.hyperplane()
+        oc = OrcCfg()
+        for name, _ in OrcCfg._fields_:
+            setattr(oc, name, getattr(cfg, name))
+        inner = 3
+        sl = (slice(3, nx + 3), slice(2, ny + 2), slice(2, nz + 2))
+        for step in range(2):
+            # factors other than 1 on every face, whichever the sensor picks
+            ctx.filter(Roe_criterion=0.007, E_high=0.7, E_low=0.3)
+            EpF = [ctx.get(nv.EPX + d) for d in range(3)]
+            face = (slice(2, nx + 2), slice(1, ny), slice(1, nz))   # x faces
+            assert set(np.unique(EpF[0][face])) <= {0.7, 0.3}
+            it, e1, en = ctx.channel_step(dt, tol=0.0, max_inner=inner)
+            assert it == inner
+            ers = oracle_step(orc, oc, f, ctx.shape, N, I, dt, inner, EpF=EpF)
+            for v in range(5):
+                assert abs(en[v] - ers[-1][v]) <= 1e-10 * abs(ers[-1][v]), (step, v, en[v], ers[-1][v])
+            got = ctx.get_many(nv.U, 5)
             for v in range(5):
                 r = rel(got[v][sl], f["U"][v][sl])
                 assert r <= TOL, (step, v, r)
