@@ -397,6 +397,43 @@ def test_fp32_residuals_track_the_fp64_oracle(nv, orc, name):
     print("worst FP32 relative error", max(worst.values()), flush=True)
 
 
+def test_fp32_scaled_dissipation_tracks_the_fp64_oracle(nv, orc):
+    """The FP32 fluxes with the sensor-scaled Roe dissipation against the
+    FP64 oracle's, at the FP32 residual tolerance."""
+    name = "ragged_20x24x17"
+    print("case", name, "FP32 sensor-scaled dissipation", flush=True)
+    case = make_case(name)
+    sl = case.slab
+    f = make_inputs(orc, sl, SEED + 10)
+    rng = np.random.default_rng(17)
+    EpF = [np.where(rng.random(sl.shape) < 0.5, 1.0, 0.1) for _ in range(3)]
+    from oracle_harness import met9
+    want = {}
+    for d, tag in enumerate(("_u", "_v", "_w")):
+        U_ = [a.copy() for a in f["U_"]]
+        MR, ML, F = sl.zeros(5), sl.zeros(5), sl.zeros(5)
+        fn = (orc.orc_flux_x_scaled, orc.orc_flux_y_scaled, orc.orc_flux_z_scaled)[d]
+        fn(C.byref(case.cfg(orc, xi_mode=1)), 2, C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML),
+           dp(f["J"]), dp(f["J" + tag]), ptrs(met9(f, tag)), ptrs(F), dp(EpF[d]))
+        want["inF" + "xyz"[d]] = F
+    reg = regions(sl)
+    with open_ctx(nv, orc, case, xi_mode=1, precision=1) as ctx:
+        upload_inputs(ctx, f, with_levels=False)
+        for d in range(3):
+            ctx.set(nv.EPX + d, EpF[d])
+        ctx.set_dissipation_scaling("sensor")
+        ctx.flux_x(1.0, 1e-4)
+        ctx.flux_y(1.0, 1e-4)
+        ctx.flux_z(1.0, 1e-4)
+        for d, fid in enumerate((nv.INFX, nv.INFY, nv.INFZ)):
+            key = "inF" + "xyz"[d]
+            got = ctx.get_many(fid, 5)
+            for v in range(5):
+                r = rel(got[v][reg[key]], want[key][v][reg[key]])
+                assert r <= FP32_TOL["inF"], (key, v, r)
+                assert r > 0.0   # really single precision
+
+
 @pytest.mark.parametrize("name", ["ref_12x16x10", "ragged_20x24x17"])
 def test_mixed_precision_iteration_tracks_the_fp64_oracle(nv, orc, name):
     """One inner iteration with cfg.precision = 1 (FP32 residual kernels and
