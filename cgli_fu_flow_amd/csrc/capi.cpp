@@ -477,6 +477,9 @@ int cgli_filter(cgli_ctx *c, double Roe_criterion, double E_high, double E_low)
         void *p = nullptr;
         if (hipMalloc(&p, sizeof(double) * filter_scratch_doubles(c->g)) != hipSuccess) return CGLI_E_NOMEM;
         c->filt = static_cast<double *>(p);
+        // NaN (FP64 and FP32 alike): a work cell read before the passes
+        // write it would spoil the sensor at once
+        HIPCHK(hipMemsetAsync(c->filt, 0xFF, sizeof(double) * filter_scratch_doubles(c->g), c->stream));
     }
     tbegin(c, T_FILTER);
     launch_filter(c->g, Roe_criterion, E_high, E_low, c5(c, CGLI_U_), c->f[CGLI_J], c->f[CGLI_YPOINT], c->filt,

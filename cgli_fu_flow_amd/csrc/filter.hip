@@ -38,8 +38,11 @@ static void level_range(const Geo &g, int lvl, int &lo, int &hi)
 }
 
 // every work array starts as U_ on i=0..gend+3, j=1..NYY, k=1..NZZ
-// (Filter.cpp:117-134); only their ghost shells are read before being
-// overwritten, but the whole box is copied like the reference does
+// (Filter.cpp:117-134).  Only the box's ghost shell (the three x planes at
+// each end, j=1 and j=NYY, k=1 and k=NZZ) is read before the passes
+// overwrite it, so only the shell is copied: the interior would be
+// overwritten unread.  (The work arrays start as NaN when allocated, so a
+// read of an unwritten interior cell would spoil the sensor.)
 template <typename Real>
 __global__ void k_filt_init(Geo g, R4 U, Real *__restrict__ work, long N)
 {
@@ -47,6 +50,8 @@ __global__ void k_filt_init(Geo g, R4 U, Real *__restrict__ work, long N)
     const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
     const int i = blockIdx.z;
     if (k > g.NZZ || j > g.NYY) return;
+    const bool shell = i <= 2 || i >= g.gend + 1 || j == 1 || j == g.NYY || k == 1 || k == g.NZZ;
+    if (!shell) return;
     const long c = ix(g, i, j, k);
 #pragma unroll
     for (int v = 0; v < 4; v++) {
