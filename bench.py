@@ -1,13 +1,13 @@
 """Benchmark: physical steps/s of the compressible channel at a 1e-6 inner drop.
 
-Workload (the plain FP64 GPU path on a curvilinear grid):
-channel 128x128x128, periodic x (4 pi) and z (2 pi), isothermal walls,
-tanh stretching beta=2.2, skew x' = x + 0.05 sin(2 pi y) sin(2 pi z/Lz),
-Re_tau=180, Mach 0.5, acoustic CFL 0.02, laminar Poiseuille + seeded +-10%
-noise.  Every physical step iterates the dual-time LU-SGS inner loop until
-max_v er_v/er_v(first) <= 1e-6 (or --max-inner), after one evaluation of
-the filter sensor.  State is resident in HBM before the timed region starts.
-`value` is this path's physical steps/s.
+Workload (the plain FP64 GPU path): channel 256x192x256, periodic x (4 pi)
+and z (2 pi), isothermal walls, tanh stretching beta=2.4, Re_tau=395,
+Mach 0.5, acoustic CFL 0.5, preconditioning floor e = Mach^2, laminar
+Poiseuille + seeded +-10% noise (--skew adds the sinusoidal mesh skew of the
+curvilinear configurations).  Every physical step iterates the dual-time
+LU-SGS inner loop until max_v er_v/er_v(first) <= 1e-6 (or --max-inner),
+after one evaluation of the filter sensor.  State is resident in HBM before
+the timed region starts.  `value` is this path's physical steps/s.
 
 `variants` then compares, from one saved state, plain FP64 (each step
 starting from U^n, as the reference does) with FP32 residuals plus the
@@ -26,9 +26,10 @@ library's stream over the timed region;
 `graph_replay` runs the same steps from one saved state eagerly and with
 each inner iteration replayed from a captured HIP graph;
 `traffic` is the HBM bytes per launch of that kernel measured by two
-rocprofv3 counter passes (profiles/channel_128_traffic.md) for the default
-workload; `cpu_baseline` is the oracle (C restatement, OpenMP) on this host
-for a bounded number of inner iterations at the same size.
+rocprofv3 counter passes (profiles/channel_<size>_traffic.md) for the same
+workload; `hbm_copy` is the measured copy bandwidth of the box (two copy
+kernels, 2 GiB); `cpu_baseline` is the oracle (C restatement, OpenMP) on
+this host for a bounded number of inner iterations at the same size.
 """
 import argparse
 import ctypes as C
@@ -48,8 +49,18 @@ HBM_PEAK_GBS = 8000.0
 # (each input element read once, each output written once)
 ALGO_BYTES = {"flux_x": 120, "flux_y": 120, "flux_z": 120, "viscous": 320, "rhs_diag": 472,
               "lusgs_fwd": 240, "lusgs_bwd": 240, "update_norm": 128, "split_coef": 560, "filter": 72,
-              "march_scatter": 120}
-TRAFFIC = os.path.join(ROOT, "profiles", "channel_128_traffic.json")
+              "march_scatter": 80}
+
+
+def traffic_file(nx, ny, nz):
+    return os.path.join(ROOT, "profiles", "channel_%dx%dx%d_traffic.json" % (nx, ny, nz))
+
+
+def workload(a, ch):
+    return ("compressible channel %dx%dx%d, beta=%.1f, skew=%.2f, Re_tau=%g, Mach %.1f, acoustic CFL %.2f, "
+            "preconditioning floor e=%.3g, %s dual-time LU-SGS to a %.0e inner drop"
+            % (a.nx, a.ny, a.nz, a.beta, a.skew, a.re_tau, ch.mach, ch.cfl, ch.precond_e,
+               "plain FP64" if a.precision == "fp64" else "FP32-residual", a.tol))
 
 
 def parse():
@@ -57,15 +68,19 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nx", type=int, default=128)
-    ap.add_argument("--ny", type=int, default=128)
-    ap.add_argument("--nz", type=int, default=128)
-    ap.add_argument("--beta", type=float, default=2.2)
-    ap.add_argument("--skew", type=float, default=0.05)
-    ap.add_argument("--re-tau", type=float, default=180.0)
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--ny", type=int, default=192)
+    ap.add_argument("--nz", type=int, default=256)
+    ap.add_argument("--beta", type=float, default=2.4)
+    ap.add_argument("--skew", type=float, default=0.0)
+    ap.add_argument("--re-tau", type=float, default=395.0)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-inner", type=int, default=200)
-    ap.add_argument("--cfl", type=float, default=0.02)
+    ap.add_argument("--cfl", type=float, default=0.5, help="acoustic CFL of the physical step")
+    ap.add_argument("--precond-e", type=float, default=0.0,
+                    help="preconditioning floor e (0: Mach^2, the channel default; the reference's own air "
+                         "case uses 1e-4)")
+    ap.add_argument("--no-copy", action="store_true", help="skip the measured copy bandwidth")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-filter", action="store_true", help="skip the per-step filter sensor")
@@ -84,7 +99,7 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(ctx, nv, cfg, dt, inner_per_step, budget_s, with_filter):
+def cpu_baseline(ctx, nv, cfg, dt, e, inner_per_step, budget_s, with_filter):
     """Oracle inner iterations on the host, same grid and state as the GPU run
     (plus one filter evaluation per physical step when the GPU run has one)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -113,15 +128,15 @@ def cpu_baseline(ctx, nv, cfg, dt, inner_per_step, budget_s, with_filter):
         orc.orc_bc_y(C.byref(oc), ptrs(U_), dp(f["J"]), C.c_double(1.0), C.c_double(1.0), -(2 ** 31), 2 ** 31 - 1)
         orc.orc_bc_z(C.byref(oc), ptrs(U_))
         orc.orc_bc_x_periodic(C.byref(oc), ptrs(U_))
-        orc.orc_flux_x(C.byref(oc), C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]),
+        orc.orc_flux_x(C.byref(oc), C.c_double(1.0), C.c_double(e), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]),
                        dp(f["J_u"]), ptrs(f["met_u"]), ptrs(Fx))
-        orc.orc_flux_y(C.byref(oc), C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]),
+        orc.orc_flux_y(C.byref(oc), C.c_double(1.0), C.c_double(e), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]),
                        dp(f["J_v"]), ptrs(f["met_v"]), ptrs(Fy))
-        orc.orc_flux_z(C.byref(oc), C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]),
+        orc.orc_flux_z(C.byref(oc), C.c_double(1.0), C.c_double(e), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]),
                        dp(f["J_w"]), ptrs(f["met_w"]), ptrs(Fz))
         orc.orc_viscous(C.byref(oc), ptrs(U_), ptrs(vF), dp(f["J"]), dp(f["J_u"]), ptrs(f["met_u"]), dp(f["J_v"]),
                         ptrs(f["met_v"]), dp(f["J_w"]), ptrs(f["met_w"]), ptrs(LL), ptrs(MLv), ptrs(NLv))
-        orc.orc_dual_time_stepping(C.byref(oc), C.c_double(dt), C.c_double(1e-4), er, N.ctypes.data_as(C.POINTER(C.c_int)),
+        orc.orc_dual_time_stepping(C.byref(oc), C.c_double(dt), C.c_double(e), er, N.ctypes.data_as(C.POINTER(C.c_int)),
                                    I.ctypes.data_as(C.POINTER(C.c_int)), ptrs(U_), ptrs(f["U"]), ptrs(f["Uq"]),
                                    dp(f["J"]), ptrs(f["met"]), ptrs(Fx), ptrs(Fy), ptrs(Fz), ptrs(vF), ptrs(MR),
                                    ptrs(NR), ptrs(NL), ptrs(Res), dp(f["met_v"][4]), dp(f["J_v"]), 1)
@@ -276,9 +291,14 @@ def main():
               "(requested --gpus %d)" % a.gpus, file=sys.stderr)
         sys.exit(2)
     from cgli_fu_flow_amd import native as nv
-    ch = nv.channel(a.re_tau, beta=a.beta, skew=a.skew, cfl=a.cfl)
+    ch = nv.channel(a.re_tau, beta=a.beta, skew=a.skew, cfl=a.cfl, precond_e=a.precond_e)
     cfg = nv.channel_config(ch, a.nx, a.ny, a.nz)
     cfg.precision = 1 if a.precision == "mixed" else 0
+    copy = None
+    if not a.no_copy:
+        copy = nv.stream_copy(1 << 31, 20)
+        print("measured copy: %.0f / %.0f GB/s" % (copy["GBps_one_double_per_lane"], copy["GBps_16B_per_lane"]),
+              file=sys.stderr, flush=True)
     ctx = nv.Context(cfg, device=0)
     if a.lusgs_mode is not None:
         ctx.set_lusgs_mode(a.lusgs_mode)
@@ -325,8 +345,11 @@ def main():
     dom_avg_ms = tot / cnt
     achieved = ALGO_BYTES[dom] * ncell / (dom_avg_ms * 1e-3) / 1e9
     traffic = None
-    if (a.nx, a.ny, a.nz) == (128, 128, 128) and os.path.exists(TRAFFIC):
-        traffic = json.load(open(TRAFFIC))["bytes_per_launch"].get(dom)
+    tf = traffic_file(a.nx, a.ny, a.nz)
+    if os.path.exists(tf):
+        tj = json.load(open(tf))
+        if tj.get("workload") == workload(a, ch):
+            traffic = tj["bytes_per_launch"].get(dom)
     out = {
         "metric": METRIC,
         "value": steps_per_s,
@@ -340,10 +363,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if a.precision == "fp64" else "f64 (f32 residuals and sweep records)",
         "data": "synthetic: laminar Poiseuille + seeded +-10%% velocity noise (splitmix64 seed %d)" % ch.seed,
-        "config": {"workload": "compressible channel %dx%dx%d, beta=%.1f, skew=%.2f, Re_tau=%g, Mach %.1f, "
-                               "acoustic CFL %.2f, %s dual-time LU-SGS to a %.0e inner drop"
-                               % (a.nx, a.ny, a.nz, a.beta, a.skew, a.re_tau, ch.mach, ch.cfl,
-                                  "plain FP64" if a.precision == "fp64" else "FP32-residual", a.tol),
+        "config": {"workload": workload(a, ch),
                    "cells": ncell, "deltaT": dt, "max_inner": a.max_inner, "filter_every_step": not a.no_filter},
         "mcell_updates_per_s": ncell * steps_per_s / 1e6,
         "inner_iterations_per_step": float(np.mean(inner)),
@@ -351,14 +371,18 @@ def main():
         "kernels": kernels,   # one fully timed step before the timed region
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "measured_copy_peak": copy["GBps_16B_per_lane"] if copy else None,
+                     "frac_of_measured_copy": achieved / copy["GBps_16B_per_lane"] if copy else None,
                      "algorithmic_bytes_per_launch": ALGO_BYTES[dom] * ncell,
                      "avg_ms_in_timed_region": dom_avg_ms, "launches_in_timed_region": cnt},
         "cpu_baseline": None,
+        "hbm_copy": copy,
     }
     if not a.no_graph:
         out["graph_replay"] = graph_replay(a, nv, ctx, dt)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(ctx, nv, cfg, dt, float(np.mean(inner)), a.cpu_seconds, not a.no_filter)
+        out["cpu_baseline"] = cpu_baseline(ctx, nv, cfg, dt, ch.precond_e, float(np.mean(inner)), a.cpu_seconds,
+                                           not a.no_filter)
     if not a.no_variants and a.precision == "fp64":
         out["variants"] = variants(a, nv, ch, ctx, dt, ncell)
     ctx.close()

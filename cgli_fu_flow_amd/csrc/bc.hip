@@ -116,6 +116,21 @@ void launch_copy_field(double *dst, const double *src, long n, hipStream_t s)
     hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, n);
 }
 
+// STREAM-style copy for the measured HBM bandwidth: 16 bytes per lane and
+// iteration, grid-stride
+__global__ __launch_bounds__(256) void k_stream_copy(double2 *__restrict__ dst, const double2 *__restrict__ src,
+                                                     long n2)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += stride) dst[t] = src[t];
+}
+
+void launch_stream_copy(double *dst, const double *src, long n, int blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(256), 0, s, (double2 *)dst,
+                       (const double2 *)src, n / 2);
+}
+
 // One block of the restart file: the interior cells of one field in (i, j, k)
 // order with k fastest, Q = U*J (Output.cpp:183-201).
 __global__ void k_pack_q(Geo g, const double *__restrict__ U, const double *__restrict__ J,

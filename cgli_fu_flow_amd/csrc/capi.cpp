@@ -380,6 +380,52 @@ int cgli_device_memory(int device, size_t *free_bytes, size_t *total_bytes)
     return CGLI_OK;
 }
 
+int cgli_stream_copy(int device, size_t bytes, int reps, double ms[2])
+{
+    if (!ms || reps < 1 || bytes < 4096) return CGLI_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CGLI_E_NODEV;
+    if (device < 0 || device >= ndev) return CGLI_E_ARG;
+    HIPCHK(hipSetDevice(device));
+    const long n = (long)(bytes / 16) * 2;
+    double *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n * sizeof(double)) != hipSuccess) return CGLI_E_NOMEM;
+    if (hipMalloc(&b, n * sizeof(double)) != hipSuccess) {
+        (void)hipFree(a);
+        return CGLI_E_NOMEM;
+    }
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = CGLI_OK;
+    if (hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = CGLI_E_HIP;
+    if (!rc) {
+        launch_zero(a, n, s);
+        // [0]: one double per lane (the field copy of the time shift);
+        // [1]: 16 bytes per lane, grid-stride over 2048 blocks
+        for (int v = 0; v < 2 && !rc; v++) {
+            auto run = [&] {
+                if (v == 0) launch_copy_field(b, a, n, s);
+                else launch_stream_copy(b, a, n, 2048, s);
+            };
+            run();
+            hipEventRecord(e0, s);
+            for (int r = 0; r < reps; r++) run();
+            hipEventRecord(e1, s);
+            float t = 0.f;
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+                rc = CGLI_E_HIP;
+            ms[v] = (double)t / reps;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}
+
 int cgli_synchronize(cgli_ctx *c)
 {
     if (!c) return CGLI_E_ARG;

@@ -86,6 +86,8 @@ void launch_zero(double *p, long n, hipStream_t s);
 // copy plane 0 of dst to planes 1..nplanes-1
 void launch_replicate_planes(double *dst, long plane, int nplanes, hipStream_t s);
 void launch_copy_field(double *dst, const double *src, long n, hipStream_t s);
+// n even; blocks of 256 lanes, grid-stride over n/2 16-byte elements
+void launch_stream_copy(double *dst, const double *src, long n, int blocks, hipStream_t s);
 
 // filter sensor (filter.hip); work holds filter_scratch_doubles(g) doubles
 long filter_scratch_doubles(const Geo &g);
@@ -198,6 +200,7 @@ int ctx_dts_wait(cgli_ctx *c, double er[5]);
 bool ctx_timing_on(cgli_ctx *c);
 // per-context state of the channel driver (channel.cpp)
 struct ChannelState {
+    double e = 1e-4;         // preconditioning floor of the step's routines (set by cgli_channel_init)
     int rule = 0;            // 0: per-step drop, 1: running maximum
     double E[5] = {0, 0, 0, 0, 0};
     int guess = 0;           // 0: U^n, 1: linear, 2: network, 3: quadratic extrapolation

@@ -40,7 +40,8 @@ class Config(C.Structure):
 
 class Channel(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("Lx", "Lz", "beta", "skew", "mach", "re_tau", "re_c", "Pr", "suth_S",
-                                          "cfl", "noise")] + [("seed", C.c_ulonglong), ("xi_mode", C.c_int)]
+                                          "cfl", "noise", "precond_e")] + [("seed", C.c_ulonglong),
+                                                                           ("xi_mode", C.c_int)]
 
 
 _lib = None
@@ -101,6 +102,7 @@ def lib():
         L.cgli_timing_get.argtypes = [vp, C.c_int, dp, C.POINTER(C.c_long)]
         L.cgli_channel_set_rule.argtypes = [vp, C.c_int]
         L.cgli_device_memory.argtypes = [C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+        L.cgli_stream_copy.argtypes = [C.c_int, C.c_size_t, C.c_int, dp]
         L.cgli_channel_rule_state.argtypes = [vp, dp, C.c_int]
         L.cgli_channel_set_guess.argtypes = [vp, C.c_int]
         L.cgli_channel_attach_net.argtypes = [vp, vp]
@@ -131,6 +133,16 @@ def device_memory(device=0):
     return f.value, t.value
 
 
+def stream_copy(nbytes=1 << 31, reps=20, device=0):
+    """Measured copy bandwidth in GB/s (2 x nbytes per copy): one double per
+    lane, and 16 bytes per lane on a grid-stride loop (cgli_stream_copy)."""
+    ms = (C.c_double * 2)()
+    _check(lib().cgli_stream_copy(device, nbytes, reps, ms), "stream_copy")
+    return {"bytes": nbytes, "reps": reps, "ms": [ms[0], ms[1]],
+            "GBps_one_double_per_lane": 2 * nbytes / (ms[0] * 1e-3) / 1e9,
+            "GBps_16B_per_lane": 2 * nbytes / (ms[1] * 1e-3) / 1e9}
+
+
 def reference_config(nx, ny, nz):
     c = Config()
     lib().cgli_config_reference(C.byref(c), nx, ny, nz)
@@ -143,6 +155,8 @@ def channel(re_tau=180.0, **kw):
     lib().cgli_channel_defaults(C.byref(ch), re_tau)
     for k, v in kw.items():
         setattr(ch, k, v)
+    if ch.precond_e <= 0:
+        ch.precond_e = ch.mach * ch.mach   # the library's default, made explicit for callers
     return ch
 
 

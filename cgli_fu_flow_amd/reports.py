@@ -127,7 +127,7 @@ def main(argv=None):
     ap.add_argument("--re-tau", type=float, default=395.0)
     ap.add_argument("--beta", type=float, default=2.4)
     ap.add_argument("--skew", type=float, default=0.05)
-    ap.add_argument("--cfl", type=float, default=0.02)
+    ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-inner", type=int, default=200)
     ap.add_argument("--steps", type=int, default=10)

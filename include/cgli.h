@@ -98,6 +98,12 @@ int cgli_field_device_ptr(cgli_ctx *ctx, int field, void **dptr);
 int cgli_synchronize(cgli_ctx *ctx);
 /* device memory of one device in bytes (free, total), for footprint reports */
 int cgli_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
+/* Measured HBM bandwidth (no counterpart in the reference): copies of
+ * `bytes` between two device buffers, `reps` times after one warm-up;
+ * ms[0] per copy with one double per lane (the kernel of the time-level
+ * shift), ms[1] with 16 bytes per lane on a grid-stride loop.  Bandwidth =
+ * 2 * bytes / time.  Allocates and frees the two buffers. */
+int cgli_stream_copy(int device, size_t bytes, int reps, double ms[2]);
 
 /* ---- hot path: each replaces the reference routine named ------------- */
 /* Flux_X.cpp:16-360 / Flux_Y.cpp:15-415 / Flux_Z.cpp:15-360:
@@ -218,8 +224,13 @@ typedef struct {
     double mach;             /* centreline Mach number */
     double re_tau, re_c;     /* friction and centreline Reynolds numbers */
     double Pr, suth_S;       /* Prandtl number; Sutherland constant / T_ref */
-    double cfl;              /* physical CFL number that sets deltaT */
+    double cfl;              /* acoustic CFL number that sets deltaT (default 0.5):
+                                deltaT = cfl / max over cells of sum_d (|U_d| + c |grad d|) / delta_d */
     double noise;            /* uniform velocity perturbation, fraction of U_c */
+    double precond_e;        /* floor e of the low-Mach preconditioning parameter
+                                beta = max(|u|^2 / c^2, e) passed to the flux and
+                                dual-time routines (the reference's main.cpp:57 uses 1e-4
+                                for its near-incompressible air); <= 0 selects mach^2 */
     unsigned long long seed; /* splitmix64 seed of the perturbation */
     int xi_mode;             /* as cgli_config.xi_mode */
 } cgli_channel;

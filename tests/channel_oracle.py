@@ -102,8 +102,10 @@ def initial_state(c, f, seed=20261203, noise=0.1):
 
 
 class CpuChannel:
-    def __init__(self, nx, ny, nz, beta=2.2, skew=0.05, cfl=0.5, **kw):
+    def __init__(self, nx, ny, nz, beta=2.2, skew=0.05, e=None, **kw):
         self.orc = load_oracle()
+        # preconditioning floor passed to the flux and dual-time routines
+        self.e = e if e is not None else kw.get("mach", 0.5) ** 2
         self.c = channel_cfg(nx, ny, nz, **kw)
         self.f = metrics(self.c, beta, skew)
         U = initial_state(self.c, self.f)
@@ -128,13 +130,13 @@ class CpuChannel:
         o.orc_bc_z(C.byref(c), ptrs(U_))
         o.orc_bc_x_periodic(C.byref(c), ptrs(U_))
         for fn, tag, F in ((o.orc_flux_x, "_u", self.Fx), (o.orc_flux_y, "_v", self.Fy), (o.orc_flux_z, "_w", self.Fz)):
-            fn(C.byref(c), C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(self.MR), ptrs(self.ML), dp(f["J"]),
+            fn(C.byref(c), C.c_double(1.0), C.c_double(self.e), ptrs(U_), ptrs(self.MR), ptrs(self.ML), dp(f["J"]),
                dp(f["J" + tag]), ptrs([f[n + tag] for n in MET]), ptrs(F))
         o.orc_viscous(C.byref(c), ptrs(U_), ptrs(self.vF), dp(f["J"]), dp(f["J_u"]), ptrs([f[n + "_u"] for n in MET]),
                       dp(f["J_v"]), ptrs([f[n + "_v"] for n in MET]), dp(f["J_w"]), ptrs([f[n + "_w"] for n in MET]),
                       ptrs(self.LL), ptrs(self.MLv), ptrs(self.NLv))
         er = (C.c_double * 5)()
-        o.orc_dual_time_stepping(C.byref(c), C.c_double(dt), C.c_double(1e-4), er,
+        o.orc_dual_time_stepping(C.byref(c), C.c_double(dt), C.c_double(self.e), er,
                                  self.N.ctypes.data_as(C.POINTER(C.c_int)), self.I.ctypes.data_as(C.POINTER(C.c_int)),
                                  ptrs(U_), ptrs(f["U"]), ptrs(f["Uq"]), dp(f["J"]), ptrs([f[n] for n in MET]),
                                  ptrs(self.Fx), ptrs(self.Fy), ptrs(self.Fz), ptrs(self.vF), ptrs(self.MR),

@@ -28,7 +28,7 @@ def oracle_state(ctx, nv):
     return f
 
 
-def oracle_step(orc, oc, f, shape, N, I, dt, inner, EpF=None):
+def oracle_step(orc, oc, f, shape, N, I, dt, inner, e, EpF=None):
     z = lambda n=5: [np.zeros(shape) for _ in range(n)]
     MR, ML, Fx, Fy, Fz, NL, NR, Res, LL, MLv, NLv = (z() for _ in range(11))
     vF = z(4)
@@ -42,15 +42,15 @@ def oracle_step(orc, oc, f, shape, N, I, dt, inner, EpF=None):
         for d, (fn, jk, mk, F) in enumerate(((orc.orc_flux_x, "J_u", "met_u", Fx), (orc.orc_flux_y, "J_v", "met_v", Fy),
                                              (orc.orc_flux_z, "J_w", "met_w", Fz))):
             if EpF is None:
-                fn(C.byref(oc), C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]), dp(f[jk]),
+                fn(C.byref(oc), C.c_double(1.0), C.c_double(e), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]), dp(f[jk]),
                    ptrs(f[mk]), ptrs(F))
             else:   # the sensor-scaled Roe dissipation
                 fs = (orc.orc_flux_x_scaled, orc.orc_flux_y_scaled, orc.orc_flux_z_scaled)[d]
-                fs(C.byref(oc), 2, C.c_double(1.0), C.c_double(1e-4), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]),
+                fs(C.byref(oc), 2, C.c_double(1.0), C.c_double(e), ptrs(U_), ptrs(MR), ptrs(ML), dp(f["J"]),
                    dp(f[jk]), ptrs(f[mk]), ptrs(F), dp(EpF[d]))
         orc.orc_viscous(C.byref(oc), ptrs(U_), ptrs(vF), dp(f["J"]), dp(f["J_u"]), ptrs(f["met_u"]), dp(f["J_v"]),
                         ptrs(f["met_v"]), dp(f["J_w"]), ptrs(f["met_w"]), ptrs(LL), ptrs(MLv), ptrs(NLv))
-        orc.orc_dual_time_stepping(C.byref(oc), C.c_double(dt), C.c_double(1e-4), er,
+        orc.orc_dual_time_stepping(C.byref(oc), C.c_double(dt), C.c_double(e), er,
                                    N.ctypes.data_as(C.POINTER(C.c_int)), I.ctypes.data_as(C.POINTER(C.c_int)),
                                    ptrs(U_), ptrs(f["U"]), ptrs(f["Uq"]), dp(f["J"]), ptrs(f["met"]), ptrs(Fx),
                                    ptrs(Fy), ptrs(Fz), ptrs(vF), ptrs(MR), ptrs(NR), ptrs(NL), ptrs(Res),
@@ -82,7 +82,7 @@ def test_channel_steps_match_oracle(size, skew, xi_mode):
         for step in range(2):
             it, e1, en = ctx.channel_step(dt, tol=0.0, max_inner=inner)
             assert it == inner
-            ers = oracle_step(orc, oc, f, ctx.shape, N, I, dt, inner)
+            ers = oracle_step(orc, oc, f, ctx.shape, N, I, dt, inner, ch.precond_e)
             for v in range(5):
                 assert abs(en[v] - ers[-1][v]) <= 1e-10 * abs(ers[-1][v]), (step, v, en[v], ers[-1][v])
             got = ctx.get_many(nv.U, 5)
@@ -121,7 +121,7 @@ def test_channel_steps_with_sensor_scaled_dissipation_match_oracle():
             assert set(np.unique(EpF[0][face])) <= {0.7, 0.3}
             it, e1, en = ctx.channel_step(dt, tol=0.0, max_inner=inner)
             assert it == inner
-            ers = oracle_step(orc, oc, f, ctx.shape, N, I, dt, inner, EpF=EpF)
+            ers = oracle_step(orc, oc, f, ctx.shape, N, I, dt, inner, ch.precond_e, EpF=EpF)
             for v in range(5):
                 assert abs(en[v] - ers[-1][v]) <= 1e-10 * abs(ers[-1][v]), (step, v, en[v], ers[-1][v])
             got = ctx.get_many(nv.U, 5)
