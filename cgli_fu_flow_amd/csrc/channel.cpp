@@ -168,10 +168,17 @@ int cgli_channel_init(cgli_ctx *ctx, const cgli_channel *ch, double *deltaT)
             ctx_mark(ctx, id, 1);
         }
     }
-    // y coordinate of the cell centres (Filter input)
+    // y coordinate of the cell centres (Filter input).  Ghost rows mirror the
+    // first and last interior rows about the walls y = -1 and y = 1, the
+    // rule of Grid.cpp:88-89 (whose walls are y = 0 and y = high).
     {
-        for (int j = 0; j < Ym; j++)
-            for (int k = 0; k < Zm; k++) pl[j * Zm + k] = map.y((j - 1.5) * cfg->deta);
+        for (int j = 0; j < Ym; j++) {
+            double yy;
+            if (j < 2) yy = -2.0 - map.y((3 - j - 1.5) * cfg->deta);
+            else if (j > NY) yy = 2.0 - map.y((2 * NY + 1 - j - 1.5) * cfg->deta);
+            else yy = map.y((j - 1.5) * cfg->deta);
+            for (int k = 0; k < Zm; k++) pl[j * Zm + k] = yy;
+        }
         double *d = nullptr;
         int r = ctx_field(ctx, CGLI_YPOINT, &d);
         if (r) return r;

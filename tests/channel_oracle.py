@@ -74,7 +74,32 @@ def metrics(c, beta, skew, Lz=2 * np.pi):
         out["J" + tag] = np.broadcast_to(1.0 / ye, shape).copy()
         if tag == "":
             out["y"] = y
+    # Filter input: cell-centre y, ghost rows mirrored about the walls y = -1
+    # and y = 1 (Grid.cpp:88-89 mirrors about its walls y = 0 and y = high)
+    yc = out["y"][:, 0].copy()
+    NY = c.ny + 1
+    yc[1], yc[0] = -2.0 - yc[2], -2.0 - yc[3]
+    yc[NY + 1], yc[NY + 2] = 2.0 - yc[NY], 2.0 - yc[NY - 1]
+    out["Y_point"] = np.broadcast_to(yc[None, :, None], shape).copy()
     return out
+
+
+def delta_t(c, f, U, cfl):
+    """Physical step at acoustic CFL number cfl: cfl over the largest
+    sum_d (|U_d| + c |grad d|) / delta_d of the interior cells, with U_d the
+    contravariant velocity and c the speed of sound of the state U (Q/J)."""
+    sl = (slice(3, c.nx + 3), slice(2, c.ny + 2), slice(2, c.nz + 2))
+    J = f["J"][sl]
+    rho = U[0][sl] * J
+    u, v, w = U[1][sl] / U[0][sl], U[2][sl] / U[0][sl], U[3][sl] / U[0][sl]
+    p = (U[4][sl] * J - 0.5 * rho * (u * u + v * v + w * w)) * (c.K - 1)
+    cs = np.sqrt(c.K * p / rho)
+    lam = 0.0
+    for d, dd in enumerate((c.dxi, c.deta, c.dzeta)):
+        m = [f[MET[3 * d + t]][sl] for t in range(3)]
+        Ud = m[0] * u + m[1] * v + m[2] * w
+        lam = lam + (np.abs(Ud) + cs * np.sqrt(m[0] ** 2 + m[1] ** 2 + m[2] ** 2)) / dd
+    return cfl / float(np.max(lam))
 
 
 def initial_state(c, f, seed=20261203, noise=0.1):

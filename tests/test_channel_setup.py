@@ -1,0 +1,103 @@
+"""The channel workload's grid, metrics, initial state and time step.
+
+The reference has only the analytic orthogonal tanh metric
+(Jacobian.cpp:66-103) and a quiescent initial state; the channel's skewed
+tanh mapping, Poiseuille + seeded-noise state and CFL time step are this
+build's own.  tests/channel_oracle.py restates them in numpy from the
+mapping's closed form.  Here that restatement is checked against finite
+differences of the mapping itself (CPU), and the library's
+cgli_channel_init is checked against the restatement (GPU).
+"""
+import numpy as np
+import pytest
+
+from channel_oracle import MET, channel_cfg, delta_t, initial_state, metrics, splitmix_stream
+
+
+def mapping(beta, skew, Lz):
+    def xyz(xi, eta, zeta):
+        y = -np.tanh(beta * (1 - 2 * eta)) / np.tanh(beta)
+        x = xi + skew * np.sin(2 * np.pi * y) * np.sin(2 * np.pi * zeta / Lz)
+        return np.array([x, y, zeta])
+    return xyz
+
+
+@pytest.mark.parametrize("beta,skew", [(2.2, 0.05), (2.6, 0.05), (2.0, 0.0)])
+def test_metrics_invert_the_mapping(beta, skew):
+    """The restated metrics times the mapping's Jacobian d(x,y,z)/d(xi,eta,zeta)
+    (central differences) is the identity, and J = 1/det, on all four point
+    sets (cell centres, x, y and z faces) including ghost rows."""
+    c = channel_cfg(6, 10, 8)
+    Lz = 2 * np.pi
+    f = metrics(c, beta, skew, Lz)
+    xyz = mapping(beta, skew, Lz)
+    h = 1e-6
+    for tag, eo, zo in (("", 1.5, 1.5), ("_u", 0.5, 0.5), ("_v", 1.0, 0.5), ("_w", 0.5, 1.0)):
+        for j in range(c.ny + 4):
+            for k in range(c.nz + 4):
+                eta, zeta, xi = (j - eo) * c.deta, (k - zo) * c.dzeta, 0.3
+                M = np.zeros((3, 3))
+                for d, step in enumerate(([h, 0, 0], [0, h, 0], [0, 0, h])):
+                    p = xyz(xi + step[0], eta + step[1], zeta + step[2])
+                    m = xyz(xi - step[0], eta - step[1], zeta - step[2])
+                    M[:, d] = (p - m) / (2 * h)
+                met = np.array([f[n + tag][4, j, k] for n in MET]).reshape(3, 3)
+                err = np.max(np.abs(met @ M - np.eye(3)))
+                assert err < 1e-7, (tag, j, k, err)
+                assert abs(f["J" + tag][4, j, k] * np.linalg.det(M) - 1) < 1e-7
+
+
+def test_y_point_ghosts_mirror_the_walls():
+    c = channel_cfg(4, 12, 4)
+    f = metrics(c, 2.4, 0.0)
+    Y = f["Y_point"][3, :, 3]
+    NY = c.ny + 1
+    assert np.all(np.diff(Y) > 0)
+    assert Y[1] + Y[2] == pytest.approx(-2.0, abs=1e-15)
+    assert Y[NY] + Y[NY + 1] == pytest.approx(2.0, abs=1e-15)
+    assert np.allclose(Y[2:NY + 1], f["y"][2:NY + 1, 0], rtol=0, atol=0)
+
+
+def test_noise_streams_are_splitmix64():
+    """Known outputs of splitmix64 (Vigna's reference generator) from state 0:
+    0xe220a8397b1dcdaf, 0x6e789e6aa1b965f4, 0x06c45d188009454f."""
+    u = splitmix_stream(0, 3)
+    want = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    for a, b in zip(u, want):
+        assert a == (b >> 11) * 2.0 ** -53
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("size,re_tau,beta,skew", [((12, 20, 10), 180.0, 2.2, 0.05), ((10, 16, 12), 395.0, 2.4, 0.0),
+                                                   ((8, 18, 8), 590.0, 2.6, 0.05)])
+def test_channel_init_matches_the_numpy_construction(size, re_tau, beta, skew):
+    from cgli_fu_flow_amd import native as nv
+    nx, ny, nz = size
+    print("channel init", size, re_tau, beta, skew, flush=True)
+    ch = nv.channel(re_tau, beta=beta, skew=skew)
+    cfg = nv.channel_config(ch, nx, ny, nz)
+    c = channel_cfg(nx, ny, nz, re_tau=re_tau, re_c=ch.re_c)
+    for name in ("dxi", "deta", "dzeta", "K", "R", "Cv", "mu_L", "Pr_L", "suth_S", "f_const"):
+        assert getattr(cfg, name) == pytest.approx(getattr(c, name), rel=1e-15), name
+    f = metrics(c, beta, skew)
+    U = initial_state(c, f)
+    with nv.Context(cfg) as ctx:
+        dt = ctx.channel_init(ch)
+        for tag, jid, mid in (("", nv.J, nv.MET), ("_u", nv.J_U, nv.MET_U), ("_v", nv.J_V, nv.MET_V),
+                              ("_w", nv.J_W, nv.MET_W)):
+            J = ctx.get(jid)
+            assert np.max(np.abs(J - f["J" + tag])) <= 1e-14 * np.max(np.abs(f["J" + tag])), tag
+            got = ctx.get_many(mid, 9)
+            for q, n in enumerate(MET):
+                want = f[n + tag]
+                scale = max(1.0, float(np.max(np.abs(want))))
+                assert np.max(np.abs(got[q] - want)) <= 1e-13 * scale, (n + tag)
+        assert np.max(np.abs(ctx.get(nv.YPOINT) - f["Y_point"])) <= 1e-15
+        sl = (slice(3, nx + 3), slice(2, ny + 2), slice(2, nz + 2))
+        for base in (nv.U_, nv.U, nv.UQ):
+            got = ctx.get_many(base, 5)
+            for q in range(5):
+                assert np.max(np.abs(got[q][sl] - U[q][sl])) <= 1e-14 * np.max(np.abs(U[q][sl])), (base, q)
+        want_dt = delta_t(c, f, U, ch.cfl)
+        print("deltaT", dt, want_dt, flush=True)
+        assert dt == pytest.approx(want_dt, rel=1e-13)
