@@ -17,7 +17,8 @@ using namespace cgli;
 
 namespace {
 
-enum { T_FLUX_X, T_FLUX_Y, T_FLUX_Z, T_VISCOUS, T_RHS, T_FWD, T_BWD, T_UPDATE, T_COEF, T_FILTER, T_SCATTER, T_NCLASS };
+enum { T_FLUX_X, T_FLUX_Y, T_FLUX_Z, T_VISCOUS, T_RHS, T_FWD, T_BWD, T_UPDATE, T_COEF, T_FILTER, T_SCATTER, T_GATHER,
+       T_NCLASS };
 
 struct EventPair { hipEvent_t a, b; };
 
@@ -59,13 +60,20 @@ struct cgli_ctx {
     double *er_mapped = nullptr;     // device address of er_host, where the runtime maps it
     int *flags_mapped = nullptr;     // device address of counters_host
     bool norm_wave = false;      // ... with the wavefront's timeout flag
-    int lusgs_mode = 3;          // 0: one launch per hyperplane, 1: tiled wavefront, 2: with loader waves,
-                                 // 3: tiled wavefront on march-ordered records (default)
+    int lusgs_mode = 4;          // 0: one launch per hyperplane, 1: tiled wavefront, 3: tiled wavefront on
+                                 // march-ordered records, 4: march computing its coefficients (default)
     double *coef = nullptr;      // split-coefficient blocks of the wavefront (float in the mixed mode)
     double *dgn = nullptr;       // diagonal blocks of the wavefront (FP64)
     void *coefM = nullptr;       // the same records in march order (mode 3)
     double *dgnM = nullptr;
     double *marchF = nullptr;    // mode 3: MR, NL and NR in march order
+    void *geo4 = nullptr;        // mode 4: J and the 9 metrics in march order
+    void *u4 = nullptr;          // mode 4: U_ in march order
+    void *x4 = nullptr;          // mode 4: MR, NL and NR in march order
+    double *xbufC = nullptr;     // mode 4: exchange of the k products (xbuf holds the j products)
+    long geo_version = 0;        // bumped when J or a cell metric is written
+    long geo4_version = -1;      // geo_version of geo4's contents
+    bool geo_volatile = false;   // a device pointer to J or a metric was handed out
     int *morder_dev = nullptr;   // mode 3: tile order of its tiles
     int mtiles = 0;
     double *xbuf = nullptr;      // tile-edge exchange of the wavefront: 5 values per cell
@@ -134,6 +142,15 @@ static void nl_sync(cgli_ctx *c)
                          c->stream);
 }
 
+// J or a cell metric written: the march-ordered copy of the geometry is stale
+static void geo_written(cgli_ctx *c, int id, int n)
+{
+    if (id <= CGLI_MET + 8 && id + n > CGLI_J) {
+        c->geo_version++;
+        c->generation++;   // a captured gather of the old geometry is stale
+    }
+}
+
 static F5 f5(cgli_ctx *c, int id) { return F5{{c->f[id], c->f[id + 1], c->f[id + 2], c->f[id + 3], c->f[id + 4]}}; }
 static C5 c5(cgli_ctx *c, int id) { return C5{{c->f[id], c->f[id + 1], c->f[id + 2], c->f[id + 3], c->f[id + 4]}}; }
 static M9 m9(cgli_ctx *c, int id)
@@ -143,7 +160,7 @@ static M9 m9(cgli_ctx *c, int id)
     return m;
 }
 
-static int scratch(cgli_ctx *c, long n)
+static int ensure_scratch(cgli_ctx *c, long n)
 {
     if (c->scratch_len >= n) return CGLI_OK;
     if (c->capturing) return CGLI_E_STATE;
@@ -194,7 +211,11 @@ void ctx_set_fp32(cgli_ctx *c, bool on) { c->p.fp32 = on ? 1 : 0; }
 bool ctx_timing_on(cgli_ctx *c) { return c->timing; }
 
 void ctx_set_last_fp32(cgli_ctx *c, int n) { c->last_fp32 = n; }
-void ctx_mark(cgli_ctx *c, int id, int n) { mark(c, id, n); }
+void ctx_mark(cgli_ctx *c, int id, int n)
+{
+    mark(c, id, n);
+    geo_written(c, id, n);
+}
 const Geo &ctx_geo(cgli_ctx *c) { return c->g; }
 ChannelState &ctx_channel(cgli_ctx *c) { return c->chan; }
 }  // namespace cgli
@@ -304,6 +325,10 @@ int cgli_destroy(cgli_ctx *c)
     if (c->coefM) hipFree(c->coefM);
     if (c->dgnM) hipFree(c->dgnM);
     if (c->marchF) hipFree(c->marchF);
+    if (c->geo4) hipFree(c->geo4);
+    if (c->u4) hipFree(c->u4);
+    if (c->x4) hipFree(c->x4);
+    if (c->xbufC) hipFree(c->xbufC);
     if (c->morder_dev) hipFree(c->morder_dev);
     if (c->xbuf) hipFree(c->xbuf);
     if (c->trash) hipFree(c->trash);
@@ -337,10 +362,12 @@ int cgli_set_field(cgli_ctx *c, int id, const double *host)
     HIPCHK(hipSetDevice(c->device));
     int r = ensure(c, id);
     if (r) return r;
-    if (id >= CGLI_NL && id < CGLI_NL + 5) nl_sync(c);   // the other components stay consistent
+    // NL still to be copied from NR: copy it before either is overwritten
+    if (id >= CGLI_NL && id < CGLI_NR + 5) nl_sync(c);
     HIPCHK(hipMemcpyAsync(c->f[id], host, sizeof(double) * c->g.N, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->valid[id] = true;
+    geo_written(c, id, 1);
     return CGLI_OK;
 }
 
@@ -361,10 +388,13 @@ int cgli_field_device_ptr(cgli_ctx *c, int id, void **dptr)
 {
     if (!c || !dptr || id < 0 || id >= CGLI_NFIELDS) return CGLI_E_ARG;
     HIPCHK(hipSetDevice(c->device));
+    // the deferred copy of NR into NL cannot be settled inside a capture
+    if (c->capturing && id >= CGLI_NL && id < CGLI_NR + 5) return CGLI_E_STATE;
     int r = ensure(c, id);
     if (r) return r;
-    if (id >= CGLI_NL && id < CGLI_NL + 5) nl_sync(c);
+    if (id >= CGLI_NL && id < CGLI_NR + 5) nl_sync(c);
     c->valid[id] = true;   // the caller writes it
+    if (id == CGLI_J || (id >= CGLI_MET && id < CGLI_MET + 9)) c->geo_volatile = true;
     *dptr = c->f[id];
     return CGLI_OK;
 }
@@ -665,8 +695,33 @@ static int wave_setup(cgli_ctx *c)
 {
     void *p = nullptr;
     if (c->lusgs_mode == 3 && !march_fits(c->g, c->cfg.precision == 1)) return CGLI_E_ARG;
-    if (c->capturing && ((c->lusgs_mode == 3 && !c->coefM) || (c->lusgs_mode != 3 && !c->coef) || !c->xbuf))
+    if (c->capturing && ((c->lusgs_mode == 3 && !c->coefM) || (c->lusgs_mode == 4 && !c->geo4) ||
+                         (c->lusgs_mode < 3 && !c->coef) || !c->xbuf))
         return CGLI_E_STATE;
+    if (c->lusgs_mode >= 3 && !c->morder_dev) {
+        std::vector<int> order;
+        c->mtiles = wave_tiles(c->g, MARCH_TJ, MARCH_TK, order);
+        if (hipMalloc(&p, sizeof(int) * order.size()) != hipSuccess) return CGLI_E_NOMEM;
+        c->morder_dev = static_cast<int *>(p);
+        HIPCHK(hipMemcpy(c->morder_dev, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+    }
+    if (c->lusgs_mode == 4 && !c->geo4) {
+        const long gb = 16 * march4_geo_pairs(c->g), fb = 16 * march4_field_pairs(c->g);
+        if (hipMalloc(&p, gb) != hipSuccess) return CGLI_E_NOMEM;
+        c->geo4 = p;
+        if (hipMalloc(&p, fb) != hipSuccess) return CGLI_E_NOMEM;
+        c->u4 = p;
+        if (hipMalloc(&p, 3 * fb) != hipSuccess) return CGLI_E_NOMEM;
+        c->x4 = p;
+        // slots with no cell are never written: NaN in them
+        HIPCHK(hipMemset(c->geo4, 0xFF, gb));
+        HIPCHK(hipMemset(c->u4, 0xFF, fb));
+        HIPCHK(hipMemset(c->x4, 0xFF, 3 * fb));
+        c->geo4_version = -1;
+        if (hipMalloc(&p, sizeof(double) * 5 * c->g.N) != hipSuccess) return CGLI_E_NOMEM;
+        c->xbufC = static_cast<double *>(p);
+        HIPCHK(hipMemset(c->xbufC, 0, sizeof(double) * 5 * c->g.N));
+    }
     if (c->lusgs_mode == 3 && !c->coefM) {
         // FP64 pairs are 16 bytes: the FP32 records use half
         if (hipMalloc(&p, 16 * march_coef_pairs(c->g)) != hipSuccess) return CGLI_E_NOMEM;
@@ -680,13 +735,8 @@ static int wave_setup(cgli_ctx *c)
         HIPCHK(hipMemset(c->coefM, 0xFF, 16 * march_coef_pairs(c->g)));
         HIPCHK(hipMemset(c->dgnM, 0xFF, 16 * march_diag_pairs(c->g)));
         HIPCHK(hipMemset(c->marchF, 0xFF, sizeof(double) * 3 * march_field_doubles(c->g)));
-        std::vector<int> order;
-        c->mtiles = wave_tiles(c->g, MARCH_TJ, MARCH_TK, order);
-        if (hipMalloc(&p, sizeof(int) * order.size()) != hipSuccess) return CGLI_E_NOMEM;
-        c->morder_dev = static_cast<int *>(p);
-        HIPCHK(hipMemcpy(c->morder_dev, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
     }
-    if (c->lusgs_mode != 3 && !c->coef) {
+    if (c->lusgs_mode < 3 && !c->coef) {
         if (hipMalloc(&p, sizeof(double) * coef_doubles(c->g)) != hipSuccess) return CGLI_E_NOMEM;
         c->coef = static_cast<double *>(p);
         if (hipMalloc(&p, sizeof(double) * diag_doubles(c->g)) != hipSuccess) return CGLI_E_NOMEM;
@@ -722,7 +772,7 @@ static int wave_setup(cgli_ctx *c)
 
 int cgli_set_lusgs_mode(cgli_ctx *c, int mode)
 {
-    if (!c || mode < 0 || mode > 3) return CGLI_E_ARG;
+    if (!c || mode < 0 || mode > 4 || mode == 2) return CGLI_E_ARG;
     NO_CAPTURE(c);
     c->lusgs_mode = mode;
     c->generation++;
@@ -853,7 +903,7 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
     const Geo &g = c->g;
     const long ncell = (long)g.nx * g.ny * g.nz;
     const long nb = (ncell + 255) / 256;
-    r = scratch(c, 5 * nb + (long)g.NYY * g.NZZ + 64);
+    r = ensure_scratch(c, 5 * nb + (long)g.NYY * g.NZZ + 64);
     if (r) return r;
     const hipStream_t s = c->stream;
     if (c->lusgs_mode >= 1) {
@@ -873,11 +923,20 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
         launch_rhs_diag(g, c->p, deltaT, e, c->cfg.sponge, c5(c, CGLI_U_), c5(c, CGLI_U), c5(c, CGLI_UQ),
                         c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_INFX), c5(c, CGLI_INFY), c5(c, CGLI_INFZ),
                         C4{{c->f[CGLI_VF], c->f[CGLI_VF + 1], c->f[CGLI_VF + 2], c->f[CGLI_VF + 3]}},
-                        c5(c, CGLI_RES), f5(c, CGLI_MR), f5(c, CGLI_NL), f5(c, CGLI_NR), s, c->lusgs_mode != 3);
+                        c5(c, CGLI_RES), f5(c, CGLI_MR), f5(c, CGLI_NL), f5(c, CGLI_NR), s, c->lusgs_mode < 3);
         tend(c, T_RHS);
     }
     const int mlast = g.gend + g.NY + g.NZ;
-    if (c->lusgs_mode >= 1) {
+    const long f4b = 16 * march4_field_pairs(g);
+    char *const x4 = static_cast<char *>(c->x4);   // mode 4: MR, NL, NR in march order
+    if (c->lusgs_mode == 4) {
+        tbegin(c, T_GATHER);
+        const bool geo = c->geo_volatile || c->geo4_version != c->geo_version;
+        launch_gather4(g, c5(c, CGLI_U_), c5(c, CGLI_MR), c->f[CGLI_J], m9(c, CGLI_MET), c->u4, x4,
+                       geo ? c->geo4 : nullptr, s);
+        c->geo4_version = c->geo_version;
+        tend(c, T_GATHER);
+    } else if (c->lusgs_mode >= 1) {
         tbegin(c, T_COEF);
         if (c->lusgs_mode == 3)
             launch_split_march(g, c->p, deltaT, e, c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_MR),
@@ -888,7 +947,27 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
         tend(c, T_COEF);
     }
     for (int sw = 0; sw < sweeps; sw++) {
-        if (c->lusgs_mode == 3) {
+        if (c->lusgs_mode == 4) {
+            launch_march_prep(g, 1, c->xbuf, c->counters, s, c->xbufC);
+            tbegin(c, T_FWD);
+            launch_lusgs_fused(g, c->p, deltaT, e, 1, c->geo4, c->u4, x4, x4 + f4b, c5(c, CGLI_U_), c->f[CGLI_J],
+                               m9(c, CGLI_MET), c5(c, CGLI_NL), c->xbuf, c->xbufC, c->trash, c->morder_dev,
+                               c->mtiles, c->counters, c->max_spin, MARCH_SLACK, s);
+            tend(c, T_FWD);
+            launch_march_prep(g, 0, c->xbuf, c->counters, s, c->xbufC);
+            tbegin(c, T_BWD);
+            launch_lusgs_fused(g, c->p, deltaT, e, 0, c->geo4, c->u4, x4 + f4b, x4 + 2 * f4b, c5(c, CGLI_U_),
+                               c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_NR), c->xbuf, c->xbufC, c->trash,
+                               c->morder_dev, c->mtiles, c->counters, c->max_spin, MARCH_SLACK, s);
+            tend(c, T_BWD);
+            if (sw == sweeps - 1) {
+                tbegin(c, T_SCATTER);
+                launch_scatter4(g, x4 + 2 * f4b, F5{{nullptr, nullptr, nullptr, nullptr, nullptr}}, f5(c, CGLI_NR), s);
+                c->nl_from_nr = true;
+                c->nr_scatters++;
+                tend(c, T_SCATTER);
+            }
+        } else if (c->lusgs_mode == 3) {
             launch_march_prep(g, 1, c->xbuf, c->counters, s);
             tbegin(c, T_FWD);
             const long mf = march_field_doubles(g);
@@ -914,12 +993,12 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
             launch_lusgs_wave_prep(g, c->xbuf, c->counters, s);
             tbegin(c, T_FWD);
             launch_lusgs_wave(g, 1, c->coef, c->dgn, c->cfg.precision == 1, c5(c, CGLI_MR), f5(c, CGLI_NL), c->xbuf, c->trash, c->order_dev,
-                              c->ntiles, c->counters, c->max_spin, c->lusgs_mode == 2, s);
+                              c->ntiles, c->counters, c->max_spin, s);
             tend(c, T_FWD);
             launch_lusgs_wave_prep(g, c->xbuf, c->counters, s);
             tbegin(c, T_BWD);
             launch_lusgs_wave(g, 0, c->coef, c->dgn, c->cfg.precision == 1, c5(c, CGLI_NL), f5(c, CGLI_NR), c->xbuf, c->trash, c->order_dev,
-                              c->ntiles, c->counters, c->max_spin, c->lusgs_mode == 2, s);
+                              c->ntiles, c->counters, c->max_spin, s);
             tend(c, T_BWD);
         } else {
             tbegin(c, T_FWD);
@@ -941,8 +1020,8 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
             }
             tend(c, T_BWD);
         }
-        // NL = NR at the end of a sweep pair (schedule 3: where NL is read)
-        if (c->lusgs_mode != 3) {
+        // NL = NR at the end of a sweep pair (schedules 3, 4: where NL is read)
+        if (c->lusgs_mode < 3) {
             launch_copy_interior(g, c5(c, CGLI_NR), f5(c, CGLI_NL), s);
             c->nl_from_nr = false;
         }
@@ -974,7 +1053,7 @@ int cgli_dplusgs(cgli_ctx *c, double deltaT, double er[5], double e, int sweeps)
     const Geo &g = c->g;
     const long ncell = (long)g.nx * g.ny * g.nz;
     const long nb = (ncell + 255) / 256;
-    r = scratch(c, 5 * nb + 64);
+    r = ensure_scratch(c, 5 * nb + 64);
     if (r) return r;
     const hipStream_t s = c->stream;
     Phys pd = c->p;
@@ -1166,14 +1245,14 @@ int ctx_rhs_sumsq(cgli_ctx *c, double deltaT, double e, double out[5])
     const Geo &g = c->g;
     const long ncell = (long)g.nx * g.ny * g.nz;
     const long nb = (ncell + 255) / 256;
-    r = scratch(c, 5 * nb + (long)g.NYY * g.NZZ + 64);
-    if (!r) r = scratch(c, 5L * g.nx * g.ny + 16);
+    r = ensure_scratch(c, 5 * nb + (long)g.NYY * g.NZZ + 64);
+    if (!r) r = ensure_scratch(c, 5L * g.nx * g.ny + 16);
     if (r) return r;
     const hipStream_t s = c->stream;
     launch_rhs_diag(g, c->p, deltaT, e, 0, c5(c, CGLI_U_), c5(c, CGLI_U), c5(c, CGLI_UQ),
                     c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_INFX), c5(c, CGLI_INFY), c5(c, CGLI_INFZ),
                     C4{{c->f[CGLI_VF], c->f[CGLI_VF + 1], c->f[CGLI_VF + 2], c->f[CGLI_VF + 3]}},
-                    c5(c, CGLI_RES), f5(c, CGLI_MR), f5(c, CGLI_NL), f5(c, CGLI_NR), s, c->lusgs_mode != 3);
+                    c5(c, CGLI_RES), f5(c, CGLI_MR), f5(c, CGLI_NL), f5(c, CGLI_NR), s, c->lusgs_mode < 3);
     double *dev = c->scratch + c->scratch_len - 8;
     launch_sumsq5(g, c5(c, CGLI_MR), c->scratch, dev, s);
     HIPCHK(hipGetLastError());

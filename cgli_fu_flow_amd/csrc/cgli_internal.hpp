@@ -125,14 +125,14 @@ long diag_doubles(const Geo &g);
 // before each sweep: exchange sentinels and the tile ticket
 void launch_lusgs_wave_prep(const Geo &g, double *xbuf, int *counters, hipStream_t s);
 // before a march sweep: sentinels into the exchange entries it polls, ticket
-// counters[0] to zero
-void launch_march_prep(const Geo &g, int forward, double *xbuf, int *counters, hipStream_t s);
+// counters[0] to zero; xcol: the buffer of the k-edge entries when it is not
+// xbuf (schedule 4 exchanges different j and k products)
+void launch_march_prep(const Geo &g, int forward, double *xbuf, int *counters, hipStream_t s,
+                       double *xcol = nullptr);
 // coef: split blocks (float when fp32), dgn: FP64 diagonal blocks
-// loaders: two loader waves per tile keep the coefficient stream in flight
-// (slower than the computing wave's own prefetch at 128^3: 2.17 vs 1.94 ms)
 void launch_lusgs_wave(const Geo &g, int forward, const void *coef, const double *dgn, int fp32, C5 R, F5 X,
                        double *xbuf, double *trash, const int *order, int ntiles, int *counters, int max_spin,
-                       int loaders, hipStream_t s);
+                       hipStream_t s);
 void launch_reset_ints(int *p, int n, hipStream_t s);
 // march layout (implicit.hip): coefficient records, right-hand sides and
 // increments in the order the wavefront reads them; sizes in CT pairs, FP64
@@ -149,6 +149,21 @@ void launch_lusgs_march(const Geo &g, int forward, const void *coefM, const doub
 // NR (march order) into the fields NR and NL: the end of a sweep pair
 // NL with null pointers: NR only
 void launch_march_scatter(const Geo &g, const double *nrM, F5 NL, F5 NR, hipStream_t s);
+// schedule 4 (implicit.hip): march-ordered pairs of the geometry (J and 9
+// metrics, 5 pairs per slot) and of five-value fields (3 pairs per slot)
+long march4_geo_pairs(const Geo &g);
+long march4_field_pairs(const Geo &g);
+// U_ and R (when Rm) and the geometry (when Gm) into march order
+void launch_gather4(const Geo &g, C5 U_, C5 R, const double *J, M9 met, void *Um, void *Rm, void *Gm, hipStream_t s);
+// march-ordered increments Xm into NR (and NL unless its pointers are null)
+void launch_scatter4(const Geo &g, const void *Xm, F5 NL, F5 NR, hipStream_t s);
+// one sweep: right-hand sides Rm (forward MR, backward NL) into increments
+// Xm; U_, J, met and the increment field Xg supply the ghost cell upstream
+// in i; xbB / xbC: exchange buffers of the j and k products (5 per cell)
+void launch_lusgs_fused(const Geo &g, const Phys &p, double deltaT, double e, int forward, const void *Gm,
+                        const void *Um, const void *Rm, void *Xm, C5 U_, const double *J, M9 met, C5 Xg, double *xbB,
+                        double *xbC, double *trash, const int *order, int ntiles, int *counters, int max_spin,
+                        int slack, hipStream_t s);
 
 
 // learned initial guess (cnn.hip): see that file for layouts

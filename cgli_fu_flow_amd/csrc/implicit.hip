@@ -804,8 +804,11 @@ __global__ void k_fill_sentinel(Geo g, double *__restrict__ xbuf)
 // the rows (j-2) % TJ == rj and columns (k-2) % TK == rk of every x plane
 // (rj = TJ-1, rk = TK-1 forward; 0, 0 backward); no other entry is read.
 // blockIdx.y 0: the rows, 5 nz consecutive doubles each; 1: the columns.
-__global__ void k_fill_sentinel_edges(Geo g, double *__restrict__ xbuf, int TJ, int TK, int rj, int rk,
-                                      int *__restrict__ ticket)
+// The rows are polled in xrow, the columns in xcol (one buffer for both when
+// the increments themselves are exchanged, two when the j and k products
+// differ).
+__global__ void k_fill_sentinel_edges(Geo g, double *__restrict__ xrow, double *__restrict__ xcol, int TJ, int TK,
+                                      int rj, int rk, int *__restrict__ ticket)
 {
     const int i = 3 + blockIdx.z;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -817,12 +820,12 @@ __global__ void k_fill_sentinel_edges(Geo g, double *__restrict__ xbuf, int TJ, 
         const long per = 5L * g.nz;
         if (t >= per * nrow) return;
         const int j = 2 + rj + (int)(t / per) * TJ;
-        xbuf[5 * ix(g, i, j, 2) + t % per] = S;
+        xrow[5 * ix(g, i, j, 2) + t % per] = S;
     } else {
         if (t >= 5L * g.ny * ncol) return;
         const long cell = t / 5;
         const int j = 2 + (int)(cell / ncol), k = 2 + rk + (int)(cell % ncol) * TK;
-        xbuf[5 * ix(g, i, j, k) + t % 5] = S;
+        xcol[5 * ix(g, i, j, k) + t % 5] = S;
     }
 }
 
@@ -1109,327 +1112,6 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_wave(Geo g, const CT *__restri
     for (int s = 0; s < S; s += 2) {
         wave_step<TJ, TK, FWD, 0>(w, L, s, prev, sx, P0, P1, nb0, nb1, nc0, nc1);
         if (s + 1 < S) wave_step<TJ, TK, FWD, 1>(w, L, s + 1, prev, sx, P1, P0, nb1, nb0, nc1, nc0);
-    }
-}
-
-// ---------------------------------------------------------------- loader waves
-// The same march with its coefficient stream moved out of the computing
-// wave.  A workgroup is one computing wave (the TJ x TK columns, as in
-// k_lusgs_wave) and two loader waves.  Loader lane l serves column l:
-// loader 1 fetches the x and y split records, loader 2 the z record, the
-// diagonal and R.  Each keeps LDEPTH steps of loads in flight in register
-// sets used in rotation; one step before the computing wave needs a step's
-// data, its loader waits for that set and copies it into one of two LDS
-// slots.  The per-step barrier the march already has orders the slot
-// hand-over, so the computing wave reads its coefficients from LDS and only
-// its tile-edge exchange still waits on memory.
-constexpr int LDEPTH = 3;   // steps of coefficient loads in flight per loader
-
-template <typename CT>
-struct LdSlot {
-    typedef typename Pair2<CT>::t V2;
-    V2 ab[12][64];      // x then y direction: 6 pairs each
-    V2 cz[6][64];       // z direction
-    double2 dg[2][64];  // (d11, d22), (d51, pad)
-    double r[5][64];
-};
-
-// one loader's half of a step: loader 1 holds the x record (6 pairs), the
-// first 3 pairs of the y record and R[0..1]; loader 2 the last 3 pairs of the
-// y record, the z record (6 pairs), the diagonal and R[2..4]
-template <typename CT>
-struct LdSet { CT p[18]; double d[4]; double r[3]; };   // scalars: 9 pairs, 2 diagonal pairs
-
-template <bool FWD, typename CT>
-__device__ __forceinline__ void ld_dir6(typename Pair2<CT>::t *o, const CT *__restrict__ r)
-{
-    typedef typename Pair2<CT>::t V2;
-    const V2 *b = reinterpret_cast<const V2 *>(r);
-#pragma unroll
-    for (int q = 0; q < 5; q++) o[q] = b[q * CB];
-    o[5] = b[(FWD ? 5 : 6) * CB];
-}
-
-// the computing wave's step: as wave_step, with the coefficients of this
-// step read from LDS slot `sl`
-template <int TJ, int TK, bool FWD, int PAR, typename CT>
-__device__ __forceinline__ void wave_step_ld(const WaveArgs<CT> &w, const Lane &L, int s, double prev[5],
-                                             double (*sx)[5][TJ * TK], const LdSlot<CT> &sl,
-                                             unsigned long long *nbc, unsigned long long *nbn,
-                                             unsigned long long *ncc, unsigned long long *ncn)
-{
-    const int il = s - L.jl - L.kl;
-    const bool act = L.col && il >= 0 && il < w.g.nx;
-    const long c = cell_at<FWD>(w, L, il);
-    double xb[5], xc[5];
-    const int tb = L.tid >= TK ? L.tid - TK : L.tid, tc = L.tid >= 1 ? L.tid - 1 : L.tid;
-#pragma unroll
-    for (int v = 0; v < 5; v++) {
-        xb[v] = sx[PAR ^ 1][v][tb];
-        xc[v] = sx[PAR ^ 1][v][tc];
-    }
-    settle5(w.xbuf, w.g.N, c + w.d * w.g.Zm, act && L.jl == 0, nbc, xb, w.max_spin, w.err);
-    settle5(w.xbuf, w.g.N, c + w.d, act && L.kl == 0, ncc, xc, w.max_spin, w.err);
-    {   // the next step's tile-edge values (only the lanes that consume them)
-        const long cn = cell_at<FWD>(w, L, il + 1);
-        if (L.jl == 0) issue5(w.xbuf, w.g.N, cn + w.d * w.g.Zm, nbn);
-        if (L.kl == 0) issue5(w.xbuf, w.g.N, cn + w.d, ncn);
-    }
-    CT A[12], B[12], Cc[12];
-#pragma unroll
-    for (int q = 0; q < 6; q++) {
-        const typename Pair2<CT>::t a = sl.ab[q][L.tid], b = sl.ab[6 + q][L.tid], cz = sl.cz[q][L.tid];
-        const int o = q < 5 ? 2 * q : 10;
-        A[o] = a.x; A[o + 1] = a.y;
-        B[o] = b.x; B[o + 1] = b.y;
-        Cc[o] = cz.x; Cc[o + 1] = cz.y;
-    }
-    const double2 d0 = sl.dg[0][L.tid], d1 = sl.dg[1][L.tid];
-    const double d11 = d0.x, d22 = d0.y, d51 = d1.x;
-    double r[5];
-#pragma unroll
-    for (int v = 0; v < 5; v++) r[v] = sl.r[v][L.tid];
-    CT xa[5], xbc[5], xcc[5];
-#pragma unroll
-    for (int v = 0; v < 5; v++) {
-        xa[v] = (CT)prev[v];
-        xbc[v] = (CT)xb[v];
-        xcc[v] = (CT)xc[v];
-    }
-    double out[5];
-    if (FWD) {
-        const CT LL0 = (-A[10] * xa[0] - A[0] * xa[1] - A[1] * xa[2] - A[2] * xa[3]) +
-                           (-B[10] * xbc[0] - B[0] * xbc[1] - B[1] * xbc[2] - B[2] * xbc[3]) +
-                           (-Cc[10] * xcc[0] - Cc[0] * xcc[1] - Cc[1] * xcc[2] - Cc[2] * xcc[3]);
-        const CT LL1 = (-A[3] * xa[0] - A[11] * xa[1]) + (-B[3] * xbc[0] - B[11] * xbc[1]) +
-                           (-Cc[3] * xcc[0] - Cc[11] * xcc[1]);
-        const CT LL2 = (-A[4] * xa[0] - A[11] * xa[2]) + (-B[4] * xbc[0] - B[11] * xbc[2]) +
-                           (-Cc[4] * xcc[0] - Cc[11] * xcc[2]);
-        const CT LL3 = (-A[5] * xa[0] - A[11] * xa[3]) + (-B[5] * xbc[0] - B[11] * xbc[3]) +
-                           (-Cc[5] * xcc[0] - Cc[11] * xcc[3]);
-        const CT LL4 = (-A[6] * xa[0] - A[7] * xa[1] - A[8] * xa[2] - A[9] * xa[3] - A[11] * xa[4]) +
-                           (-B[6] * xbc[0] - B[7] * xbc[1] - B[8] * xbc[2] - B[9] * xbc[3] - B[11] * xbc[4]) +
-                           (-Cc[6] * xcc[0] - Cc[7] * xcc[1] - Cc[8] * xcc[2] - Cc[9] * xcc[3] - Cc[11] * xcc[4]);
-        out[0] = d11 * (r[0] - (double)LL0);
-        out[1] = d22 * (r[1] - (double)LL1);
-        out[2] = d22 * (r[2] - (double)LL2);
-        out[3] = d22 * (r[3] - (double)LL3);
-        out[4] = d51 * (r[0] - (double)LL0) + d22 * (r[4] - (double)LL4);
-    } else {
-        const CT UU0 = (+A[10] * xa[0] + A[0] * xa[1] + A[1] * xa[2] + A[2] * xa[3]) +
-                           (+B[10] * xbc[0] + B[0] * xbc[1] + B[1] * xbc[2] + B[2] * xbc[3]) +
-                           (+Cc[10] * xcc[0] + Cc[0] * xcc[1] + Cc[1] * xcc[2] + Cc[2] * xcc[3]);
-        const CT UU1 = (+A[3] * xa[0] + A[11] * xa[1]) + (+B[3] * xbc[0] + B[11] * xbc[1]) +
-                           (+Cc[3] * xcc[0] + Cc[11] * xcc[1]);
-        const CT UU2 = (+A[4] * xa[0] + A[11] * xa[2]) + (+B[4] * xbc[0] + B[11] * xbc[2]) +
-                           (+Cc[4] * xcc[0] + Cc[11] * xcc[2]);
-        const CT UU3 = (+A[5] * xa[0] + A[11] * xa[3]) + (+B[5] * xbc[0] + B[11] * xbc[3]) +
-                           (+Cc[5] * xcc[0] + Cc[11] * xcc[3]);
-        const CT UU4 = (+A[6] * xa[0] + A[7] * xa[1] + A[8] * xa[2] + A[9] * xa[3] + A[11] * xa[4]) +
-                           (+B[6] * xbc[0] + B[7] * xbc[1] + B[8] * xbc[2] + B[9] * xbc[3] + B[11] * xbc[4]) +
-                           (+Cc[6] * xcc[0] + Cc[7] * xcc[1] + Cc[8] * xcc[2] + Cc[9] * xcc[3] + Cc[11] * xcc[4]);
-        out[0] = r[0] - d11 * (double)UU0;
-        out[1] = r[1] - d22 * (double)UU1;
-        out[2] = r[2] - d22 * (double)UU2;
-        out[3] = r[3] - d22 * (double)UU3;
-        out[4] = r[4] - (d51 * (double)UU0 + d22 * (double)UU4);
-    }
-#pragma unroll
-    for (int v = 0; v < 5; v++) {
-        double *px = act ? w.X.v[v] + c : w.trash + L.tline + v;
-        *px = out[v];
-        sx[PAR][v][L.tid] = out[v];
-        prev[v] = act ? out[v] : prev[v];
-    }
-    if (act && L.edge_out) {
-#pragma unroll
-        for (int v = 0; v < 5; v++)
-            __hip_atomic_store(reinterpret_cast<unsigned long long *>(w.xbuf + 5 * c + v),
-                               (unsigned long long)__double_as_longlong(out[v]), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-    }
-}
-
-// issue one loader's loads of local step il into a set (pairs are split into
-// scalars at once: the register sets never hold vector structs)
-template <typename CT>
-__device__ __forceinline__ void put_pair(CT *o, typename Pair2<CT>::t v)
-{
-    o[0] = v.x;
-    o[1] = v.y;
-}
-
-template <bool FWD, typename CT>
-__device__ __forceinline__ void ld_issue(LdSet<CT> &S, int role, const WaveArgs<CT> &w, const Lane &L, int il)
-{
-    typedef typename Pair2<CT>::t V2;
-    const long ic = icl(w, il);
-    const CT *rec = w.coef + L.r0 + ic * w.dcol;
-    const long c = cell_at<FWD>(w, L, il);
-    const V2 *b = reinterpret_cast<const V2 *>(rec + w.oB + 7 * CPAIR);
-    if (role == 1) {
-        const V2 *a = reinterpret_cast<const V2 *>(rec + w.oA);
-#pragma unroll
-        for (int q = 0; q < 5; q++) put_pair<CT>(S.p + 2 * q, a[q * CB]);
-        put_pair<CT>(S.p + 10, a[(FWD ? 5 : 6) * CB]);
-#pragma unroll
-        for (int q = 0; q < 3; q++) put_pair<CT>(S.p + 12 + 2 * q, b[q * CB]);
-        S.r[0] = w.R.v[0][c];
-        S.r[1] = w.R.v[1][c];
-    } else {
-        put_pair<CT>(S.p, b[3 * CB]);
-        put_pair<CT>(S.p + 2, b[4 * CB]);
-        put_pair<CT>(S.p + 4, b[(FWD ? 5 : 6) * CB]);
-        const V2 *cz = reinterpret_cast<const V2 *>(rec + w.oC + 14 * CPAIR);
-#pragma unroll
-        for (int q = 0; q < 5; q++) put_pair<CT>(S.p + 6 + 2 * q, cz[q * CB]);
-        put_pair<CT>(S.p + 16, cz[(FWD ? 5 : 6) * CB]);
-        const double2 *db = reinterpret_cast<const double2 *>(w.dgn + L.d0 + ic * w.ddcol);
-        put_pair<double>(S.d, db[0]);
-        put_pair<double>(S.d + 2, db[CB]);
-#pragma unroll
-        for (int v = 0; v < 3; v++) S.r[v] = w.R.v[2 + v][c];
-    }
-}
-
-template <typename CT>
-__device__ __forceinline__ void ld_put(LdSlot<CT> &sl, const LdSet<CT> &S, int role, int lane)
-{
-    if (role == 1) {
-#pragma unroll
-        for (int q = 0; q < 9; q++) sl.ab[q][lane] = Pair2<CT>::make(S.p[2 * q], S.p[2 * q + 1]);
-        sl.r[0][lane] = S.r[0];
-        sl.r[1][lane] = S.r[1];
-    } else {
-#pragma unroll
-        for (int q = 0; q < 3; q++) sl.ab[9 + q][lane] = Pair2<CT>::make(S.p[2 * q], S.p[2 * q + 1]);
-#pragma unroll
-        for (int q = 0; q < 6; q++) sl.cz[q][lane] = Pair2<CT>::make(S.p[6 + 2 * q], S.p[7 + 2 * q]);
-        sl.dg[0][lane] = make_double2(S.d[0], S.d[1]);
-        sl.dg[1][lane] = make_double2(S.d[2], S.d[3]);
-#pragma unroll
-        for (int v = 0; v < 3; v++) sl.r[2 + v][lane] = S.r[v];
-    }
-}
-
-// step s0+u of a group of LCM(2, LDEPTH) = 6 steps: every set index and
-// LDS parity is a compile-time constant
-template <int q, typename CT>
-__device__ __forceinline__ LdSet<CT> &pick_set(LdSet<CT> &a, LdSet<CT> &b, LdSet<CT> &c)
-{
-    if constexpr (q == 0) return a;
-    else if constexpr (q == 1) return b;
-    else return c;
-}
-
-template <int u, int TJ, int TK, bool FWD, typename CT>
-__device__ __forceinline__ void ld_group_step(const WaveArgs<CT> &w, const Lane &L, int s0, int S, int wave, int lane,
-                                              int il0, double prev[5], double (*sx)[5][TJ * TK], LdSlot<CT> *slot,
-                                              LdSet<CT> &set0, LdSet<CT> &set1, LdSet<CT> &set2,
-                                              unsigned long long *nb0, unsigned long long *nb1,
-                                              unsigned long long *nc0, unsigned long long *nc1)
-{
-    static_assert(LDEPTH == 3, "three named register sets");
-    const int s = s0 + u;
-    if (s >= S) return;
-    __syncthreads();
-    if (wave == 0) {
-        if (u & 1)
-            wave_step_ld<TJ, TK, FWD, 1, CT>(w, L, s, prev, sx, slot[1], nb1, nb0, nc1, nc0);
-        else
-            wave_step_ld<TJ, TK, FWD, 0, CT>(w, L, s, prev, sx, slot[0], nb0, nb1, nc0, nc1);
-    } else if (s + 1 < S) {
-        constexpr int q = (u + 1) % LDEPTH;
-        LdSet<CT> &st = pick_set<q, CT>(set0, set1, set2);
-        ld_put(slot[(u + 1) & 1], st, wave, lane);
-        ld_issue<FWD, CT>(st, wave, w, L, il0 + s + 1 + LDEPTH);
-    }
-}
-
-template <int TJ, int TK, bool FWD, typename CT>
-__global__ void __launch_bounds__(TJ *TK + 128) k_lusgs_wave_ld(Geo g, const CT *__restrict__ coef,
-                                                               const double *__restrict__ dgn, C5 R, F5 X,
-                                                               double *xbuf, double *trash,
-                                                               const int *__restrict__ order, int *ticket, int *err,
-                                                               int max_spin)
-{
-    static_assert(TJ * TK == 64, "one computing wave per tile");
-    constexpr int U = 6;   // steps per unrolled group (ld_group_step<0..5>): LCM(2 LDS parities, LDEPTH sets)
-    static_assert(U % 2 == 0 && U % LDEPTH == 0, "group length");
-    __shared__ int s_t;
-    __shared__ double sx[2][5][TJ * TK];
-    __shared__ LdSlot<CT> slot[2];
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    Lane L;
-    L.tid = lane;   // every wave maps its lane to the same column
-    L.kl = lane % TK;
-    L.jl = lane / TK;
-    if (threadIdx.x == 0) s_t = atomicAdd(ticket, 1);
-    __syncthreads();
-    const int packed = order[s_t];
-    const int TJi = packed >> 16, TKi = packed & 0xffff;
-    const int jo = TJi * TJ + L.jl, ko = TKi * TK + L.kl;
-    L.col = (jo < g.ny) && (ko < g.nz);
-    const int joc = jo < g.ny ? jo : g.ny - 1, koc = ko < g.nz ? ko : g.nz - 1;
-    const int j = FWD ? 2 + joc : g.NY - joc;
-    const int k = FWD ? 2 + koc : g.NZ - koc;
-    L.c0 = ix(g, 0, j, k);
-    L.edge_out = (L.jl == TJ - 1) || (L.kl == TK - 1);
-    L.tline = ((long)blockIdx.x * 16) * 16;
-    WaveArgs<CT> w;
-    w.g = g;
-    w.coef = coef;
-    w.dgn = dgn;
-    w.R = R;
-    w.X = X;
-    w.xbuf = xbuf;
-    w.trash = trash;
-    w.err = err;
-    w.max_spin = max_spin;
-    w.sxs = (long)g.Ym * g.Zm;
-    w.d = FWD ? -1 : 1;
-    {
-        const long nkb = coef_nkb(g);
-        const int i0 = FWD ? 3 : g.gend;
-        w.dcol = -w.d * g.Ym * nkb * CBLK;
-        w.ddcol = -w.d * g.Ym * nkb * DBLK;
-        w.oA = -w.dcol;
-        w.oB = w.d * nkb * CBLK;
-        L.r0 = block_at(g, nkb, CBLK, i0, j, k);
-        L.d0 = block_at(g, nkb, DBLK, i0, j, k);
-        w.oC = block_at(g, nkb, CBLK, i0, j, k + (int)w.d) - L.r0;
-    }
-    const int S = g.nx + TJ + TK - 2;
-    const int il0 = -(L.jl + L.kl);   // local step of march step 0
-    // computing wave state
-    double prev[5];
-    unsigned long long nb0[5], nb1[5], nc0[5], nc1[5];
-    // loader state: LDEPTH register sets
-    LdSet<CT> set0, set1, set2;
-    if (wave == 0) {
-        const long c = cell_at<FWD>(w, L, 0);
-#pragma unroll
-        for (int v = 0; v < 5; v++) prev[v] = X.v[v][c + w.d * w.sxs];   // ghost plane (zero)
-        const long cs = cell_at<FWD>(w, L, il0);
-        if (L.jl == 0) issue5(xbuf, g.N, cs + w.d * g.Zm, nb0);
-        if (L.kl == 0) issue5(xbuf, g.N, cs + w.d, nc0);
-    } else {
-        ld_issue<FWD, CT>(set0, wave, w, L, il0);
-        ld_issue<FWD, CT>(set1, wave, w, L, il0 + 1);
-        ld_issue<FWD, CT>(set2, wave, w, L, il0 + 2);
-        ld_put(slot[0], set0, wave, lane);                       // step 0
-        ld_issue<FWD, CT>(set0, wave, w, L, il0 + LDEPTH);       // set 0 now holds step LDEPTH
-    }
-    // step s: one barrier, then the computing wave runs step s from slot
-    // s&1 while the loaders move step s+1 (register set (s+1) % LDEPTH)
-    // into slot (s+1)&1 and reuse that set for step s+1+LDEPTH
-    for (int s0 = 0; s0 < S; s0 += U) {
-        ld_group_step<0, TJ, TK, FWD, CT>(w, L, s0, S, wave, lane, il0, prev, sx, slot, set0, set1, set2, nb0, nb1, nc0, nc1);
-        ld_group_step<1, TJ, TK, FWD, CT>(w, L, s0, S, wave, lane, il0, prev, sx, slot, set0, set1, set2, nb0, nb1, nc0, nc1);
-        ld_group_step<2, TJ, TK, FWD, CT>(w, L, s0, S, wave, lane, il0, prev, sx, slot, set0, set1, set2, nb0, nb1, nc0, nc1);
-        ld_group_step<3, TJ, TK, FWD, CT>(w, L, s0, S, wave, lane, il0, prev, sx, slot, set0, set1, set2, nb0, nb1, nc0, nc1);
-        ld_group_step<4, TJ, TK, FWD, CT>(w, L, s0, S, wave, lane, il0, prev, sx, slot, set0, set1, set2, nb0, nb1, nc0, nc1);
-        ld_group_step<5, TJ, TK, FWD, CT>(w, L, s0, S, wave, lane, il0, prev, sx, slot, set0, set1, set2, nb0, nb1, nc0, nc1);
     }
 }
 
@@ -1971,6 +1653,459 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_march(Geo g, March M, const CT
     if (late) atomicOr(err, 1);
 }
 
+// ==================================================================
+// Schedule 4: the march without coefficient records.
+//
+// The sweep reads, per cell and step, the cell's state U_, its Jacobian and
+// 9 metrics and its right-hand side (MR forward, NL backward), all held in
+// march order (J and the metrics gathered once, they do not change), and
+// computes the cell's split coefficients and diagonal itself, one step
+// before the step that needs them, so that this arithmetic runs while the
+// step's exchange entries are on their way.  What travels between cells
+// are the products of an increment with the split block its downstream
+// neighbour needs along each direction -- the per-direction sums of
+// Dual_time_stepping.cpp:1172-1232 (forward: plus blocks, negated) and
+// 1529-1590 (backward: minus diagonal entries) -- already divided by the
+// grid spacing: along i in a register of the same lane, along j and k
+// through LDS inside a tile and through two exchange buffers (j products,
+// k products) across tile edges.  Compared with the plane order (which
+// follows the reference's expressions literally) the divisions by U1, rho,
+// the grid spacings and the diagonal's denominators are multiplications by
+// reciprocals: results agree to rounding.
+//
+// March-ordered arrays hold 16-byte pairs [row][pair][lane]: the geometry
+// (J, m0) (m1, m2) (m3, m4) (m5, m6) (m7, m8), and five-value fields (U_,
+// MR, NL, NR) as (v0, v1) (v2, v3) (v4, -).
+constexpr int G4P = 5, F4P = 3;
+constexpr long G4ROWB = (long)G4P * 64 * 16, F4ROWB = (long)F4P * 64 * 16;
+
+long march4_geo_pairs(const Geo &g) { return march_rows(g) * G4P * 64; }
+long march4_field_pairs(const Geo &g) { return march_rows(g) * F4P * 64; }
+
+// U_ and a right-hand side R (when Rm) and the geometry (when Gm) into
+// march order; the slots of cells only (the others keep their fill)
+__global__ void __launch_bounds__(256) k_gather4(Geo g, March M, C5 U_, C5 R, const double *__restrict__ J, M9 met,
+                                                 double2 *__restrict__ Um, double2 *__restrict__ Rm,
+                                                 double2 *__restrict__ Gm)
+{
+    const MGather q = mgather_at(g, M);
+    if (!q.ok) return;
+    const long c = ix(g, 3 + q.il, 2 + q.jo, 2 + q.ko);
+    const long row = march_row(M, q.T, q.t);
+    const long o = row * F4P * 64 + q.l;
+    Um[o] = make_double2(U_.v[0][c], U_.v[1][c]);
+    Um[o + 64] = make_double2(U_.v[2][c], U_.v[3][c]);
+    Um[o + 128] = make_double2(U_.v[4][c], 0.0);
+    if (Rm) {
+        Rm[o] = make_double2(R.v[0][c], R.v[1][c]);
+        Rm[o + 64] = make_double2(R.v[2][c], R.v[3][c]);
+        Rm[o + 128] = make_double2(R.v[4][c], 0.0);
+    }
+    if (Gm) {
+        const long og = row * G4P * 64 + q.l;
+        Gm[og] = make_double2(J[c], met.m[0][c]);
+        Gm[og + 64] = make_double2(met.m[1][c], met.m[2][c]);
+        Gm[og + 128] = make_double2(met.m[3][c], met.m[4][c]);
+        Gm[og + 192] = make_double2(met.m[5][c], met.m[6][c]);
+        Gm[og + 256] = make_double2(met.m[7][c], met.m[8][c]);
+    }
+}
+
+// march-ordered increments back into NR (and NL when NL.v[0] is not null)
+__global__ void __launch_bounds__(256) k_scatter4(Geo g, March M, const double2 *__restrict__ Xm, F5 NL, F5 NR)
+{
+    const bool nl = NL.v[0] != nullptr;
+    const MGather q = mgather_at(g, M);
+    if (!q.ok) return;
+    const long c = ix(g, 3 + q.il, 2 + q.jo, 2 + q.ko);
+    const long o = march_row(M, q.T, q.t) * F4P * 64 + q.l;
+    const double2 a = Xm[o], b = Xm[o + 64], d = Xm[o + 128];
+    const double x[5] = {a.x, a.y, b.x, b.y, d.x};
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        if (nl) NL.v[v][c] = x[v];
+        NR.v[v][c] = x[v];
+    }
+}
+
+// the loads of one step and the coefficients computed from them
+struct Ld4 { double2 g[G4P], u[F4P], r[F4P]; };
+// a[d]: g11, p12, p13, p14, p21, g22, p31, p41, p51, p52, p53, p54 of
+// direction d divided by its spacing; g11, g22 = p11, p22 forward, m11, m22
+// backward
+struct Cf4 { double a[3][12]; double d11, d22, d51; double r[5]; };
+
+struct F4Lane {
+    int tid, jl, kl;
+    bool col;                 // the lane's (j, k) column lies in the slab
+    bool hasb, hasc;          // in-tile upstream neighbour along j / k
+    bool need_b, need_c;      // upstream neighbour across a tile edge (exchange)
+    bool out_b, out_c;        // downstream neighbour across a tile edge (publish)
+    int ib, ic;               // LDS slots of the in-tile upstream neighbours
+    long x0;                  // 5 x field index of (i = 3 - jl - kl, j, k)
+};
+
+struct M4Args {
+    Geo g;
+    int S;
+    double K, e, deltaT;
+    double h[3];              // 1 / dxi, 1 / deta, 1 / dzeta
+    __amdgpu_buffer_rsrc_t grow, urow, rrow;   // rows from (T, t = 0)
+    char *xrow0;              // output row of (T, t = 0)
+    double *xbB, *xbC;        // exchange: j products, k products (5 per cell)
+    double *trash;            // this wave's scratch line
+    int max_spin;
+    long sxs5, dnb5, dnc5;    // exchange strides (times 5): one i plane, the upstream j and k cells
+};
+
+__device__ __forceinline__ double2 ld_pair(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
+{
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
+}
+
+template <bool FWD>
+__device__ __forceinline__ int m4_t(const M4Args &w, int s)
+{
+    const int sc = s < w.S ? s : w.S - 1;   // past the end: the last step's row again
+    return FWD ? sc : w.S - 1 - sc;
+}
+
+template <bool FWD>
+__device__ __forceinline__ void load4(Ld4 &L, const M4Args &w, unsigned vo, int s)
+{
+    const int t = m4_t<FWD>(w, s);
+    const unsigned sg = (unsigned)(t * G4ROWB), sf = (unsigned)(t * F4ROWB);
+#pragma unroll
+    for (int p = 0; p < G4P; p++) L.g[p] = ld_pair(w.grow, vo, sg + p * 1024);
+#pragma unroll
+    for (int p = 0; p < F4P; p++) L.u[p] = ld_pair(w.urow, vo, sf + p * 1024);
+#pragma unroll
+    for (int p = 0; p < F4P; p++) L.r[p] = ld_pair(w.rrow, vo, sf + p * 1024);
+}
+
+// primitives (Dual_time_stepping.cpp:759-769), the split coefficients of the
+// three directions (1054-1170, 1384-1524) and the diagonal (856-895) of one
+// cell; the directions' spectral radii are shared by the split and the
+// diagonal
+template <bool FWD>
+__device__ __forceinline__ void coef_cell(double a[3][12], double &d11, double &d22, double &d51, double J,
+                                          const double m[9], const double U[5], const M4Args &w)
+{
+    const double K = w.K;
+    const double iU1 = 1.0 / U[0];
+    const double rho = U[0] * J;
+    const double u = U[1] * iU1, v = U[2] * iU1, ww = U[3] * iU1;
+    const double VV = u * u + v * v + ww * ww;
+    const double P = (U[4] * J - 0.5 * rho * VV) * (K - 1);
+    const double irho = 1.0 / rho;
+    const double C = K * P * irho;
+    const double T = P * irho;
+    const double rr = VV / C;
+    const double b = rr > w.e ? rr : w.e;
+    double lam = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double mx = m[3 * d], my = m[3 * d + 1], mz = m[3 * d + 2];
+        const double Un = mx * u + my * v + mz * ww;
+        const double S = sqrt(Un * Un * (b - 1) * (b - 1) + 4 * b * C * (mx * mx + my * my + mz * mz));
+        const double l = ((b + 1) * fabs(Un) + S) * 0.5;
+        const double h = w.h[d];
+        lam += l * h;
+        const double dd = fabs(l);
+        a[d][0] = FWD ? 0.5 * (b * Un + dd) * h : 0.5 * (b * Un - dd) * h;
+        a[d][5] = FWD ? 0.5 * (Un + dd) * h : 0.5 * (Un - dd) * h;
+        const double pk = 0.5 * P * K * b * h;
+        a[d][1] = pk * mx;
+        a[d][2] = pk * my;
+        a[d][3] = pk * mz;
+        const double ir = 0.5 * irho * h;
+        a[d][4] = ir * mx;
+        a[d][6] = ir * my;
+        a[d][7] = ir * mz;
+        a[d][8] = Un * (K - 1) * (b - 1) * (ir / K);
+        const double bt = 0.5 * b * T * (K - 1) * h;
+        a[d][9] = bt * mx;
+        a[d][10] = bt * my;
+        a[d][11] = bt * mz;
+    }
+    const double dt = w.deltaT;
+    const double rA = 1.0 / (3 + 2 * lam * dt), rB = 1.0 / (3 * b + 2 * lam * dt);
+    d11 = 2 * dt * rB;
+    d22 = 2 * dt * rA;
+    d51 = -(6 * (K - 1) / K) * (b - 1) * dt * rA * rB * irho;
+}
+
+template <bool FWD>
+__device__ __forceinline__ void coef4(Cf4 &C, const Ld4 &L, const M4Args &w)
+{
+    const double m[9] = {L.g[0].y, L.g[1].x, L.g[1].y, L.g[2].x, L.g[2].y, L.g[3].x, L.g[3].y, L.g[4].x, L.g[4].y};
+    const double U[5] = {L.u[0].x, L.u[0].y, L.u[1].x, L.u[1].y, L.u[2].x};
+    coef_cell<FWD>(C.a, C.d11, C.d22, C.d51, L.g[0].x, m, U, w);
+    C.r[0] = L.r[0].x;
+    C.r[1] = L.r[0].y;
+    C.r[2] = L.r[1].x;
+    C.r[3] = L.r[1].y;
+    C.r[4] = L.r[2].x;
+}
+
+// the product of increment x with one direction's block: forward the
+// negated plus block (a lower neighbour's contribution), backward the block
+// with the minus diagonal entries (an upper neighbour's)
+template <bool FWD>
+__device__ __forceinline__ void prod4(const double a[12], const double x[5], double y[5])
+{
+    if (FWD) {
+        y[0] = -a[0] * x[0] - a[1] * x[1] - a[2] * x[2] - a[3] * x[3];
+        y[1] = -a[4] * x[0] - a[5] * x[1];
+        y[2] = -a[6] * x[0] - a[5] * x[2];
+        y[3] = -a[7] * x[0] - a[5] * x[3];
+        y[4] = -a[8] * x[0] - a[9] * x[1] - a[10] * x[2] - a[11] * x[3] - a[5] * x[4];
+    } else {
+        y[0] = a[0] * x[0] + a[1] * x[1] + a[2] * x[2] + a[3] * x[3];
+        y[1] = a[4] * x[0] + a[5] * x[1];
+        y[2] = a[6] * x[0] + a[5] * x[2];
+        y[3] = a[7] * x[0] + a[5] * x[3];
+        y[4] = a[8] * x[0] + a[9] * x[1] + a[10] * x[2] + a[11] * x[3] + a[5] * x[4];
+    }
+}
+
+// one march step s; C holds step s's coefficients, Ln the loads of step
+// s + 1 (issued a step ago), Lf receives the loads of step s + 2; C leaves
+// with step s + 1's coefficients
+template <int TJ, int TK, bool FWD, int PAR>
+__device__ __forceinline__ void step4(const M4Args &w, const F4Lane &L, int s, Cf4 &C, const Ld4 &Ln, Ld4 &Lf,
+                                      double sa[5], double (*sB)[5][TJ * TK], double (*sC)[5][TJ * TK],
+                                      unsigned long long nb[5], unsigned long long nc[5], bool &late)
+{
+    const int t = FWD ? s : w.S - 1 - s;
+    const int il = t - L.jl - L.kl;
+    const bool act = L.col && il >= 0 && il < w.g.nx;
+    __syncthreads();
+    double yb[5], yc[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        yb[v] = L.hasb ? sB[PAR ^ 1][v][L.ib] : 0.0;
+        yc[v] = L.hasc ? sC[PAR ^ 1][v][L.ic] : 0.0;
+    }
+    const long xc5 = L.x0 + (long)t * w.sxs5;   // dereferenced only where act
+    settle5m(w.xbB + xc5 + w.dnb5, act && L.need_b, nb, yb, w.max_spin, late);
+    settle5m(w.xbC + xc5 + w.dnc5, act && L.need_c, nc, yc, w.max_spin, late);
+    double x[5];
+    if (FWD) {
+        const double LL0 = sa[0] + yb[0] + yc[0], LL1 = sa[1] + yb[1] + yc[1], LL2 = sa[2] + yb[2] + yc[2];
+        const double LL3 = sa[3] + yb[3] + yc[3], LL4 = sa[4] + yb[4] + yc[4];
+        x[0] = C.d11 * (C.r[0] - LL0);
+        x[1] = C.d22 * (C.r[1] - LL1);
+        x[2] = C.d22 * (C.r[2] - LL2);
+        x[3] = C.d22 * (C.r[3] - LL3);
+        x[4] = C.d51 * (C.r[0] - LL0) + C.d22 * (C.r[4] - LL4);
+    } else {
+        const double UU0 = sa[0] + yb[0] + yc[0], UU1 = sa[1] + yb[1] + yc[1], UU2 = sa[2] + yb[2] + yc[2];
+        const double UU3 = sa[3] + yb[3] + yc[3], UU4 = sa[4] + yb[4] + yc[4];
+        x[0] = C.r[0] - C.d11 * UU0;
+        x[1] = C.r[1] - C.d22 * UU1;
+        x[2] = C.r[2] - C.d22 * UU2;
+        x[3] = C.r[3] - C.d22 * UU3;
+        x[4] = C.r[4] - (C.d51 * UU0 + C.d22 * UU4);
+    }
+    double pa[5], pb[5], pc[5];
+    prod4<FWD>(C.a[0], x, pa);
+    prod4<FWD>(C.a[1], x, pb);
+    prod4<FWD>(C.a[2], x, pc);
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        // a column outside the slab stands for ghost cells: zero products
+        sB[PAR][v][L.tid] = L.col ? pb[v] : 0.0;
+        sC[PAR][v][L.tid] = L.col ? pc[v] : 0.0;
+        sa[v] = act ? pa[v] : sa[v];
+    }
+    // every lane stores its slot (a slot with no cell is never read as a
+    // value) and ten exchange values (lanes with nothing to publish write
+    // the wave's scratch line), so no branch splits the stores
+    double2 *xo = reinterpret_cast<double2 *>(w.xrow0 + (long)t * F4ROWB) + L.tid;
+    xo[0] = make_double2(x[0], x[1]);
+    xo[64] = make_double2(x[2], x[3]);
+    xo[128] = make_double2(x[4], 0.0);
+    double *eb = act && L.out_b ? w.xbB + xc5 : w.trash;
+    double *ec = act && L.out_c ? w.xbC + xc5 : w.trash + 8;
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        __hip_atomic_store(reinterpret_cast<unsigned long long *>(eb + v), (unsigned long long)__double_as_longlong(pb[v]),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(reinterpret_cast<unsigned long long *>(ec + v), (unsigned long long)__double_as_longlong(pc[v]),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    {   // the next step's exchange polls, the loads two steps ahead, then
+        // the next step's coefficients while both are in flight
+        const int tn = FWD ? s + 1 : w.S - 2 - s;
+        const int iln = tn - L.jl - L.kl;
+        const bool actn = s + 1 < w.S && L.col && iln >= 0 && iln < w.g.nx;
+        const long xn5 = L.x0 + (long)tn * w.sxs5;
+        if (actn && L.need_b) issue5m(w.xbB + xn5 + w.dnb5, nb);
+        if (actn && L.need_c) issue5m(w.xbC + xn5 + w.dnc5, nc);
+    }
+    load4<FWD>(Lf, w, (unsigned)L.tid * 16u, s + 2);
+    coef4<FWD>(C, Ln, w);
+}
+
+// the x-direction product of the ghost cell upstream in i (its increment Xg
+// from the field: zero on a periodic or end slab), which starts each column
+template <bool FWD>
+__device__ __forceinline__ void ghost_product4(const M4Args &w, C5 U_, const double *__restrict__ J, M9 met, C5 Xg,
+                                               long cg, double y[5])
+{
+    const double m[9] = {met.m[0][cg], met.m[1][cg], met.m[2][cg], 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double U[5] = {U_.v[0][cg], U_.v[1][cg], U_.v[2][cg], U_.v[3][cg], U_.v[4][cg]};
+    double a[3][12], d11, d22, d51;
+    coef_cell<FWD>(a, d11, d22, d51, J[cg], m, U, w);
+    const double x[5] = {Xg.v[0][cg], Xg.v[1][cg], Xg.v[2][cg], Xg.v[3][cg], Xg.v[4][cg]};
+    prod4<FWD>(a[0], x, y);
+}
+
+template <int TJ, int TK, bool FWD>
+__global__ void __launch_bounds__(TJ *TK) k_lusgs_fused(Geo g, March M, double K, double e, double deltaT, double hx,
+                                                      double hy, double hz, const double2 *__restrict__ Gm,
+                                                      const double2 *__restrict__ Um, const double2 *Rm, double2 *Xm,
+                                                      C5 U_, const double *__restrict__ J, M9 met, C5 Xg,
+                                                      double *xbB, double *xbC, double *trash,
+                                                      const int *__restrict__ order, int ntiles, int *ticket, int *err,
+                                                      int max_spin, int slack)
+{
+    static_assert(TJ * TK == 64, "one wave per tile");
+    __shared__ int s_t;
+    __shared__ double sB[2][5][TJ * TK], sC[2][5][TJ * TK];
+    F4Lane L;
+    L.tid = threadIdx.x;
+    L.kl = L.tid % TK;
+    L.jl = L.tid / TK;
+    if (L.tid == 0) s_t = atomicAdd(ticket, 1);
+    __syncthreads();
+    // forward: anti-diagonals ascending; backward: the same list reversed
+    const int tk = __builtin_amdgcn_readfirstlane(s_t);
+    const int packed = order[FWD ? tk : ntiles - 1 - tk];
+    const int TJi = packed >> 16, TKi = packed & 0xffff;
+    const int T = TJi * M.ntk + TKi;
+    const int jo = TJi * TJ + L.jl, ko = TKi * TK + L.kl;
+    L.col = (jo < g.ny) && (ko < g.nz);
+    const int joc = jo < g.ny ? jo : g.ny - 1, koc = ko < g.nz ? ko : g.nz - 1;
+    const long c0 = ix(g, 0, 2 + joc, 2 + koc);
+    if (FWD) {
+        L.hasb = L.jl >= 1;
+        L.hasc = L.kl >= 1;
+        L.need_b = L.jl == 0 && TJi >= 1;
+        L.need_c = L.kl == 0 && TKi >= 1;
+        L.out_b = L.jl == TJ - 1;
+        L.out_c = L.kl == TK - 1;
+        L.ib = L.tid - TK;
+        L.ic = L.tid - 1;
+    } else {
+        L.hasb = L.jl <= TJ - 2;
+        L.hasc = L.kl <= TK - 2;
+        L.need_b = L.jl == TJ - 1 && TJi + 1 < M.ntj;
+        L.need_c = L.kl == TK - 1 && TKi + 1 < M.ntk;
+        L.out_b = L.jl == 0;
+        L.out_c = L.kl == 0;
+        L.ib = L.tid + TK;
+        L.ic = L.tid + 1;
+    }
+    M4Args w;
+    w.g = g;
+    w.S = M.S;
+    w.K = K;
+    w.e = e;
+    w.deltaT = deltaT;
+    w.h[0] = hx;
+    w.h[1] = hy;
+    w.h[2] = hz;
+    constexpr int F = 0x00020000;   // raw buffer, 32-bit data format
+    const long row0 = march_row(M, T, 0);
+    w.grow = __builtin_amdgcn_make_buffer_rsrc((void *)(Gm + row0 * G4P * 64), (short)0, -1, F);
+    w.urow = __builtin_amdgcn_make_buffer_rsrc((void *)(Um + row0 * F4P * 64), (short)0, -1, F);
+    w.rrow = __builtin_amdgcn_make_buffer_rsrc((void *)(Rm + row0 * F4P * 64), (short)0, -1, F);
+    w.xrow0 = reinterpret_cast<char *>(Xm + row0 * F4P * 64);
+    w.xbB = xbB;
+    w.xbC = xbC;
+    w.trash = trash + (long)blockIdx.x * 256;
+    w.max_spin = max_spin;
+    const long sxs = (long)g.Ym * g.Zm;
+    const int d = FWD ? -1 : 1;
+    w.sxs5 = 5 * sxs;
+    w.dnb5 = 5L * d * g.Zm;
+    w.dnc5 = 5L * d;
+    L.x0 = 5 * (c0 + (long)(3 - L.jl - L.kl) * sxs);
+    const int S = M.S;
+    bool late = false;
+    unsigned long long nb[5], nc[5];
+    // start `slack` march steps later than the dependencies require (as the
+    // schedule-3 march): the upstream tiles have then published the entries
+    // of the first steps' polls
+    if (slack > 0 && slack < S) {
+        const int ts = FWD ? slack : S - 1 - slack;
+        const int ils = ts - L.jl - L.kl;
+        const bool acts = L.col && ils >= 0 && ils < g.nx;
+        const long xs5 = L.x0 + (long)ts * w.sxs5;
+        unsigned long long sb[5] = {0, 0, 0, 0, 0}, sc[5] = {0, 0, 0, 0, 0};
+        double tmp[5] = {0, 0, 0, 0, 0};
+        if (acts && L.need_b) issue5m(xbB + xs5 + w.dnb5, sb);
+        if (acts && L.need_c) issue5m(xbC + xs5 + w.dnc5, sc);
+        settle5m(xbB + xs5 + w.dnb5, acts && L.need_b, sb, tmp, max_spin, late);
+        settle5m(xbC + xs5 + w.dnc5, acts && L.need_c, sc, tmp, max_spin, late);
+    }
+    double sa[5] = {0, 0, 0, 0, 0};
+    if (L.col) ghost_product4<FWD>(w, U_, J, met, Xg, c0 + (long)(FWD ? 2 : g.gend + 1) * sxs, sa);
+    Ld4 La, Lb;
+    Cf4 C;
+    load4<FWD>(La, w, (unsigned)L.tid * 16u, 0);
+    load4<FWD>(Lb, w, (unsigned)L.tid * 16u, 1);
+    {
+        const int t0 = FWD ? 0 : S - 1;
+        const int il0 = t0 - L.jl - L.kl;
+        const bool act0 = L.col && il0 >= 0 && il0 < g.nx;
+        const long x5 = L.x0 + (long)t0 * w.sxs5;
+        if (act0 && L.need_b) issue5m(xbB + x5 + w.dnb5, nb);
+        if (act0 && L.need_c) issue5m(xbC + x5 + w.dnc5, nc);
+    }
+    coef4<FWD>(C, La, w);
+    for (int s = 0; s < S; s += 2) {
+        step4<TJ, TK, FWD, 0>(w, L, s, C, Lb, La, sa, sB, sC, nb, nc, late);
+        if (s + 1 < S) step4<TJ, TK, FWD, 1>(w, L, s + 1, C, La, Lb, sa, sB, sC, nb, nc, late);
+    }
+    if (late) atomicOr(err, 1);
+}
+
+void launch_gather4(const Geo &g, C5 U_, C5 R, const double *J, M9 met, void *Um, void *Rm, void *Gm, hipStream_t s)
+{
+    const March M = march_of(g);
+    hipLaunchKernelGGL(k_gather4, mgather_grid(M), dim3(256), 0, s, g, M, U_, R, J, met, static_cast<double2 *>(Um),
+                       static_cast<double2 *>(Rm), static_cast<double2 *>(Gm));
+}
+
+void launch_scatter4(const Geo &g, const void *Xm, F5 NL, F5 NR, hipStream_t s)
+{
+    const March M = march_of(g);
+    hipLaunchKernelGGL(k_scatter4, mgather_grid(M), dim3(256), 0, s, g, M, static_cast<const double2 *>(Xm), NL, NR);
+}
+
+void launch_lusgs_fused(const Geo &g, const Phys &p, double deltaT, double e, int forward, const void *Gm,
+                        const void *Um, const void *Rm, void *Xm, C5 U_, const double *J, M9 met, C5 Xg, double *xbB,
+                        double *xbC, double *trash, const int *order, int ntiles, int *counters, int max_spin,
+                        int slack, hipStream_t s)
+{
+    const March M = march_of(g);
+    const dim3 gr(ntiles), bl(MARCH_TJ * MARCH_TK);
+    const double2 *G = static_cast<const double2 *>(Gm), *Uu = static_cast<const double2 *>(Um),
+                  *R = static_cast<const double2 *>(Rm);
+    double2 *X = static_cast<double2 *>(Xm);
+    if (forward)
+        hipLaunchKernelGGL((k_lusgs_fused<MARCH_TJ, MARCH_TK, true>), gr, bl, 0, s, g, M, p.K, e, deltaT, 1.0 / p.dxi,
+                           1.0 / p.deta, 1.0 / p.dzeta, G, Uu, R, X, U_, J, met, Xg, xbB, xbC, trash, order, ntiles,
+                           counters, counters + 1, max_spin, slack);
+    else
+        hipLaunchKernelGGL((k_lusgs_fused<MARCH_TJ, MARCH_TK, false>), gr, bl, 0, s, g, M, p.K, e, deltaT, 1.0 / p.dxi,
+                           1.0 / p.deta, 1.0 / p.dzeta, G, Uu, R, X, U_, J, met, Xg, xbB, xbC, trash, order, ntiles,
+                           counters, counters + 1, max_spin, slack);
+}
+
 // the launchers take the allocations (march_*_pairs / march_field_doubles)
 void launch_split_march(const Geo &g, const Phys &p, double deltaT, double e, C5 U_, const double *J, M9 met,
                         C5 MR, void *coefM, double *dgnM, double *mrM, int fp32, hipStream_t s)
@@ -2050,14 +2185,15 @@ void launch_split_coef(const Geo &g, const Phys &p, double deltaT, double e, C5 
                            static_cast<double *>(coef), dgn);
 }
 
-void launch_march_prep(const Geo &g, int forward, double *xbuf, int *counters, hipStream_t s)
+void launch_march_prep(const Geo &g, int forward, double *xbuf, int *counters, hipStream_t s, double *xcol)
 {
     const int rj = forward ? MARCH_TJ - 1 : 0, rk = forward ? MARCH_TK - 1 : 0;
     const long nrow = rj < g.ny ? (g.ny - 1 - rj) / MARCH_TJ + 1 : 0;
     const long ncol = rk < g.nz ? (g.nz - 1 - rk) / MARCH_TK + 1 : 0;
     const long n = std::max(std::max(5L * g.nz * nrow, 5L * g.ny * ncol), 1L);
     const dim3 gr((unsigned)((n + 255) / 256), 2, g.nx);
-    hipLaunchKernelGGL(k_fill_sentinel_edges, gr, dim3(256), 0, s, g, xbuf, MARCH_TJ, MARCH_TK, rj, rk, counters);
+    hipLaunchKernelGGL(k_fill_sentinel_edges, gr, dim3(256), 0, s, g, xbuf, xcol ? xcol : xbuf, MARCH_TJ, MARCH_TK,
+                       rj, rk, counters);
 }
 
 void launch_lusgs_wave_prep(const Geo &g, double *xbuf, int *counters, hipStream_t s)
@@ -2071,28 +2207,12 @@ void launch_lusgs_wave_prep(const Geo &g, double *xbuf, int *counters, hipStream
 
 void launch_lusgs_wave(const Geo &g, int forward, const void *coef, const double *dgn, int fp32, C5 R, F5 X,
                        double *xbuf, double *trash, const int *order, int ntiles, int *counters, int max_spin,
-                       int loaders, hipStream_t s)
+                       hipStream_t s)
 {
     // counters[0]: ticket, counters[1]: error flag (cleared by the caller)
     const dim3 gr(ntiles), bl(WAVE_TJ * WAVE_TK);
     const float *cf = static_cast<const float *>(coef);
     const double *cd = static_cast<const double *>(coef);
-    if (loaders) {
-        const dim3 bl3(WAVE_TJ * WAVE_TK + 128);
-        if (forward && fp32)
-            hipLaunchKernelGGL((k_lusgs_wave_ld<WAVE_TJ, WAVE_TK, true, float>), gr, bl3, 0, s, g, cf, dgn, R, X, xbuf,
-                               trash, order, counters, counters + 1, max_spin);
-        else if (forward)
-            hipLaunchKernelGGL((k_lusgs_wave_ld<WAVE_TJ, WAVE_TK, true, double>), gr, bl3, 0, s, g, cd, dgn, R, X,
-                               xbuf, trash, order, counters, counters + 1, max_spin);
-        else if (fp32)
-            hipLaunchKernelGGL((k_lusgs_wave_ld<WAVE_TJ, WAVE_TK, false, float>), gr, bl3, 0, s, g, cf, dgn, R, X,
-                               xbuf, trash, order, counters, counters + 1, max_spin);
-        else
-            hipLaunchKernelGGL((k_lusgs_wave_ld<WAVE_TJ, WAVE_TK, false, double>), gr, bl3, 0, s, g, cd, dgn, R, X,
-                               xbuf, trash, order, counters, counters + 1, max_spin);
-        return;
-    }
     if (forward && fp32)
         hipLaunchKernelGGL((k_lusgs_wave<WAVE_TJ, WAVE_TK, true, float>), gr, bl, 0, s, g, cf, dgn, R, X, xbuf, trash,
                            order, counters, counters + 1, max_spin);

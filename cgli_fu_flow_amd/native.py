@@ -370,12 +370,12 @@ class Context:
 
     # -- timing (HIP events around each launch class)
     TIMING_CLASSES = ["flux_x", "flux_y", "flux_z", "viscous", "rhs_diag", "lusgs_fwd", "lusgs_bwd",
-                      "update_norm", "split_coef", "filter", "march_scatter"]
+                      "update_norm", "split_coef", "filter", "march_scatter", "march_gather"]
 
     def set_lusgs_mode(self, mode):
-        """0: one launch per hyperplane, 1: tiled wavefront, 2: the wavefront
-        with loader waves, 3: the wavefront on march-ordered records
-        (default)."""
+        """0: one launch per hyperplane, 1: tiled wavefront, 3: the wavefront
+        on march-ordered coefficient records, 4: the wavefront on
+        march-ordered state computing its own coefficients (default)."""
         _check(lib().cgli_set_lusgs_mode(self.h, mode), "set_lusgs_mode")
 
     def set_spin_limit(self, n):

@@ -174,13 +174,16 @@ int cgli_graph_launch(cgli_graph *graph);
 int cgli_graph_destroy(cgli_graph *graph);
 
 /* LU-SGS schedule: 0 = one launch per i+j+k hyperplane (the reference's
- * order), 1 = tiled wavefront in one launch per sweep, 2 = the
- * same wavefront with two loader waves per tile streaming the coefficients
- * through LDS, three steps ahead, 3 = the wavefront with its coefficient
- * records, right-hand sides and increments stored in the order the march
- * reads them (one contiguous block per tile and step; the default).  The wavefronts agree with each other bit for bit and
- * with the plane order to rounding.  The wavefront's bounded waits give up after
- * max_spin polls each (then CGLI_E_TIMEOUT). */
+ * order), 1 = tiled wavefront in one launch per sweep on precomputed
+ * coefficient records, 3 = the wavefront with its coefficient records,
+ * right-hand sides and increments stored in the order the march reads them
+ * (one contiguous block per tile and step), 4 = the wavefront on the
+ * march-ordered state, each cell computing its own split coefficients and
+ * diagonal and passing on the products of its increment with them (no
+ * coefficient records; the default).  Schedules 1 and 3 agree with each
+ * other bit for bit, and all with the plane order to rounding.  The
+ * wavefronts' bounded waits give up after max_spin polls each (then
+ * CGLI_E_TIMEOUT).  Schedule 2 (retired) is CGLI_E_ARG. */
 int cgli_set_lusgs_mode(cgli_ctx *ctx, int mode);
 int cgli_set_spin_limit(cgli_ctx *ctx, int max_spin);
 /* DPLUSGS.cpp:18-1238: Jacobi (DP-LUR) variant with `sweeps` sweeps (the
@@ -356,8 +359,8 @@ int cgli_net_timing(cgli_net *net, double *predict_ms, long *n_predict, double *
 /* Average device time (ms) of the last `n` launches of a kernel class
  * recorded since cgli_timing_reset; classes: 0 flux_x 1 flux_y 2 flux_z
  * 3 viscous 4 rhs_diag 5 lusgs_fwd 6 lusgs_bwd 7 update_norm 8 split_coef
- * 9 filter 10 march_scatter (LU-SGS schedule 3: increments back to their
- * fields). */
+ * 9 filter 10 march_scatter (LU-SGS schedules 3 and 4: increments back to
+ * their fields) 11 march_gather (schedule 4: U_ and MR into march order). */
 int cgli_timing_enable(cgli_ctx *ctx, int on);
 /* classes timed while timing is on: bit c of mask for class c (default all).
  * Each timed launch costs two event records on the stream, so a run that

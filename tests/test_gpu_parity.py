@@ -277,17 +277,18 @@ def test_missing_input_is_an_error(nv, orc):
 
 @pytest.mark.parametrize("name", [n for n in sorted(CASES) if CASES[n][3] == 1])
 def test_wavefront_and_plane_schedules_agree(nv, orc, name):
-    """The tiled wavefront visits cells in another topological order of the
-    same dependencies as the reference's plane order, with the grid-spacing
-    division folded into its coefficients: results agree to rounding.  The
-    wavefront without loader waves (mode 1), with them (mode 2) and on
-    march-ordered records (mode 3) do the same arithmetic on the same
-    values: identical bits."""
+    """The tiled wavefronts visit cells in another topological order of the
+    same dependencies as the reference's plane order: results agree to
+    rounding.  The wavefront on coefficient records in field blocks (mode 1)
+    and in march order (mode 3) do the same arithmetic on the same values:
+    identical bits.  The march that computes its own coefficients (mode 4,
+    the default) multiplies by reciprocals where the others divide: it
+    agrees with both to rounding."""
     print("case", name, flush=True)
     case = make_case(name)
     f = make_inputs(orc, case.slab, SEED + 6)
     outs = []
-    for mode in (0, 1, 2, 3):
+    for mode in (0, 1, 3, 4):
         with open_ctx(nv, orc, case) as ctx:
             ctx.set_lusgs_mode(mode)
             upload_inputs(ctx, f)
@@ -299,14 +300,25 @@ def test_wavefront_and_plane_schedules_agree(nv, orc, name):
             outs.append((er, ctx.get_many(nv.NL, 5), ctx.get_many(nv.NR, 5), ctx.get_many(nv.U_, 5)))
     reg = regions(case.slab)["cell"]
     for v in range(5):
-        assert abs(outs[0][0][v] - outs[1][0][v]) <= 1e-13 * abs(outs[0][0][v])
-        assert outs[1][0][v] == outs[2][0][v] == outs[3][0][v]
+        assert outs[1][0][v] == outs[2][0][v]
+        for m in (1, 3):
+            assert abs(outs[0][0][v] - outs[m][0][v]) <= 1e-13 * abs(outs[0][0][v]), (m, v)
         for q in (1, 2, 3):
-            a, b = outs[0][q][v][reg], outs[1][q][v][reg]
-            assert np.all(np.isfinite(b))
-            assert rel(b, a) <= 1e-13, (q, v, rel(b, a))
+            a = outs[0][q][v][reg]
+            for m in (1, 3):
+                b = outs[m][q][v][reg]
+                assert np.all(np.isfinite(b))
+                assert rel(b, a) <= 1e-13, (m, q, v, rel(b, a))
             assert np.array_equal(outs[1][q][v], outs[2][q][v])
-            assert np.array_equal(outs[1][q][v], outs[3][q][v]), (q, v, rel(outs[3][q][v][reg], b))
+
+
+def test_unknown_schedule_is_an_error(nv, orc):
+    case = make_case("ref_8x10x6_both_ends")
+    with open_ctx(nv, orc, case) as ctx:
+        for mode in (-1, 2, 5):
+            with pytest.raises(nv.CgliError) as ei:
+                ctx.set_lusgs_mode(mode)
+            assert ei.value.code == -1
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -446,7 +458,7 @@ def test_mixed_precision_iteration_tracks_the_fp64_oracle(nv, orc, name):
     case.overrides = {"sponge": 0}
     want = run_chain("orc", orc, case, f, dts=True, sweeps=1)
     got = []
-    for mode in (1, 3):
+    for mode in (1, 3, 4):
         with open_ctx(nv, orc, case, sponge=0, precision=1) as ctx:
             ctx.set_lusgs_mode(mode)
             upload_inputs(ctx, f)
@@ -455,16 +467,17 @@ def test_mixed_precision_iteration_tracks_the_fp64_oracle(nv, orc, name):
             ctx.flux_z(1.0, 1e-4)
             ctx.viscous()
             got.append((ctx.dual_time_stepping(2e-4, 1e-4, sweeps=1), ctx.get_many(nv.NR, 5)))
-    (er, NR), (er3, NR3) = got
+    (er, NR), (er3, NR3), (er4, NR4) = got
     assert list(er) == list(er3)
     for v in range(5):
         assert np.array_equal(NR[v], NR3[v])
     reg = regions(case.slab)["cell"]
     worst = 0.0
     for v in range(5):
-        r = rel(NR[v][reg], want["NR"][v][reg])
-        worst = max(worst, r)
-        assert r <= 1e-4, ("NR", v, r)
-        assert abs(er[v] - want["er"][v]) <= 1e-4 * abs(want["er"][v])
+        for e_, N_ in ((er, NR), (er4, NR4)):   # mode 4 sweeps in FP64 on FP32 residuals
+            r = rel(N_[v][reg], want["NR"][v][reg])
+            worst = max(worst, r)
+            assert r <= 1e-4, ("NR", v, r)
+            assert abs(e_[v] - want["er"][v]) <= 1e-4 * abs(want["er"][v])
     print("worst mixed-precision increment error", worst, flush=True)
     assert worst > 1e-12   # the FP32 kernels really ran
