@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -98,6 +99,38 @@ struct cgli_ctx {
         hipError_t e_ = (x);                           \
         if (e_ != hipSuccess) return CGLI_E_HIP;       \
     } while (0)
+
+// Scratch of the allocation-free entry point cgli_dplusgs_noalloc: the
+// residual norm's partial sums (5 per block of 256 cells) and, in its last
+// 8 doubles, the five norms.  Each device holds its own copy for the life of
+// the process; every call on a device shares it, so calls that overlap on
+// one device race.  Host code never passes the array's name to a kernel (it
+// is a host-side stand-in there): it uses the device address noalloc_ptr.
+constexpr long NOALLOC_DOUBLES = 1L << 19;   // 4 MiB
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex noalloc_mu;
+static double *noalloc_ptr[NOALLOC_MAX_DEVICES];
+static bool noalloc_queried[NOALLOC_MAX_DEVICES];
+
+// once per device, with that device current (the first cgli_create on it)
+static void noalloc_setup(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (noalloc_queried[device]) return;
+    noalloc_queried[device] = true;
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess) noalloc_ptr[device] = static_cast<double *>(ptr);
+    else (void)hipGetLastError();
+}
+
+static double *noalloc_scratch(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    return noalloc_ptr[device];
+}
 
 static int ensure(cgli_ctx *c, int id)
 {
@@ -251,6 +284,7 @@ const char *cgli_error_string(int code)
     case CGLI_E_NODEV: return "no HIP device";
     case CGLI_E_TIMEOUT: return "bounded device wait expired";
     case CGLI_E_IO: return "restart file could not be opened, read or written";
+    case CGLI_E_SIZE: return "the grid exceeds the fixed scratch of the allocation-free entry point";
     default: return "unknown error";
     }
 }
@@ -272,6 +306,7 @@ int cgli_create(const cgli_config *cfg, int device, cgli_ctx **out)
         delete c;
         return CGLI_E_HIP;
     }
+    noalloc_setup(device);
     Geo &g = c->g;
     g.nx = cfg->nx; g.ny = cfg->ny; g.nz = cfg->nz;
     g.Ym = cfg->ny + 4; g.Zm = cfg->nz + 4;
@@ -1076,6 +1111,57 @@ int cgli_dplusgs(cgli_ctx *c, double deltaT, double er[5], double e, int sweeps)
     HIPCHK(hipGetLastError());
     mark(c, CGLI_MR, 15);
     return finish_norm(c, nblk, er);
+}
+
+int cgli_dplusgs_noalloc(cgli_ctx *c, double deltaT, double er[5], double e, int sweeps)
+{
+    if (!c || !er || sweeps < 1) return CGLI_E_ARG;
+    NO_CAPTURE(c);
+    const Geo &g = c->g;
+    const long ncell = (long)g.nx * g.ny * g.nz;
+    const long nb = (ncell + 255) / 256;
+    if (5 * nb + 8 > NOALLOC_DOUBLES) return CGLI_E_SIZE;
+    HIPCHK(hipSetDevice(c->device));
+    double *ptr = noalloc_scratch(c->device);
+    if (!ptr) return CGLI_E_HIP;
+    // inputs and outputs already on the device: nothing is allocated here
+    if (!have(c, CGLI_U_, 5) || !have(c, CGLI_U, 5) || !have(c, CGLI_UQ, 5) || !have(c, CGLI_J, 1) ||
+        !have(c, CGLI_MET, 9) || !have(c, CGLI_INFX, 15) || !have(c, CGLI_VF, 4))
+        return CGLI_E_STATE;
+    for (int id = CGLI_MR; id < CGLI_RES + 5; id++)
+        if (!c->f[id]) return CGLI_E_STATE;
+    const hipStream_t s = c->stream;
+    Phys pd = c->p;
+    pd.grav = 0.0;            // DPLUSGS.cpp:417,420 use the constant f of prm.h:11
+    pd.f_const = c->cfg.f_dplusgs;
+    launch_zero_increment_ghosts(g, f5(c, CGLI_NL), f5(c, CGLI_NR), nullptr, s);
+    launch_rhs_diag(g, pd, deltaT, e, 0, c5(c, CGLI_U_), c5(c, CGLI_U), c5(c, CGLI_UQ), c->f[CGLI_J],
+                    m9(c, CGLI_MET), c5(c, CGLI_INFX), c5(c, CGLI_INFY), c5(c, CGLI_INFZ),
+                    C4{{c->f[CGLI_VF], c->f[CGLI_VF + 1], c->f[CGLI_VF + 2], c->f[CGLI_VF + 3]}},
+                    c5(c, CGLI_RES), f5(c, CGLI_MR), f5(c, CGLI_NL), f5(c, CGLI_NR), s);
+    for (int sw = 0; sw < sweeps; sw++) {
+        launch_dplur_sweep(g, c->p, deltaT, e, c5(c, CGLI_U_), c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_MR),
+                           c5(c, CGLI_NL), f5(c, CGLI_NR), s);
+        launch_copy_interior(g, c5(c, CGLI_NR), f5(c, CGLI_NL), s);
+        c->nl_from_nr = false;
+    }
+    for (int v = 0; v < 5; v++) launch_zero(c->f[CGLI_RES + v], g.N, s);
+    // every partial the final reduction reads is written by this call
+    const int nblk = launch_update_norm(g, c->p, f5(c, CGLI_U_), c->f[CGLI_J], c5(c, CGLI_NR), f5(c, CGLI_RES),
+                                        1, 1, ptr, s);
+    double *er_dev = ptr + NOALLOC_DOUBLES - 8;
+    const double DN = 1. / (double)((long)c->cfg.nx_global * c->cfg.ny * c->cfg.nz);
+    launch_finish_norm(nblk, ptr, DN, 0, er_dev, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->er_host, er_dev, sizeof(double) * 5, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    mark(c, CGLI_MR, 15);
+    bool finite = true;
+    for (int v = 0; v < 5; v++) {
+        er[v] = c->er_host[v];
+        if (!std::isfinite(er[v])) finite = false;
+    }
+    return finite ? CGLI_OK : CGLI_E_NONFINITE;
 }
 
 int cgli_bc_y(cgli_ctx *c, double T_low, double T_high, int heated_lo, int heated_hi)

@@ -21,7 +21,8 @@ METRIC_NAMES = ["xidx", "xidy", "xidz", "etdx", "etdy", "etdz", "ztdx", "ztdy", 
 
 ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "device allocation failed",
           -4: "non-finite residual norm", -5: "a required input field was never set", -6: "no HIP device",
-          -7: "bounded device wait expired", -8: "restart file could not be opened, read or written"}
+          -7: "bounded device wait expired", -8: "restart file could not be opened, read or written",
+          -9: "the grid exceeds the fixed scratch of the allocation-free entry point"}
 
 
 class CgliError(RuntimeError):
@@ -76,6 +77,7 @@ def lib():
         L.cgli_hyperplane.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.cgli_dual_time_stepping.argtypes = [vp, C.c_double, C.c_double, dp, C.c_int, C.c_int]
         L.cgli_dplusgs.argtypes = [vp, C.c_double, dp, C.c_double, C.c_int]
+        L.cgli_dplusgs_noalloc.argtypes = [vp, C.c_double, dp, C.c_double, C.c_int]
         L.cgli_dual_time_stepping_async.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int]
         L.cgli_dual_time_stepping_wait.argtypes = [vp, dp]
         L.cgli_graph_begin.argtypes = [vp]
@@ -293,6 +295,12 @@ class Context:
     def dplusgs(self, deltaT, e, sweeps=4):
         er = (C.c_double * 5)()
         _check(lib().cgli_dplusgs(self.h, deltaT, er, e, sweeps), "dplusgs")
+        return np.array(list(er))
+
+    def dplusgs_noalloc(self, deltaT, e, sweeps=4):
+        """cgli_dplusgs_noalloc: the same, allocating nothing (fields must exist)"""
+        er = (C.c_double * 5)()
+        _check(lib().cgli_dplusgs_noalloc(self.h, deltaT, er, e, sweeps), "dplusgs_noalloc")
         return np.array(list(er))
 
     def hyperplane(self):

@@ -29,6 +29,7 @@ extern "C" {
 #define CGLI_E_NODEV (-6)      /* no HIP device */
 #define CGLI_E_TIMEOUT (-7)    /* a bounded device wait gave up */
 #define CGLI_E_IO (-8)         /* a restart file could not be opened, read or written */
+#define CGLI_E_SIZE (-9)       /* the grid exceeds a fixed maximum (cgli_dplusgs_noalloc) */
 
 typedef struct cgli_ctx cgli_ctx;
 typedef struct cgli_net cgli_net;
@@ -189,6 +190,17 @@ int cgli_set_spin_limit(cgli_ctx *ctx, int max_spin);
 /* DPLUSGS.cpp:18-1238: Jacobi (DP-LUR) variant with `sweeps` sweeps (the
  * reference runs 4). */
 int cgli_dplusgs(cgli_ctx *ctx, double deltaT, double er[5], double e, int sweeps);
+/* No counterpart in the reference: cgli_dplusgs for callers that cannot
+ * allocate device memory while they run (harnesses that forbid hipMalloc
+ * after set-up, serving loops that must not touch the allocator).  Same
+ * kernels and results as cgli_dplusgs; it allocates nothing: every field it
+ * reads or writes (U_, U, Uq, J, the 9 cell metrics, inFx/inFy/inFz, vF,
+ * MR, NL, NR, Residual) must already be on the device (cgli_set_field, or an
+ * earlier call that wrote it), else CGLI_E_STATE; the norm's partial sums go
+ * to a fixed 4 MiB per-device buffer inside the library, which limits the
+ * slab to nx*ny*nz <= 26,843,136 cells (else CGLI_E_SIZE).  Calls on one
+ * device share that buffer: calls that overlap on a device race. */
+int cgli_dplusgs_noalloc(cgli_ctx *ctx, double deltaT, double er[5], double e, int sweeps);
 
 /* ---- ghost cells ----------------------------------------------------- */
 /* Y_boundary_condition.cpp:98-182: zero-gradient copy, then isothermal
