@@ -72,6 +72,8 @@ struct cgli_ctx {
     void *u4 = nullptr;          // mode 4: U_ in march order
     void *x4 = nullptr;          // mode 4: MR, NL and NR in march order
     double *xbufC = nullptr;     // mode 4: exchange of the k products (xbuf holds the j products)
+    void *gcol4 = nullptr;       // mode 4: each tile lane's column geometry, when the metrics do not vary along i
+    bool geo_cols = false;       // the metrics do not vary along i (as of geo4_version)
     long geo_version = 0;        // bumped when J or a cell metric is written
     long geo4_version = -1;      // geo_version of geo4's contents
     bool geo_volatile = false;   // a device pointer to J or a metric was handed out
@@ -364,6 +366,7 @@ int cgli_destroy(cgli_ctx *c)
     if (c->u4) hipFree(c->u4);
     if (c->x4) hipFree(c->x4);
     if (c->xbufC) hipFree(c->xbufC);
+    if (c->gcol4) hipFree(c->gcol4);
     if (c->morder_dev) hipFree(c->morder_dev);
     if (c->xbuf) hipFree(c->xbuf);
     if (c->trash) hipFree(c->trash);
@@ -756,6 +759,8 @@ static int wave_setup(cgli_ctx *c)
         if (hipMalloc(&p, sizeof(double) * 5 * c->g.N) != hipSuccess) return CGLI_E_NOMEM;
         c->xbufC = static_cast<double *>(p);
         HIPCHK(hipMemset(c->xbufC, 0, sizeof(double) * 5 * c->g.N));
+        if (hipMalloc(&p, 16 * march4_col_pairs(c->g)) != hipSuccess) return CGLI_E_NOMEM;
+        c->gcol4 = p;
     }
     if (c->lusgs_mode == 3 && !c->coefM) {
         // FP64 pairs are 16 bytes: the FP32 records use half
@@ -965,11 +970,30 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
     const long f4b = 16 * march4_field_pairs(g);
     char *const x4 = static_cast<char *>(c->x4);   // mode 4: MR, NL, NR in march order
     if (c->lusgs_mode == 4) {
+        // the geometry in march order: per column when the metrics do not
+        // vary along i (checked once per geometry version), per cell
+        // otherwise, and every call while a caller holds a device pointer to
+        // them (it may write them at any time)
+        bool full_geo = false;
+        if (c->geo_volatile) {
+            full_geo = true;
+            c->geo_cols = false;
+        } else if (c->geo4_version != c->geo_version) {
+            if (c->capturing) return CGLI_E_STATE;
+            int *flag = c->counters + 2;
+            HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), s));
+            launch_geo_icheck(g, c->f[CGLI_J], m9(c, CGLI_MET), flag, s);
+            int differs = 0;
+            HIPCHK(hipMemcpyAsync(&differs, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            c->geo_cols = differs == 0;
+            if (c->geo_cols) launch_gather_cols4(g, c->f[CGLI_J], m9(c, CGLI_MET), c->gcol4, s);
+            else full_geo = true;
+            c->geo4_version = c->geo_version;
+        }
         tbegin(c, T_GATHER);
-        const bool geo = c->geo_volatile || c->geo4_version != c->geo_version;
         launch_gather4(g, c5(c, CGLI_U_), c5(c, CGLI_MR), c->f[CGLI_J], m9(c, CGLI_MET), c->u4, x4,
-                       geo ? c->geo4 : nullptr, s);
-        c->geo4_version = c->geo_version;
+                       full_geo ? c->geo4 : nullptr, s);
         tend(c, T_GATHER);
     } else if (c->lusgs_mode >= 1) {
         tbegin(c, T_COEF);
@@ -985,15 +1009,17 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
         if (c->lusgs_mode == 4) {
             launch_march_prep(g, 1, c->xbuf, c->counters, s, c->xbufC);
             tbegin(c, T_FWD);
-            launch_lusgs_fused(g, c->p, deltaT, e, 1, c->geo4, c->u4, x4, x4 + f4b, c5(c, CGLI_U_), c->f[CGLI_J],
+            launch_lusgs_fused(g, c->p, deltaT, e, 1, c->geo4, c->geo_cols ? c->gcol4 : nullptr, c->u4, x4, x4 + f4b,
+                               c5(c, CGLI_U_), c->f[CGLI_J],
                                m9(c, CGLI_MET), c5(c, CGLI_NL), c->xbuf, c->xbufC, c->trash, c->morder_dev,
-                               c->mtiles, c->counters, c->max_spin, MARCH_SLACK, s);
+                               c->mtiles, c->counters, c->max_spin, -1, s);
             tend(c, T_FWD);
             launch_march_prep(g, 0, c->xbuf, c->counters, s, c->xbufC);
             tbegin(c, T_BWD);
-            launch_lusgs_fused(g, c->p, deltaT, e, 0, c->geo4, c->u4, x4 + f4b, x4 + 2 * f4b, c5(c, CGLI_U_),
+            launch_lusgs_fused(g, c->p, deltaT, e, 0, c->geo4, c->geo_cols ? c->gcol4 : nullptr, c->u4, x4 + f4b,
+                               x4 + 2 * f4b, c5(c, CGLI_U_),
                                c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_NR), c->xbuf, c->xbufC, c->trash,
-                               c->morder_dev, c->mtiles, c->counters, c->max_spin, MARCH_SLACK, s);
+                               c->morder_dev, c->mtiles, c->counters, c->max_spin, -1, s);
             tend(c, T_BWD);
             if (sw == sweeps - 1) {
                 tbegin(c, T_SCATTER);

@@ -159,11 +159,20 @@ void launch_gather4(const Geo &g, C5 U_, C5 R, const double *J, M9 met, void *Um
 void launch_scatter4(const Geo &g, const void *Xm, F5 NL, F5 NR, hipStream_t s);
 // one sweep: right-hand sides Rm (forward MR, backward NL) into increments
 // Xm; U_, J, met and the increment field Xg supply the ghost cell upstream
-// in i; xbB / xbC: exchange buffers of the j and k products (5 per cell)
+// in i; xbB / xbC: exchange buffers of the j and k products (5 per cell);
+// slack < 0: the schedule's own lead per hop
+// Gc not null: the metrics do not vary along i and Gc holds each tile
+// lane's column geometry (launch_gather_cols4); Gm is then not read
 void launch_lusgs_fused(const Geo &g, const Phys &p, double deltaT, double e, int forward, const void *Gm,
-                        const void *Um, const void *Rm, void *Xm, C5 U_, const double *J, M9 met, C5 Xg, double *xbB,
-                        double *xbC, double *trash, const int *order, int ntiles, int *counters, int max_spin,
-                        int slack, hipStream_t s);
+                        const void *Gc, const void *Um, const void *Rm, void *Xm, C5 U_, const double *J, M9 met, C5 Xg,
+                        double *xbB, double *xbC, double *trash, const int *order, int ntiles, int *counters,
+                        int max_spin, int slack, hipStream_t s);
+// *flag = 1 where an interior cell's J or metrics differ from its column's
+// at the first interior plane (the caller zeroes *flag)
+void launch_geo_icheck(const Geo &g, const double *J, M9 met, int *flag, hipStream_t s);
+// pairs of the per-column geometry array of the GC sweep, and its gather
+long march4_col_pairs(const Geo &g);
+void launch_gather_cols4(const Geo &g, const double *J, M9 met, void *Gc, hipStream_t s);
 
 
 // learned initial guess (cnn.hip): see that file for layouts

@@ -1677,6 +1677,10 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_march(Geo g, March M, const CT
 // (J, m0) (m1, m2) (m3, m4) (m5, m6) (m7, m8), and five-value fields (U_,
 // MR, NL, NR) as (v0, v1) (v2, v3) (v4, -).
 constexpr int G4P = 5, F4P = 3;
+#ifndef CGLI_FUSED_SLACK
+#define CGLI_FUSED_SLACK 3
+#endif
+constexpr int FUSED_SLACK = CGLI_FUSED_SLACK;   // march steps of lead per hop (polls issued two steps ahead)
 constexpr long G4ROWB = (long)G4P * 64 * 16, F4ROWB = (long)F4P * 64 * 16;
 
 long march4_geo_pairs(const Geo &g) { return march_rows(g) * G4P * 64; }
@@ -1756,6 +1760,7 @@ struct M4Args {
     double *trash;            // this wave's scratch line
     int max_spin;
     long sxs5, dnb5, dnc5;    // exchange strides (times 5): one i plane, the upstream j and k cells
+    double2 gcol[G4P];        // GC: the column's geometry, the same in every plane
 };
 
 __device__ __forceinline__ double2 ld_pair(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
@@ -1770,13 +1775,16 @@ __device__ __forceinline__ int m4_t(const M4Args &w, int s)
     return FWD ? sc : w.S - 1 - sc;
 }
 
-template <bool FWD>
+// GC: the grid's metrics do not vary along i (the channel's mapping does not
+// depend on xi), so each lane holds its column's geometry in registers for
+// the whole sweep instead of loading it at every step
+template <bool FWD, bool GC>
 __device__ __forceinline__ void load4(Ld4 &L, const M4Args &w, unsigned vo, int s)
 {
     const int t = m4_t<FWD>(w, s);
     const unsigned sg = (unsigned)(t * G4ROWB), sf = (unsigned)(t * F4ROWB);
 #pragma unroll
-    for (int p = 0; p < G4P; p++) L.g[p] = ld_pair(w.grow, vo, sg + p * 1024);
+    for (int p = 0; p < G4P; p++) L.g[p] = GC ? w.gcol[p] : ld_pair(w.grow, vo, sg + p * 1024);
 #pragma unroll
     for (int p = 0; p < F4P; p++) L.u[p] = ld_pair(w.urow, vo, sf + p * 1024);
 #pragma unroll
@@ -1871,8 +1879,9 @@ __device__ __forceinline__ void prod4(const double a[12], const double x[5], dou
 
 // one march step s; C holds step s's coefficients, Ln the loads of step
 // s + 1 (issued a step ago), Lf receives the loads of step s + 2; C leaves
-// with step s + 1's coefficients
-template <int TJ, int TK, bool FWD, int PAR>
+// with step s + 1's coefficients.  nb / nc hold the exchange polls of step
+// s (issued two steps ago) and receive those of step s + 2.
+template <int TJ, int TK, bool FWD, bool GC, int PAR>
 __device__ __forceinline__ void step4(const M4Args &w, const F4Lane &L, int s, Cf4 &C, const Ld4 &Ln, Ld4 &Lf,
                                       double sa[5], double (*sB)[5][TJ * TK], double (*sC)[5][TJ * TK],
                                       unsigned long long nb[5], unsigned long long nc[5], bool &late)
@@ -1919,6 +1928,20 @@ __device__ __forceinline__ void step4(const M4Args &w, const F4Lane &L, int s, C
         sC[PAR][v][L.tid] = L.col ? pc[v] : 0.0;
         sa[v] = act ? pa[v] : sa[v];
     }
+    {   // the exchange polls and the loads two steps ahead, issued before this
+        // step's stores: the vector-memory counter completes in order, so a
+        // later wait for these loads does not wait for those stores too
+        const int tn = FWD ? s + 2 : w.S - 3 - s;
+        const int iln = tn - L.jl - L.kl;
+        const bool actn = s + 2 < w.S && L.col && iln >= 0 && iln < w.g.nx;
+        const long xn5 = L.x0 + (long)tn * w.sxs5;
+        // every lane polls (lanes with nothing to wait for read the wave's
+        // scratch line), so no branch splits the loads in flight
+        issue5m(actn && L.need_b ? w.xbB + xn5 + w.dnb5 : w.trash + 16, nb);
+        issue5m(actn && L.need_c ? w.xbC + xn5 + w.dnc5 : w.trash + 24, nc);
+        load4<FWD, GC>(Lf, w, (unsigned)L.tid * 16u, s + 2);
+    }
+    __builtin_amdgcn_sched_barrier(0);
     // every lane stores its slot (a slot with no cell is never read as a
     // value) and ten exchange values (lanes with nothing to publish write
     // the wave's scratch line), so no branch splits the stores
@@ -1935,16 +1958,8 @@ __device__ __forceinline__ void step4(const M4Args &w, const F4Lane &L, int s, C
         __hip_atomic_store(reinterpret_cast<unsigned long long *>(ec + v), (unsigned long long)__double_as_longlong(pc[v]),
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    {   // the next step's exchange polls, the loads two steps ahead, then
-        // the next step's coefficients while both are in flight
-        const int tn = FWD ? s + 1 : w.S - 2 - s;
-        const int iln = tn - L.jl - L.kl;
-        const bool actn = s + 1 < w.S && L.col && iln >= 0 && iln < w.g.nx;
-        const long xn5 = L.x0 + (long)tn * w.sxs5;
-        if (actn && L.need_b) issue5m(w.xbB + xn5 + w.dnb5, nb);
-        if (actn && L.need_c) issue5m(w.xbC + xn5 + w.dnc5, nc);
-    }
-    load4<FWD>(Lf, w, (unsigned)L.tid * 16u, s + 2);
+    __builtin_amdgcn_sched_barrier(0);
+    // the next step's coefficients while all of it is in flight
     coef4<FWD>(C, Ln, w);
 }
 
@@ -1962,9 +1977,10 @@ __device__ __forceinline__ void ghost_product4(const M4Args &w, C5 U_, const dou
     prod4<FWD>(a[0], x, y);
 }
 
-template <int TJ, int TK, bool FWD>
+template <int TJ, int TK, bool FWD, bool GC>
 __global__ void __launch_bounds__(TJ *TK) k_lusgs_fused(Geo g, March M, double K, double e, double deltaT, double hx,
                                                       double hy, double hz, const double2 *__restrict__ Gm,
+                                                      const double2 *__restrict__ Gc,
                                                       const double2 *__restrict__ Um, const double2 *Rm, double2 *Xm,
                                                       C5 U_, const double *__restrict__ J, M9 met, C5 Xg,
                                                       double *xbB, double *xbC, double *trash,
@@ -2035,10 +2051,12 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_fused(Geo g, March M, double K
     L.x0 = 5 * (c0 + (long)(3 - L.jl - L.kl) * sxs);
     const int S = M.S;
     bool late = false;
-    unsigned long long nb[5], nc[5];
+    // exchange polls in flight two steps ahead: one register set per parity
+    unsigned long long nb0[5], nc0[5], nb1[5], nc1[5];
     // start `slack` march steps later than the dependencies require (as the
     // schedule-3 march): the upstream tiles have then published the entries
-    // of the first steps' polls
+    // of the first steps' polls, and each poll issued two steps ahead finds
+    // its entry written
     if (slack > 0 && slack < S) {
         const int ts = FWD ? slack : S - 1 - slack;
         const int ils = ts - L.jl - L.kl;
@@ -2053,22 +2071,27 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_fused(Geo g, March M, double K
     }
     double sa[5] = {0, 0, 0, 0, 0};
     if (L.col) ghost_product4<FWD>(w, U_, J, met, Xg, c0 + (long)(FWD ? 2 : g.gend + 1) * sxs, sa);
+    if (GC) {
+#pragma unroll
+        for (int p = 0; p < G4P; p++) w.gcol[p] = Gc[((long)T * G4P + p) * 64 + L.tid];
+    }
     Ld4 La, Lb;
     Cf4 C;
-    load4<FWD>(La, w, (unsigned)L.tid * 16u, 0);
-    load4<FWD>(Lb, w, (unsigned)L.tid * 16u, 1);
-    {
-        const int t0 = FWD ? 0 : S - 1;
+    load4<FWD, GC>(La, w, (unsigned)L.tid * 16u, 0);
+    load4<FWD, GC>(Lb, w, (unsigned)L.tid * 16u, 1);
+#pragma unroll
+    for (int s0 = 0; s0 < 2; s0++) {
+        const int t0 = FWD ? s0 : S - 1 - s0;
         const int il0 = t0 - L.jl - L.kl;
-        const bool act0 = L.col && il0 >= 0 && il0 < g.nx;
+        const bool act0 = s0 < S && L.col && il0 >= 0 && il0 < g.nx;
         const long x5 = L.x0 + (long)t0 * w.sxs5;
-        if (act0 && L.need_b) issue5m(xbB + x5 + w.dnb5, nb);
-        if (act0 && L.need_c) issue5m(xbC + x5 + w.dnc5, nc);
+        issue5m(act0 && L.need_b ? xbB + x5 + w.dnb5 : w.trash + 16, s0 ? nb1 : nb0);
+        issue5m(act0 && L.need_c ? xbC + x5 + w.dnc5 : w.trash + 24, s0 ? nc1 : nc0);
     }
     coef4<FWD>(C, La, w);
     for (int s = 0; s < S; s += 2) {
-        step4<TJ, TK, FWD, 0>(w, L, s, C, Lb, La, sa, sB, sC, nb, nc, late);
-        if (s + 1 < S) step4<TJ, TK, FWD, 1>(w, L, s + 1, C, La, Lb, sa, sB, sC, nb, nc, late);
+        step4<TJ, TK, FWD, GC, 0>(w, L, s, C, Lb, La, sa, sB, sC, nb0, nc0, late);
+        if (s + 1 < S) step4<TJ, TK, FWD, GC, 1>(w, L, s + 1, C, La, Lb, sa, sB, sC, nb1, nc1, late);
     }
     if (late) atomicOr(err, 1);
 }
@@ -2087,23 +2110,78 @@ void launch_scatter4(const Geo &g, const void *Xm, F5 NL, F5 NR, hipStream_t s)
 }
 
 void launch_lusgs_fused(const Geo &g, const Phys &p, double deltaT, double e, int forward, const void *Gm,
-                        const void *Um, const void *Rm, void *Xm, C5 U_, const double *J, M9 met, C5 Xg, double *xbB,
-                        double *xbC, double *trash, const int *order, int ntiles, int *counters, int max_spin,
-                        int slack, hipStream_t s)
+                        const void *Gc, const void *Um, const void *Rm, void *Xm, C5 U_, const double *J, M9 met, C5 Xg,
+                        double *xbB, double *xbC, double *trash, const int *order, int ntiles, int *counters,
+                        int max_spin, int slack, hipStream_t s)
 {
     const March M = march_of(g);
     const dim3 gr(ntiles), bl(MARCH_TJ * MARCH_TK);
-    const double2 *G = static_cast<const double2 *>(Gm), *Uu = static_cast<const double2 *>(Um),
-                  *R = static_cast<const double2 *>(Rm);
+    const double2 *G = static_cast<const double2 *>(Gm), *Gcol = static_cast<const double2 *>(Gc),
+                  *Uu = static_cast<const double2 *>(Um), *R = static_cast<const double2 *>(Rm);
     double2 *X = static_cast<double2 *>(Xm);
-    if (forward)
-        hipLaunchKernelGGL((k_lusgs_fused<MARCH_TJ, MARCH_TK, true>), gr, bl, 0, s, g, M, p.K, e, deltaT, 1.0 / p.dxi,
-                           1.0 / p.deta, 1.0 / p.dzeta, G, Uu, R, X, U_, J, met, Xg, xbB, xbC, trash, order, ntiles,
-                           counters, counters + 1, max_spin, slack);
-    else
-        hipLaunchKernelGGL((k_lusgs_fused<MARCH_TJ, MARCH_TK, false>), gr, bl, 0, s, g, M, p.K, e, deltaT, 1.0 / p.dxi,
-                           1.0 / p.deta, 1.0 / p.dzeta, G, Uu, R, X, U_, J, met, Xg, xbB, xbC, trash, order, ntiles,
-                           counters, counters + 1, max_spin, slack);
+    const int sl = slack < 0 ? FUSED_SLACK : slack;
+    const double hx = 1.0 / p.dxi, hy = 1.0 / p.deta, hz = 1.0 / p.dzeta;
+#define CGLI_FUSED(FW, GCOL)                                                                                         \
+    hipLaunchKernelGGL((k_lusgs_fused<MARCH_TJ, MARCH_TK, FW, GCOL>), gr, bl, 0, s, g, M, p.K, e, deltaT, hx, hy, hz, \
+                       G, Gcol, Uu, R, X, U_, J, met, Xg, xbB, xbC, trash, order, ntiles, counters, counters + 1,   \
+                       max_spin, sl)
+    if (forward && Gc) CGLI_FUSED(true, true);
+    else if (forward) CGLI_FUSED(true, false);
+    else if (Gc) CGLI_FUSED(false, true);
+    else CGLI_FUSED(false, false);
+#undef CGLI_FUSED
+}
+
+// 1 into *flag where a cell's J or metrics differ from those of the first
+// interior plane of its column (interior cells only: the sweep reads no
+// other geometry per step)
+__global__ void k_geo_icheck(Geo g, const double *__restrict__ J, M9 met, int *flag)
+{
+    const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = 4 + blockIdx.z;
+    if (k > g.NZ || j > g.NY || i > g.gend) return;
+    const long c = ix(g, i, j, k), c3 = ix(g, 3, j, k);
+    bool same = J[c] == J[c3];
+#pragma unroll
+    for (int q = 0; q < 9; q++) same = same && met.m[q][c] == met.m[q][c3];
+    if (!same) *flag = 1;
+}
+
+// the column geometry of every tile lane (plane i = 3), for the GC sweep
+__global__ void k_gather_cols4(Geo g, March M, const double *__restrict__ J, M9 met, double2 *__restrict__ Gc)
+{
+    const int T = blockIdx.x, l = threadIdx.x;
+    const int TJi = T / M.ntk, TKi = T % M.ntk;
+    const int jo = TJi * MARCH_TJ + l / MARCH_TK, ko = TKi * MARCH_TK + l % MARCH_TK;
+    const int joc = jo < g.ny ? jo : g.ny - 1, koc = ko < g.nz ? ko : g.nz - 1;
+    const long c = ix(g, 3, 2 + joc, 2 + koc);
+    double2 *o = Gc + (long)T * G4P * 64 + l;
+    o[0] = make_double2(J[c], met.m[0][c]);
+    o[64] = make_double2(met.m[1][c], met.m[2][c]);
+    o[128] = make_double2(met.m[3][c], met.m[4][c]);
+    o[192] = make_double2(met.m[5][c], met.m[6][c]);
+    o[256] = make_double2(met.m[7][c], met.m[8][c]);
+}
+
+void launch_geo_icheck(const Geo &g, const double *J, M9 met, int *flag, hipStream_t s)
+{
+    if (g.nx < 2) return;
+    const dim3 b(64, 4);
+    const dim3 gr((g.nz + b.x - 1) / b.x, (g.ny + b.y - 1) / b.y, g.nx - 1);
+    hipLaunchKernelGGL(k_geo_icheck, gr, b, 0, s, g, J, met, flag);
+}
+
+long march4_col_pairs(const Geo &g)
+{
+    const March M = march_of(g);
+    return (long)M.ntj * M.ntk * G4P * 64;
+}
+
+void launch_gather_cols4(const Geo &g, const double *J, M9 met, void *Gc, hipStream_t s)
+{
+    const March M = march_of(g);
+    hipLaunchKernelGGL(k_gather_cols4, dim3(M.ntj * M.ntk), dim3(64), 0, s, g, M, J, met, static_cast<double2 *>(Gc));
 }
 
 // the launchers take the allocations (march_*_pairs / march_field_doubles)
