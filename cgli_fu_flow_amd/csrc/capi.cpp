@@ -604,6 +604,31 @@ int cgli_filter(cgli_ctx *c, double Roe_criterion, double E_high, double E_low)
     return CGLI_OK;
 }
 
+int cgli_filter_energies(cgli_ctx *c, double *Eh, double *E2h)
+{
+    if (!c || !Eh || !E2h) return CGLI_E_ARG;
+    NO_CAPTURE(c);
+    HIPCHK(hipSetDevice(c->device));
+    if (!c->filt || !c->valid[CGLI_EPX]) return CGLI_E_STATE;
+    const long N = c->g.N;
+    // the passes' work set ends with E_h and E_2h (FP32 in the mixed mode)
+    if (c->cfg.precision == 1) {
+        const float *w = reinterpret_cast<const float *>(c->filt);
+        std::vector<float> tmp(2 * N);
+        HIPCHK(hipMemcpyAsync(tmp.data(), w + 32 * N, sizeof(float) * 2 * N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (long q = 0; q < N; q++) {
+            Eh[q] = tmp[q];
+            E2h[q] = tmp[N + q];
+        }
+    } else {
+        HIPCHK(hipMemcpyAsync(Eh, c->filt + 32 * N, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(E2h, c->filt + 33 * N, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return CGLI_OK;
+}
+
 int cgli_channel_last_step(cgli_ctx *c, int *inner_fp32)
 {
     if (!c || !inner_fp32) return CGLI_E_ARG;

@@ -77,6 +77,7 @@ def lib():
         L.cgli_hyperplane.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.cgli_dual_time_stepping.argtypes = [vp, C.c_double, C.c_double, dp, C.c_int, C.c_int]
         L.cgli_dplusgs.argtypes = [vp, C.c_double, dp, C.c_double, C.c_int]
+        L.cgli_filter_energies.argtypes = [vp, dp, dp]
         L.cgli_dplusgs_noalloc.argtypes = [vp, C.c_double, dp, C.c_double, C.c_int]
         L.cgli_dual_time_stepping_async.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int]
         L.cgli_dual_time_stepping_wait.argtypes = [vp, dp]
@@ -291,6 +292,13 @@ class Context:
     def set_graph(self, on=True):
         """Replay channel_step's inner iterations from captured graphs."""
         _check(lib().cgli_channel_set_graph(self.h, 1 if on else 0), "channel_set_graph")
+
+    def filter_energies(self):
+        """E_h and E_2h of the last filter call (field layout)."""
+        Eh, E2h = np.zeros(self.shape), np.zeros(self.shape)
+        _check(lib().cgli_filter_energies(self.h, Eh.ctypes.data_as(C.POINTER(C.c_double)),
+                                          E2h.ctypes.data_as(C.POINTER(C.c_double))), "filter_energies")
+        return Eh, E2h
 
     def dplusgs(self, deltaT, e, sweeps=4):
         er = (C.c_double * 5)()

@@ -132,6 +132,12 @@ int cgli_filter(cgli_ctx *ctx, double Roe_criterion, double E_high, double E_low
 /* (with cfg.precision = 1 the filter passes, energies and sensor ratio run
  * in FP32; faces whose ratio lies within FP32 rounding of the criterion may
  * then take the other value) */
+/* The filtered kinetic energies of the last cgli_filter call, per cell in the
+ * field layout: E_h (Filter.cpp's h scale) and E_2h (the 2h scale), whose
+ * face ratio the sensor compares with Roe_criterion (no counterpart in the
+ * reference's interface: its Filter keeps them in local arrays).  Cells the
+ * sensor does not read may hold anything. */
+int cgli_filter_energies(cgli_ctx *ctx, double *Eh, double *E2h);
 /* Statistic.cpp:130-164: the mean and raw second-moment profiles U V W T
  * UU VV WW TT UV UT VT of U_, averaged over x and z (the channel's
  * homogeneous directions) for every interior j; out[11*ny] on the host,
@@ -144,7 +150,12 @@ int cgli_hyperplane(cgli_ctx *ctx, int *Nijk, int *ijk);
  * update (RHS, Gamma^-1, diagonal, hyperplane LU-SGS, update of U_, norm).
  * er[5] receives the reference's residual norms.  `sweeps` symmetric
  * sweeps (the reference runs 2; on one slab the second repeats the first).
- * write_residual != 0 also stores the squared increments in Residual. */
+ * write_residual != 0 also stores the squared increments in Residual.
+ * On an interior slab of a multi-slab problem (first = last = x_periodic =
+ * 0) the x ghost planes of NL and NR are used as the caller left them: the
+ * reference exchanges them with the neighbouring ranks before each sweep
+ * (Dual_time_stepping.cpp:942-976), which this single-device library does
+ * not do, so with sweeps > 1 such a slab differs from a multi-rank run. */
 int cgli_dual_time_stepping(cgli_ctx *ctx, double deltaT, double e, double er[5], int sweeps,
                             int write_residual);
 /* The same in two halves, so that an inner iteration can be queued (or

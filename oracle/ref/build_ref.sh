@@ -11,13 +11,17 @@
 # tree with `-I<tmp> -I-`, so that the quoted includes resolve to the resized
 # headers.  Nothing from the reference is written into the repository; only
 # the linked library lands in oracle/_ref/.  Flags follow the reference's
-# GNU Makefile (Makefile:9: mpicxx -O3, no OpenMP).
+# GNU Makefile (Makefile:9: mpicxx -O3, no OpenMP); with CGLI_REF_OPENMP=1 the
+# routines are compiled with -fopenmp as in the reference's tuned build
+# (Makefile_K:12), into libref_<tag>_omp.so, for timing only.
 set -euo pipefail
 REF=${CGLI_REFERENCE:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
 OUT=$(cd "$HERE/.." && pwd)/_ref
 X=$1; Y=$2; Z=$3; NP=$4; NIN=$5; NOUT=$6
 TAG="${X}x${Y}x${Z}_np${NP}_in${NIN}_out${NOUT}"
+OMPF=""
+if [ "${CGLI_REF_OPENMP:-0}" = 1 ]; then OMPF="-fopenmp"; TAG="${TAG}_omp"; fi
 if [ ! -f "$REF/Resolution.h" ]; then echo "reference tree not found at $REF" >&2; exit 3; fi
 MPIINC=/opt/conda/include; MPILIB=/opt/conda/lib
 mkdir -p "$OUT"
@@ -37,8 +41,8 @@ grep -q "#define X_out $X\b" "$TMP/Resolution.h" && grep -q "#define np $NP\b" "
 INC="-I$TMP -I- -I$REF -I$MPIINC"
 TUS="Flux_X Flux_Y Flux_Z Viscous_terms Dual_time_stepping Hyperplane Jacobian Jacobian_u Jacobian_v Jacobian_w Y_boundary_condition Z_boundary_condition Filter Output Initial_condition"
 for t in $TUS; do
-  g++ -O3 -fPIC -w $INC -c "$REF/$t.cpp" -o "$TMP/$t.o"
+  g++ -O3 -fPIC -w $OMPF $INC -c "$REF/$t.cpp" -o "$TMP/$t.o"
 done
 g++ -O3 -fPIC -w $INC -c "$HERE/ref_harness.cpp" -o "$TMP/ref_harness.o"
-g++ -shared -o "$OUT/libref_${TAG}.so" "$TMP"/*.o -L$MPILIB -Wl,-rpath,$MPILIB -lmpi
+g++ -shared $OMPF -o "$OUT/libref_${TAG}.so" "$TMP"/*.o -L$MPILIB -Wl,-rpath,$MPILIB -lmpi
 echo "$OUT/libref_${TAG}.so"

@@ -2,6 +2,10 @@
 TEST INFRASTRUCTURE, run by hand (not collected by pytest):
 
     OMP_NUM_THREADS=1 python tests/cpu_calibration.py [nx ny nz]
+    OMP_NUM_THREADS=8 python tests/cpu_calibration.py nx ny nz --omp
+
+(--omp: the reference's routines compiled with -fopenmp, as in its tuned
+build, so that both run on the same OpenMP thread count)
 
 Times one inner iteration (the three fluxes, the viscous terms and one
 Dual_time_stepping call with its two hyperplane sweeps) of the C oracle and
@@ -27,12 +31,15 @@ from oracle_harness import REF_DIR, Case, load_oracle, make_inputs, ref_cfg_from
 
 
 def main():
-    nx, ny, nz = (int(a) for a in sys.argv[1:4]) if len(sys.argv) >= 4 else (128, 128, 128)
-    tag = "%dx%dx%d_np1_in3_out3" % (nx, ny, nz)
+    omp = "--omp" in sys.argv
+    args = [a for a in sys.argv[1:] if a != "--omp"]
+    nx, ny, nz = (int(a) for a in args[:3]) if len(args) >= 3 else (128, 128, 128)
+    tag = "%dx%dx%d_np1_in3_out3%s" % (nx, ny, nz, "_omp" if omp else "")
     path = os.path.join(REF_DIR, "libref_%s.so" % tag)
     if not os.path.exists(path):
+        env = dict(os.environ, CGLI_REF_OPENMP="1" if omp else "0")
         subprocess.run([os.path.join(os.path.dirname(HERE), "oracle", "ref", "build_ref.sh"), str(nx), str(ny),
-                        str(nz), "1", "3", "3"], check=True, stdout=subprocess.DEVNULL)
+                        str(nz), "1", "3", "3"], check=True, stdout=subprocess.DEVNULL, env=env)
     ref = C.CDLL(path)
     ref.ref_init()
     orc = load_oracle()
@@ -50,6 +57,7 @@ def main():
         times[backend] = best
         print("%s: %.3f s per inner iteration" % (backend, best), file=sys.stderr, flush=True)
     print(json.dumps({"grid": [nx, ny, nz], "threads": int(os.environ.get("OMP_NUM_THREADS", "0")) or None,
+                      "reference_openmp": omp,
                       "reference_s_per_iteration": times["reference"], "oracle_s_per_iteration": times["oracle"],
                       "oracle_over_reference": times["oracle"] / times["reference"]}))
 

@@ -62,13 +62,19 @@ def oracle_step(orc, oc, f, shape, N, I, dt, inner, e, EpF=None):
     return ers
 
 
-@pytest.mark.parametrize("size,skew,xi_mode", [((16, 24, 12), 0.05, 1), ((12, 20, 16), 0.0, 0)])
-def test_channel_steps_match_oracle(size, skew, xi_mode):
+# Re_tau 180 (beta 2.2, the curvilinear 128^3 physics), 395 (beta 2.4, the
+# 256x192x256 physics) and 590 (beta 2.6 with skew, the 512x384x512 physics),
+# at reduced size
+@pytest.mark.parametrize("size,skew,xi_mode,re_tau,beta", [((16, 24, 12), 0.05, 1, 180.0, 2.2),
+                                                          ((12, 20, 16), 0.0, 0, 180.0, 2.2),
+                                                          ((12, 24, 16), 0.0, 1, 395.0, 2.4),
+                                                          ((12, 28, 12), 0.05, 1, 590.0, 2.6)])
+def test_channel_steps_match_oracle(size, skew, xi_mode, re_tau, beta):
     from cgli_fu_flow_amd import native as nv
     nx, ny, nz = size
-    print("channel", size, "skew", skew, "xi_mode", xi_mode, flush=True)
+    print("channel", size, "skew", skew, "xi_mode", xi_mode, "Re_tau", re_tau, "beta", beta, flush=True)
     orc = load_oracle()
-    ch = nv.channel(180.0, beta=2.2, skew=skew, xi_mode=xi_mode)
+    ch = nv.channel(re_tau, beta=beta, skew=skew, xi_mode=xi_mode)
     cfg = nv.channel_config(ch, nx, ny, nz)
     with nv.Context(cfg) as ctx:
         dt = ctx.channel_init(ch)
@@ -222,15 +228,17 @@ def test_fp32_filter_decisions():
                 assert diff.sum() <= max(2, 1e-3 * diff.size)
 
 
-def test_fp32_residuals_keep_the_mean_profiles():
+@pytest.mark.parametrize("re_tau,beta,skew", [(180.0, 2.2, 0.05), (395.0, 2.4, 0.0)])
+def test_fp32_residuals_keep_the_mean_profiles(re_tau, beta, skew):
     """The channel in the mixed mode (FP32 residual kernels until the drop
-    nears the FP32 floor, then FP64; FP32 off-diagonal sweep; FP64 update,
-    diagonal and norm) against the same run in FP64: both reach the 1e-6
-    inner drop every step, and after the steps the mean-velocity and
-    Reynolds-stress profiles differ by at most 1e-4 relative (the target for
-    the mixed path)."""
+    nears the FP32 floor, then FP64; FP64 sweep, update, diagonal and norm)
+    against the same run in FP64: both reach the 1e-6 inner drop every step,
+    and after the steps the mean-velocity and Reynolds-stress profiles
+    differ by at most 1e-4 relative (the target for the mixed path), at the
+    physics of the 128^3 and the 256x192x256 configurations."""
     from cgli_fu_flow_amd import native as nv
-    ch = nv.channel(180.0, beta=2.2, skew=0.05)
+    print("mixed vs FP64 profiles, Re_tau", re_tau, "beta", beta, flush=True)
+    ch = nv.channel(re_tau, beta=beta, skew=skew)
     res = {}
     for prec in (0, 1):
         cfg = nv.channel_config(ch, 16, 24, 12)

@@ -326,13 +326,23 @@ def test_unknown_schedule_is_an_error(nv, orc):
 def test_filter_matches_oracle(nv, orc, name, periodic):
     """Filter sensor: identical E_high/E_low decisions on every face of the
     reference's windows, for both thresholds (the second splits the faces
-    roughly in half)."""
+    roughly in half); and the filtered energies E_h, E_2h of every cell a
+    face compares, finite and equal to the oracle's at 1e-12 (a work cell
+    read before it is written holds NaN, which the decisions alone would
+    hide where the oracle picks E_low)."""
     print("case", name, "periodic", periodic, flush=True)
     case = make_case(name)
     f = make_inputs(orc, case.slab, SEED + 7)
     over = {"x_periodic": 1, "first": 0, "last": 0} if periodic else {}
     case.overrides = dict(over)
     reg = regions(case.slab)
+    sl = case.slab
+    g, NY, NZ = sl.gend, sl.NY, sl.NZ
+    # the cells the x, y and z faces compare (Filter.cpp:1564-1643)
+    cells = [(slice(2, g + 2), slice(2, NY + 1), slice(2, NZ + 1)), (slice(3, g + 1), slice(1, NY + 2), slice(2, NZ + 1)),
+             (slice(3, g + 1), slice(2, NY + 1), slice(1, NZ + 2))]
+    Eh_o, E2h_o = sl.zeros(), sl.zeros()
+    orc.orc_filter_energy(C.byref(case.cfg(orc)), ptrs(f["U_"]), dp(f["Y_point"]), dp(f["J"]), dp(Eh_o), dp(E2h_o))
     with open_ctx(nv, orc, case, **over) as ctx:
         ctx.set_many(nv.U_, f["U_"])
         ctx.set(nv.J, f["J"])
@@ -342,8 +352,13 @@ def test_filter_matches_oracle(nv, orc, name, periodic):
             ctx.filter(crit, 1.0, 0.1)
             got = ctx.get_many(nv.EPX, 3)
             for d, nm in enumerate(("EpX", "EpY", "EpZ")):
-                g, w = got[d][reg[nm]], want[d][reg[nm]]
-                assert np.array_equal(g, w), (nm, crit, int(np.sum(g != w)))
+                gg, w = got[d][reg[nm]], want[d][reg[nm]]
+                assert np.array_equal(gg, w), (nm, crit, int(np.sum(gg != w)))
+        Eh, E2h = ctx.filter_energies()
+        for win in cells:
+            for a, b in ((Eh, Eh_o), (E2h, E2h_o)):
+                assert np.all(np.isfinite(a[win]))
+                assert rel(a[win], b[win]) <= TOL, rel(a[win], b[win])
 
 
 def test_filter_needs_its_inputs(nv, orc):
