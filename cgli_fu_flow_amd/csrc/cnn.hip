@@ -44,27 +44,34 @@ __device__ __forceinline__ u16 f2bf(float f)
 }
 
 __device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+
+// GELU of the convolution epilogue, z Phi(z), branch-free.  Phi comes from
+// erfc(|z| / sqrt 2) by Abramowitz and Stegun 7.1.26,
+//   erfc(x) = t (a1 + a2 t + ... + a5 t^4) exp(-x^2),  t = 1 / (1 + p x),
+// absolute error <= 1.5e-7 -- within a few FP32 ulps of erff near 1 and far
+// below the BF16 rounding its result goes through next.  Here p carries the
+// 1 / sqrt 2 and the a_k the factor 1/2 of Phi = erfc(-z / sqrt 2) / 2.
+__device__ __forceinline__ float gelu_fast(float z)
+{
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f * 0.70710678118654752f, fabsf(z), 1.0f));   // v_rcp_f32
+    const float poly =
+        t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 0.5307027145f, -0.7265760135f), 0.7107068705f), -0.142248368f),
+                 0.127414796f);
+    const float h = poly * __builtin_amdgcn_exp2f((-0.72134752044448170f * z) * z);   // erfc(|z|/sqrt2)/2; v_exp_f32
+    return z * (z >= 0.0f ? 1.0f - h : h);
+}
+
+// two FP32 values to packed BF16 (round to nearest even, the gfx950
+// conversion instruction)
+__device__ __forceinline__ unsigned pack_bf16x2(float a, float b)
+{
+    return (unsigned)__builtin_bit_cast(u16, (__bf16)a) | ((unsigned)__builtin_bit_cast(u16, (__bf16)b) << 16);
+}
 __device__ __forceinline__ float gelu_grad(float z)
 {
     const float Phi = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
     const float phi = 0.39894228040143268f * expf(-0.5f * z * z);
     return Phi + z * phi;
-}
-
-__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u)); }
-
-// 8 channels c0..c0+7 of voxel (b, d, h, w) of a [voxel][C] BF16 tensor;
-// zero outside the sample unless `per` (then periodic in d and w)
-__device__ __forceinline__ bf16x8 frag(const u16 *__restrict__ X, int C, const NetDims &m, int b, int d, int h,
-                                       int w, int c0, int per)
-{
-    if (per) {
-        d = d < 0 ? d + m.D : d >= m.D ? d - m.D : d;
-        w = w < 0 ? w + m.W : w >= m.W ? w - m.W : w;
-    }
-    if (d < 0 || d >= m.D || h < 0 || h >= m.H || w < 0 || w >= m.W) return zero8();
-    const long v = (((long)b * m.D + d) * m.H + h) * m.W + w;
-    return *reinterpret_cast<const bf16x8 *>(X + v * C + c0);
 }
 
 // Per-lane K fragment of one tap: CIN = 32 -> 8 channels (16 bytes) for
@@ -96,130 +103,249 @@ template <> struct KF<16> {
     }
 };
 
-// voxel of (b, d+dd, h, w+dw) for tap offsets, and whether it lies in the
-// sample (periodic in d and w when `per`); an outside voxel is replaced by
-// `safe` so that every load stays in bounds
-template <bool PER>
-__device__ __forceinline__ int tap_voxel(const NetDims &m, int b, int d, int dd, int h, int w, int dw, int safe,
-                                         bool &ok)
+// Block tile of a convolution: CT_D planes x CT_H rows x CT_W voxels of the
+// output, computed by 8 waves of 1 plane x 4 rows x 16 voxels each.  Its
+// input, the tile plus a 1-voxel halo (CS_D planes of CS_H x CS_W voxels),
+// sits in LDS in CS_D plane slots.
+constexpr int CT_D = 2, CT_H = 8, CT_W = 32, CT_THREADS = 512;
+constexpr int CS_D = CT_D + 2, CS_H = CT_H + 2, CS_W = CT_W + 2;
+constexpr int CS_P = CS_H * CS_W;   // voxels of a staged plane (340)
+constexpr int CS_N = CS_D * CS_P;   // 1360
+
+struct ConvTile { int b, d0, h0, w0; };
+
+// tiles are numbered plane-fastest: tile T and T + 1 of one column (same
+// b, h0, w0) are CT_D planes apart
+__device__ __forceinline__ ConvTile conv_tile(int T, const NetDims &m)
 {
-    int d2 = d + dd, w2 = w + dw;
-    if constexpr (PER) {
-        d2 = d2 < 0 ? d2 + m.D : (d2 >= m.D ? d2 - m.D : d2);
-        w2 = w2 < 0 ? w2 + m.W : (w2 >= m.W ? w2 - m.W : w2);
-    }
-    ok = d2 >= 0 && d2 < m.D && h >= 0 && h < m.H && w2 >= 0 && w2 < m.W;
-    const int v = ((b * m.D + d2) * m.H + h) * m.W + w2;
-    return ok ? v : safe;
+    const int nd = (m.D + CT_D - 1) / CT_D, nh = (m.H + CT_H - 1) / CT_H, nw = (m.W + CT_W - 1) / CT_W;
+    ConvTile c;
+    c.d0 = (T % nd) * CT_D;
+    int r = T / nd;
+    c.h0 = (r % nh) * CT_H;
+    r /= nh;
+    c.w0 = (r % nw) * CT_W;
+    c.b = r / nw;
+    return c;
 }
 
-// one tap of an item: the RW rows' fragments of tap t into F (in flight)
-template <int CIN, int RW, bool PER>
-__device__ __forceinline__ void tap_load(typename KF<CIN>::raw *F, bool *ok, const u16 *__restrict__ X,
-                                         const NetDims &m, int t, int b, int d, int hb, int w, int c0, int safe)
+// 16 bytes (8 channels) into a [kq][item][CPL] LDS array: one fragment when
+// CPL = 8, two (kq and kq + 1, `stride` elements apart) when CPL = 4
+template <int CPL>
+__device__ __forceinline__ void put_chunk(u16 *dst, int stride, uint4 v)
 {
-    const int dd = t / 9 - 1, dh = (t / 3) % 3 - 1, dw = t % 3 - 1;
+    if constexpr (CPL == 8) {
+        *reinterpret_cast<uint4 *>(dst) = v;
+    } else {
+        *reinterpret_cast<uint2 *>(dst) = make_uint2(v.x, v.y);
+        *reinterpret_cast<uint2 *>(dst + stride) = make_uint2(v.z, v.w);
+    }
+}
+
+// LDS stride (elements) of one kq quarter of the staged input: padded so
+// that consecutive quarters start 64 bytes apart modulo the 256 bytes the
+// banks span, which keeps the staging stores free of bank conflicts
+template <int CIN>
+constexpr int stage_plane()
+{
+    constexpr int b = CS_N * (CIN / 4) * 2;
+    return (b + ((64 - b % 256) + 256) % 256) / 2;
+}
+
+// 16 zero bytes: the source of staged chunks outside the sample
+__device__ const uint4 k_zero16 = {0u, 0u, 0u, 0u};
+
+// Staging moves two planes at a time, CT_THREADS threads x PF 16-byte chunks.
+// Chunk e = thread + CT_THREADS * s is chunk e % CPR of staged row e / CPR
+// (CPR chunks per row of CS_W voxels): plane pp = row / CS_H (0 or 1 of the
+// pair), row r = row % CS_H, voxel x of the row and 8-channel group ch.
+// code[s] = pp << 24 | r << 16 | x << 8 | ch (-1 past the last chunk) and
+// soff[s], its LDS element offset from the pair's first slot, do not depend
+// on the tile.
+template <int CIN, int PF>
+__device__ __forceinline__ void stage_codes(int *code, int *soff)
+{
+    constexpr int CPL = CIN / 4, CPV = CIN / 8, CPR = CS_W * CPV, NCH = 2 * CS_H * CPR, KS = stage_plane<CIN>();
 #pragma unroll
-    for (int q = 0; q < RW; q++) {
-        const int v = tap_voxel<PER>(m, b, d, dd, hb + q + dh, w, dw, safe, ok[q]);
-        F[q] = *reinterpret_cast<const typename KF<CIN>::raw *>(X + (long)v * CIN + c0);
+    for (int s = 0; s < PF; s++) {
+        const int e = threadIdx.x + CT_THREADS * s;
+        const int R = e / CPR, j = e % CPR, pp = R / CS_H, r = R % CS_H, x = j / CPV, ch = j % CPV;
+        code[s] = e < NCH ? pp << 24 | r << 16 | x << 8 | ch : -1;
+        soff[s] = (8 * ch / CPL) * KS + (pp * CS_P + r * CS_W + x) * CPL;
     }
 }
 
-// y = conv(x, W) + bias for all voxels.  Persistent blocks: the layer's
-// weights (COUT x 27 x CIN BF16, rows padded by 8 elements against LDS bank
-// conflicts) are staged once per block in LDS; then each wave takes items of
-// RW rows (h .. h+RW-1) x 16 voxels (w0 .. w0+15) at one (b, d), striding
-// over the items.  The K loop runs over the 27 taps: per tap MT weight
-// fragments from LDS, RW input fragments (loaded one tap ahead into the
-// other of two register buffers), RW x MT MFMAs.  Outputs (either may be
-// null): Z FP32 [v][COUT] (the pre-activation), A BF16 [v][COUT] = GELU(Z)
-// when ACT, else Z; channels co >= cout are written as 0.
-template <int CIN, int MT, bool ACT, int RW, bool PER>
-__global__ void __launch_bounds__(256) k_conv(const u16 *__restrict__ X, const u16 *__restrict__ Wt,
-                                              const float *__restrict__ bias, float *__restrict__ Z,
-                                              u16 *__restrict__ A, NetDims m, int cout, long nitems)
+// the chunks of tile-relative planes p0 and p0 + 1 of tile c (plane p holds
+// d0 - 1 + p) into registers: zero outside the sample unless PER (periodic
+// in d and w), always zero outside in h
+template <int CIN, int PF, bool PER>
+__device__ __forceinline__ void stage_load(uint4 *pf, const u16 *__restrict__ X, const NetDims &m, const ConvTile &c,
+                                           int p0, const int *code)
+{
+    int dA = c.d0 - 1 + p0, dB = dA + 1;
+    if constexpr (PER) {
+        dA = dA < 0 ? dA + m.D : (dA >= m.D ? dA - m.D : dA);
+        dB = dB < 0 ? dB + m.D : (dB >= m.D ? dB - m.D : dB);
+        if (m.D < 2) dA = dB = 0;
+    }
+    const bool okA = dA >= 0 && dA < m.D, okB = dB >= 0 && dB < m.D;
+    const long plane = (long)m.H * m.W * CIN;
+    const u16 *PA = X + ((long)c.b * m.D + (okA ? dA : 0)) * plane;
+    const u16 *PB = X + ((long)c.b * m.D + (okB ? dB : 0)) * plane;
+    const int hm = c.h0 - 1, wm = c.w0 - 1;
+    const bool inner = c.w0 >= 1 && c.w0 + CS_W - 1 <= m.W;   // voxels w0-1 .. w0+CT_W inside, no wrap
+#pragma unroll
+    for (int s = 0; s < PF; s++) {
+        const int cd = code[s];
+        const int pp = (cd >> 24) & 1, h = hm + ((cd >> 16) & 255), ch = cd & 255;
+        int w = wm + ((cd >> 8) & 255);
+        bool ok = cd >= 0 && (pp ? okB : okA) && (unsigned)h < (unsigned)m.H;
+        if constexpr (PER) {
+            w = w < 0 ? w + m.W : w;
+            w = w >= m.W ? w - m.W : w;
+            if (m.W < CS_W) w %= m.W;   // narrow samples: more than one period in a staged row
+        } else {
+            ok = ok && (inner || (unsigned)w < (unsigned)m.W);
+        }
+        const u16 *src = (pp ? PB : PA) + (long)(h * m.W + w) * CIN + 8 * ch;
+        pf[s] = *(ok ? reinterpret_cast<const uint4 *>(src) : &k_zero16);
+    }
+}
+
+// the chunks of stage_load into LDS plane slots slot0 and slot0 + 1
+template <int CIN, int PF>
+__device__ __forceinline__ void stage_store(u16 *sx, const uint4 *pf, int slot0, const int *code, const int *soff)
+{
+    constexpr int CPL = CIN / 4, KS = stage_plane<CIN>();
+    u16 *base = sx + slot0 * CS_P * CPL;
+#pragma unroll
+    for (int s = 0; s < PF; s++)
+        if (code[s] >= 0) put_chunk<CPL>(base + soff[s], KS, pf[s]);
+}
+
+// y = conv(x, W) + bias for all voxels.  Persistent blocks of 8 waves, one
+// per CU: the layer's weights (COUT x 27 x CIN BF16) are staged once per
+// block in LDS as [tap][kq][co][CPL], and the staged input as
+// [kq][voxel][CPL] (both put a wave's 16 lanes of one kq on consecutive
+// fragments).  Each block walks a contiguous range of tiles, so that
+// consecutive tiles mostly continue one column: the two planes they share
+// stay in their LDS slots (a ring of four) and only two new planes are
+// staged, loaded into registers while the previous tile is computed.  A
+// wave's K loop runs over the 9 (dd, dw) tap pairs: the 3 taps dh = -1..1
+// of the pair are held in registers and each of the 6 input rows its 4
+// output rows touch is read once from LDS and used by up to 3 of them.
+// Blocks on one XCD take neighbouring ranges (workgroups are dealt to the 8
+// XCDs round robin), so that halo rows they share meet in that XCD's L2.
+// Outputs (either may be null): Z FP32 [v][COUT] (the pre-activation),
+// A BF16 [v][COUT] = GELU(Z) when ACT, else Z; channels co >= cout are
+// written as 0.
+template <int CIN, int MT, bool ACT, bool PER>
+__global__ void __launch_bounds__(CT_THREADS) k_conv(const u16 *__restrict__ X, const u16 *__restrict__ Wt,
+                                                     const float *__restrict__ bias, float *__restrict__ Z,
+                                                     u16 *__restrict__ A, NetDims m, int cout, int ntiles)
 {
     constexpr int COUT = 16 * MT;
-    constexpr int CPL = CIN / 4;                  // channels per lane and tap
-    constexpr int WROW = 27 * CIN + 8;            // padded LDS weight row
+    constexpr int CPL = CIN / 4;                                           // channels per lane fragment
+    constexpr int PF = (2 * CS_H * CS_W * CIN / 8 + CT_THREADS - 1) / CT_THREADS;   // chunks per thread, 2 planes
+    constexpr int KS = stage_plane<CIN>();                                 // LDS stride of a kq quarter
     typedef KF<CIN> K;
     typedef typename K::raw R;
-    __shared__ __align__(16) u16 sw[COUT * WROW];
-    for (int t = threadIdx.x; t < COUT * 27 * CIN / 8; t += blockDim.x) {
-        const int e = 8 * t, co = e / (27 * CIN), r = e % (27 * CIN);
-        *reinterpret_cast<uint4 *>(&sw[co * WROW + r]) = reinterpret_cast<const uint4 *>(Wt)[t];
+    __shared__ __align__(16) u16 sw[27 * CIN * COUT];
+    __shared__ __align__(16) u16 sx[4 * KS];
+    for (int t = threadIdx.x; t < COUT * 27 * CIN / 8; t += CT_THREADS) {
+        const int e = 8 * t, co = e / (27 * CIN), r = e % (27 * CIN), tap = r / CIN, kq = (r % CIN) / CPL;
+        put_chunk<CPL>(sw + ((tap * 4 + kq) * COUT + co) * CPL, COUT * CPL, reinterpret_cast<const uint4 *>(Wt)[t]);
     }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int n = lane & 15, kq = lane >> 4, c0 = CPL * kq;
-    const int rowsW = m.W / 16, hgroups = (m.H + RW - 1) / RW;
-    for (long item = (long)blockIdx.x * 4 + wave; item < nitems; item += (long)gridDim.x * 4) {
-        const int w0 = (int)(item % rowsW) * 16;
-        long r = item / rowsW;
-        const int hb = (int)(r % hgroups) * RW;
-        r /= hgroups;
-        const int d = (int)(r % m.D), b = (int)(r / m.D);
-        const int w = w0 + n;
-        const int safe = ((b * m.D + d) * m.H + hb) * m.W + w;   // (b, d, hb, w): always inside
-        f32x4 acc[RW][MT];
+    const int G = gridDim.x;
+    const int rix = G % 8 == 0 ? (int)(blockIdx.x & 7) * (G >> 3) + (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+    const int T0 = (int)((long)rix * ntiles / G), T1 = (int)((long)(rix + 1) * ntiles / G);
+    const int nd = (m.D + CT_D - 1) / CT_D;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int n = lane & 15, kq = lane >> 4;
+    const int pd = wv & 1, hg = (wv >> 1) & 1, wh = wv >> 2;
+    const u16 *xq = sx + kq * KS;
+    const u16 *wq = sw + (kq * COUT + n) * CPL;
+    float bz[MT][4];   // this lane's biases (0 for padded channels)
 #pragma unroll
-        for (int q = 0; q < RW; q++)
+    for (int u = 0; u < MT; u++)
 #pragma unroll
-            for (int t = 0; t < MT; t++) acc[q][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-        R F0[RW], F1[RW];
-        bool ok0[RW], ok1[RW];
-        tap_load<CIN, RW, PER>(F0, ok0, X, m, 0, b, d, hb, w, c0, safe);
-        // taps in pairs: tap t from F0 while t+1 loads into F1, then t+1
-        // from F1 while t+2 loads into F0; fully unrolled, so that every wait
-        // for a tap's loads leaves the next tap's loads in flight
+        for (int rr = 0; rr < 4; rr++) {
+            const int ch = 16 * u + 4 * kq + rr;
+            bz[u][rr] = bias && ch < cout ? bias[ch] : 0.0f;
+        }
+    int code[PF], soff[PF];
+    stage_codes<CIN, PF>(code, soff);
+    uint4 pf[PF];
+    int base = 0;   // LDS slot of tile-relative plane 0
+    if (T0 < T1) stage_load<CIN, PF, PER>(pf, X, m, conv_tile(T0, m), 0, code);
+    for (int T = T0; T < T1; T++) {
+        const ConvTile c = conv_tile(T, m);
+        __syncthreads();   // the previous tile's LDS reads (and the weights' staging) are done
+        if (T > T0 && T % nd != 0) {
+            // same column as tile T - 1: its planes 2, 3 are this tile's 0, 1
+            base = (base + 2) & 3;
+            stage_store<CIN, PF>(sx, pf, (base + 2) & 3, code, soff);
+        } else {
+            base = 0;
+            stage_store<CIN, PF>(sx, pf, 0, code, soff);
+            stage_load<CIN, PF, PER>(pf, X, m, c, 2, code);
+            stage_store<CIN, PF>(sx, pf, 2, code, soff);
+        }
+        __syncthreads();
+        if (T + 1 < T1) stage_load<CIN, PF, PER>(pf, X, m, conv_tile(T + 1, m), (T + 1) % nd != 0 ? 2 : 0, code);
+        const int d = c.d0 + pd, hb = c.h0 + 4 * hg, w = c.w0 + 16 * wh + n;
+        if (d >= m.D || hb >= m.H || c.w0 + 16 * wh >= m.W) continue;   // wave-uniform
+        f32x4 acc[4][MT];
 #pragma unroll
-        for (int t = 0; t < 27; t += 2) {
-            if (t + 1 < 27) tap_load<CIN, RW, PER>(F1, ok1, X, m, t + 1, b, d, hb, w, c0, safe);
-            {
-                R af[MT];
+        for (int q = 0; q < 4; q++)
 #pragma unroll
-                for (int u = 0; u < MT; u++) af[u] = *reinterpret_cast<const R *>(&sw[(16 * u + n) * WROW + t * CIN + c0]);
+            for (int u = 0; u < MT; u++) acc[q][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int dd = 0; dd < 3; dd++) {   // not unrolled: bounds the LDS reads in flight
+            const u16 *xp = xq + (((pd + dd + base) & 3) * CS_P + 4 * hg * CS_W + 16 * wh + n) * CPL;
 #pragma unroll
-                for (int q = 0; q < RW; q++) {
-                    const R bq = K::sel(ok0[q], F0[q]);
+            for (int dw = 0; dw < 3; dw++) {
+                R a[3][MT];
 #pragma unroll
-                    for (int u = 0; u < MT; u++) acc[q][u] = K::mfma(af[u], bq, acc[q][u]);
-                }
-            }
-            if (t + 1 < 27) {
-                if (t + 2 < 27) tap_load<CIN, RW, PER>(F0, ok0, X, m, t + 2, b, d, hb, w, c0, safe);
-                R af[MT];
+                for (int dh = 0; dh < 3; dh++)
 #pragma unroll
-                for (int u = 0; u < MT; u++)
-                    af[u] = *reinterpret_cast<const R *>(&sw[(16 * u + n) * WROW + (t + 1) * CIN + c0]);
+                    for (int u = 0; u < MT; u++)
+                        a[dh][u] = *reinterpret_cast<const R *>(wq + ((dd * 9 + dh * 3 + dw) * 4 * COUT + 16 * u) * CPL);
 #pragma unroll
-                for (int q = 0; q < RW; q++) {
-                    const R bq = K::sel(ok1[q], F1[q]);
+                for (int rp = 0; rp < 6; rp++) {
+                    const R bq = *reinterpret_cast<const R *>(xp + (rp * CS_W + dw) * CPL);
 #pragma unroll
-                    for (int u = 0; u < MT; u++) acc[q][u] = K::mfma(af[u], bq, acc[q][u]);
+                    for (int dh = 0; dh < 3; dh++) {
+                        const int q = rp - dh;
+                        if (q < 0 || q >= 4) continue;
+#pragma unroll
+                        for (int u = 0; u < MT; u++) acc[q][u] = K::mfma(a[dh][u], bq, acc[q][u]);
+                    }
                 }
             }
         }
+        const bool padded = cout < COUT;   // uniform
 #pragma unroll
-        for (int q = 0; q < RW; q++) {
+        for (int q = 0; q < 4; q++) {
             if (hb + q >= m.H) break;
-            const long v = (((long)b * m.D + d) * m.H + hb + q) * m.W + w;
+            const long v = (((long)c.b * m.D + d) * m.H + hb + q) * m.W + w;
 #pragma unroll
             for (int u = 0; u < MT; u++) {
                 const int cb = 16 * u + 4 * kq;   // channels cb .. cb+3 of this lane
-                float z[4];
-                unsigned short a16[4];
+                float z[4], g[4];
 #pragma unroll
                 for (int rr = 0; rr < 4; rr++) {
-                    const bool on = cb + rr < cout;
-                    z[rr] = on ? acc[q][u][rr] + (bias ? bias[cb + rr] : 0.0f) : 0.0f;
-                    a16[rr] = on ? f2bf(ACT ? gelu(z[rr]) : z[rr]) : (u16)0;
+                    z[rr] = acc[q][u][rr] + bz[u][rr];
+                    g[rr] = ACT ? gelu_fast(z[rr]) : z[rr];
+                }
+                if (padded) {
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++)
+                        if (cb + rr >= cout) z[rr] = g[rr] = 0.0f;
                 }
                 if (Z) *reinterpret_cast<float4 *>(Z + v * COUT + cb) = make_float4(z[0], z[1], z[2], z[3]);
                 if (A)
-                    *reinterpret_cast<uint2 *>(A + v * COUT + cb) =
-                        make_uint2((unsigned)a16[0] | ((unsigned)a16[1] << 16), (unsigned)a16[2] | ((unsigned)a16[3] << 16));
+                    *reinterpret_cast<uint2 *>(A + v * COUT + cb) = make_uint2(pack_bf16x2(g[0], g[1]), pack_bf16x2(g[2], g[3]));
             }
         }
     }
@@ -464,28 +590,44 @@ __global__ void k_gather(const float *__restrict__ S, const float *__restrict__ 
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nv = (long)m.B * m.D * m.H * m.W;
     if (t >= nv) return;
-    const int w = (int)(t % m.W), h = (int)((t / m.W) % m.H), d = (int)((t / ((long)m.W * m.H)) % m.D);
-    const int b = (int)(t / ((long)m.W * m.H * m.D));
-    // whole_rec >= 0: one sample, the whole grid of record whole_rec
-    const int o[4] = {whole_rec >= 0 ? whole_rec : org[4 * b], whole_rec >= 0 ? 0 : org[4 * b + 1],
-                      whole_rec >= 0 ? 0 : org[4 * b + 2], whole_rec >= 0 ? 0 : org[4 * b + 3]};
-    const int i = (o[1] + d) % nx, k = (o[3] + w) % nz, j = o[2] + h;
-    const long cell = ((long)i * ny + j) * nz + k;
-    const int rec = o[0];
+    long cell;
+    int rec, d = 0, h = 0, w = 0, oj = 0;
+    if (whole_rec >= 0) {
+        // one sample, the whole grid of record whole_rec: voxel t is cell t
+        cell = t;
+        rec = whole_rec;
+    } else {
+        // crops (a batch of crops holds fewer than 2^31 voxels)
+        const int tt = (int)t;
+        w = tt % m.W;
+        int r = tt / m.W;
+        h = r % m.H;
+        r /= m.H;
+        d = r % m.D;
+        const int b = r / m.D;
+        rec = org[4 * b];
+        oj = org[4 * b + 2];
+        const int i = (org[4 * b + 1] + d) % nx, k = (org[4 * b + 3] + w) % nz;
+        cell = ((long)i * ny + oj + h) * nz + k;
+    }
     const float *sc = rs + (long)(rec % cap) * NET_NSCALE;
     const float *s0 = S + ((long)(rec % cap) * vol + cell) * 5;
     const float *s1 = S + ((long)((rec - 1) % cap) * vol + cell) * 5;
     const float *s2 = S + ((long)((rec - 2) % cap) * vol + cell) * 5;
-    u16 *x = X + t * 16;
+    float f[16];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
         const float a = s0[q], bq = s1[q], c = s2[q];
         const float d1 = a - bq, d1p = bq - c;
-        x[q] = f2bf(a * sc[q]);
-        x[5 + q] = f2bf(d1 * sc[5 + q]);
-        x[10 + q] = f2bf((d1 - d1p) * sc[10 + q]);
+        f[q] = a * sc[q];
+        f[5 + q] = d1 * sc[5 + q];
+        f[10 + q] = (d1 - d1p) * sc[10 + q];
     }
-    x[15] = 0;
+    f[15] = 0.0f;
+    uint4 *x = reinterpret_cast<uint4 *>(X + t * 16);
+    x[0] = make_uint4(pack_bf16x2(f[0], f[1]), pack_bf16x2(f[2], f[3]), pack_bf16x2(f[4], f[5]), pack_bf16x2(f[6], f[7]));
+    x[1] = make_uint4(pack_bf16x2(f[8], f[9]), pack_bf16x2(f[10], f[11]), pack_bf16x2(f[12], f[13]),
+                      pack_bf16x2(f[14], f[15]));
     if (T) {
         const float *s3 = S + ((long)((rec + 1) % cap) * vol + cell) * 5;
 #pragma unroll
@@ -494,7 +636,7 @@ __global__ void k_gather(const float *__restrict__ S, const float *__restrict__ 
     if (mask) {
         const bool okd = d >= halo && d < m.D - halo;
         const bool okw = w >= halo && w < m.W - halo;
-        const bool okh = (h >= halo || o[2] == 0) && (h < m.H - halo || o[2] + m.H == ny);
+        const bool okh = (h >= halo || oj == 0) && (h < m.H - halo || oj + m.H == ny);
         mask[t] = (okd && okw && okh) ? 1 : 0;
     }
 }
@@ -585,9 +727,9 @@ __global__ void k_guess(Geo g, C5 U, C5 Uq, const double *__restrict__ J, const 
 // ---------------------------------------------------------------- launchers
 static unsigned blocks_of(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
-constexpr int CONV_RW = 4;   // rows per wave item
-
-static int conv_blocks(long nitems)
+// one persistent block per CU (a block's LDS holds the weights and one
+// staged tile)
+static int conv_grid(int ntiles)
 {
     static int ncu = 0;
     if (!ncu) {
@@ -596,25 +738,27 @@ static int conv_blocks(long nitems)
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
             ncu = 256;
     }
-    const long want = (nitems + 3) / 4;
-    return (int)std::max(1L, std::min<long>(want, 2L * ncu));
+    return std::max(1, std::min(ntiles, ncu));
 }
 
 template <bool PER>
 static int conv_launch(int cin_pad, int cout_pad, bool act, const u16 *x, const u16 *w, const float *bias, float *Z,
-                       u16 *a, const NetDims &m, int cout, long nitems, hipStream_t s)
+                       u16 *a, const NetDims &m, int cout, hipStream_t s)
 {
-    const dim3 gr(conv_blocks(nitems)), bl(256);
+    const long nt = (long)m.B * ((m.D + CT_D - 1) / CT_D) * ((m.H + CT_H - 1) / CT_H) * ((m.W + CT_W - 1) / CT_W);
+    if (nt < 1 || nt > 0x7fffffffL) return -1;
+    const int ntiles = (int)nt;
+    const dim3 gr(conv_grid(ntiles)), bl(CT_THREADS);
     if (cin_pad == 16 && cout_pad == 32 && act)
-        hipLaunchKernelGGL((k_conv<16, 2, true, CONV_RW, PER>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems);
+        hipLaunchKernelGGL((k_conv<16, 2, true, PER>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, ntiles);
     else if (cin_pad == 32 && cout_pad == 32 && act)
-        hipLaunchKernelGGL((k_conv<32, 2, true, CONV_RW, PER>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems);
+        hipLaunchKernelGGL((k_conv<32, 2, true, PER>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, ntiles);
     else if (cin_pad == 32 && cout_pad == 16 && !act)
-        hipLaunchKernelGGL((k_conv<32, 1, false, CONV_RW, PER>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems);
+        hipLaunchKernelGGL((k_conv<32, 1, false, PER>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, ntiles);
     else if (cin_pad == 16 && cout_pad == 32 && !act)
-        hipLaunchKernelGGL((k_conv<16, 2, false, CONV_RW, PER>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems);
+        hipLaunchKernelGGL((k_conv<16, 2, false, PER>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, ntiles);
     else if (cin_pad == 32 && cout_pad == 32 && !act)
-        hipLaunchKernelGGL((k_conv<32, 2, false, CONV_RW, PER>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, nitems);
+        hipLaunchKernelGGL((k_conv<32, 2, false, PER>), gr, bl, 0, s, x, w, bias, Z, a, m, cout, ntiles);
     else
         return -1;
     return 0;
@@ -624,11 +768,10 @@ int net_conv(int cin_pad, int cout_pad, bool act, const void *X, const void *W, 
              const NetDims &m, int cout, int per, hipStream_t s)
 {
     if (m.W % 16) return -1;
-    const long nitems = (long)m.B * m.D * ((m.H + CONV_RW - 1) / CONV_RW) * (m.W / 16);
     const u16 *x = static_cast<const u16 *>(X), *w = static_cast<const u16 *>(W);
     u16 *a = static_cast<u16 *>(A);
-    return per ? conv_launch<true>(cin_pad, cout_pad, act, x, w, bias, Z, a, m, cout, nitems, s)
-               : conv_launch<false>(cin_pad, cout_pad, act, x, w, bias, Z, a, m, cout, nitems, s);
+    return per ? conv_launch<true>(cin_pad, cout_pad, act, x, w, bias, Z, a, m, cout, s)
+               : conv_launch<false>(cin_pad, cout_pad, act, x, w, bias, Z, a, m, cout, s);
 }
 
 void net_gelu_back(const float *dA, const float *Z, void *dZ, float *dZf, long n, hipStream_t s)
