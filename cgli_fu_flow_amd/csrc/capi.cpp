@@ -42,8 +42,6 @@ struct cgli_ctx {
     Phys p;
     double *f[CGLI_NFIELDS] = {};
     bool valid[CGLI_NFIELDS] = {};
-    double *prim4 = nullptr;     // 4 fields: u, v, w, T (viscous pass 1)
-    double *faces12 = nullptr;   // 12 fields: viscous face fluxes
     double *scratch = nullptr;   // reductions
     long scratch_len = 0;
     double *er_host = nullptr;   // pinned
@@ -353,8 +351,6 @@ int cgli_destroy(cgli_ctx *c)
     channel_drop_graphs(c->chan);
     for (int q = 0; q < CGLI_NFIELDS; q++)
         if (c->f[q]) hipFree(c->f[q]);
-    if (c->prim4) hipFree(c->prim4);
-    if (c->faces12) hipFree(c->faces12);
     if (c->scratch) hipFree(c->scratch);
     if (c->ijk_dev) hipFree(c->ijk_dev);
     if (c->coef) hipFree(c->coef);
@@ -557,21 +553,10 @@ int cgli_viscous(cgli_ctx *c)
         return CGLI_E_STATE;
     int r = ensure_range(c, CGLI_VF, 4);
     if (r) return r;
-    if ((!c->prim4 || !c->faces12) && c->capturing) return CGLI_E_STATE;
-    if (!c->prim4) {
-        void *p = nullptr;
-        if (hipMalloc(&p, sizeof(double) * 4 * c->g.N) != hipSuccess) return CGLI_E_NOMEM;
-        c->prim4 = static_cast<double *>(p);
-    }
-    if (!c->faces12) {
-        void *p = nullptr;
-        if (hipMalloc(&p, sizeof(double) * 12 * c->g.N) != hipSuccess) return CGLI_E_NOMEM;
-        c->faces12 = static_cast<double *>(p);
-    }
     F4 vF{{c->f[CGLI_VF], c->f[CGLI_VF + 1], c->f[CGLI_VF + 2], c->f[CGLI_VF + 3]}};
     tbegin(c, T_VISCOUS);
     launch_viscous(c->g, c->p, c5(c, CGLI_U_), c->f[CGLI_J], c->f[CGLI_J_U], m9(c, CGLI_MET_U), c->f[CGLI_J_V],
-                   m9(c, CGLI_MET_V), c->f[CGLI_J_W], m9(c, CGLI_MET_W), c->prim4, c->faces12, vF, c->stream);
+                   m9(c, CGLI_MET_V), c->f[CGLI_J_W], m9(c, CGLI_MET_W), vF, c->stream);
     tend(c, T_VISCOUS);
     HIPCHK(hipGetLastError());
     mark(c, CGLI_VF, 4);

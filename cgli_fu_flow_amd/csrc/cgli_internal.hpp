@@ -50,8 +50,7 @@ void launch_flux_y(const Geo &g, const Phys &p, double Ep, double e, int xi_mode
 void launch_flux_z(const Geo &g, const Phys &p, double e, int xi_mode, C5 U, const double *J, const double *Jf,
                    M9 mf, F5 F, double epS, const double *EpF, hipStream_t s);
 void launch_viscous(const Geo &g, const Phys &p, C5 U, const double *J, const double *Ju, M9 mu,
-                    const double *Jv, M9 mv, const double *Jw, M9 mw, double *prim4, double *faces12,
-                    F4 vF, hipStream_t s);
+                    const double *Jv, M9 mv, const double *Jw, M9 mw, F4 vF, hipStream_t s);
 // zeroes counters[0..1] too when counters is not null
 void launch_zero_increment_ghosts(const Geo &g, F5 NL, F5 NR, int *counters, hipStream_t s);
 void launch_sponge(const Geo &g, const Phys &p, double e, int gstart, int gend0, int nx_global, int nx_inlet,

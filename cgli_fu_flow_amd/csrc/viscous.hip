@@ -1,41 +1,26 @@
-// viscous.hip -- viscous and heat-flux terms on gfx950.  The face pass is
-// templated on its arithmetic type (double: the reference path; float: the
-// FP32 residual path, loading the FP64 primitives and metrics and storing
-// FP64 face fluxes).
+// viscous.hip -- viscous and heat-flux terms on gfx950, templated on the
+// arithmetic type of the face fluxes (double: the reference path; float: the
+// FP32 residual path, reading the FP64 primitives and metrics and summing
+// the divergence of FP64-stored face fluxes).
 //
-// Three passes (Viscous_terms.cpp:183-905):
-//   1. primitives (u, v, w, T) of every cell the stencil touches, once;
-//   2. per cell, the fluxes through its low x, y and z faces (each face only
-//      where the divergence uses it), from second-order face gradients with
-//      four-point cross averages and Sutherland viscosity;
+// The reference's three stages (Viscous_terms.cpp:183-905) in one kernel:
+//   1. primitives (u, v, w, T) of every cell the stencil touches, once per
+//      block;
+//   2. per cell, the fluxes through its low x, y and z faces, from
+//      second-order face gradients with four-point cross averages and
+//      Sutherland viscosity;
 //   3. vF = dF/dxi + dG/deta + dH/dzeta on interior cells.
-// Pass 1 replaces the reference's 16 primitive reconstructions per cell
-// with one per cell; the values are the same doubles.
+// A block owns a (j, k) tile and marches along i through a segment of
+// planes: a ring of three planes of primitives (tile plus a 1-cell halo) and
+// the tile's y and z face fluxes live in LDS, the x face flux of the cell
+// below travels in registers, so that U, J and each face's metrics are read
+// once and vF is the only store.  The values are the same doubles as the
+// reference's 16 primitive reconstructions per cell.
+#include <algorithm>
+
 #include "cgli_internal.hpp"
 
 namespace cgli {
-
-__global__ void __launch_bounds__(256) k_prim4(Geo g, double K, double R, C5 U, const double *__restrict__ J,
-                                               double *__restrict__ P4)
-{
-    // box i=2..gend+1, j=1..NYY, k=1..NZZ; (j, k) of a plane flattened, k
-    // fastest, so that rows of NZZ cells fill whole waves
-    const int nk = g.NZZ;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nk * g.NYY) return;
-    const int k = 1 + t % nk;
-    const int j = 1 + t / nk;
-    const int i = 2 + blockIdx.z;
-    const long c = ix(g, i, j, k);
-    const double r = U.v[0][c];
-    const double u = U.v[1][c] / r, v = U.v[2][c] / r, w = U.v[3][c] / r;
-    const double P = (U.v[4][c] - 0.5 * r * (u * u + v * v + w * w)) * (K - 1) * J[c];
-    const double T = P / (r * R) / J[c];
-    P4[c] = u;
-    P4[c + g.N] = v;
-    P4[c + 2 * g.N] = w;
-    P4[c + 3 * g.N] = T;
-}
 
 template <typename Real>
 struct Pt { Real u, v, w, T; };
@@ -153,174 +138,245 @@ __device__ __forceinline__ Real suth(const Phys &p, Real T)
     return (Real)p.mu_L * pow((T / Ts), Real(1.5)) * (Ts + S) / (T + S);
 }
 
-// Face fluxes of cell (i,j,k): x face i-1/2 for i<=gend+1 (j<=NY, k<=NZ),
-// y face j-1/2 for j<=NYY (i<=gend, k<=NZ), z face k-1/2 for k<=NZZ
-// (i<=gend, j<=NY).  Faces go to FF[var*3+dir] (vars 2..5 -> 0..3).
-// DIR is the face direction (one launch each): a thread computes one face,
-// so it holds one direction's stencil and metric groups (the three
-// together needed about 250 VGPRs, two waves per SIMD).
-template <typename Real, int DIR>
-__global__ void __launch_bounds__(256) k_visc_faces(Geo g, Phys p, const double *__restrict__ P4,
-                                                    const double *__restrict__ Ju, M9 mu,
-                                                    const double *__restrict__ Jv, M9 mv,
-                                                    const double *__restrict__ Jw, M9 mw,
-                                                    double *__restrict__ FF)
+// The face fluxes (variables 2..5 of the reference, here 0..3).  x face
+// i-1/2 of cell (i, j, k): O = P(i, j, k), iO = P(i-1, j, k), and the j and k
+// neighbours of both; Jf and m are the face point's Jacobian and metrics.
+template <typename Real>
+__device__ __forceinline__ void xflux(const Phys &p, const Pt<Real> &O, const Pt<Real> &iO, const Pt<Real> &Oj,
+                                      const Pt<Real> &iOj, const Pt<Real> &jO, const Pt<Real> &ijO, const Pt<Real> &Ok,
+                                      const Pt<Real> &iOk, const Pt<Real> &kO, const Pt<Real> &ikO, const Real m[9],
+                                      Real Jf, double F[4])
 {
-    // (j, k) = (2..NYY, 2..NZZ) of a plane flattened, k fastest
-    const int nk = g.NZZ - 1;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nk * (g.NYY - 1)) return;
-    const int k = 2 + t % nk;
-    const int j = 2 + t / nk;
-    const int i = 3 + blockIdx.z;
-    constexpr int dir = DIR;
-    const bool xf = dir == 0 && (j <= g.NY && k <= g.NZ);
-    const bool yf = dir == 1 && (i <= g.gend && k <= g.NZ);
-    const bool zf = dir == 2 && (i <= g.gend && j <= g.NY);
-    if (!(xf || yf || zf)) return;
-    const long N = g.N, sx = g.Ym * g.Zm, sy = g.Zm;
-    const long c = ix(g, i, j, k);
-    const long fid = ix(g, i - 1, j - 1, k - 1);
-    const Real invXI = (Real)(1. / (p.dxi)), invET = (Real)(1. / (p.deta)), invZT = (Real)(1. / (p.dzeta));
-    const Real inv4XI = (Real)(1. / (4 * p.dxi)), inv4ET = (Real)(1. / (4 * p.deta)),
-               inv4ZT = (Real)(1. / (4 * p.dzeta));
+    const Real invXI = (Real)(1. / (p.dxi)), inv4ET = (Real)(1. / (4 * p.deta)), inv4ZT = (Real)(1. / (4 * p.dzeta));
     const Real CvK = (Real)(p.Cv * p.K), Pr = (Real)p.Pr_L;
-    const Pt<Real> O = ld<Real>(P4, N, c);
-    const Pt<Real> iO = ld<Real>(P4, N, c - sx), jO = ld<Real>(P4, N, c - sy), kO = ld<Real>(P4, N, c - 1);
-    Real m[9];
     Grp<Real> G;
-    if (xf) {
-        const Pt<Real> Oj = ld<Real>(P4, N, c + sy), iOj = ld<Real>(P4, N, c - sx + sy), ijO = ld<Real>(P4, N, c - sx - sy);
-        const Pt<Real> Ok = ld<Real>(P4, N, c + 1), iOk = ld<Real>(P4, N, c - sx + 1), ikO = ld<Real>(P4, N, c - sx - 1);
-        const D10<Real> X = d2(O, iO, invXI);
-        const D10<Real> Y = d4(Oj, iOj, jO, ijO, inv4ET);
-        const D10<Real> Z = d4(Ok, iOk, kO, ikO, inv4ZT);
-#pragma unroll
-        for (int q = 0; q < 9; q++) m[q] = (Real)mu.m[q][fid];
-        groups(m, G);
-        const Real Ux = Real(0.5) * (O.u + iO.u), Vx = Real(0.5) * (O.v + iO.v), Wx = Real(0.5) * (O.w + iO.w);
-        const Real Tx = Real(0.5) * (O.T + iO.T);
-        const Real mu_ = suth<Real>(p, Tx);
-        const Real Jf = (Real)Ju[fid];
-        FF[(0 * 3 + 0) * N + c] = mu_ * (G.a[1] * X.u + G.a[5] * X.v + G.a[7] * X.w + G.d[1] * Y.u + G.d[7] * Y.v +
-                                         G.d[9] * Y.w + G.e[1] * Z.u + G.e[7] * Z.v + G.e[9] * Z.w) / Jf;
-        FF[(1 * 3 + 0) * N + c] = mu_ * (G.a[5] * X.u + G.a[2] * X.v + G.a[6] * X.w + G.d[5] * Y.u + G.d[2] * Y.v +
-                                         G.d[10] * Y.w + G.e[5] * Z.u + G.e[2] * Z.v + G.e[10] * Z.w) / Jf;
-        FF[(2 * 3 + 0) * N + c] = mu_ * (G.a[7] * X.u + G.a[6] * X.v + G.a[3] * X.w + G.d[6] * Y.u + G.d[8] * Y.v +
-                                         G.d[3] * Y.w + G.e[6] * Z.u + G.e[8] * Z.v + G.e[3] * Z.w) / Jf;
-        FF[(3 * 3 + 0) * N + c] =
-            mu_ * (Real(0.5) * G.a[1] * X.u2 + Real(0.5) * G.a[2] * X.v2 + Real(0.5) * G.a[3] * X.w2 + G.a[5] * X.uv + G.a[6] * X.vw +
-                   G.a[7] * X.uw + CvK * G.a[4] * X.T / (Pr) + Real(0.5) * G.d[1] * Y.u2 + Real(0.5) * G.d[2] * Y.v2 +
-                   Real(0.5) * G.d[3] * Y.w2 + G.d[5] * Vx * Y.u + G.d[6] * Wx * Y.u + G.d[7] * Ux * Y.v +
-                   G.d[8] * Wx * Y.v + G.d[9] * Ux * Y.w + G.d[10] * Vx * Y.w + CvK * G.d[4] * Y.T / (Pr) +
-                   Real(0.5) * G.e[1] * Z.u2 + Real(0.5) * G.e[2] * Z.v2 + Real(0.5) * G.e[3] * Z.w2 + G.e[5] * Vx * Z.u +
-                   G.e[6] * Wx * Z.u + G.e[7] * Ux * Z.v + G.e[8] * Wx * Z.v + G.e[9] * Ux * Z.w +
-                   G.e[10] * Vx * Z.w + CvK * G.e[4] * Z.T / (Pr)) /
-            Jf;
-    }
-    if (yf) {
-        const Pt<Real> Oi = ld<Real>(P4, N, c + sx), jOi = ld<Real>(P4, N, c + sx - sy), ijO = ld<Real>(P4, N, c - sx - sy);
-        const Pt<Real> Ok = ld<Real>(P4, N, c + 1), jOk = ld<Real>(P4, N, c - sy + 1), jkO = ld<Real>(P4, N, c - sy - 1);
-        const D10<Real> X = d4(Oi, jOi, iO, ijO, inv4XI);
-        const D10<Real> Y = d2(O, jO, invET);
-        const D10<Real> Z = d4(Ok, jOk, kO, jkO, inv4ZT);
-#pragma unroll
-        for (int q = 0; q < 9; q++) m[q] = (Real)mv.m[q][fid];
-        groups(m, G);
-        const Real Uy = Real(0.5) * (O.u + jO.u), Vy = Real(0.5) * (O.v + jO.v), Wy = Real(0.5) * (O.w + jO.w);
-        const Real Ty = Real(0.5) * (O.T + jO.T);
-        const Real mu_ = suth<Real>(p, Ty);
-        const Real Jf = (Real)Jv[fid];
-        FF[(0 * 3 + 1) * N + c] = mu_ * (G.d[1] * X.u + G.d[5] * X.v + G.d[6] * X.w + G.b[1] * Y.u + G.b[5] * Y.v +
-                                         G.b[7] * Y.w + G.f[1] * Z.u + G.f[7] * Z.v + G.f[9] * Z.w) / Jf;
-        FF[(1 * 3 + 1) * N + c] = mu_ * (G.d[7] * X.u + G.d[2] * X.v + G.d[8] * X.w + G.b[5] * Y.u + G.b[2] * Y.v +
-                                         G.b[6] * Y.w + G.f[5] * Z.u + G.f[2] * Z.v + G.f[10] * Z.w) / Jf;
-        FF[(2 * 3 + 1) * N + c] = mu_ * (G.d[9] * X.u + G.d[10] * X.v + G.d[3] * X.w + G.b[7] * Y.u + G.b[6] * Y.v +
-                                         G.b[3] * Y.w + G.f[6] * Z.u + G.f[8] * Z.v + G.f[3] * Z.w) / Jf;
-        FF[(3 * 3 + 1) * N + c] =
-            mu_ * (Real(0.5) * G.b[1] * Y.u2 + Real(0.5) * G.b[2] * Y.v2 + Real(0.5) * G.b[3] * Y.w2 + G.b[5] * Y.uv + G.b[6] * Y.vw +
-                   G.b[7] * Y.uw + CvK * G.b[4] * Y.T / (Pr) + Real(0.5) * G.d[1] * X.u2 + Real(0.5) * G.d[2] * X.v2 +
-                   Real(0.5) * G.d[3] * X.w2 + G.d[5] * Uy * X.v + G.d[6] * Uy * X.w + G.d[7] * Vy * X.u +
-                   G.d[8] * Vy * X.w + G.d[9] * Wy * X.u + G.d[10] * Wy * X.v + CvK * G.d[4] * X.T / (Pr) +
-                   Real(0.5) * G.f[1] * Z.u2 + Real(0.5) * G.f[2] * Z.v2 + Real(0.5) * G.f[3] * Z.w2 + G.f[5] * Vy * Z.u +
-                   G.f[6] * Wy * Z.u + G.f[7] * Uy * Z.v + G.f[8] * Wy * Z.v + G.f[9] * Uy * Z.w +
-                   G.f[10] * Vy * Z.w + CvK * G.f[4] * Z.T / (Pr)) /
-            Jf;
-    }
-    if (zf) {
-        const Pt<Real> Oi = ld<Real>(P4, N, c + sx), kOi = ld<Real>(P4, N, c + sx - 1), ikO = ld<Real>(P4, N, c - sx - 1);
-        const Pt<Real> Oj = ld<Real>(P4, N, c + sy), kOj = ld<Real>(P4, N, c + sy - 1), jkO = ld<Real>(P4, N, c - sy - 1);
-        const Pt<Real> iOj = ld<Real>(P4, N, c - sx + sy);
-        const D10<Real> X = d4(Oi, kOi, iO, ikO, inv4XI);
-        D10<Real> Y = d4(Oj, kOj, jO, jkO, inv4ET);
-        // Viscous_terms.cpp:744-746 pair kUj with iVj / iWj
-        Y.uv = (Oj.u * Oj.v + kOj.u * iOj.v - jO.u * jO.v - jkO.u * jkO.v) * inv4ET;
-        Y.vw = (Oj.v * Oj.w + kOj.v * iOj.w - jO.v * jO.w - jkO.v * jkO.w) * inv4ET;
-        Y.uw = (Oj.u * Oj.w + kOj.u * iOj.w - jO.u * jO.w - jkO.u * jkO.w) * inv4ET;
-        const D10<Real> Z = d2(O, kO, invZT);
-#pragma unroll
-        for (int q = 0; q < 9; q++) m[q] = (Real)mw.m[q][fid];
-        groups(m, G);
-        const Real Uz = Real(0.5) * (O.u + kO.u), Vz = Real(0.5) * (O.v + kO.v), Wz = Real(0.5) * (O.w + kO.w);
-        const Real Tz = Real(0.5) * (O.T + kO.T);
-        const Real mu_ = suth<Real>(p, Tz);
-        const Real Jf = (Real)Jw[fid];
-        FF[(0 * 3 + 2) * N + c] = mu_ * (G.e[1] * X.u + G.e[5] * X.v + G.e[6] * X.w + G.f[1] * Y.u + G.f[5] * Y.v +
-                                         G.f[6] * Y.w + G.c[1] * Z.u + G.c[5] * Z.v + G.c[7] * Z.w) / Jf;
-        FF[(1 * 3 + 2) * N + c] = mu_ * (G.e[7] * X.u + G.e[2] * X.v + G.e[8] * X.w + G.f[7] * Y.u + G.f[2] * Y.v +
-                                         G.f[8] * Y.w + G.c[5] * Z.u + G.c[2] * Z.v + G.c[6] * Z.w) / Jf;
-        FF[(2 * 3 + 2) * N + c] = mu_ * (G.e[9] * X.u + G.e[10] * X.v + G.e[3] * X.w + G.f[9] * Y.u + G.f[10] * Y.v +
-                                         G.f[3] * Y.w + G.c[7] * Z.u + G.c[6] * Z.v + G.c[3] * Z.w) / Jf;
-        FF[(3 * 3 + 2) * N + c] =
-            mu_ * (Real(0.5) * G.c[1] * Z.u2 + Real(0.5) * G.c[2] * Z.v2 + Real(0.5) * G.c[3] * Z.w2 + G.c[5] * Z.uv + G.c[6] * Z.vw +
-                   G.c[7] * Z.uw + CvK * G.c[4] * Z.T / (Pr) + Real(0.5) * G.e[1] * X.u2 + Real(0.5) * G.e[2] * X.v2 +
-                   Real(0.5) * G.e[3] * X.w2 + G.e[5] * Uz * X.v + G.e[6] * Uz * X.w + G.e[7] * Vz * X.u +
-                   G.e[8] * Vz * X.w + G.e[9] * Wz * X.u + G.e[10] * Wz * X.v + CvK * G.e[4] * X.T / (Pr) +
-                   Real(0.5) * G.f[1] * Y.u2 + Real(0.5) * G.f[2] * Y.v2 + Real(0.5) * G.f[3] * Y.w2 + G.f[5] * Uz * Y.v +
-                   G.f[6] * Uz * Y.w + G.f[7] * Vz * Y.u + G.f[8] * Vz * Y.w + G.f[9] * Wz * Y.u +
-                   G.f[10] * Wz * Y.v + CvK * G.f[4] * Y.T / (Pr)) /
-            Jf;
-    }
-    (void)ix;
+    const D10<Real> X = d2(O, iO, invXI);
+    const D10<Real> Y = d4(Oj, iOj, jO, ijO, inv4ET);
+    const D10<Real> Z = d4(Ok, iOk, kO, ikO, inv4ZT);
+    groups(m, G);
+    const Real Ux = Real(0.5) * (O.u + iO.u), Vx = Real(0.5) * (O.v + iO.v), Wx = Real(0.5) * (O.w + iO.w);
+    const Real Tx = Real(0.5) * (O.T + iO.T);
+    const Real mu_ = suth<Real>(p, Tx);
+    F[0] = mu_ * (G.a[1] * X.u + G.a[5] * X.v + G.a[7] * X.w + G.d[1] * Y.u + G.d[7] * Y.v + G.d[9] * Y.w +
+                  G.e[1] * Z.u + G.e[7] * Z.v + G.e[9] * Z.w) / Jf;
+    F[1] = mu_ * (G.a[5] * X.u + G.a[2] * X.v + G.a[6] * X.w + G.d[5] * Y.u + G.d[2] * Y.v + G.d[10] * Y.w +
+                  G.e[5] * Z.u + G.e[2] * Z.v + G.e[10] * Z.w) / Jf;
+    F[2] = mu_ * (G.a[7] * X.u + G.a[6] * X.v + G.a[3] * X.w + G.d[6] * Y.u + G.d[8] * Y.v + G.d[3] * Y.w +
+                  G.e[6] * Z.u + G.e[8] * Z.v + G.e[3] * Z.w) / Jf;
+    F[3] = mu_ * (Real(0.5) * G.a[1] * X.u2 + Real(0.5) * G.a[2] * X.v2 + Real(0.5) * G.a[3] * X.w2 + G.a[5] * X.uv +
+                  G.a[6] * X.vw + G.a[7] * X.uw + CvK * G.a[4] * X.T / (Pr) + Real(0.5) * G.d[1] * Y.u2 +
+                  Real(0.5) * G.d[2] * Y.v2 + Real(0.5) * G.d[3] * Y.w2 + G.d[5] * Vx * Y.u + G.d[6] * Wx * Y.u +
+                  G.d[7] * Ux * Y.v + G.d[8] * Wx * Y.v + G.d[9] * Ux * Y.w + G.d[10] * Vx * Y.w +
+                  CvK * G.d[4] * Y.T / (Pr) + Real(0.5) * G.e[1] * Z.u2 + Real(0.5) * G.e[2] * Z.v2 +
+                  Real(0.5) * G.e[3] * Z.w2 + G.e[5] * Vx * Z.u + G.e[6] * Wx * Z.u + G.e[7] * Ux * Z.v +
+                  G.e[8] * Wx * Z.v + G.e[9] * Ux * Z.w + G.e[10] * Vx * Z.w + CvK * G.e[4] * Z.T / (Pr)) /
+           Jf;
 }
 
-__global__ void __launch_bounds__(256) k_visc_div(Geo g, Phys p, const double *__restrict__ FF, F4 vF)
+// y face j-1/2 of cell (i, j, k): O = P(i, j, k), jO = P(i, j-1, k), and the
+// i and k neighbours of both
+template <typename Real>
+__device__ __forceinline__ void yflux(const Phys &p, const Pt<Real> &O, const Pt<Real> &jO, const Pt<Real> &Oi,
+                                      const Pt<Real> &jOi, const Pt<Real> &iO, const Pt<Real> &ijO, const Pt<Real> &Ok,
+                                      const Pt<Real> &jOk, const Pt<Real> &kO, const Pt<Real> &jkO, const Real m[9],
+                                      Real Jf, double F[4])
 {
-    const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = 3 + blockIdx.z;
-    if (k > g.NZ || j > g.NY) return;
-    const long N = g.N, sx = g.Ym * g.Zm, sy = g.Zm;
-    const long c = ix(g, i, j, k);
-    const double invXI = 1. / (p.dxi), invET = 1. / (p.deta), invZT = 1. / (p.dzeta);
+    const Real inv4XI = (Real)(1. / (4 * p.dxi)), invET = (Real)(1. / (p.deta)), inv4ZT = (Real)(1. / (4 * p.dzeta));
+    const Real CvK = (Real)(p.Cv * p.K), Pr = (Real)p.Pr_L;
+    Grp<Real> G;
+    const D10<Real> X = d4(Oi, jOi, iO, ijO, inv4XI);
+    const D10<Real> Y = d2(O, jO, invET);
+    const D10<Real> Z = d4(Ok, jOk, kO, jkO, inv4ZT);
+    groups(m, G);
+    const Real Uy = Real(0.5) * (O.u + jO.u), Vy = Real(0.5) * (O.v + jO.v), Wy = Real(0.5) * (O.w + jO.w);
+    const Real Ty = Real(0.5) * (O.T + jO.T);
+    const Real mu_ = suth<Real>(p, Ty);
+    F[0] = mu_ * (G.d[1] * X.u + G.d[5] * X.v + G.d[6] * X.w + G.b[1] * Y.u + G.b[5] * Y.v + G.b[7] * Y.w +
+                  G.f[1] * Z.u + G.f[7] * Z.v + G.f[9] * Z.w) / Jf;
+    F[1] = mu_ * (G.d[7] * X.u + G.d[2] * X.v + G.d[8] * X.w + G.b[5] * Y.u + G.b[2] * Y.v + G.b[6] * Y.w +
+                  G.f[5] * Z.u + G.f[2] * Z.v + G.f[10] * Z.w) / Jf;
+    F[2] = mu_ * (G.d[9] * X.u + G.d[10] * X.v + G.d[3] * X.w + G.b[7] * Y.u + G.b[6] * Y.v + G.b[3] * Y.w +
+                  G.f[6] * Z.u + G.f[8] * Z.v + G.f[3] * Z.w) / Jf;
+    F[3] = mu_ * (Real(0.5) * G.b[1] * Y.u2 + Real(0.5) * G.b[2] * Y.v2 + Real(0.5) * G.b[3] * Y.w2 + G.b[5] * Y.uv +
+                  G.b[6] * Y.vw + G.b[7] * Y.uw + CvK * G.b[4] * Y.T / (Pr) + Real(0.5) * G.d[1] * X.u2 +
+                  Real(0.5) * G.d[2] * X.v2 + Real(0.5) * G.d[3] * X.w2 + G.d[5] * Uy * X.v + G.d[6] * Uy * X.w +
+                  G.d[7] * Vy * X.u + G.d[8] * Vy * X.w + G.d[9] * Wy * X.u + G.d[10] * Wy * X.v +
+                  CvK * G.d[4] * X.T / (Pr) + Real(0.5) * G.f[1] * Z.u2 + Real(0.5) * G.f[2] * Z.v2 +
+                  Real(0.5) * G.f[3] * Z.w2 + G.f[5] * Vy * Z.u + G.f[6] * Wy * Z.u + G.f[7] * Uy * Z.v +
+                  G.f[8] * Wy * Z.v + G.f[9] * Uy * Z.w + G.f[10] * Vy * Z.w + CvK * G.f[4] * Z.T / (Pr)) /
+           Jf;
+}
+
+// z face k-1/2 of cell (i, j, k): O = P(i, j, k), kO = P(i, j, k-1), and the
+// i and j neighbours of both; iOj = P(i-1, j+1, k) enters through the
+// reference's pairing of kUj with iVj / iWj (Viscous_terms.cpp:744-746)
+template <typename Real>
+__device__ __forceinline__ void zflux(const Phys &p, const Pt<Real> &O, const Pt<Real> &kO, const Pt<Real> &Oi,
+                                      const Pt<Real> &kOi, const Pt<Real> &iO, const Pt<Real> &ikO, const Pt<Real> &Oj,
+                                      const Pt<Real> &kOj, const Pt<Real> &jO, const Pt<Real> &jkO, const Pt<Real> &iOj,
+                                      const Real m[9], Real Jf, double F[4])
+{
+    const Real inv4XI = (Real)(1. / (4 * p.dxi)), inv4ET = (Real)(1. / (4 * p.deta)), invZT = (Real)(1. / (p.dzeta));
+    const Real CvK = (Real)(p.Cv * p.K), Pr = (Real)p.Pr_L;
+    Grp<Real> G;
+    const D10<Real> X = d4(Oi, kOi, iO, ikO, inv4XI);
+    D10<Real> Y = d4(Oj, kOj, jO, jkO, inv4ET);
+    Y.uv = (Oj.u * Oj.v + kOj.u * iOj.v - jO.u * jO.v - jkO.u * jkO.v) * inv4ET;
+    Y.vw = (Oj.v * Oj.w + kOj.v * iOj.w - jO.v * jO.w - jkO.v * jkO.w) * inv4ET;
+    Y.uw = (Oj.u * Oj.w + kOj.u * iOj.w - jO.u * jO.w - jkO.u * jkO.w) * inv4ET;
+    const D10<Real> Z = d2(O, kO, invZT);
+    groups(m, G);
+    const Real Uz = Real(0.5) * (O.u + kO.u), Vz = Real(0.5) * (O.v + kO.v), Wz = Real(0.5) * (O.w + kO.w);
+    const Real Tz = Real(0.5) * (O.T + kO.T);
+    const Real mu_ = suth<Real>(p, Tz);
+    F[0] = mu_ * (G.e[1] * X.u + G.e[5] * X.v + G.e[6] * X.w + G.f[1] * Y.u + G.f[5] * Y.v + G.f[6] * Y.w +
+                  G.c[1] * Z.u + G.c[5] * Z.v + G.c[7] * Z.w) / Jf;
+    F[1] = mu_ * (G.e[7] * X.u + G.e[2] * X.v + G.e[8] * X.w + G.f[7] * Y.u + G.f[2] * Y.v + G.f[8] * Y.w +
+                  G.c[5] * Z.u + G.c[2] * Z.v + G.c[6] * Z.w) / Jf;
+    F[2] = mu_ * (G.e[9] * X.u + G.e[10] * X.v + G.e[3] * X.w + G.f[9] * Y.u + G.f[10] * Y.v + G.f[3] * Y.w +
+                  G.c[7] * Z.u + G.c[6] * Z.v + G.c[3] * Z.w) / Jf;
+    F[3] = mu_ * (Real(0.5) * G.c[1] * Z.u2 + Real(0.5) * G.c[2] * Z.v2 + Real(0.5) * G.c[3] * Z.w2 + G.c[5] * Z.uv +
+                  G.c[6] * Z.vw + G.c[7] * Z.uw + CvK * G.c[4] * Z.T / (Pr) + Real(0.5) * G.e[1] * X.u2 +
+                  Real(0.5) * G.e[2] * X.v2 + Real(0.5) * G.e[3] * X.w2 + G.e[5] * Uz * X.v + G.e[6] * Uz * X.w +
+                  G.e[7] * Vz * X.u + G.e[8] * Vz * X.w + G.e[9] * Wz * X.u + G.e[10] * Wz * X.v +
+                  CvK * G.e[4] * X.T / (Pr) + Real(0.5) * G.f[1] * Y.u2 + Real(0.5) * G.f[2] * Y.v2 +
+                  Real(0.5) * G.f[3] * Y.w2 + G.f[5] * Uz * Y.v + G.f[6] * Uz * Y.w + G.f[7] * Vz * Y.u +
+                  G.f[8] * Vz * Y.w + G.f[9] * Wz * Y.u + G.f[10] * Wz * Y.v + CvK * G.f[4] * Y.T / (Pr)) /
+           Jf;
+}
+
+// Tile of the fused kernel: the threads are VT_J x VT_K face positions (j, k)
+// (k fastest); the cells whose four side faces all lie among them, the
+// first VT_J - 1 rows and VT_K - 1 columns, are the tile's output.
+constexpr int VT_J = 8, VT_K = 32, VT_THREADS = VT_J * VT_K;
+constexpr int VH_J = VT_J + 2, VH_K = VT_K + 2;   // staged primitives: positions plus a 1-cell halo
+
+// primitives (u, v, w, T) of plane i of the tile's halo into ring slot sl
+__device__ __forceinline__ void visc_plane(const Geo &g, const Phys &p, C5 U, const double *__restrict__ J, int i,
+                                           int j0, int k0, double (*sP)[4][VH_J][VH_K])
+{
+    for (int t = threadIdx.x; t < VH_J * VH_K; t += VT_THREADS) {
+        const int hj = t / VH_K, hk = t % VH_K;
+        const int j = j0 - 1 + hj, k = k0 - 1 + hk;
+        if (j > g.NYY || k > g.NZZ) continue;   // outside the primitive box: never read for a used face
+        const long c = ix(g, i, j, k);
+        const double r = U.v[0][c];
+        const double u = U.v[1][c] / r, v = U.v[2][c] / r, w = U.v[3][c] / r;
+        const double P = (U.v[4][c] - 0.5 * r * (u * u + v * v + w * w)) * (p.K - 1) * J[c];
+        const double T = P / (r * p.R) / J[c];
+        double(*pl)[VH_J][VH_K] = sP[i % 3];
+        pl[0][hj][hk] = u;
+        pl[1][hj][hk] = v;
+        pl[2][hj][hk] = w;
+        pl[3][hj][hk] = T;
+    }
+}
+
+template <typename Real>
+__device__ __forceinline__ Pt<Real> lp(const double (*sP)[4][VH_J][VH_K], int i, int hj, int hk)
+{
+    const double(*pl)[VH_J][VH_K] = sP[i % 3];
+    return Pt<Real>{(Real)pl[0][hj][hk], (Real)pl[1][hj][hk], (Real)pl[2][hj][hk], (Real)pl[3][hj][hk]};
+}
+
+// x face of cell (i, j, k) (between planes i-1 and i) at the thread's position
+template <typename Real>
+__device__ __forceinline__ void xface(const Geo &g, const Phys &p, const double (*sP)[4][VH_J][VH_K], int i, int j,
+                                      int k, int hj, int hk, const double *__restrict__ Ju, const M9 &mu, double F[4])
+{
+    const long fid = ix(g, i - 1, j - 1, k - 1);
+    Real m[9];
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const double *X = FF + (v * 3 + 0) * N, *Y = FF + (v * 3 + 1) * N, *Z = FF + (v * 3 + 2) * N;
-        vF.v[v][c] = (X[c + sx] - X[c]) * invXI + (Y[c + sy] - Y[c]) * invET + (Z[c + 1] - Z[c]) * invZT;
+    for (int q = 0; q < 9; q++) m[q] = (Real)mu.m[q][fid];
+    xflux<Real>(p, lp<Real>(sP, i, hj, hk), lp<Real>(sP, i - 1, hj, hk), lp<Real>(sP, i, hj + 1, hk),
+                lp<Real>(sP, i - 1, hj + 1, hk), lp<Real>(sP, i, hj - 1, hk), lp<Real>(sP, i - 1, hj - 1, hk),
+                lp<Real>(sP, i, hj, hk + 1), lp<Real>(sP, i - 1, hj, hk + 1), lp<Real>(sP, i, hj, hk - 1),
+                lp<Real>(sP, i - 1, hj, hk - 1), m, (Real)Ju[fid], F);
+}
+
+// vF of the interior cells.  Grid: (tiles in k, tiles in j, segments of seg
+// planes in i); every face the reference's divergence reads is computed
+// (x faces i <= gend+1, y faces j <= NYY, z faces k <= NZZ).
+template <typename Real>
+__global__ void __launch_bounds__(VT_THREADS, 2) k_viscous(Geo g, Phys p, C5 U, const double *__restrict__ J,
+                                                        const double *__restrict__ Ju, M9 mu,
+                                                        const double *__restrict__ Jv, M9 mv,
+                                                        const double *__restrict__ Jw, M9 mw, F4 vF, int seg)
+{
+    __shared__ double sP[3][4][VH_J][VH_K];    // primitives of planes i-1, i, i+1 (slot i % 3)
+    __shared__ double sF[2][4][VT_J][VT_K];    // y and z face fluxes of the current plane
+    const int tk = threadIdx.x % VT_K, tj = threadIdx.x / VT_K;
+    const int j0 = 2 + blockIdx.y * (VT_J - 1), k0 = 2 + blockIdx.x * (VT_K - 1);
+    const int j = j0 + tj, k = k0 + tk, hj = tj + 1, hk = tk + 1;
+    const int iS = 3 + blockIdx.z * seg, iE = min(iS + seg, g.gend + 1);
+    const bool out = tj < VT_J - 1 && tk < VT_K - 1 && j <= g.NY && k <= g.NZ;   // an output cell
+    const bool yok = j <= g.NYY && k <= g.NZ, zok = j <= g.NY && k <= g.NZZ;   // its low y / z face exists
+    const double invXI = 1. / (p.dxi), invET = 1. / (p.deta), invZT = 1. / (p.dzeta);
+    visc_plane(g, p, U, J, iS - 1, j0, k0, sP);
+    visc_plane(g, p, U, J, iS, j0, k0, sP);
+    __syncthreads();
+    double Fx[4] = {0.0, 0.0, 0.0, 0.0};   // x face below the current cell
+    if (out) xface<Real>(g, p, sP, iS, j, k, hj, hk, Ju, mu, Fx);
+    for (int i = iS; i < iE; i++) {
+        visc_plane(g, p, U, J, i + 1, j0, k0, sP);
+        __syncthreads();
+        // the three faces one after the other: the scheduling barriers keep the
+        // compiler from hoisting all their stencil reads at once
+        double Fh[4] = {0.0, 0.0, 0.0, 0.0};
+        if (out) xface<Real>(g, p, sP, i + 1, j, k, hj, hk, Ju, mu, Fh);
+        __builtin_amdgcn_sched_barrier(0);
+        if (yok) {
+            const long fid = ix(g, i - 1, j - 1, k - 1);
+            Real m[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) m[q] = (Real)mv.m[q][fid];
+            double F[4];
+            yflux<Real>(p, lp<Real>(sP, i, hj, hk), lp<Real>(sP, i, hj - 1, hk), lp<Real>(sP, i + 1, hj, hk),
+                        lp<Real>(sP, i + 1, hj - 1, hk), lp<Real>(sP, i - 1, hj, hk), lp<Real>(sP, i - 1, hj - 1, hk),
+                        lp<Real>(sP, i, hj, hk + 1), lp<Real>(sP, i, hj - 1, hk + 1), lp<Real>(sP, i, hj, hk - 1),
+                        lp<Real>(sP, i, hj - 1, hk - 1), m, (Real)Jv[fid], F);
+#pragma unroll
+            for (int v = 0; v < 4; v++) sF[0][v][tj][tk] = F[v];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (zok) {
+            const long fid = ix(g, i - 1, j - 1, k - 1);
+            Real m[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) m[q] = (Real)mw.m[q][fid];
+            double F[4];
+            zflux<Real>(p, lp<Real>(sP, i, hj, hk), lp<Real>(sP, i, hj, hk - 1), lp<Real>(sP, i + 1, hj, hk),
+                        lp<Real>(sP, i + 1, hj, hk - 1), lp<Real>(sP, i - 1, hj, hk), lp<Real>(sP, i - 1, hj, hk - 1),
+                        lp<Real>(sP, i, hj + 1, hk), lp<Real>(sP, i, hj + 1, hk - 1), lp<Real>(sP, i, hj - 1, hk),
+                        lp<Real>(sP, i, hj - 1, hk - 1), lp<Real>(sP, i - 1, hj + 1, hk), m, (Real)Jw[fid], F);
+#pragma unroll
+            for (int v = 0; v < 4; v++) sF[1][v][tj][tk] = F[v];
+        }
+        __syncthreads();
+        if (out) {
+            const long c = ix(g, i, j, k);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                vF.v[v][c] = (Fh[v] - Fx[v]) * invXI + (sF[0][v][tj + 1][tk] - sF[0][v][tj][tk]) * invET +
+                             (sF[1][v][tj][tk + 1] - sF[1][v][tj][tk]) * invZT;
+                Fx[v] = Fh[v];
+            }
+        }
     }
 }
 
 void launch_viscous(const Geo &g, const Phys &p, C5 U, const double *J, const double *Ju, M9 mu,
-                    const double *Jv, M9 mv, const double *Jw, M9 mw, double *prim4, double *faces12,
-                    F4 vF, hipStream_t s)
+                    const double *Jv, M9 mv, const double *Jw, M9 mw, F4 vF, hipStream_t s)
 {
-    const dim3 b1(256);
-    const int np = g.NZZ * g.NYY, nf = (g.NZZ - 1) * (g.NYY - 1);
-    dim3 gp((np + 255) / 256, 1, g.gend);   // i=2..gend+1
-    hipLaunchKernelGGL(k_prim4, gp, b1, 0, s, g, p.K, p.R, U, J, prim4);
-    dim3 gf((nf + 255) / 256, 1, g.gend - 1);   // i=3..gend+1
-    if (p.fp32) {
-        hipLaunchKernelGGL((k_visc_faces<float, 0>), gf, b1, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
-        hipLaunchKernelGGL((k_visc_faces<float, 1>), gf, b1, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
-        hipLaunchKernelGGL((k_visc_faces<float, 2>), gf, b1, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
-    } else {
-        hipLaunchKernelGGL((k_visc_faces<double, 0>), gf, b1, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
-        hipLaunchKernelGGL((k_visc_faces<double, 1>), gf, b1, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
-        hipLaunchKernelGGL((k_visc_faces<double, 2>), gf, b1, 0, s, g, p, prim4, Ju, mu, Jv, mv, Jw, mw, faces12);
-    }
-    dim3 b(64, 4);
-    dim3 gd((g.NZ - 1 + b.x - 1) / b.x, (g.NY - 1 + b.y - 1) / b.y, g.gend - 2);   // i=3..gend
-    hipLaunchKernelGGL(k_visc_div, gd, b, 0, s, g, p, faces12, vF);
+    const int tj = (g.NY - 1 + VT_J - 2) / (VT_J - 1), tk = (g.NZ - 1 + VT_K - 2) / (VT_K - 1);
+    const int ni = g.gend - 2;   // cell planes i = 3..gend
+    // segments of at least 8 planes, enough blocks to fill the chip three times over
+    int seg = (int)std::min<long>(ni, std::max<long>(8, (long)ni * tj * tk / 2304));
+    const dim3 gr(tk, tj, (ni + seg - 1) / seg), bl(VT_THREADS);
+    if (p.fp32)
+        hipLaunchKernelGGL(k_viscous<float>, gr, bl, 0, s, g, p, U, J, Ju, mu, Jv, mv, Jw, mw, vF, seg);
+    else
+        hipLaunchKernelGGL(k_viscous<double>, gr, bl, 0, s, g, p, U, J, Ju, mu, Jv, mv, Jw, mw, vF, seg);
 }
 
 }  // namespace cgli
