@@ -98,6 +98,40 @@ def test_channel_steps_match_oracle(size, skew, xi_mode, re_tau, beta):
                 assert r <= TOL, (step, v, r)
 
 
+def test_channel_graph_steps_match_oracle():
+    """The channel driver replaying its inner iterations from a HIP graph
+    against the oracle sequence: after the first (eager) iteration of each
+    step, every later one is a replay of the captured iteration, with the
+    host's work on the step between the replays.  (Mixed precision replays
+    are compared with the eager mixed run bit for bit in the graph tests.)"""
+    from cgli_fu_flow_amd import native as nv
+    nx, ny, nz = 16, 24, 12
+    print("channel graph vs oracle", (nx, ny, nz), flush=True)
+    orc = load_oracle()
+    ch = nv.channel(180.0, beta=2.2, skew=0.05, xi_mode=1)
+    cfg = nv.channel_config(ch, nx, ny, nz)
+    with nv.Context(cfg) as ctx:
+        dt = ctx.channel_init(ch)
+        ctx.set_graph(True)
+        f = oracle_state(ctx, nv)
+        N, I = ctx.hyperplane()
+        oc = OrcCfg()
+        for name, _ in OrcCfg._fields_:
+            setattr(oc, name, getattr(cfg, name))
+        inner = 5
+        sl = (slice(3, nx + 3), slice(2, ny + 2), slice(2, nz + 2))
+        for step in range(3):
+            it, e1, en = ctx.channel_step(dt, tol=0.0, max_inner=inner)
+            assert it == inner
+            ers = oracle_step(orc, oc, f, ctx.shape, N, I, dt, inner, ch.precond_e)
+            for v in range(5):
+                assert abs(en[v] - ers[-1][v]) <= 1e-10 * abs(ers[-1][v]), (step, v, en[v], ers[-1][v])
+            got = ctx.get_many(nv.U, 5)
+            for v in range(5):
+                r = rel(got[v][sl], f["U"][v][sl])
+                assert r <= TOL, (step, v, r)
+
+
 def test_channel_steps_with_sensor_scaled_dissipation_match_oracle():
     """The channel driver with the Roe dissipation scaled by the filter
     sensor: each step runs the filter, then the steps' fluxes use its face
