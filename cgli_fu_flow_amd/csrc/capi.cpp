@@ -61,6 +61,7 @@ struct cgli_ctx {
     bool norm_wave = false;      // ... with the wavefront's timeout flag
     int lusgs_mode = 4;          // 0: one launch per hyperplane, 1: tiled wavefront, 3: tiled wavefront on
                                  // march-ordered records, 4: march computing its coefficients (default)
+    int sweep_waves = 1;         // schedule 4: waves per tile (1: one wave; 2-4: a chain wave and helpers)
     double *coef = nullptr;      // split-coefficient blocks of the wavefront (float in the mixed mode)
     double *dgn = nullptr;       // diagonal blocks of the wavefront (FP64)
     void *coefM = nullptr;       // the same records in march order (mode 3)
@@ -69,7 +70,8 @@ struct cgli_ctx {
     void *geo4 = nullptr;        // mode 4: J and the 9 metrics in march order
     void *u4 = nullptr;          // mode 4: U_ in march order
     void *x4 = nullptr;          // mode 4: MR, NL and NR in march order
-    double *xbufC = nullptr;     // mode 4: exchange of the k products (xbuf holds the j products)
+    void *rec4 = nullptr;        // mode 4: tile-edge exchange records (j products, then k products)
+    long rec4_bytes = 0;         // bytes of each of the two
     void *gcol4 = nullptr;       // mode 4: each tile lane's column geometry, when the metrics do not vary along i
     bool geo_cols = false;       // the metrics do not vary along i (as of geo4_version)
     long geo_version = 0;        // bumped when J or a cell metric is written
@@ -361,7 +363,7 @@ int cgli_destroy(cgli_ctx *c)
     if (c->geo4) hipFree(c->geo4);
     if (c->u4) hipFree(c->u4);
     if (c->x4) hipFree(c->x4);
-    if (c->xbufC) hipFree(c->xbufC);
+    if (c->rec4) hipFree(c->rec4);
     if (c->gcol4) hipFree(c->gcol4);
     if (c->morder_dev) hipFree(c->morder_dev);
     if (c->xbuf) hipFree(c->xbuf);
@@ -766,9 +768,11 @@ static int wave_setup(cgli_ctx *c)
         HIPCHK(hipMemset(c->u4, 0xFF, fb));
         HIPCHK(hipMemset(c->x4, 0xFF, 3 * fb));
         c->geo4_version = -1;
-        if (hipMalloc(&p, sizeof(double) * 5 * c->g.N) != hipSuccess) return CGLI_E_NOMEM;
-        c->xbufC = static_cast<double *>(p);
-        HIPCHK(hipMemset(c->xbufC, 0, sizeof(double) * 5 * c->g.N));
+        // the exchange records: every offset a sweep forms stays below 2 GiB
+        c->rec4_bytes = march4_rec_bytes(c->g);
+        if (2 * c->rec4_bytes + 64 > (1L << 31)) return CGLI_E_ARG;
+        if (hipMalloc(&p, 2 * c->rec4_bytes) != hipSuccess) return CGLI_E_NOMEM;
+        c->rec4 = p;
         if (hipMalloc(&p, 16 * march4_col_pairs(c->g)) != hipSuccess) return CGLI_E_NOMEM;
         c->gcol4 = p;
     }
@@ -825,6 +829,15 @@ int cgli_set_lusgs_mode(cgli_ctx *c, int mode)
     if (!c || mode < 0 || mode > 4 || mode == 2) return CGLI_E_ARG;
     NO_CAPTURE(c);
     c->lusgs_mode = mode;
+    c->generation++;
+    return CGLI_OK;
+}
+
+int cgli_set_sweep_waves(cgli_ctx *c, int waves)
+{
+    if (!c || waves < 1 || waves > 4) return CGLI_E_ARG;
+    NO_CAPTURE(c);
+    c->sweep_waves = waves;
     c->generation++;
     return CGLI_OK;
 }
@@ -1017,19 +1030,19 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
     }
     for (int sw = 0; sw < sweeps; sw++) {
         if (c->lusgs_mode == 4) {
-            launch_march_prep(g, 1, c->xbuf, c->counters, s, c->xbufC);
+            launch_rec4_prep(c->rec4, c->rec4_bytes, c->counters, s);
             tbegin(c, T_FWD);
             launch_lusgs_fused(g, c->p, deltaT, e, 1, c->geo4, c->geo_cols ? c->gcol4 : nullptr, c->u4, x4, x4 + f4b,
                                c5(c, CGLI_U_), c->f[CGLI_J],
-                               m9(c, CGLI_MET), c5(c, CGLI_NL), c->xbuf, c->xbufC, c->trash, c->morder_dev,
-                               c->mtiles, c->counters, c->max_spin, -1, s);
+                               m9(c, CGLI_MET), c5(c, CGLI_NL), c->rec4, c->rec4_bytes, c->morder_dev,
+                               c->mtiles, c->counters, c->max_spin, -1, c->sweep_waves, s);
             tend(c, T_FWD);
-            launch_march_prep(g, 0, c->xbuf, c->counters, s, c->xbufC);
+            launch_rec4_prep(c->rec4, c->rec4_bytes, c->counters, s);
             tbegin(c, T_BWD);
             launch_lusgs_fused(g, c->p, deltaT, e, 0, c->geo4, c->geo_cols ? c->gcol4 : nullptr, c->u4, x4 + f4b,
                                x4 + 2 * f4b, c5(c, CGLI_U_),
-                               c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_NR), c->xbuf, c->xbufC, c->trash,
-                               c->morder_dev, c->mtiles, c->counters, c->max_spin, -1, s);
+                               c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_NR), c->rec4, c->rec4_bytes,
+                               c->morder_dev, c->mtiles, c->counters, c->max_spin, -1, c->sweep_waves, s);
             tend(c, T_BWD);
             if (sw == sweeps - 1) {
                 tbegin(c, T_SCATTER);

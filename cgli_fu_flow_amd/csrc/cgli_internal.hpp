@@ -156,16 +156,23 @@ long march4_field_pairs(const Geo &g);
 void launch_gather4(const Geo &g, C5 U_, C5 R, const double *J, M9 met, void *Um, void *Rm, void *Gm, hipStream_t s);
 // march-ordered increments Xm into NR (and NL unless its pointers are null)
 void launch_scatter4(const Geo &g, const void *Xm, F5 NL, F5 NR, hipStream_t s);
+// bytes of one of the two tile-edge exchange record buffers of a sweep (the
+// j products, then the k products); the sweep needs 2 * march4_rec_bytes
+// below 2 GiB
+long march4_rec_bytes(const Geo &g);
+// the sentinel into both record buffers and the sweep's tile ticket to zero
+void launch_rec4_prep(void *rec, long rec_bytes, int *counters, hipStream_t s);
 // one sweep: right-hand sides Rm (forward MR, backward NL) into increments
 // Xm; U_, J, met and the increment field Xg supply the ghost cell upstream
-// in i; xbB / xbC: exchange buffers of the j and k products (5 per cell);
-// slack < 0: the schedule's own lead per hop
+// in i; rec: the two exchange record buffers (launch_rec4_prep before each
+// sweep); slack < 0: the schedule's own lead per hop; waves: workgroup waves
+// per tile (1, or a chain wave and 1-3 helpers)
 // Gc not null: the metrics do not vary along i and Gc holds each tile
 // lane's column geometry (launch_gather_cols4); Gm is then not read
 void launch_lusgs_fused(const Geo &g, const Phys &p, double deltaT, double e, int forward, const void *Gm,
                         const void *Gc, const void *Um, const void *Rm, void *Xm, C5 U_, const double *J, M9 met, C5 Xg,
-                        double *xbB, double *xbC, double *trash, const int *order, int ntiles, int *counters,
-                        int max_spin, int slack, hipStream_t s);
+                        void *rec, long rec_bytes, const int *order, int ntiles, int *counters,
+                        int max_spin, int slack, int waves, hipStream_t s);
 // *flag = 1 where an interior cell's J or metrics differ from its column's
 // at the first interior plane (the caller zeroes *flag)
 void launch_geo_icheck(const Geo &g, const double *J, M9 met, int *flag, hipStream_t s);

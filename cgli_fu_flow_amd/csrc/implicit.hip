@@ -1746,7 +1746,8 @@ struct F4Lane {
     bool need_b, need_c;      // upstream neighbour across a tile edge (exchange)
     bool out_b, out_c;        // downstream neighbour across a tile edge (publish)
     int ib, ic;               // LDS slots of the in-tile upstream neighbours
-    long x0;                  // 5 x field index of (i = 3 - jl - kl, j, k)
+    unsigned pb, pc;          // byte offsets of the polled j / k records at step 0 (XR_OOB: none)
+    unsigned ob, oc;          // byte offsets of the published j / k records at step 0
 };
 
 struct M4Args {
@@ -1756,12 +1757,83 @@ struct M4Args {
     double h[3];              // 1 / dxi, 1 / deta, 1 / dzeta
     __amdgpu_buffer_rsrc_t grow, urow, rrow;   // rows from (T, t = 0)
     char *xrow0;              // output row of (T, t = 0)
-    double *xbB, *xbC;        // exchange: j products, k products (5 per cell)
-    double *trash;            // this wave's scratch line
+    __amdgpu_buffer_rsrc_t rb, rc;   // the j and k exchange records
+    int dtb, dtc;             // a polled record's step minus the polling cell's step
     int max_spin;
-    long sxs5, dnb5, dnc5;    // exchange strides (times 5): one i plane, the upstream j and k cells
     double2 gcol[G4P];        // GC: the column's geometry, the same in every plane
 };
+
+// Tile-edge exchange records.  A tile's cell on its downstream j edge
+// publishes, at its march step t, the product of its increment with its j
+// block (five doubles and a zero pad, 48 bytes) as entry e = its k position of
+// record (T, t) in the j buffer; the k edge likewise into the k buffer.  The
+// eight entries of one record are contiguous, so an edge's publication and
+// its consumer's poll touch three 128-byte lines.  The consumer cell of the
+// same column one tile downstream needs that entry at its own step t - dtb
+// (a tile edge's cells are tile-width steps apart in the skewed march).
+// All accesses are 16-byte buffer operations at device scope (write-through
+// stores, L2-bypassing loads); lanes with nothing to publish or poll use an
+// offset past the buffer, which the hardware discards (loads return zero),
+// so every lane issues the same instructions and no memory request is made
+// for them.  Each 8-byte value is validated on its own against the sentinel
+// the buffers are filled with before each sweep.
+typedef unsigned int xr_u32x4 __attribute__((ext_vector_type(4)));
+constexpr unsigned XR_OOB = 0x80000000u;
+constexpr int XR_CPOL = 16;                           // sc1: device scope
+constexpr unsigned XR_ENTRY = 48, XR_STEP = 8 * XR_ENTRY;
+
+__device__ __forceinline__ unsigned long long xr_lo(xr_u32x4 q) { return q.x | ((unsigned long long)q.y << 32); }
+__device__ __forceinline__ unsigned long long xr_hi(xr_u32x4 q) { return q.z | ((unsigned long long)q.w << 32); }
+
+__device__ __forceinline__ void rec_issue(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned long long b[6])
+{
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        const xr_u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16u * p, 0, XR_CPOL);
+        b[2 * p] = xr_lo(q);
+        b[2 * p + 1] = xr_hi(q);
+    }
+}
+
+// where `need`, re-poll the record entry until none of its five values holds
+// the sentinel (bounded: `late` records an expired wait) and take them
+__device__ __forceinline__ void rec_settle(__amdgpu_buffer_rsrc_t r, unsigned vo, bool need, unsigned long long b[6],
+                                           double out[5], int max_spin, bool &late)
+{
+    bool pend = need & ((b[0] == SENTINEL) | (b[1] == SENTINEL) | (b[2] == SENTINEL) | (b[3] == SENTINEL) |
+                        (b[4] == SENTINEL));
+    if (pend) {
+        for (int n = 0; pend && n < max_spin; n++) {
+            __builtin_amdgcn_s_sleep(1);
+            rec_issue(r, vo, b);
+            pend = (b[0] == SENTINEL) | (b[1] == SENTINEL) | (b[2] == SENTINEL) | (b[3] == SENTINEL) |
+                   (b[4] == SENTINEL);
+        }
+        late |= pend;
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) out[v] = need ? __longlong_as_double((long long)b[v]) : out[v];
+}
+
+__device__ __forceinline__ void rec_store(__amdgpu_buffer_rsrc_t r, unsigned vo, const double p[5])
+{
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(xr_u32x4, make_double2(p[0], p[1])), r, vo, 0, XR_CPOL);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(xr_u32x4, make_double2(p[2], p[3])), r, vo + 16u, 0,
+                                           XR_CPOL);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(xr_u32x4, make_double2(p[4], 0.0)), r, vo + 32u, 0,
+                                           XR_CPOL);
+}
+
+// the record offsets of step t for a lane: polled (where act and needed) and
+// published (where act and on the edge)
+__device__ __forceinline__ unsigned rec_poll(unsigned base, bool on, int t, int dt)
+{
+    return on ? base + (unsigned)(t + dt) * XR_STEP : XR_OOB;
+}
+__device__ __forceinline__ unsigned rec_pub(unsigned base, bool on, int t)
+{
+    return on ? base + (unsigned)t * XR_STEP : XR_OOB;
+}
 
 __device__ __forceinline__ double2 ld_pair(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
 {
@@ -1794,10 +1866,15 @@ __device__ __forceinline__ void load4(Ld4 &L, const M4Args &w, unsigned vo, int 
 // primitives (Dual_time_stepping.cpp:759-769), the split coefficients of the
 // three directions (1054-1170, 1384-1524) and the diagonal (856-895) of one
 // cell; the directions' spectral radii are shared by the split and the
-// diagonal
+// diagonal.  The compact set holds per direction the entries that do not
+// multiply a metric (a0 = g11 or m11, a5 = g22 or m22, a8) and the three
+// factors that do (pk, ir, bt), then d11, d22, d51; coef_expand forms the
+// twelve entries of each direction from it with the same products.
+constexpr int CF_N = 21;
+
 template <bool FWD>
-__device__ __forceinline__ void coef_cell(double a[3][12], double &d11, double &d22, double &d51, double J,
-                                          const double m[9], const double U[5], const M4Args &w)
+__device__ __forceinline__ void coef_compact(double c[CF_N], double J, const double m[9], const double U[5],
+                                             const M4Args &w)
 {
     const double K = w.K;
     const double iU1 = 1.0 / U[0];
@@ -1820,27 +1897,53 @@ __device__ __forceinline__ void coef_cell(double a[3][12], double &d11, double &
         const double h = w.h[d];
         lam += l * h;
         const double dd = fabs(l);
-        a[d][0] = FWD ? 0.5 * (b * Un + dd) * h : 0.5 * (b * Un - dd) * h;
-        a[d][5] = FWD ? 0.5 * (Un + dd) * h : 0.5 * (Un - dd) * h;
-        const double pk = 0.5 * P * K * b * h;
-        a[d][1] = pk * mx;
-        a[d][2] = pk * my;
-        a[d][3] = pk * mz;
+        double *a = c + 6 * d;
+        a[0] = FWD ? 0.5 * (b * Un + dd) * h : 0.5 * (b * Un - dd) * h;
+        a[1] = FWD ? 0.5 * (Un + dd) * h : 0.5 * (Un - dd) * h;
+        a[2] = 0.5 * P * K * b * h;
         const double ir = 0.5 * irho * h;
-        a[d][4] = ir * mx;
-        a[d][6] = ir * my;
-        a[d][7] = ir * mz;
-        a[d][8] = Un * (K - 1) * (b - 1) * (ir / K);
-        const double bt = 0.5 * b * T * (K - 1) * h;
-        a[d][9] = bt * mx;
-        a[d][10] = bt * my;
-        a[d][11] = bt * mz;
+        a[3] = ir;
+        a[4] = Un * (K - 1) * (b - 1) * (ir / K);
+        a[5] = 0.5 * b * T * (K - 1) * h;
     }
     const double dt = w.deltaT;
     const double rA = 1.0 / (3 + 2 * lam * dt), rB = 1.0 / (3 * b + 2 * lam * dt);
-    d11 = 2 * dt * rB;
-    d22 = 2 * dt * rA;
-    d51 = -(6 * (K - 1) / K) * (b - 1) * dt * rA * rB * irho;
+    c[18] = 2 * dt * rB;
+    c[19] = 2 * dt * rA;
+    c[20] = -(6 * (K - 1) / K) * (b - 1) * dt * rA * rB * irho;
+}
+
+__device__ __forceinline__ void coef_expand(double a[3][12], const double c[CF_N], const double m[9])
+{
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double mx = m[3 * d], my = m[3 * d + 1], mz = m[3 * d + 2];
+        const double *q = c + 6 * d;
+        a[d][0] = q[0];
+        a[d][5] = q[1];
+        a[d][1] = q[2] * mx;
+        a[d][2] = q[2] * my;
+        a[d][3] = q[2] * mz;
+        a[d][4] = q[3] * mx;
+        a[d][6] = q[3] * my;
+        a[d][7] = q[3] * mz;
+        a[d][8] = q[4];
+        a[d][9] = q[5] * mx;
+        a[d][10] = q[5] * my;
+        a[d][11] = q[5] * mz;
+    }
+}
+
+template <bool FWD>
+__device__ __forceinline__ void coef_cell(double a[3][12], double &d11, double &d22, double &d51, double J,
+                                          const double m[9], const double U[5], const M4Args &w)
+{
+    double c[CF_N];
+    coef_compact<FWD>(c, J, m, U, w);
+    coef_expand(a, c, m);
+    d11 = c[18];
+    d22 = c[19];
+    d51 = c[20];
 }
 
 template <bool FWD>
@@ -1884,7 +1987,7 @@ __device__ __forceinline__ void prod4(const double a[12], const double x[5], dou
 template <int TJ, int TK, bool FWD, bool GC, int PAR>
 __device__ __forceinline__ void step4(const M4Args &w, const F4Lane &L, int s, Cf4 &C, const Ld4 &Ln, Ld4 &Lf,
                                       double sa[5], double (*sB)[5][TJ * TK], double (*sC)[5][TJ * TK],
-                                      unsigned long long nb[5], unsigned long long nc[5], bool &late)
+                                      unsigned long long nb[6], unsigned long long nc[6], bool &late)
 {
     const int t = FWD ? s : w.S - 1 - s;
     const int il = t - L.jl - L.kl;
@@ -1896,9 +1999,8 @@ __device__ __forceinline__ void step4(const M4Args &w, const F4Lane &L, int s, C
         yb[v] = L.hasb ? sB[PAR ^ 1][v][L.ib] : 0.0;
         yc[v] = L.hasc ? sC[PAR ^ 1][v][L.ic] : 0.0;
     }
-    const long xc5 = L.x0 + (long)t * w.sxs5;   // dereferenced only where act
-    settle5m(w.xbB + xc5 + w.dnb5, act && L.need_b, nb, yb, w.max_spin, late);
-    settle5m(w.xbC + xc5 + w.dnc5, act && L.need_c, nc, yc, w.max_spin, late);
+    rec_settle(w.rb, rec_poll(L.pb, act && L.need_b, t, w.dtb), act && L.need_b, nb, yb, w.max_spin, late);
+    rec_settle(w.rc, rec_poll(L.pc, act && L.need_c, t, w.dtc), act && L.need_c, nc, yc, w.max_spin, late);
     double x[5];
     if (FWD) {
         const double LL0 = sa[0] + yb[0] + yc[0], LL1 = sa[1] + yb[1] + yc[1], LL2 = sa[2] + yb[2] + yc[2];
@@ -1934,30 +2036,19 @@ __device__ __forceinline__ void step4(const M4Args &w, const F4Lane &L, int s, C
         const int tn = FWD ? s + 2 : w.S - 3 - s;
         const int iln = tn - L.jl - L.kl;
         const bool actn = s + 2 < w.S && L.col && iln >= 0 && iln < w.g.nx;
-        const long xn5 = L.x0 + (long)tn * w.sxs5;
-        // every lane polls (lanes with nothing to wait for read the wave's
-        // scratch line), so no branch splits the loads in flight
-        issue5m(actn && L.need_b ? w.xbB + xn5 + w.dnb5 : w.trash + 16, nb);
-        issue5m(actn && L.need_c ? w.xbC + xn5 + w.dnc5 : w.trash + 24, nc);
+        rec_issue(w.rb, rec_poll(L.pb, actn && L.need_b, tn, w.dtb), nb);
+        rec_issue(w.rc, rec_poll(L.pc, actn && L.need_c, tn, w.dtc), nc);
         load4<FWD, GC>(Lf, w, (unsigned)L.tid * 16u, s + 2);
     }
     __builtin_amdgcn_sched_barrier(0);
     // every lane stores its slot (a slot with no cell is never read as a
-    // value) and ten exchange values (lanes with nothing to publish write
-    // the wave's scratch line), so no branch splits the stores
+    // value) and issues the six record stores, so no branch splits them
     double2 *xo = reinterpret_cast<double2 *>(w.xrow0 + (long)t * F4ROWB) + L.tid;
     xo[0] = make_double2(x[0], x[1]);
     xo[64] = make_double2(x[2], x[3]);
     xo[128] = make_double2(x[4], 0.0);
-    double *eb = act && L.out_b ? w.xbB + xc5 : w.trash;
-    double *ec = act && L.out_c ? w.xbC + xc5 : w.trash + 8;
-#pragma unroll
-    for (int v = 0; v < 5; v++) {
-        __hip_atomic_store(reinterpret_cast<unsigned long long *>(eb + v), (unsigned long long)__double_as_longlong(pb[v]),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(reinterpret_cast<unsigned long long *>(ec + v), (unsigned long long)__double_as_longlong(pc[v]),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    rec_store(w.rb, rec_pub(L.ob, act && L.out_b, t), pb);
+    rec_store(w.rc, rec_pub(L.oc, act && L.out_c, t), pc);
     __builtin_amdgcn_sched_barrier(0);
     // the next step's coefficients while all of it is in flight
     coef4<FWD>(C, Ln, w);
@@ -1977,34 +2068,25 @@ __device__ __forceinline__ void ghost_product4(const M4Args &w, C5 U_, const dou
     prod4<FWD>(a[0], x, y);
 }
 
-template <int TJ, int TK, bool FWD, bool GC>
-__global__ void __launch_bounds__(TJ *TK) k_lusgs_fused(Geo g, March M, double K, double e, double deltaT, double hx,
-                                                      double hy, double hz, const double2 *__restrict__ Gm,
-                                                      const double2 *__restrict__ Gc,
-                                                      const double2 *__restrict__ Um, const double2 *Rm, double2 *Xm,
-                                                      C5 U_, const double *__restrict__ J, M9 met, C5 Xg,
-                                                      double *xbB, double *xbC, double *trash,
-                                                      const int *__restrict__ order, int ntiles, int *ticket, int *err,
-                                                      int max_spin, int slack)
+// the lane's place in its tile and the tile's march arguments (both
+// schedule-4 kernels); returns the tile's index T in the tile grid and the
+// field index of the lane's column at i = 0 in c0
+template <int TJ, int TK, bool FWD>
+__device__ __forceinline__ int fused_setup(F4Lane &L, M4Args &w, long &c0, int lane, int packed, const Geo &g,
+                                           const March &M, double K, double e, double deltaT, double hx, double hy,
+                                           double hz, const double2 *Gm, const double2 *Um, const double2 *Rm,
+                                           double2 *Xm, double2 *rec, unsigned rec_bytes, int max_spin)
 {
-    static_assert(TJ * TK == 64, "one wave per tile");
-    __shared__ int s_t;
-    __shared__ double sB[2][5][TJ * TK], sC[2][5][TJ * TK];
-    F4Lane L;
-    L.tid = threadIdx.x;
+    L.tid = lane;
     L.kl = L.tid % TK;
     L.jl = L.tid / TK;
-    if (L.tid == 0) s_t = atomicAdd(ticket, 1);
-    __syncthreads();
-    // forward: anti-diagonals ascending; backward: the same list reversed
-    const int tk = __builtin_amdgcn_readfirstlane(s_t);
-    const int packed = order[FWD ? tk : ntiles - 1 - tk];
     const int TJi = packed >> 16, TKi = packed & 0xffff;
     const int T = TJi * M.ntk + TKi;
     const int jo = TJi * TJ + L.jl, ko = TKi * TK + L.kl;
     L.col = (jo < g.ny) && (ko < g.nz);
     const int joc = jo < g.ny ? jo : g.ny - 1, koc = ko < g.nz ? ko : g.nz - 1;
-    const long c0 = ix(g, 0, 2 + joc, 2 + koc);
+    c0 = ix(g, 0, 2 + joc, 2 + koc);
+    int Tb, Tc;
     if (FWD) {
         L.hasb = L.jl >= 1;
         L.hasc = L.kl >= 1;
@@ -2014,6 +2096,10 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_fused(Geo g, March M, double K
         L.out_c = L.kl == TK - 1;
         L.ib = L.tid - TK;
         L.ic = L.tid - 1;
+        Tb = T - M.ntk;
+        Tc = T - 1;
+        w.dtb = TJ - 1;
+        w.dtc = TK - 1;
     } else {
         L.hasb = L.jl <= TJ - 2;
         L.hasc = L.kl <= TK - 2;
@@ -2023,8 +2109,18 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_fused(Geo g, March M, double K
         L.out_c = L.kl == 0;
         L.ib = L.tid + TK;
         L.ic = L.tid + 1;
+        Tb = T + M.ntk;
+        Tc = T + 1;
+        w.dtb = -(TJ - 1);
+        w.dtc = -(TK - 1);
     }
-    M4Args w;
+    // record (tile, step t) starts at (tile * S + t) * XR_STEP; the host
+    // keeps every offset below XR_OOB
+    const unsigned S = (unsigned)M.S;
+    L.ob = (unsigned)T * S * XR_STEP + (unsigned)L.kl * XR_ENTRY;
+    L.oc = (unsigned)T * S * XR_STEP + (unsigned)L.jl * XR_ENTRY;
+    L.pb = L.need_b ? (unsigned)Tb * S * XR_STEP + (unsigned)L.kl * XR_ENTRY : XR_OOB;
+    L.pc = L.need_c ? (unsigned)Tc * S * XR_STEP + (unsigned)L.jl * XR_ENTRY : XR_OOB;
     w.g = g;
     w.S = M.S;
     w.K = K;
@@ -2039,38 +2135,81 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_fused(Geo g, March M, double K
     w.urow = __builtin_amdgcn_make_buffer_rsrc((void *)(Um + row0 * F4P * 64), (short)0, -1, F);
     w.rrow = __builtin_amdgcn_make_buffer_rsrc((void *)(Rm + row0 * F4P * 64), (short)0, -1, F);
     w.xrow0 = reinterpret_cast<char *>(Xm + row0 * F4P * 64);
-    w.xbB = xbB;
-    w.xbC = xbC;
-    w.trash = trash + (long)blockIdx.x * 256;
+    w.rb = __builtin_amdgcn_make_buffer_rsrc((void *)rec, (short)0, (int)rec_bytes, F);
+    w.rc = __builtin_amdgcn_make_buffer_rsrc((void *)(reinterpret_cast<char *>(rec) + rec_bytes), (short)0,
+                                             (int)rec_bytes, F);
     w.max_spin = max_spin;
-    const long sxs = (long)g.Ym * g.Zm;
-    const int d = FWD ? -1 : 1;
-    w.sxs5 = 5 * sxs;
-    w.dnb5 = 5L * d * g.Zm;
-    w.dnc5 = 5L * d;
-    L.x0 = 5 * (c0 + (long)(3 - L.jl - L.kl) * sxs);
-    const int S = M.S;
-    bool late = false;
-    // exchange polls in flight two steps ahead: one register set per parity
-    unsigned long long nb0[5], nc0[5], nb1[5], nc1[5];
-    // start `slack` march steps later than the dependencies require (as the
-    // schedule-3 march): the upstream tiles have then published the entries
-    // of the first steps' polls, and each poll issued two steps ahead finds
-    // its entry written
+    return T;
+}
+
+// the march start: wait until the upstream tiles have published the records
+// of step `slack` (so that the polls issued two steps ahead find theirs
+// written), the x product of the ghost cell upstream in i, and the polls of
+// steps 0 and 1
+template <bool FWD>
+__device__ __forceinline__ void fused_start(const M4Args &w, const F4Lane &L, int slack, C5 U_,
+                                            const double *__restrict__ J, M9 met, C5 Xg, long c0, double sa[5],
+                                            unsigned long long nb0[6], unsigned long long nc0[6],
+                                            unsigned long long nb1[6], unsigned long long nc1[6], bool &late)
+{
+    const Geo &g = w.g;
+    const int S = w.S;
     if (slack > 0 && slack < S) {
         const int ts = FWD ? slack : S - 1 - slack;
         const int ils = ts - L.jl - L.kl;
         const bool acts = L.col && ils >= 0 && ils < g.nx;
-        const long xs5 = L.x0 + (long)ts * w.sxs5;
-        unsigned long long sb[5] = {0, 0, 0, 0, 0}, sc[5] = {0, 0, 0, 0, 0};
+        unsigned long long sb[6], sc[6];
         double tmp[5] = {0, 0, 0, 0, 0};
-        if (acts && L.need_b) issue5m(xbB + xs5 + w.dnb5, sb);
-        if (acts && L.need_c) issue5m(xbC + xs5 + w.dnc5, sc);
-        settle5m(xbB + xs5 + w.dnb5, acts && L.need_b, sb, tmp, max_spin, late);
-        settle5m(xbC + xs5 + w.dnc5, acts && L.need_c, sc, tmp, max_spin, late);
+        const unsigned vb = rec_poll(L.pb, acts && L.need_b, ts, w.dtb);
+        const unsigned vc = rec_poll(L.pc, acts && L.need_c, ts, w.dtc);
+        rec_issue(w.rb, vb, sb);
+        rec_issue(w.rc, vc, sc);
+        rec_settle(w.rb, vb, acts && L.need_b, sb, tmp, w.max_spin, late);
+        rec_settle(w.rc, vc, acts && L.need_c, sc, tmp, w.max_spin, late);
     }
-    double sa[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int v = 0; v < 5; v++) sa[v] = 0.0;
+    const long sxs = (long)g.Ym * g.Zm;
     if (L.col) ghost_product4<FWD>(w, U_, J, met, Xg, c0 + (long)(FWD ? 2 : g.gend + 1) * sxs, sa);
+#pragma unroll
+    for (int s0 = 0; s0 < 2; s0++) {
+        const int t0 = FWD ? s0 : S - 1 - s0;
+        const int il0 = t0 - L.jl - L.kl;
+        const bool act0 = s0 < S && L.col && il0 >= 0 && il0 < g.nx;
+        rec_issue(w.rb, rec_poll(L.pb, act0 && L.need_b, t0, w.dtb), s0 ? nb1 : nb0);
+        rec_issue(w.rc, rec_poll(L.pc, act0 && L.need_c, t0, w.dtc), s0 ? nc1 : nc0);
+    }
+}
+
+template <int TJ, int TK, bool FWD, bool GC>
+__global__ void __launch_bounds__(TJ *TK) k_lusgs_fused(Geo g, March M, double K, double e, double deltaT, double hx,
+                                                      double hy, double hz, const double2 *__restrict__ Gm,
+                                                      const double2 *__restrict__ Gc,
+                                                      const double2 *__restrict__ Um, const double2 *Rm, double2 *Xm,
+                                                      C5 U_, const double *__restrict__ J, M9 met, C5 Xg,
+                                                      double2 *rec, unsigned rec_bytes,
+                                                      const int *__restrict__ order, int ntiles, int *ticket, int *err,
+                                                      int max_spin, int slack)
+{
+    static_assert(TJ * TK == 64, "one wave per tile");
+    __shared__ int s_t;
+    __shared__ double sB[2][5][TJ * TK], sC[2][5][TJ * TK];
+    if (threadIdx.x == 0) s_t = atomicAdd(ticket, 1);
+    __syncthreads();
+    // forward: anti-diagonals ascending; backward: the same list reversed
+    const int tk = __builtin_amdgcn_readfirstlane(s_t);
+    const int packed = order[FWD ? tk : ntiles - 1 - tk];
+    F4Lane L;
+    M4Args w;
+    long c0;
+    const int T = fused_setup<TJ, TK, FWD>(L, w, c0, threadIdx.x, packed, g, M, K, e, deltaT, hx, hy, hz, Gm, Um, Rm,
+                                           Xm, rec, rec_bytes, max_spin);
+    const int S = M.S;
+    bool late = false;
+    // exchange polls in flight two steps ahead: one register set per parity
+    unsigned long long nb0[6], nc0[6], nb1[6], nc1[6];
+    double sa[5];
+    fused_start<FWD>(w, L, slack, U_, J, met, Xg, c0, sa, nb0, nc0, nb1, nc1, late);
     if (GC) {
 #pragma unroll
         for (int p = 0; p < G4P; p++) w.gcol[p] = Gc[((long)T * G4P + p) * 64 + L.tid];
@@ -2079,21 +2218,282 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_fused(Geo g, March M, double K
     Cf4 C;
     load4<FWD, GC>(La, w, (unsigned)L.tid * 16u, 0);
     load4<FWD, GC>(Lb, w, (unsigned)L.tid * 16u, 1);
-#pragma unroll
-    for (int s0 = 0; s0 < 2; s0++) {
-        const int t0 = FWD ? s0 : S - 1 - s0;
-        const int il0 = t0 - L.jl - L.kl;
-        const bool act0 = s0 < S && L.col && il0 >= 0 && il0 < g.nx;
-        const long x5 = L.x0 + (long)t0 * w.sxs5;
-        issue5m(act0 && L.need_b ? xbB + x5 + w.dnb5 : w.trash + 16, s0 ? nb1 : nb0);
-        issue5m(act0 && L.need_c ? xbC + x5 + w.dnc5 : w.trash + 24, s0 ? nc1 : nc0);
-    }
     coef4<FWD>(C, La, w);
     for (int s = 0; s < S; s += 2) {
         step4<TJ, TK, FWD, GC, 0>(w, L, s, C, Lb, La, sa, sB, sC, nb0, nc0, late);
         if (s + 1 < S) step4<TJ, TK, FWD, GC, 1>(w, L, s + 1, C, La, Lb, sa, sB, sC, nb1, nc1, late);
     }
     if (late) atomicOr(err, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Schedule 4 with helper waves.  The march step of k_lusgs_fused spends about
+// half its time on the coefficients of the next cell, work that does not
+// depend on the sweep.  Here a tile's workgroup holds one chain wave and H
+// helper waves.  Helper h loads the state of steps h, h + H, h + 2H, ...,
+// computes each cell's compact coefficient set (coef_compact) and leaves it,
+// with the residual, in one of R LDS slots; the chain wave runs the march as
+// k_lusgs_fused does, taking each step's coefficients from its slot and
+// forming the metric products itself, so the increments are the same bits.
+// Slot protocol (all in LDS, whose operations from one wave complete in
+// issue order): a helper writes slot s % R once the chain wave has posted
+// `consumed` > s - R, then sets ready[s % R] = s; the chain wave posts
+// consumed = s when it starts step s (the slots of earlier steps have been
+// read by then) and reads slot s % R after seeing ready[s % R] == s.  Both
+// waits are bounded spins that set the error flag on expiry.
+#ifndef FH_R
+#define FH_R 3                 // LDS coefficient slots per tile
+#endif
+constexpr int FH_NP_GC = 13;   // slot pairs per lane: 21 coefficients + 5 residuals (+ 1 pad)
+constexpr int FH_NP = 18;      // and the nine metrics where they vary along i
+
+__device__ __forceinline__ void lds_fence() { asm volatile("" ::: "memory"); }
+// relaxed workgroup-scope accesses to the slot flags: single LDS operations
+// with no wait attached (ordering comes from LDS completing a wave's
+// operations in issue order, and lds_fence keeps the compiler's order)
+__device__ __forceinline__ int flag_get(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void flag_put(int *p, int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <bool FWD, bool GC, int R, int H, int NP>
+__device__ void fused_helper(const M4Args &w, int h, int tid, double2 (*slot)[NP][64], int *ready, int *consumed,
+                             bool &late)
+{
+    const int S = w.S;
+    const unsigned vo = (unsigned)tid * 16u;
+    Ld4 A, B;
+    load4<FWD, GC>(A, w, vo, h);
+    auto one = [&](int s, const Ld4 &L) {
+        const double m[9] = {L.g[0].y, L.g[1].x, L.g[1].y, L.g[2].x, L.g[2].y, L.g[3].x, L.g[3].y, L.g[4].x, L.g[4].y};
+        const double U[5] = {L.u[0].x, L.u[0].y, L.u[1].x, L.u[1].y, L.u[2].x};
+        double c[CF_N];
+        coef_compact<FWD>(c, L.g[0].x, m, U, w);
+        const int need = s - R + 1;   // the chain wave has started step s - R + 1
+        if (need > 0 && __builtin_amdgcn_readfirstlane(flag_get(consumed)) < need) {
+            int n = 0;
+            while (__builtin_amdgcn_readfirstlane(flag_get(consumed)) < need && n < w.max_spin) {
+                __builtin_amdgcn_s_sleep(1);
+                n++;
+            }
+            late |= n >= w.max_spin;
+        }
+        lds_fence();
+        double2 *o = &slot[s % R][0][tid];
+#pragma unroll
+        for (int p = 0; p < 10; p++) o[p * 64] = make_double2(c[2 * p], c[2 * p + 1]);
+        o[10 * 64] = make_double2(c[20], L.r[0].x);
+        o[11 * 64] = make_double2(L.r[0].y, L.r[1].x);
+        o[12 * 64] = make_double2(L.r[1].y, L.r[2].x);
+        if constexpr (!GC) {
+#pragma unroll
+            for (int p = 0; p < 4; p++) o[(13 + p) * 64] = make_double2(m[2 * p], m[2 * p + 1]);
+            o[17 * 64] = make_double2(m[8], 0.0);
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        flag_put(ready + s % R, s);
+        lds_fence();
+    };
+    for (int s = h; s < S; s += 2 * H) {
+        load4<FWD, GC>(B, w, vo, s + H);
+        one(s, A);
+        if (s + H >= S) break;
+        load4<FWD, GC>(A, w, vo, s + 2 * H);
+        one(s + H, B);
+    }
+}
+
+// one march step of the chain wave: step4 with the coefficients from the
+// step's slot instead of computed, and no workgroup barrier (the chain
+// wave's own LDS operations complete in issue order)
+template <int TJ, int TK, bool FWD, bool GC, int PAR, int R, int NP>
+__device__ __forceinline__ void step_chain(const M4Args &w, const F4Lane &L, int s, const double mc[9],
+                                           const double2 (*slot)[NP][64], int *ready, int *consumed,
+                                           double sa[5], double (*sB)[5][TJ * TK],
+                                           double (*sC)[5][TJ * TK], unsigned long long nb[6],
+                                           unsigned long long nc[6], bool &late)
+{
+    const int t = FWD ? s : w.S - 1 - s;
+    const int il = t - L.jl - L.kl;
+    const bool act = L.col && il >= 0 && il < w.g.nx;
+    const int r = s % R;
+    lds_fence();
+    flag_put(consumed, s);
+    const int rd = flag_get(ready + r);
+    lds_fence();
+    double2 q[NP];
+#pragma unroll
+    for (int p = 0; p < NP; p++) q[p] = slot[r][p][L.tid];
+    double yb[5], yc[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        yb[v] = L.hasb ? sB[PAR ^ 1][v][L.ib] : 0.0;
+        yc[v] = L.hasc ? sC[PAR ^ 1][v][L.ic] : 0.0;
+    }
+    if (__builtin_amdgcn_readfirstlane(rd) != s) {
+        int n = 0;
+        while (__builtin_amdgcn_readfirstlane(flag_get(ready + r)) != s && n < w.max_spin) {
+            __builtin_amdgcn_s_sleep(1);
+            n++;
+        }
+        late |= n >= w.max_spin;
+        lds_fence();
+#pragma unroll
+        for (int p = 0; p < NP; p++) q[p] = slot[r][p][L.tid];
+    }
+    rec_settle(w.rb, rec_poll(L.pb, act && L.need_b, t, w.dtb), act && L.need_b, nb, yb, w.max_spin, late);
+    rec_settle(w.rc, rec_poll(L.pc, act && L.need_c, t, w.dtc), act && L.need_c, nc, yc, w.max_spin, late);
+    double c[CF_N], rr[5], m[9];
+#pragma unroll
+    for (int p = 0; p < 10; p++) {
+        c[2 * p] = q[p].x;
+        c[2 * p + 1] = q[p].y;
+    }
+    c[20] = q[10].x;
+    rr[0] = q[10].y;
+    rr[1] = q[11].x;
+    rr[2] = q[11].y;
+    rr[3] = q[12].x;
+    rr[4] = q[12].y;
+    if constexpr (GC) {
+#pragma unroll
+        for (int v = 0; v < 9; v++) m[v] = mc[v];
+    } else {
+#pragma unroll
+        for (int v = 0; v < 9; v++) m[v] = v & 1 ? q[13 + v / 2].y : q[13 + v / 2].x;
+    }
+    double a[3][12];
+    coef_expand(a, c, m);
+    const double d11 = c[18], d22 = c[19], d51 = c[20];
+    double x[5];
+    if (FWD) {
+        const double LL0 = sa[0] + yb[0] + yc[0], LL1 = sa[1] + yb[1] + yc[1], LL2 = sa[2] + yb[2] + yc[2];
+        const double LL3 = sa[3] + yb[3] + yc[3], LL4 = sa[4] + yb[4] + yc[4];
+        x[0] = d11 * (rr[0] - LL0);
+        x[1] = d22 * (rr[1] - LL1);
+        x[2] = d22 * (rr[2] - LL2);
+        x[3] = d22 * (rr[3] - LL3);
+        x[4] = d51 * (rr[0] - LL0) + d22 * (rr[4] - LL4);
+    } else {
+        const double UU0 = sa[0] + yb[0] + yc[0], UU1 = sa[1] + yb[1] + yc[1], UU2 = sa[2] + yb[2] + yc[2];
+        const double UU3 = sa[3] + yb[3] + yc[3], UU4 = sa[4] + yb[4] + yc[4];
+        x[0] = rr[0] - d11 * UU0;
+        x[1] = rr[1] - d22 * UU1;
+        x[2] = rr[2] - d22 * UU2;
+        x[3] = rr[3] - d22 * UU3;
+        x[4] = rr[4] - (d51 * UU0 + d22 * UU4);
+    }
+    double pa[5], pb[5], pc[5];
+    prod4<FWD>(a[0], x, pa);
+    prod4<FWD>(a[1], x, pb);
+    prod4<FWD>(a[2], x, pc);
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        sB[PAR][v][L.tid] = L.col ? pb[v] : 0.0;
+        sC[PAR][v][L.tid] = L.col ? pc[v] : 0.0;
+        sa[v] = act ? pa[v] : sa[v];
+    }
+    {   // the exchange polls two steps ahead, before this step's stores
+        const int tn = FWD ? s + 2 : w.S - 3 - s;
+        const int iln = tn - L.jl - L.kl;
+        const bool actn = s + 2 < w.S && L.col && iln >= 0 && iln < w.g.nx;
+        rec_issue(w.rb, rec_poll(L.pb, actn && L.need_b, tn, w.dtb), nb);
+        rec_issue(w.rc, rec_poll(L.pc, actn && L.need_c, tn, w.dtc), nc);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    double2 *xo = reinterpret_cast<double2 *>(w.xrow0 + (long)t * F4ROWB) + L.tid;
+    xo[0] = make_double2(x[0], x[1]);
+    xo[64] = make_double2(x[2], x[3]);
+    xo[128] = make_double2(x[4], 0.0);
+    rec_store(w.rb, rec_pub(L.ob, act && L.out_b, t), pb);
+    rec_store(w.rc, rec_pub(L.oc, act && L.out_c, t), pc);
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int TJ, int TK, bool FWD, bool GC, int R, int H>
+__global__ void __launch_bounds__(64 * (1 + H)) k_lusgs_helped(Geo g, March M, double K, double e, double deltaT,
+                                                             double hx, double hy, double hz,
+                                                             const double2 *__restrict__ Gm,
+                                                             const double2 *__restrict__ Gc,
+                                                             const double2 *__restrict__ Um, const double2 *Rm,
+                                                             double2 *Xm, C5 U_, const double *__restrict__ J, M9 met,
+                                                             C5 Xg, double2 *rec, unsigned rec_bytes,
+                                                             const int *__restrict__ order, int ntiles, int *ticket,
+                                                             int *err, int max_spin, int slack)
+{
+    static_assert(TJ * TK == 64, "one chain wave per tile");
+    constexpr int NP = GC ? FH_NP_GC : FH_NP;
+    __shared__ int s_t;
+    __shared__ int s_ready[R], s_consumed;
+    __shared__ double sB[2][5][TJ * TK], sC[2][5][TJ * TK];
+    __shared__ double2 slot[R][NP][64];
+    const int wv = threadIdx.x >> 6;
+    if (threadIdx.x == 0) {
+        s_t = atomicAdd(ticket, 1);
+        s_consumed = 0;
+    }
+    if (threadIdx.x < R) s_ready[threadIdx.x] = -1;
+    __syncthreads();
+    const int tk = __builtin_amdgcn_readfirstlane(s_t);
+    const int packed = order[FWD ? tk : ntiles - 1 - tk];
+    F4Lane L;
+    M4Args w;
+    long c0;
+    const int T = fused_setup<TJ, TK, FWD>(L, w, c0, threadIdx.x & 63, packed, g, M, K, e, deltaT, hx, hy, hz, Gm, Um,
+                                           Rm, Xm, rec, rec_bytes, max_spin);
+    const int S = M.S;
+    bool late = false;
+    if (GC) {
+#pragma unroll
+        for (int p = 0; p < G4P; p++) w.gcol[p] = Gc[((long)T * G4P + p) * 64 + L.tid];
+    }
+    if (wv > 0) {
+        if (wv - 1 < S) fused_helper<FWD, GC, R, H, NP>(w, wv - 1, L.tid, slot, s_ready, &s_consumed, late);
+        if (late) atomicOr(err, 1);
+        return;
+    }
+    unsigned long long nb0[6], nc0[6], nb1[6], nc1[6];
+    double sa[5];
+    fused_start<FWD>(w, L, slack, U_, J, met, Xg, c0, sa, nb0, nc0, nb1, nc1, late);
+    double mc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if constexpr (GC) {
+        const double gm[9] = {w.gcol[0].y, w.gcol[1].x, w.gcol[1].y, w.gcol[2].x, w.gcol[2].y,
+                              w.gcol[3].x, w.gcol[3].y, w.gcol[4].x, w.gcol[4].y};
+#pragma unroll
+        for (int v = 0; v < 9; v++) mc[v] = gm[v];
+    }
+    for (int s = 0; s < S; s += 2) {
+        step_chain<TJ, TK, FWD, GC, 0, R, NP>(w, L, s, mc, slot, s_ready, &s_consumed, sa, sB, sC, nb0, nc0, late);
+        if (s + 1 < S)
+            step_chain<TJ, TK, FWD, GC, 1, R, NP>(w, L, s + 1, mc, slot, s_ready, &s_consumed, sa, sB, sC, nb1, nc1,
+                                                  late);
+    }
+    if (late) atomicOr(err, 1);
+}
+
+long march4_rec_bytes(const Geo &g)
+{
+    const March M = march_of(g);
+    return (long)M.ntj * M.ntk * M.S * XR_STEP;
+}
+
+__global__ void k_fill_rec4(ulonglong2 *rec, long n, int *ticket)
+{
+    const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i0 == 0) *ticket = 0;
+    for (long i = i0; i < n; i += (long)gridDim.x * blockDim.x) rec[i] = make_ulonglong2(SENTINEL, SENTINEL);
+}
+
+void launch_rec4_prep(void *rec, long rec_bytes, int *counters, hipStream_t s)
+{
+    const long n = 2 * rec_bytes / 16;
+    const long blocks = std::min((n + 255) / 256, 4096L);
+    hipLaunchKernelGGL(k_fill_rec4, dim3((unsigned)blocks), dim3(256), 0, s, static_cast<ulonglong2 *>(rec), n,
+                       counters);
 }
 
 void launch_gather4(const Geo &g, C5 U_, C5 R, const double *J, M9 met, void *Um, void *Rm, void *Gm, hipStream_t s)
@@ -2111,8 +2511,8 @@ void launch_scatter4(const Geo &g, const void *Xm, F5 NL, F5 NR, hipStream_t s)
 
 void launch_lusgs_fused(const Geo &g, const Phys &p, double deltaT, double e, int forward, const void *Gm,
                         const void *Gc, const void *Um, const void *Rm, void *Xm, C5 U_, const double *J, M9 met, C5 Xg,
-                        double *xbB, double *xbC, double *trash, const int *order, int ntiles, int *counters,
-                        int max_spin, int slack, hipStream_t s)
+                        void *rec, long rec_bytes, const int *order, int ntiles, int *counters,
+                        int max_spin, int slack, int waves, hipStream_t s)
 {
     const March M = march_of(g);
     const dim3 gr(ntiles), bl(MARCH_TJ * MARCH_TK);
@@ -2123,13 +2523,34 @@ void launch_lusgs_fused(const Geo &g, const Phys &p, double deltaT, double e, in
     const double hx = 1.0 / p.dxi, hy = 1.0 / p.deta, hz = 1.0 / p.dzeta;
 #define CGLI_FUSED(FW, GCOL)                                                                                         \
     hipLaunchKernelGGL((k_lusgs_fused<MARCH_TJ, MARCH_TK, FW, GCOL>), gr, bl, 0, s, g, M, p.K, e, deltaT, hx, hy, hz, \
-                       G, Gcol, Uu, R, X, U_, J, met, Xg, xbB, xbC, trash, order, ntiles, counters, counters + 1,   \
+                       G, Gcol, Uu, R, X, U_, J, met, Xg, static_cast<double2 *>(rec), (unsigned)rec_bytes,     \
+                       order, ntiles, counters, counters + 1,                                                        \
                        max_spin, sl)
-    if (forward && Gc) CGLI_FUSED(true, true);
-    else if (forward) CGLI_FUSED(true, false);
-    else if (Gc) CGLI_FUSED(false, true);
-    else CGLI_FUSED(false, false);
+#define CGLI_HELPED(FW, GCOL, H)                                                                                     \
+    hipLaunchKernelGGL((k_lusgs_helped<MARCH_TJ, MARCH_TK, FW, GCOL, FH_R, H>), gr, dim3(64 * (1 + H)), 0, s, g, M, p.K, \
+                       e, deltaT, hx, hy, hz, G, Gcol, Uu, R, X, U_, J, met, Xg, static_cast<double2 *>(rec),        \
+                       (unsigned)rec_bytes, order, ntiles,                                                           \
+                       counters, counters + 1, max_spin, sl)
+#define CGLI_HELPED_H(FW, GCOL)                     \
+    switch (waves) {                                \
+    case 2: CGLI_HELPED(FW, GCOL, 1); break;        \
+    case 3: CGLI_HELPED(FW, GCOL, 2); break;        \
+    default: CGLI_HELPED(FW, GCOL, 3); break;       \
+    }
+    if (waves <= 1) {
+        if (forward && Gc) CGLI_FUSED(true, true);
+        else if (forward) CGLI_FUSED(true, false);
+        else if (Gc) CGLI_FUSED(false, true);
+        else CGLI_FUSED(false, false);
+    } else {
+        if (forward && Gc) CGLI_HELPED_H(true, true)
+        else if (forward) CGLI_HELPED_H(true, false)
+        else if (Gc) CGLI_HELPED_H(false, true)
+        else CGLI_HELPED_H(false, false)
+    }
 #undef CGLI_FUSED
+#undef CGLI_HELPED
+#undef CGLI_HELPED_H
 }
 
 // 1 into *flag where a cell's J or metrics differ from those of the first

@@ -98,6 +98,7 @@ def lib():
         L.cgli_channel_ghosts.argtypes = [vp]
         L.cgli_channel_step.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), dp, dp]
         L.cgli_set_lusgs_mode.argtypes = [vp, C.c_int]
+        L.cgli_set_sweep_waves.argtypes = [vp, C.c_int]
         L.cgli_set_spin_limit.argtypes = [vp, C.c_int]
         L.cgli_timing_enable.argtypes = [vp, C.c_int]
         L.cgli_timing_reset.argtypes = [vp]
@@ -393,6 +394,11 @@ class Context:
         on march-ordered coefficient records, 4: the wavefront on
         march-ordered state computing its own coefficients (default)."""
         _check(lib().cgli_set_lusgs_mode(self.h, mode), "set_lusgs_mode")
+
+    def set_sweep_waves(self, waves):
+        """Schedule 4's waves per tile: 1 (one wave), or a chain wave with
+        1-3 helper waves computing the coefficients ahead."""
+        _check(lib().cgli_set_sweep_waves(self.h, waves), "set_sweep_waves")
 
     def set_spin_limit(self, n):
         _check(lib().cgli_set_spin_limit(self.h, n), "set_spin_limit")

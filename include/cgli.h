@@ -197,6 +197,12 @@ int cgli_graph_destroy(cgli_graph *graph);
  * wavefronts' bounded waits give up after max_spin polls each (then
  * CGLI_E_TIMEOUT).  Schedule 2 (retired) is CGLI_E_ARG. */
 int cgli_set_lusgs_mode(cgli_ctx *ctx, int mode);
+/* Schedule 4's workgroup per tile: 1 = one wave marching and computing each
+ * next cell's coefficients, 2-4 = one chain wave marching and 1-3 helper
+ * waves computing the coefficients of the steps ahead into LDS slots (the
+ * same increments bit for bit).  No counterpart in the reference (its sweep
+ * is one loop per hyperplane). */
+int cgli_set_sweep_waves(cgli_ctx *ctx, int waves);
 int cgli_set_spin_limit(cgli_ctx *ctx, int max_spin);
 /* DPLUSGS.cpp:18-1238: Jacobi (DP-LUR) variant with `sweeps` sweeps (the
  * reference runs 4). */

@@ -132,6 +132,37 @@ def test_channel_graph_steps_match_oracle():
                 assert r <= TOL, (step, v, r)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_channel_steps_with_helper_waves_give_the_same_bits(precision):
+    """The channel's metrics do not vary along i, so the helper-wave sweep
+    keeps each column's metrics in registers: channel steps with 2-4 waves
+    per tile equal the one-wave steps bit for bit (FP64 and mixed)."""
+    from cgli_fu_flow_amd import native as nv
+    nx, ny, nz = 16, 24, 12
+    print("channel helper waves", (nx, ny, nz), "precision", precision, flush=True)
+    ch = nv.channel(395.0, beta=2.4, skew=0.0, xi_mode=1)
+    cfg = nv.channel_config(ch, nx, ny, nz)
+    cfg.precision = precision
+    ctxs = [nv.Context(cfg) for _ in range(4)]
+    try:
+        dt = None
+        for w, ctx in enumerate(ctxs):
+            dt = ctx.channel_init(ch)
+            ctx.set_sweep_waves(w + 1)
+        for step in range(3):
+            res = [ctx.channel_step(dt, tol=1e-6, max_inner=12) for ctx in ctxs]
+            for w in range(1, 4):
+                assert res[w][0] == res[0][0] and np.array_equal(res[w][2], res[0][2]), (step, w)
+            U0 = ctxs[0].get_many(nv.U, 5)
+            for w in range(1, 4):
+                Uw = ctxs[w].get_many(nv.U, 5)
+                for v in range(5):
+                    assert np.array_equal(Uw[v], U0[v]), (step, w, v)
+    finally:
+        for ctx in ctxs:
+            ctx.close()
+
+
 def test_channel_steps_with_sensor_scaled_dissipation_match_oracle():
     """The channel driver with the Roe dissipation scaled by the filter
     sensor: each step runs the filter, then the steps' fluxes use its face

@@ -312,6 +312,42 @@ def test_wavefront_and_plane_schedules_agree(nv, orc, name):
             assert np.array_equal(outs[1][q][v], outs[2][q][v])
 
 
+@pytest.mark.parametrize("name", [n for n in sorted(CASES) if CASES[n][3] == 1])
+def test_sweep_helper_waves_give_the_same_bits(nv, orc, name):
+    """Schedule 4 with helper waves computing the coefficients ahead into
+    LDS slots does the same arithmetic on the same values as the one-wave
+    march: identical bits for every wave count (these slabs' metrics vary
+    along i, so the slots carry the metrics too)."""
+    print("case", name, flush=True)
+    case = make_case(name)
+    f = make_inputs(orc, case.slab, SEED + 6)
+    outs = []
+    for waves in (1, 2, 3, 4):
+        with open_ctx(nv, orc, case) as ctx:
+            ctx.set_sweep_waves(waves)
+            upload_inputs(ctx, f)
+            ctx.flux_x(1.0, 1e-4)
+            ctx.flux_y(1.0, 1e-4)
+            ctx.flux_z(1.0, 1e-4)
+            ctx.viscous()
+            er = ctx.dual_time_stepping(2e-4, 1e-4, sweeps=2)
+            outs.append((er, ctx.get_many(nv.NL, 5), ctx.get_many(nv.NR, 5), ctx.get_many(nv.U_, 5)))
+    for w in (1, 2, 3):
+        assert np.array_equal(outs[0][0], outs[w][0]), w
+        for q in (1, 2, 3):
+            for v in range(5):
+                assert np.array_equal(outs[0][q][v], outs[w][q][v]), (w, q, v)
+
+
+def test_unknown_sweep_wave_count_is_an_error(nv, orc):
+    case = make_case("ref_8x10x6_both_ends")
+    with open_ctx(nv, orc, case) as ctx:
+        for waves in (0, 5):
+            with pytest.raises(nv.CgliError) as ei:
+                ctx.set_sweep_waves(waves)
+            assert ei.value.code == -1
+
+
 def test_unknown_schedule_is_an_error(nv, orc):
     case = make_case("ref_8x10x6_both_ends")
     with open_ctx(nv, orc, case) as ctx:
