@@ -61,7 +61,8 @@ struct cgli_ctx {
     bool norm_wave = false;      // ... with the wavefront's timeout flag
     int lusgs_mode = 4;          // 0: one launch per hyperplane, 1: tiled wavefront, 3: tiled wavefront on
                                  // march-ordered records, 4: march computing its coefficients (default)
-    int sweep_waves = 1;         // schedule 4: waves per tile (1: one wave; 2-4: a chain wave and helpers)
+    int sweep_waves = 0;         // schedule 4: waves per tile (0: by tile count; 1: one wave; 2-4: a chain
+                                 // wave and helpers)
     double *coef = nullptr;      // split-coefficient blocks of the wavefront (float in the mixed mode)
     double *dgn = nullptr;       // diagonal blocks of the wavefront (FP64)
     void *coefM = nullptr;       // the same records in march order (mode 3)
@@ -739,6 +740,15 @@ static int finish_norm(cgli_ctx *c, int nb, double er[5], bool check_wave = fals
     return r ? r : collect_norm(c, er, check_wave);
 }
 
+// schedule 4's waves per tile: by default a chain wave with one helper when
+// there is at most one tile per CU (the march step's latency bounds the
+// sweep: 128^3 forward sweep 0.68 ms against 0.77 with one wave), one wave
+// otherwise (the memory system bounds it: 256x192x256 1.76 ms against 1.93)
+static int sweep_waves(const cgli_ctx *c)
+{
+    return c->sweep_waves ? c->sweep_waves : (c->mtiles <= 256 ? 2 : 1);
+}
+
 // the wavefront's records (in the layout of the current schedule) and its
 // exchange buffers, allocated on first use
 static int wave_setup(cgli_ctx *c)
@@ -835,7 +845,7 @@ int cgli_set_lusgs_mode(cgli_ctx *c, int mode)
 
 int cgli_set_sweep_waves(cgli_ctx *c, int waves)
 {
-    if (!c || waves < 1 || waves > 4) return CGLI_E_ARG;
+    if (!c || waves < 0 || waves > 4) return CGLI_E_ARG;
     NO_CAPTURE(c);
     c->sweep_waves = waves;
     c->generation++;
@@ -1035,14 +1045,14 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
             launch_lusgs_fused(g, c->p, deltaT, e, 1, c->geo4, c->geo_cols ? c->gcol4 : nullptr, c->u4, x4, x4 + f4b,
                                c5(c, CGLI_U_), c->f[CGLI_J],
                                m9(c, CGLI_MET), c5(c, CGLI_NL), c->rec4, c->rec4_bytes, c->morder_dev,
-                               c->mtiles, c->counters, c->max_spin, -1, c->sweep_waves, s);
+                               c->mtiles, c->counters, c->max_spin, -1, sweep_waves(c), s);
             tend(c, T_FWD);
             launch_rec4_prep(c->rec4, c->rec4_bytes, c->counters, s);
             tbegin(c, T_BWD);
             launch_lusgs_fused(g, c->p, deltaT, e, 0, c->geo4, c->geo_cols ? c->gcol4 : nullptr, c->u4, x4 + f4b,
                                x4 + 2 * f4b, c5(c, CGLI_U_),
                                c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_NR), c->rec4, c->rec4_bytes,
-                               c->morder_dev, c->mtiles, c->counters, c->max_spin, -1, c->sweep_waves, s);
+                               c->morder_dev, c->mtiles, c->counters, c->max_spin, -1, sweep_waves(c), s);
             tend(c, T_BWD);
             if (sw == sweeps - 1) {
                 tbegin(c, T_SCATTER);

@@ -396,8 +396,9 @@ class Context:
         _check(lib().cgli_set_lusgs_mode(self.h, mode), "set_lusgs_mode")
 
     def set_sweep_waves(self, waves):
-        """Schedule 4's waves per tile: 1 (one wave), or a chain wave with
-        1-3 helper waves computing the coefficients ahead."""
+        """Schedule 4's waves per tile: 1 (one wave), 2-4 (a chain wave with
+        1-3 helper waves computing the coefficients ahead), 0 (by tile
+        count, the default)."""
         _check(lib().cgli_set_sweep_waves(self.h, waves), "set_sweep_waves")
 
     def set_spin_limit(self, n):

@@ -342,10 +342,11 @@ def test_sweep_helper_waves_give_the_same_bits(nv, orc, name):
 def test_unknown_sweep_wave_count_is_an_error(nv, orc):
     case = make_case("ref_8x10x6_both_ends")
     with open_ctx(nv, orc, case) as ctx:
-        for waves in (0, 5):
+        for waves in (-1, 5):
             with pytest.raises(nv.CgliError) as ei:
                 ctx.set_sweep_waves(waves)
             assert ei.value.code == -1
+        ctx.set_sweep_waves(0)
 
 
 def test_unknown_schedule_is_an_error(nv, orc):
