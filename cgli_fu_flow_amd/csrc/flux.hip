@@ -7,10 +7,11 @@
 // right states of its face from the six cells along the sweep direction
 // (Flux_X.cpp:106-216, Flux_Y.cpp:106-249, Flux_Z.cpp:105-216) and applies
 // the low-Mach preconditioned Roe flux (Flux_X.cpp:229-350).  The
-// reference's ML/MR scratch never goes to HBM.  Operation order follows the
-// reference expression by expression and the library is compiled without
-// FMA contraction, so results match the CPU build to the last bit except
-// where libm and the device math library round sqrt/pow differently.
+// reference's ML/MR scratch never goes to HBM.  The expressions are the
+// reference's, compiled without FMA contraction, except that a divisor used
+// more than once is inverted once and multiplied (the FP64 division is the
+// costliest instruction sequence of the kernel): results agree with the CPU
+// build to rounding (a few ulps), not bit for bit.
 // Optional dissipation scaling (the older flux files, Flux_X~~.cpp:552-556):
 // each face's Roe dissipation Fav is multiplied by epS, or by EpF[face] (the
 // filter sensor, face-indexed like the fluxes) when EpF is given.  The live
@@ -19,8 +20,11 @@
 
 namespace cgli {
 
-// J-weighted reconstruction weights.  `temp` is evaluated once per stencil
-// and shared by the five variables, as in the reference loop body.
+// J-weighted reconstruction weights.  The thread forms the reciprocals
+// q[m] = 1/J of the six cells c0 + (m - 2) s along the sweep direction
+// (those its two stencils use) once, and each weight from them; `temp` of
+// the reference (Flux_X.cpp:118-121) is 1 / (sum of coefficient / J), the
+// same sum with the divisions taken once per cell.
 template <typename T>
 struct Rec {
     int kind;     // 5: 5-point left, -5: 5-point right, 3: 3rd left, -3: 3rd right, 2: average
@@ -34,42 +38,48 @@ __device__ __forceinline__ T ld(const double *p, long i)
     return (T)p[i];
 }
 
+// reciprocals of J at c0 + (m - 2) s for m = 0..5; q[0] only where the left
+// stencil is the 5-point one, q[5] only where the right one is (the other
+// entries lie in the array for every face the kernels compute)
 template <typename T>
-__device__ __forceinline__ Rec<T> rec5L(const double *J, long a, long b, long c, long d, long f)
+__device__ __forceinline__ void recip_j(const double *__restrict__ J, long c0, long s, bool m0, bool m5, T q[6])
 {
-    Rec<T> r{5, a, b, c, d, f, T(0)};
-    r.w = T(1) / (T(2) / ld<T>(J, a) - T(13) / ld<T>(J, b) + T(47) / ld<T>(J, c) + T(27) / ld<T>(J, d) -
-                  T(3) / ld<T>(J, f));
-    return r;
+    q[0] = m0 ? T(1) / ld<T>(J, c0 - 2 * s) : T(0);
+#pragma unroll
+    for (int m = 1; m < 5; m++) q[m] = T(1) / ld<T>(J, c0 + (m - 2) * s);
+    q[5] = m5 ? T(1) / ld<T>(J, c0 + 3 * s) : T(0);
+}
+
+// the stencils of face c0 + s/2 by stencil-cell position m (cell c0 + (m - 2) s)
+template <typename T>
+__device__ __forceinline__ Rec<T> rec5L(const T q[6], long c0, long s)
+{
+    return Rec<T>{5, c0 - 2 * s, c0 - s, c0, c0 + s, c0 + 2 * s,
+                  T(1) / (T(2) * q[0] - T(13) * q[1] + T(47) * q[2] + T(27) * q[3] - T(3) * q[4])};
 }
 template <typename T>
-__device__ __forceinline__ Rec<T> rec5R(const double *J, long a, long b, long c, long d, long f)
+__device__ __forceinline__ Rec<T> rec5R(const T q[6], long c0, long s)
 {
-    Rec<T> r{-5, a, b, c, d, f, T(0)};
-    r.w = T(1) / (-T(3) / ld<T>(J, a) + T(27) / ld<T>(J, b) + T(47) / ld<T>(J, c) - T(13) / ld<T>(J, d) +
-                  T(2) / ld<T>(J, f));
-    return r;
+    return Rec<T>{-5, c0 - s, c0, c0 + s, c0 + 2 * s, c0 + 3 * s,
+                  T(1) / (-T(3) * q[1] + T(27) * q[2] + T(47) * q[3] - T(13) * q[4] + T(2) * q[5])};
+}
+// 3rd-order left state from cells (m, c, p) at positions (mi, ci, pi)
+template <typename T>
+__device__ __forceinline__ Rec<T> rec3L(const T q[6], long c0, long s, int mi, int ci, int pi)
+{
+    return Rec<T>{3, c0 + (mi - 2) * s, c0 + (ci - 2) * s, c0 + (pi - 2) * s, 0, 0,
+                  T(1) / (-q[mi] + T(5) * q[ci] + T(2) * q[pi])};
 }
 template <typename T>
-__device__ __forceinline__ Rec<T> rec3L(const double *J, long m, long c, long p)
+__device__ __forceinline__ Rec<T> rec3R(const T q[6], long c0, long s, int mi, int ci, int pi)
 {
-    Rec<T> r{3, m, c, p, 0, 0, T(0)};
-    r.w = T(1) / (-T(1) / ld<T>(J, m) + T(5) / ld<T>(J, c) + T(2) / ld<T>(J, p));
-    return r;
+    return Rec<T>{-3, c0 + (mi - 2) * s, c0 + (ci - 2) * s, c0 + (pi - 2) * s, 0, 0,
+                  T(1) / (T(2) * q[mi] + T(5) * q[ci] - q[pi])};
 }
 template <typename T>
-__device__ __forceinline__ Rec<T> rec3R(const double *J, long m, long c, long p)
+__device__ __forceinline__ Rec<T> rec2(const T q[6], long c0, long s, int ai, int bi)
 {
-    Rec<T> r{-3, m, c, p, 0, 0, T(0)};
-    r.w = T(1) / (T(2) / ld<T>(J, m) + T(5) / ld<T>(J, c) - T(1) / ld<T>(J, p));
-    return r;
-}
-template <typename T>
-__device__ __forceinline__ Rec<T> rec2(const double *J, long a, long b)
-{
-    Rec<T> r{2, a, b, 0, 0, 0, T(0)};
-    r.w = T(1) / (T(1) / ld<T>(J, a) + T(1) / ld<T>(J, b));
-    return r;
+    return Rec<T>{2, c0 + (ai - 2) * s, c0 + (bi - 2) * s, 0, 0, 0, T(1) / (q[ai] + q[bi])};
 }
 
 template <typename T>
@@ -90,73 +100,82 @@ __device__ __forceinline__ T apply(const Rec<T> &r, const double *__restrict__ U
 
 // Preconditioned Roe flux through one face (Flux_X.cpp:251-350).  (mx,my,mz)
 // are the face metrics, (nx_,ny_,nz_) the vector multiplying the pressure
-// jump in the momentum dissipation.
+// jump in the momentum dissipation.  The reference's expressions, with each
+// divisor that recurs (the densities, the Roe weight sum, K - 1, C, S, Jf)
+// inverted once and multiplied, and the dissipation's division by
+// 1/sqrt(|m|^2) taken as a product with sqrt(|m|^2): the results differ from
+// the reference's by rounding only.
 template <typename T>
 __device__ __forceinline__ void roe_face(const T L[5], const T Rq[5], T mx, T my, T mz, T Jf, T e, T K, T nx_,
                                          T ny_, T nz_, T ep, T F[5], T &pL_out, T &pR_out, T &Fav3_out)
 {
-    const T insqr = T(1.0) / sqrt(mx * mx + my * my + mz * mz);
+    const T sq = sqrt(mx * mx + my * my + mz * mz);
+    const T insqr = T(1.0) / sq;
+    const T Km1 = K - T(1), iKm1 = T(1) / Km1;
 
-    const T rL = L[0];
-    const T uL = L[1] / rL, vL = L[2] / rL, wL = L[3] / rL;
+    const T rL = L[0], irL = T(1) / rL;
+    const T uL = L[1] * irL, vL = L[2] * irL, wL = L[3] * irL;
     const T UnL = mx * uL + my * vL + mz * wL;
     const T qL = uL * uL + vL * vL + wL * wL;
-    const T pL = (L[4] - T(0.5) * rL * qL) * (K - T(1));
-    const T cL = K * pL / rL;
-    const T hL = T(0.5) * qL + cL / (K - T(1));
+    const T pL = (L[4] - T(0.5) * rL * qL) * Km1;
+    const T cL = K * pL * irL;
+    const T hL = T(0.5) * qL + cL * iKm1;
 
-    const T rR = Rq[0];
-    const T uR = Rq[1] / rR, vR = Rq[2] / rR, wR = Rq[3] / rR;
+    const T rR = Rq[0], irR = T(1) / rR;
+    const T uR = Rq[1] * irR, vR = Rq[2] * irR, wR = Rq[3] * irR;
     const T UnR = mx * uR + my * vR + mz * wR;
     const T qR = uR * uR + vR * vR + wR * wR;
-    const T pR = (Rq[4] - T(0.5) * rR * qR) * (K - T(1));
-    const T cR = K * pR / rR;
-    const T hR = T(0.5) * qR + cR / (K - T(1));
+    const T pR = (Rq[4] - T(0.5) * rR * qR) * Km1;
+    const T cR = K * pR * irR;
+    const T hR = T(0.5) * qR + cR * iKm1;
 
     const T sL = sqrt(rL), sR = sqrt(rR);
-    const T ss = sL + sR;
-    const T rho = sqrt(rL * rR);
-    const T u = (sL * uL + sR * uR) / ss;
-    const T v = (sL * vL + sR * vR) / ss;
-    const T w = (sL * wL + sR * wR) / ss;
-    const T Un = (sL * UnL + sR * UnR) / ss;
+    const T iss = T(1) / (sL + sR);
+    const T rho = sL * sR;
+    const T u = (sL * uL + sR * uR) * iss;
+    const T v = (sL * vL + sR * vR) * iss;
+    const T w = (sL * wL + sR * wR) * iss;
+    const T Un = (sL * UnL + sR * UnR) * iss;
     const T q = u * u + v * v + w * w;
-    const T H = (sL * hL + sR * hR) / ss;
-    const T C = (H - T(0.5) * q) * (K - T(1));
+    const T H = (sL * hL + sR * hR) * iss;
+    const T C = (H - T(0.5) * q) * Km1;
 
     const T dU1 = rR - rL;
     const T dU2 = rR * uR - rL * uL;
     const T dU3 = rR * vR - rL * vL;
     const T dU4 = rR * wR - rL * wL;
-    const T dU5 = (pR / (K - T(1)) + T(0.5) * rR * qR) - (pL / (K - T(1)) + T(0.5) * rL * qL);
+    const T dU5 = (pR * iKm1 + T(0.5) * rR * qR) - (pL * iKm1 + T(0.5) * rL * qL);
 
-    const T qc = q / C;
-    const T beta = (qc > e) ? qc : e;
-    const T ubar = T(0.5) * (T(1) + beta) * Un * insqr;
-    const T S = T(0.5) * sqrt(T(4) * beta * C + Un * insqr * Un * insqr * (T(1) - beta) * (T(1) - beta));
     const T th = q / C;
-    const T U_p = T(0.5) * (T(1) + th) * Un * insqr;
-    const T C_p = T(0.5) * sqrt(T(4) * C * th + Un * insqr * Un * insqr * (T(1) - th) * (T(1) - th));
+    const T beta = (th > e) ? th : e;
+    const T Unn = Un * insqr;
+    const T ubar = T(0.5) * (T(1) + beta) * Unn;
+    const T S = T(0.5) * sqrt(T(4) * beta * C + Unn * Unn * (T(1) - beta) * (T(1) - beta));
+    const T iS = T(1) / S;
+    const T U_p = T(0.5) * (T(1) + th) * Unn;
+    const T C_p = T(0.5) * sqrt(T(4) * C * th + Unn * Unn * (T(1) - th) * (T(1) - th));
 
-    const T t1 = (pR - pL) / rho / beta / C;
-    const T t2 = U_p / S * (UnR - UnL) * insqr;
-    const T t3 = T(0.5) * (T(1) - beta) * Un * insqr * ubar / S;
-    const T t4 = T(0.5) * (T(1) - th) * Un * insqr * U_p / S;
-    const T dUc = (S - t3 - beta * fabs(Un) * insqr) * t1 + t2;
-    const T dPc = U_p / S * (pR - pL) + (C_p - fabs(Un) * insqr + t4) * rho * (UnR - UnL) * insqr;
+    const T t1 = (pR - pL) / (rho * beta * C);
+    const T dUn = (UnR - UnL) * insqr;
+    const T t2 = U_p * iS * dUn;
+    const T t3 = T(0.5) * (T(1) - beta) * Unn * ubar * iS;
+    const T t4 = T(0.5) * (T(1) - th) * Unn * U_p * iS;
+    const T aU = fabs(Unn);
+    const T dUc = (S - t3 - beta * aU) * t1 + t2;
+    const T dPc = U_p * iS * (pR - pL) + (C_p - aU + t4) * rho * dUn;
 
-    const T aU = fabs(Un * insqr);
     const T Fav1 = aU * dU1 + dUc * rho;
     const T Fav2 = aU * dU2 + dUc * rho * u + dPc * nx_;
     const T Fav3 = aU * dU3 + dUc * rho * v + dPc * ny_;
     const T Fav4 = aU * dU4 + dUc * rho * w + dPc * nz_;
-    const T Fav5 = aU * dU5 + dUc * rho * H + dPc * Un * insqr;
+    const T Fav5 = aU * dU5 + dUc * rho * H + dPc * Unn;
 
-    F[0] = T(0.5) * ((rL * UnL + rR * UnR) - ep * Fav1 / insqr) / Jf;
-    F[1] = T(0.5) * ((rL * uL * UnL + rR * uR * UnR + mx * (pL + pR)) - ep * Fav2 / insqr) / Jf;
-    F[2] = T(0.5) * ((rL * vL * UnL + rR * vR * UnR + my * (pL + pR)) - ep * Fav3 / insqr) / Jf;
-    F[3] = T(0.5) * ((rL * wL * UnL + rR * wR * UnR + mz * (pL + pR)) - ep * Fav4 / insqr) / Jf;
-    F[4] = T(0.5) * ((UnL * (T(3.5) * pL + T(0.5) * rL * qL) + UnR * (T(3.5) * pR + T(0.5) * rR * qR)) - ep * Fav5 / insqr) / Jf;
+    const T hJ = T(0.5) / Jf, es = ep * sq;
+    F[0] = ((rL * UnL + rR * UnR) - es * Fav1) * hJ;
+    F[1] = ((rL * uL * UnL + rR * uR * UnR + mx * (pL + pR)) - es * Fav2) * hJ;
+    F[2] = ((rL * vL * UnL + rR * vR * UnR + my * (pL + pR)) - es * Fav3) * hJ;
+    F[3] = ((rL * wL * UnL + rR * wR * UnR + mz * (pL + pR)) - es * Fav4) * hJ;
+    F[4] = ((UnL * (T(3.5) * pL + T(0.5) * rL * qL) + UnR * (T(3.5) * pR + T(0.5) * rR * qR)) - es * Fav5) * hJ;
     pL_out = pL;
     pR_out = pR;
     Fav3_out = Fav3;
@@ -179,18 +198,21 @@ __global__ void __launch_bounds__(256) k_flux_x(Geo g, double K, double e, C5 U,
     const long c0 = ix(g, i, cj, ck);
     // left state of face i+1/2; later reference writes win (last-rank patch
     // after first-rank patch after the 5-point loop)
+    const bool L5 = !(g.last && i == g.gend) && !(g.first && (i == 3 || i == 2));
+    const bool R5 = !(g.last && (i == g.gend || i == g.gend - 1)) && !(g.first && i == 2);
+    T q[6];
+    recip_j<T>(J, c0, sx, L5, R5, q);
     Rec<T> L;
-    if (g.last && i == g.gend) L = rec3L<T>(J, c0 - sx, c0, c0 + sx);
-    else if (g.first && i == 3) L = rec3L<T>(J, c0 - sx, c0, c0 + sx);
-    else if (g.first && i == 2) L = rec2<T>(J, c0, c0 + sx);
-    else L = rec5L<T>(J, c0 - 2 * sx, c0 - sx, c0, c0 + sx, c0 + 2 * sx);
+    if (g.last && i == g.gend) L = rec3L<T>(q, c0, sx, 1, 2, 3);
+    else if (g.first && i == 3) L = rec3L<T>(q, c0, sx, 1, 2, 3);
+    else if (g.first && i == 2) L = rec2<T>(q, c0, sx, 2, 3);
+    else L = rec5L<T>(q, c0, sx);
     // right state of face i+1/2 (from cell i+1)
-    const long c1 = c0 + sx;
     Rec<T> R;
-    if (g.last && i == g.gend) R = rec2<T>(J, c0, c1);
-    else if (g.last && i == g.gend - 1) R = rec3R<T>(J, c0, c1, c1 + sx);
-    else if (g.first && i == 2) R = rec3R<T>(J, c0, c1, c1 + sx);
-    else R = rec5R<T>(J, c1 - 2 * sx, c1 - sx, c1, c1 + sx, c1 + 2 * sx);
+    if (g.last && i == g.gend) R = rec2<T>(q, c0, sx, 2, 3);
+    else if (g.last && i == g.gend - 1) R = rec3R<T>(q, c0, sx, 2, 3, 4);
+    else if (g.first && i == 2) R = rec3R<T>(q, c0, sx, 2, 3, 4);
+    else R = rec5R<T>(q, c0, sx);
     T Lq[5], Rq[5];
 #pragma unroll
     for (int v = 0; v < 5; v++) {
@@ -225,20 +247,24 @@ __global__ void __launch_bounds__(256) k_flux_y(Geo g, double K, double Ep, doub
     const long c0 = ix(g, i + 1, j, k + 1);   // cell j
     const long c1 = c0 + sy;                  // cell j+1
     const int NY = g.NY;
+    const bool L5 = j != NY && j != NY - 1 && j > 3;
+    const bool R5 = j != NY && j != NY - 1 && j != NY - 2 && j > 2;
+    T q[6];
+    recip_j<T>(J, c0, sy, L5, R5, q);
     Rec<T> L;
-    if (j == NY) L = rec2<T>(J, c0, c0 - sy);
-    else if (j == NY - 1) L = rec3L<T>(J, c0 - sy, c0, c0 + sy);
-    else if (j == 3) L = rec3L<T>(J, c0 - sy, c0, c0 + sy);
-    else if (j == 2) L = rec2<T>(J, c0, c0 + sy);
-    else if (j == 1) L = rec2<T>(J, c0, c0 + sy);
-    else L = rec5L<T>(J, c0 - 2 * sy, c0 - sy, c0, c0 + sy, c0 + 2 * sy);
+    if (j == NY) L = rec2<T>(q, c0, sy, 2, 1);
+    else if (j == NY - 1) L = rec3L<T>(q, c0, sy, 1, 2, 3);
+    else if (j == 3) L = rec3L<T>(q, c0, sy, 1, 2, 3);
+    else if (j == 2) L = rec2<T>(q, c0, sy, 2, 3);
+    else if (j == 1) L = rec2<T>(q, c0, sy, 2, 3);
+    else L = rec5L<T>(q, c0, sy);
     Rec<T> R;
-    if (j == NY) R = rec2<T>(J, c0, c1);
-    else if (j == NY - 1) R = rec2<T>(J, c1, c0);
-    else if (j == NY - 2) R = rec3R<T>(J, c0, c1, c1 + sy);
-    else if (j == 2) R = rec3R<T>(J, c0, c1, c1 + sy);
-    else if (j == 1) R = rec2<T>(J, c1, c1 + sy);
-    else R = rec5R<T>(J, c1 - 2 * sy, c1 - sy, c1, c1 + sy, c1 + 2 * sy);
+    if (j == NY) R = rec2<T>(q, c0, sy, 2, 3);
+    else if (j == NY - 1) R = rec2<T>(q, c0, sy, 3, 2);
+    else if (j == NY - 2) R = rec3R<T>(q, c0, sy, 2, 3, 4);
+    else if (j == 2) R = rec3R<T>(q, c0, sy, 2, 3, 4);
+    else if (j == 1) R = rec2<T>(q, c0, sy, 3, 4);
+    else R = rec5R<T>(q, c0, sy);
     T Lq[5], Rq[5];
 #pragma unroll
     for (int v = 0; v < 5; v++) {
@@ -284,16 +310,20 @@ __global__ void __launch_bounds__(256) k_flux_z(Geo g, double K, double e, int x
     const long c0 = ix(g, i + 1, j + 1, k);   // cell k
     const long c1 = c0 + 1;                   // cell k+1
     const int NZ = g.NZ;
+    const bool L5 = k != NZ && k > 2;
+    const bool R5 = k != NZ && k != NZ - 1 && k > 1;
+    T q[6];
+    recip_j<T>(J, c0, 1, L5, R5, q);
     Rec<T> L;
-    if (k == NZ) L = rec3L<T>(J, c0 - 1, c0, c0 + 1);
-    else if (k == 2) L = rec3L<T>(J, c0 - 1, c0, c0 + 1);
-    else if (k == 1) L = rec2<T>(J, c0, c0 + 1);
-    else L = rec5L<T>(J, c0 - 2, c0 - 1, c0, c0 + 1, c0 + 2);
+    if (k == NZ) L = rec3L<T>(q, c0, 1, 1, 2, 3);
+    else if (k == 2) L = rec3L<T>(q, c0, 1, 1, 2, 3);
+    else if (k == 1) L = rec2<T>(q, c0, 1, 2, 3);
+    else L = rec5L<T>(q, c0, 1);
     Rec<T> R;
-    if (k == NZ) R = rec2<T>(J, c0, c1);
-    else if (k == NZ - 1) R = rec3R<T>(J, c0, c1, c1 + 1);
-    else if (k == 1) R = rec3R<T>(J, c0, c1, c1 + 1);
-    else R = rec5R<T>(J, c1 - 2, c1 - 1, c1, c1 + 1, c1 + 2);
+    if (k == NZ) R = rec2<T>(q, c0, 1, 2, 3);
+    else if (k == NZ - 1) R = rec3R<T>(q, c0, 1, 2, 3, 4);
+    else if (k == 1) R = rec3R<T>(q, c0, 1, 2, 3, 4);
+    else R = rec5R<T>(q, c0, 1);
     T Lq[5], Rq[5];
 #pragma unroll
     for (int v = 0; v < 5; v++) {

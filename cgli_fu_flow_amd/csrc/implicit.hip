@@ -1757,7 +1757,7 @@ struct F4Lane {
 struct M4Args {
     Geo g;
     int S;
-    double K, e, deltaT;
+    double K, iK, e, deltaT;
     double h[3];              // 1 / dxi, 1 / deta, 1 / dzeta
     __amdgpu_buffer_rsrc_t grow, urow, rrow;   // rows from (T, t = 0)
     char *xrow0;              // output row of (T, t = 0)
@@ -1873,7 +1873,9 @@ __device__ __forceinline__ void load4(Ld4 &L, const M4Args &w, unsigned vo, int 
 // diagonal.  The compact set holds per direction the entries that do not
 // multiply a metric (a0 = g11 or m11, a5 = g22 or m22, a8) and the three
 // factors that do (pk, ir, bt), then d11, d22, d51; coef_expand forms the
-// twelve entries of each direction from it with the same products.
+// twelve entries of each direction from it with the same products.  The
+// division by K is a product with 1/K and the two diagonal reciprocals come
+// from one division, so the set agrees with the reference's to rounding.
 constexpr int CF_N = 21;
 
 template <bool FWD>
@@ -1907,11 +1909,14 @@ __device__ __forceinline__ void coef_compact(double c[CF_N], double J, const dou
         a[2] = 0.5 * P * K * b * h;
         const double ir = 0.5 * irho * h;
         a[3] = ir;
-        a[4] = Un * (K - 1) * (b - 1) * (ir / K);
+        a[4] = Un * (K - 1) * (b - 1) * (ir * w.iK);
         a[5] = 0.5 * b * T * (K - 1) * h;
     }
+    // the two diagonal reciprocals from one division
     const double dt = w.deltaT;
-    const double rA = 1.0 / (3 + 2 * lam * dt), rB = 1.0 / (3 * b + 2 * lam * dt);
+    const double dA = 3 + 2 * lam * dt, dB = 3 * b + 2 * lam * dt;
+    const double rAB = 1.0 / (dA * dB);
+    const double rA = dB * rAB, rB = dA * rAB;
     c[18] = 2 * dt * rB;
     c[19] = 2 * dt * rA;
     c[20] = -(6 * (K - 1) / K) * (b - 1) * dt * rA * rB * irho;
@@ -2128,6 +2133,7 @@ __device__ __forceinline__ int fused_setup(F4Lane &L, M4Args &w, long &c0, int l
     w.g = g;
     w.S = M.S;
     w.K = K;
+    w.iK = 1.0 / K;
     w.e = e;
     w.deltaT = deltaT;
     w.h[0] = hx;

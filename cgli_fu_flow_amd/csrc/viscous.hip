@@ -13,9 +13,12 @@
 // A block owns a (j, k) tile and marches along i through a segment of
 // planes: a ring of three planes of primitives (tile plus a 1-cell halo) and
 // the tile's y and z face fluxes live in LDS, the x face flux of the cell
-// below travels in registers, so that U, J and each face's metrics are read
-// once and vF is the only store.  The values are the same doubles as the
-// reference's 16 primitive reconstructions per cell.
+// below travels in registers, so that U and each face's J and metrics are
+// read once and vF is the only store.  The arithmetic is the reference's
+// except for divisions: a face's 1/Jf and the constant Cv K / Pr are formed
+// once and multiplied, the Sutherland power (T/Ts)^1.5 is x sqrt(x), and the
+// temperature is formed without the J that cancels from it; results agree
+// with the reference's to rounding.
 #include <algorithm>
 
 #include "cgli_internal.hpp"
@@ -134,8 +137,10 @@ __device__ __forceinline__ void groups(const Real m[9], Grp<Real> &G)
 template <typename Real>
 __device__ __forceinline__ Real suth(const Phys &p, Real T)
 {
+    // (T / Ts)^1.5 as x sqrt(x): within an ulp or two of pow
     const Real Ts = (Real)p.suth_T, S = (Real)p.suth_S;
-    return (Real)p.mu_L * pow((T / Ts), Real(1.5)) * (Ts + S) / (T + S);
+    const Real x = T / Ts;
+    return (Real)p.mu_L * (x * sqrt(x)) * (Ts + S) / (T + S);
 }
 
 // The face fluxes (variables 2..5 of the reference, here 0..3).  x face
@@ -148,7 +153,7 @@ __device__ __forceinline__ void xflux(const Phys &p, const Pt<Real> &O, const Pt
                                       Real Jf, double F[4])
 {
     const Real invXI = (Real)(1. / (p.dxi)), inv4ET = (Real)(1. / (4 * p.deta)), inv4ZT = (Real)(1. / (4 * p.dzeta));
-    const Real CvK = (Real)(p.Cv * p.K), Pr = (Real)p.Pr_L;
+    const Real CvKPr = (Real)(p.Cv * p.K / p.Pr_L), iJf = Real(1) / Jf;
     Grp<Real> G;
     const D10<Real> X = d2(O, iO, invXI);
     const D10<Real> Y = d4(Oj, iOj, jO, ijO, inv4ET);
@@ -158,19 +163,19 @@ __device__ __forceinline__ void xflux(const Phys &p, const Pt<Real> &O, const Pt
     const Real Tx = Real(0.5) * (O.T + iO.T);
     const Real mu_ = suth<Real>(p, Tx);
     F[0] = mu_ * (G.a[1] * X.u + G.a[5] * X.v + G.a[7] * X.w + G.d[1] * Y.u + G.d[7] * Y.v + G.d[9] * Y.w +
-                  G.e[1] * Z.u + G.e[7] * Z.v + G.e[9] * Z.w) / Jf;
+                  G.e[1] * Z.u + G.e[7] * Z.v + G.e[9] * Z.w) * iJf;
     F[1] = mu_ * (G.a[5] * X.u + G.a[2] * X.v + G.a[6] * X.w + G.d[5] * Y.u + G.d[2] * Y.v + G.d[10] * Y.w +
-                  G.e[5] * Z.u + G.e[2] * Z.v + G.e[10] * Z.w) / Jf;
+                  G.e[5] * Z.u + G.e[2] * Z.v + G.e[10] * Z.w) * iJf;
     F[2] = mu_ * (G.a[7] * X.u + G.a[6] * X.v + G.a[3] * X.w + G.d[6] * Y.u + G.d[8] * Y.v + G.d[3] * Y.w +
-                  G.e[6] * Z.u + G.e[8] * Z.v + G.e[3] * Z.w) / Jf;
+                  G.e[6] * Z.u + G.e[8] * Z.v + G.e[3] * Z.w) * iJf;
     F[3] = mu_ * (Real(0.5) * G.a[1] * X.u2 + Real(0.5) * G.a[2] * X.v2 + Real(0.5) * G.a[3] * X.w2 + G.a[5] * X.uv +
-                  G.a[6] * X.vw + G.a[7] * X.uw + CvK * G.a[4] * X.T / (Pr) + Real(0.5) * G.d[1] * Y.u2 +
+                  G.a[6] * X.vw + G.a[7] * X.uw + CvKPr * G.a[4] * X.T + Real(0.5) * G.d[1] * Y.u2 +
                   Real(0.5) * G.d[2] * Y.v2 + Real(0.5) * G.d[3] * Y.w2 + G.d[5] * Vx * Y.u + G.d[6] * Wx * Y.u +
                   G.d[7] * Ux * Y.v + G.d[8] * Wx * Y.v + G.d[9] * Ux * Y.w + G.d[10] * Vx * Y.w +
-                  CvK * G.d[4] * Y.T / (Pr) + Real(0.5) * G.e[1] * Z.u2 + Real(0.5) * G.e[2] * Z.v2 +
+                  CvKPr * G.d[4] * Y.T + Real(0.5) * G.e[1] * Z.u2 + Real(0.5) * G.e[2] * Z.v2 +
                   Real(0.5) * G.e[3] * Z.w2 + G.e[5] * Vx * Z.u + G.e[6] * Wx * Z.u + G.e[7] * Ux * Z.v +
-                  G.e[8] * Wx * Z.v + G.e[9] * Ux * Z.w + G.e[10] * Vx * Z.w + CvK * G.e[4] * Z.T / (Pr)) /
-           Jf;
+                  G.e[8] * Wx * Z.v + G.e[9] * Ux * Z.w + G.e[10] * Vx * Z.w + CvKPr * G.e[4] * Z.T) *
+           iJf;
 }
 
 // y face j-1/2 of cell (i, j, k): O = P(i, j, k), jO = P(i, j-1, k), and the
@@ -182,7 +187,7 @@ __device__ __forceinline__ void yflux(const Phys &p, const Pt<Real> &O, const Pt
                                       Real Jf, double F[4])
 {
     const Real inv4XI = (Real)(1. / (4 * p.dxi)), invET = (Real)(1. / (p.deta)), inv4ZT = (Real)(1. / (4 * p.dzeta));
-    const Real CvK = (Real)(p.Cv * p.K), Pr = (Real)p.Pr_L;
+    const Real CvKPr = (Real)(p.Cv * p.K / p.Pr_L), iJf = Real(1) / Jf;
     Grp<Real> G;
     const D10<Real> X = d4(Oi, jOi, iO, ijO, inv4XI);
     const D10<Real> Y = d2(O, jO, invET);
@@ -192,19 +197,19 @@ __device__ __forceinline__ void yflux(const Phys &p, const Pt<Real> &O, const Pt
     const Real Ty = Real(0.5) * (O.T + jO.T);
     const Real mu_ = suth<Real>(p, Ty);
     F[0] = mu_ * (G.d[1] * X.u + G.d[5] * X.v + G.d[6] * X.w + G.b[1] * Y.u + G.b[5] * Y.v + G.b[7] * Y.w +
-                  G.f[1] * Z.u + G.f[7] * Z.v + G.f[9] * Z.w) / Jf;
+                  G.f[1] * Z.u + G.f[7] * Z.v + G.f[9] * Z.w) * iJf;
     F[1] = mu_ * (G.d[7] * X.u + G.d[2] * X.v + G.d[8] * X.w + G.b[5] * Y.u + G.b[2] * Y.v + G.b[6] * Y.w +
-                  G.f[5] * Z.u + G.f[2] * Z.v + G.f[10] * Z.w) / Jf;
+                  G.f[5] * Z.u + G.f[2] * Z.v + G.f[10] * Z.w) * iJf;
     F[2] = mu_ * (G.d[9] * X.u + G.d[10] * X.v + G.d[3] * X.w + G.b[7] * Y.u + G.b[6] * Y.v + G.b[3] * Y.w +
-                  G.f[6] * Z.u + G.f[8] * Z.v + G.f[3] * Z.w) / Jf;
+                  G.f[6] * Z.u + G.f[8] * Z.v + G.f[3] * Z.w) * iJf;
     F[3] = mu_ * (Real(0.5) * G.b[1] * Y.u2 + Real(0.5) * G.b[2] * Y.v2 + Real(0.5) * G.b[3] * Y.w2 + G.b[5] * Y.uv +
-                  G.b[6] * Y.vw + G.b[7] * Y.uw + CvK * G.b[4] * Y.T / (Pr) + Real(0.5) * G.d[1] * X.u2 +
+                  G.b[6] * Y.vw + G.b[7] * Y.uw + CvKPr * G.b[4] * Y.T + Real(0.5) * G.d[1] * X.u2 +
                   Real(0.5) * G.d[2] * X.v2 + Real(0.5) * G.d[3] * X.w2 + G.d[5] * Uy * X.v + G.d[6] * Uy * X.w +
                   G.d[7] * Vy * X.u + G.d[8] * Vy * X.w + G.d[9] * Wy * X.u + G.d[10] * Wy * X.v +
-                  CvK * G.d[4] * X.T / (Pr) + Real(0.5) * G.f[1] * Z.u2 + Real(0.5) * G.f[2] * Z.v2 +
+                  CvKPr * G.d[4] * X.T + Real(0.5) * G.f[1] * Z.u2 + Real(0.5) * G.f[2] * Z.v2 +
                   Real(0.5) * G.f[3] * Z.w2 + G.f[5] * Vy * Z.u + G.f[6] * Wy * Z.u + G.f[7] * Uy * Z.v +
-                  G.f[8] * Wy * Z.v + G.f[9] * Uy * Z.w + G.f[10] * Vy * Z.w + CvK * G.f[4] * Z.T / (Pr)) /
-           Jf;
+                  G.f[8] * Wy * Z.v + G.f[9] * Uy * Z.w + G.f[10] * Vy * Z.w + CvKPr * G.f[4] * Z.T) *
+           iJf;
 }
 
 // z face k-1/2 of cell (i, j, k): O = P(i, j, k), kO = P(i, j, k-1), and the
@@ -217,7 +222,7 @@ __device__ __forceinline__ void zflux(const Phys &p, const Pt<Real> &O, const Pt
                                       const Real m[9], Real Jf, double F[4])
 {
     const Real inv4XI = (Real)(1. / (4 * p.dxi)), inv4ET = (Real)(1. / (4 * p.deta)), invZT = (Real)(1. / (p.dzeta));
-    const Real CvK = (Real)(p.Cv * p.K), Pr = (Real)p.Pr_L;
+    const Real CvKPr = (Real)(p.Cv * p.K / p.Pr_L), iJf = Real(1) / Jf;
     Grp<Real> G;
     const D10<Real> X = d4(Oi, kOi, iO, ikO, inv4XI);
     D10<Real> Y = d4(Oj, kOj, jO, jkO, inv4ET);
@@ -230,19 +235,19 @@ __device__ __forceinline__ void zflux(const Phys &p, const Pt<Real> &O, const Pt
     const Real Tz = Real(0.5) * (O.T + kO.T);
     const Real mu_ = suth<Real>(p, Tz);
     F[0] = mu_ * (G.e[1] * X.u + G.e[5] * X.v + G.e[6] * X.w + G.f[1] * Y.u + G.f[5] * Y.v + G.f[6] * Y.w +
-                  G.c[1] * Z.u + G.c[5] * Z.v + G.c[7] * Z.w) / Jf;
+                  G.c[1] * Z.u + G.c[5] * Z.v + G.c[7] * Z.w) * iJf;
     F[1] = mu_ * (G.e[7] * X.u + G.e[2] * X.v + G.e[8] * X.w + G.f[7] * Y.u + G.f[2] * Y.v + G.f[8] * Y.w +
-                  G.c[5] * Z.u + G.c[2] * Z.v + G.c[6] * Z.w) / Jf;
+                  G.c[5] * Z.u + G.c[2] * Z.v + G.c[6] * Z.w) * iJf;
     F[2] = mu_ * (G.e[9] * X.u + G.e[10] * X.v + G.e[3] * X.w + G.f[9] * Y.u + G.f[10] * Y.v + G.f[3] * Y.w +
-                  G.c[7] * Z.u + G.c[6] * Z.v + G.c[3] * Z.w) / Jf;
+                  G.c[7] * Z.u + G.c[6] * Z.v + G.c[3] * Z.w) * iJf;
     F[3] = mu_ * (Real(0.5) * G.c[1] * Z.u2 + Real(0.5) * G.c[2] * Z.v2 + Real(0.5) * G.c[3] * Z.w2 + G.c[5] * Z.uv +
-                  G.c[6] * Z.vw + G.c[7] * Z.uw + CvK * G.c[4] * Z.T / (Pr) + Real(0.5) * G.e[1] * X.u2 +
+                  G.c[6] * Z.vw + G.c[7] * Z.uw + CvKPr * G.c[4] * Z.T + Real(0.5) * G.e[1] * X.u2 +
                   Real(0.5) * G.e[2] * X.v2 + Real(0.5) * G.e[3] * X.w2 + G.e[5] * Uz * X.v + G.e[6] * Uz * X.w +
                   G.e[7] * Vz * X.u + G.e[8] * Vz * X.w + G.e[9] * Wz * X.u + G.e[10] * Wz * X.v +
-                  CvK * G.e[4] * X.T / (Pr) + Real(0.5) * G.f[1] * Y.u2 + Real(0.5) * G.f[2] * Y.v2 +
+                  CvKPr * G.e[4] * X.T + Real(0.5) * G.f[1] * Y.u2 + Real(0.5) * G.f[2] * Y.v2 +
                   Real(0.5) * G.f[3] * Y.w2 + G.f[5] * Uz * Y.v + G.f[6] * Uz * Y.w + G.f[7] * Vz * Y.u +
-                  G.f[8] * Vz * Y.w + G.f[9] * Wz * Y.u + G.f[10] * Wz * Y.v + CvK * G.f[4] * Y.T / (Pr)) /
-           Jf;
+                  G.f[8] * Vz * Y.w + G.f[9] * Wz * Y.u + G.f[10] * Wz * Y.v + CvKPr * G.f[4] * Y.T) *
+           iJf;
 }
 
 // Tile of the fused kernel: the threads are VT_J x VT_K face positions (j, k)
@@ -251,19 +256,22 @@ __device__ __forceinline__ void zflux(const Phys &p, const Pt<Real> &O, const Pt
 constexpr int VT_J = 8, VT_K = 32, VT_THREADS = VT_J * VT_K;
 constexpr int VH_J = VT_J + 2, VH_K = VT_K + 2;   // staged primitives: positions plus a 1-cell halo
 
-// primitives (u, v, w, T) of plane i of the tile's halo into ring slot sl
-__device__ __forceinline__ void visc_plane(const Geo &g, const Phys &p, C5 U, const double *__restrict__ J, int i,
-                                           int j0, int k0, double (*sP)[4][VH_J][VH_K])
+// primitives (u, v, w, T) of plane i of the tile's halo into ring slot sl.
+// The reference's T = P / (rho R) / J with P = (U5 - rho VV / 2)(K - 1) J
+// (Viscous_terms.cpp:260-266); J cancels, so T is formed from the stored
+// state alone, with one division per cell.
+__device__ __forceinline__ void visc_plane(const Geo &g, const Phys &p, C5 U, int i, int j0, int k0,
+                                           double (*sP)[4][VH_J][VH_K])
 {
+    const double KR = (p.K - 1) / p.R;
     for (int t = threadIdx.x; t < VH_J * VH_K; t += VT_THREADS) {
         const int hj = t / VH_K, hk = t % VH_K;
         const int j = j0 - 1 + hj, k = k0 - 1 + hk;
         if (j > g.NYY || k > g.NZZ) continue;   // outside the primitive box: never read for a used face
         const long c = ix(g, i, j, k);
-        const double r = U.v[0][c];
-        const double u = U.v[1][c] / r, v = U.v[2][c] / r, w = U.v[3][c] / r;
-        const double P = (U.v[4][c] - 0.5 * r * (u * u + v * v + w * w)) * (p.K - 1) * J[c];
-        const double T = P / (r * p.R) / J[c];
+        const double r = U.v[0][c], ir = 1.0 / r;
+        const double u = U.v[1][c] * ir, v = U.v[2][c] * ir, w = U.v[3][c] * ir;
+        const double T = (U.v[4][c] - 0.5 * r * (u * u + v * v + w * w)) * KR * ir;
         double(*pl)[VH_J][VH_K] = sP[i % 3];
         pl[0][hj][hk] = u;
         pl[1][hj][hk] = v;
@@ -298,8 +306,7 @@ __device__ __forceinline__ void xface(const Geo &g, const Phys &p, const double 
 // planes in i); every face the reference's divergence reads is computed
 // (x faces i <= gend+1, y faces j <= NYY, z faces k <= NZZ).
 template <typename Real>
-__global__ void __launch_bounds__(VT_THREADS, 2) k_viscous(Geo g, Phys p, C5 U, const double *__restrict__ J,
-                                                        const double *__restrict__ Ju, M9 mu,
+__global__ void __launch_bounds__(VT_THREADS, 2) k_viscous(Geo g, Phys p, C5 U, const double *__restrict__ Ju, M9 mu,
                                                         const double *__restrict__ Jv, M9 mv,
                                                         const double *__restrict__ Jw, M9 mw, F4 vF, int seg)
 {
@@ -312,13 +319,13 @@ __global__ void __launch_bounds__(VT_THREADS, 2) k_viscous(Geo g, Phys p, C5 U, 
     const bool out = tj < VT_J - 1 && tk < VT_K - 1 && j <= g.NY && k <= g.NZ;   // an output cell
     const bool yok = j <= g.NYY && k <= g.NZ, zok = j <= g.NY && k <= g.NZZ;   // its low y / z face exists
     const double invXI = 1. / (p.dxi), invET = 1. / (p.deta), invZT = 1. / (p.dzeta);
-    visc_plane(g, p, U, J, iS - 1, j0, k0, sP);
-    visc_plane(g, p, U, J, iS, j0, k0, sP);
+    visc_plane(g, p, U, iS - 1, j0, k0, sP);
+    visc_plane(g, p, U, iS, j0, k0, sP);
     __syncthreads();
     double Fx[4] = {0.0, 0.0, 0.0, 0.0};   // x face below the current cell
     if (out) xface<Real>(g, p, sP, iS, j, k, hj, hk, Ju, mu, Fx);
     for (int i = iS; i < iE; i++) {
-        visc_plane(g, p, U, J, i + 1, j0, k0, sP);
+        visc_plane(g, p, U, i + 1, j0, k0, sP);
         __syncthreads();
         // the three faces one after the other: the scheduling barriers keep the
         // compiler from hoisting all their stencil reads at once
@@ -374,9 +381,9 @@ void launch_viscous(const Geo &g, const Phys &p, C5 U, const double *J, const do
     int seg = (int)std::min<long>(ni, std::max<long>(8, (long)ni * tj * tk / 2304));
     const dim3 gr(tk, tj, (ni + seg - 1) / seg), bl(VT_THREADS);
     if (p.fp32)
-        hipLaunchKernelGGL(k_viscous<float>, gr, bl, 0, s, g, p, U, J, Ju, mu, Jv, mv, Jw, mw, vF, seg);
+        hipLaunchKernelGGL(k_viscous<float>, gr, bl, 0, s, g, p, U, Ju, mu, Jv, mv, Jw, mw, vF, seg);
     else
-        hipLaunchKernelGGL(k_viscous<double>, gr, bl, 0, s, g, p, U, J, Ju, mu, Jv, mv, Jw, mw, vF, seg);
+        hipLaunchKernelGGL(k_viscous<double>, gr, bl, 0, s, g, p, U, Ju, mu, Jv, mv, Jw, mw, vF, seg);
 }
 
 }  // namespace cgli
