@@ -1788,14 +1788,19 @@ constexpr unsigned XR_ENTRY = 48, XR_STEP = 8 * XR_ENTRY;
 __device__ __forceinline__ unsigned long long xr_lo(xr_u32x4 q) { return q.x | ((unsigned long long)q.y << 32); }
 __device__ __forceinline__ unsigned long long xr_hi(xr_u32x4 q) { return q.z | ((unsigned long long)q.w << 32); }
 
+// the five values of an entry: two 16-byte loads and one 8-byte load (not
+// the pad: a loaded register left unused may be reused by the compiler while
+// the load is in flight, which costs a wait for every outstanding load)
 __device__ __forceinline__ void rec_issue(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned long long b[6])
 {
 #pragma unroll
-    for (int p = 0; p < 3; p++) {
+    for (int p = 0; p < 2; p++) {
         const xr_u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16u * p, 0, XR_CPOL);
         b[2 * p] = xr_lo(q);
         b[2 * p + 1] = xr_hi(q);
     }
+    b[4] = __builtin_bit_cast(unsigned long long, __builtin_amdgcn_raw_buffer_load_b64(r, vo + 32u, 0, XR_CPOL));
+    b[5] = 0;
 }
 
 // where `need`, re-poll the record entry until none of its five values holds
@@ -1996,10 +2001,8 @@ __device__ __forceinline__ void rec_settle2(const M4Args &w, unsigned vb, unsign
 // with step s + 1's coefficients.  nb / nc hold the exchange polls of step
 // s (issued two steps ago) and receive those of step s + 2.  The in-tile
 // products come from LDS (a lane with no in-tile upstream neighbour reads
-// the zero entry ZS); no scheduling barrier splits the step, so the
-// compiler interleaves the next coefficients with this step's dependent
-// chain (2.6 % fewer cycles per step than with the loads, stores and
-// coefficients kept apart).
+// the zero entry TJ * TK); the compiler interleaves the next coefficients
+// with this step's dependent chain.
 template <int TJ, int TK, bool FWD, bool GC, int PAR>
 __device__ __forceinline__ void step4(const M4Args &w, const F4Lane &L, int s, Cf4 &C, const Ld4 &Ln, Ld4 &Lf,
                                       double sa[5], double (*sB)[5][TJ * TK + 1], double (*sC)[5][TJ * TK + 1],
@@ -2057,6 +2060,9 @@ __device__ __forceinline__ void step4(const M4Args &w, const F4Lane &L, int s, C
         rec_issue(w.rc, rec_poll(L.pc, actn && L.need_c, tn, w.dtc), nc);
         load4<FWD, GC>(Lf, w, (unsigned)L.tid * 16u, s + 2);
     }
+    // loads before stores: the vector-memory counter completes in issue
+    // order, so a wait for these loads must not also wait for the stores
+    __builtin_amdgcn_sched_barrier(0);
     // every lane stores its slot (a slot with no cell is never read as a
     // value) and issues the six record stores, so no branch splits them
     double2 *xo = reinterpret_cast<double2 *>(w.xrow0 + (long)t * F4ROWB) + L.tid;
@@ -2241,6 +2247,10 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_fused(Geo g, March M, double K
     load4<FWD, GC>(La, w, (unsigned)L.tid * 16u, 0);
     load4<FWD, GC>(Lb, w, (unsigned)L.tid * 16u, 1);
     coef4<FWD>(C, La, w);
+    // nothing outstanding on entry to the march: the compiler's wait counts
+    // inside the loop then follow the loop's own loads and stores (merged
+    // with the prologue's, they were full waits at every step)
+    __builtin_amdgcn_s_waitcnt(0);
     for (int s = 0; s < S; s += 2) {
         step4<TJ, TK, FWD, GC, 0>(w, L, s, C, Lb, La, sa, sB, sC, nb0, nc0, late);
         if (s + 1 < S) step4<TJ, TK, FWD, GC, 1>(w, L, s + 1, C, La, Lb, sa, sB, sC, nb1, nc1, late);
@@ -2424,6 +2434,7 @@ __device__ __forceinline__ void step_chain(const M4Args &w, const F4Lane &L, int
         rec_issue(w.rb, rec_poll(L.pb, actn && L.need_b, tn, w.dtb), nb);
         rec_issue(w.rc, rec_poll(L.pc, actn && L.need_c, tn, w.dtc), nc);
     }
+    __builtin_amdgcn_sched_barrier(0);   // loads before stores
     double2 *xo = reinterpret_cast<double2 *>(w.xrow0 + (long)t * F4ROWB) + L.tid;
     xo[0] = make_double2(x[0], x[1]);
     xo[64] = make_double2(x[2], x[3]);
@@ -2488,6 +2499,7 @@ __global__ void __launch_bounds__(64 * (1 + H)) k_lusgs_helped(Geo g, March M, d
 #pragma unroll
         for (int v = 0; v < 9; v++) mc[v] = gm[v];
     }
+    __builtin_amdgcn_s_waitcnt(0);   // nothing outstanding on entry (as in k_lusgs_fused)
     for (int s = 0; s < S; s += 2) {
         step_chain<TJ, TK, FWD, GC, 0, R, NP>(w, L, s, mc, slot, s_ready, &s_consumed, sa, sB, sC, nb0, nc0, late);
         if (s + 1 < S)
