@@ -8,15 +8,19 @@
 // (Flux_X.cpp:106-216, Flux_Y.cpp:106-249, Flux_Z.cpp:105-216) and applies
 // the low-Mach preconditioned Roe flux (Flux_X.cpp:229-350).  The
 // reference's ML/MR scratch never goes to HBM.  The expressions are the
-// reference's, compiled without FMA contraction, except that a divisor used
-// more than once is inverted once and multiplied (the FP64 division is the
-// costliest instruction sequence of the kernel): results agree with the CPU
+// reference's, except that a divisor used more than once is inverted once
+// and multiplied (the FP64 division is the costliest instruction sequence
+// of the kernel) and products and sums fuse: results agree with the CPU
 // build to rounding (a few ulps), not bit for bit.
 // Optional dissipation scaling (the older flux files, Flux_X~~.cpp:552-556):
 // each face's Roe dissipation Fav is multiplied by epS, or by EpF[face] (the
 // filter sensor, face-indexed like the fluxes) when EpF is given.  The live
 // reference has epS = 1 and no EpF; multiplying by 1 is exact.
 #include "cgli_internal.hpp"
+
+// Products and sums in this file contract to fused multiply-adds; the
+// results agree with the separately rounded reference to rounding.
+#pragma clang fp contract(fast)
 
 namespace cgli {
 
