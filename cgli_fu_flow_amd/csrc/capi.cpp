@@ -603,15 +603,15 @@ int cgli_filter_energies(cgli_ctx *c, double *Eh, double *E2h)
     if (c->cfg.precision == 1) {
         const float *w = reinterpret_cast<const float *>(c->filt);
         std::vector<float> tmp(2 * N);
-        HIPCHK(hipMemcpyAsync(tmp.data(), w + 32 * N, sizeof(float) * 2 * N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(tmp.data(), w + 24 * N, sizeof(float) * 2 * N, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         for (long q = 0; q < N; q++) {
             Eh[q] = tmp[q];
             E2h[q] = tmp[N + q];
         }
     } else {
-        HIPCHK(hipMemcpyAsync(Eh, c->filt + 32 * N, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(E2h, c->filt + 33 * N, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(Eh, c->filt + 24 * N, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(E2h, c->filt + 25 * N, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
     return CGLI_OK;

@@ -2,10 +2,10 @@
 //
 // Two scales (h: 1/8 (1,6,1); 2h: 1/4 (1,2,1)), six levels each of the
 // separable filter G = Gz Gy Gx applied to U1..U4 (U5 is filtered by the
-// reference but never used).  One launch per pass:
-//   k_filt_x    Gx = x pass of the level input      (Filter.cpp:162, 841)
-//   k_filt_y    Gy = J/Y_point-weighted y pass of Gx (Filter.cpp:199, 875)
-//   k_filt_z    level output = z pass of Gy         (Filter.cpp:231, 902)
+// reference but never used).  Launches:
+//   k_filt_level  one level: Gx = x pass of the level input, Gy =
+//                 J/Y_point-weighted y pass of Gx, output = z pass of Gy,
+//                 fused (Filter.cpp:162/199/231, 841/875/902)
 //   k_filt_exch GG halo, periodic single slab       (Filter.cpp:433-466)
 //   k_filt_energy 6G-15G2+20G3-15G4+6G5-G6, then E = |u - u_f|^2
 //                 (Filter.cpp:769-800, 1444-1475)
@@ -14,11 +14,14 @@
 // Expressions keep the reference's operation order, so results are
 // bit-identical with the reference built by g++ on x86 (no FMA
 // contraction: the library is compiled with -ffp-contract=off).
-// Bytes per pass: 4 fields in, 4 out (the y pass adds J and Y_point).
+// Bytes per level: 4 fields in (plus a 1-cell halo in j and k), J and
+// Y_point, 4 fields out.
 // Every kernel is templated on the arithmetic type Real: FP64, or FP32 for
 // the mixed-precision path (work arrays, passes, energies and the sensor
 // ratio in float; U_, J and Y_point read as FP64 and rounded once; the
 // E_high/E_low outputs are FP64 fields either way).
+#include <algorithm>
+
 #include "cgli_internal.hpp"
 
 namespace cgli {
@@ -37,7 +40,7 @@ static void level_range(const Geo &g, int lvl, int &lo, int &hi)
     hi = g.last ? g.gend : g.gend + w;
 }
 
-// every work array starts as U_ on i=0..gend+3, j=1..NYY, k=1..NZZ
+// every level array starts as U_ on i=0..gend+3, j=1..NYY, k=1..NZZ
 // (Filter.cpp:117-134).  Only the box's ghost shell (the three x planes at
 // each end, j=1 and j=NYY, k=1 and k=NZZ) is read before the passes
 // overwrite it, so only the shell is copied: the interior would be
@@ -57,68 +60,161 @@ __global__ void k_filt_init(Geo g, R4 U, Real *__restrict__ work, long N)
     for (int v = 0; v < 4; v++) {
         const Real u = (Real)U.v[v][c];
 #pragma unroll
-        for (int a = 0; a < 8; a++) work[(a * 4 + v) * N + c] = u;
+        for (int a = 0; a < 6; a++) work[(a * 4 + v) * N + c] = u;
     }
 }
 
-// In: the level's input type (U_ itself, FP64, at level 0)
+// One level of one scale (Filter.cpp:162-257 and 841-902 for the two
+// scales), its three passes fused:
+//   Gx = x pass of the level input         1/8 (1,6,1) or 1/4 (1,2,1)
+//   Gy = J/Y_point-weighted y pass of Gx
+//   level output = z pass of Gy
+// on j = 2..NY, k = 2..NZ and the level's planes lo..hi.  A block owns an
+// 8 x 32 tile of (j, k) and marches along i through a segment of planes:
+// the level input of planes i-1, i, i+1 on the tile plus a 1-cell halo
+// lives in an LDS ring (each plane read from HBM once per block), Gx and Gy
+// of plane i stay in LDS, and the level output is the only store.  Where
+// the reference reads Gx on the ghost rows j = 1, NYY or Gy on the ghost
+// columns k = 1, NZZ, which no pass writes and which hold U_ from its
+// initial copy (Filter.cpp:117-134), this kernel reads U_.  Expressions and
+// operation order are the reference's.
+constexpr int FT_J = 8, FT_K = 32, FT_THREADS = FT_J * FT_K;
+constexpr int FH_K = FT_K + 2, FH_N = (FT_J + 2) * FH_K;
+
 template <bool TWOH, typename Real, typename In>
-__global__ void k_filt_x(Geo g, int lo, R4T<In> in, L4T<Real> out)
+__global__ void __launch_bounds__(FT_THREADS) k_filt_level(Geo g, int lo, int hi, int seg, R4T<In> prev, R4 U,
+                                                          const double *__restrict__ J, const double *__restrict__ Y,
+                                                          L4T<Real> out)
 {
-    const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = lo + blockIdx.z;
-    if (k > g.NZ || j > g.NY) return;
-    const long c = ix(g, i, j, k), sx = g.Ym * g.Zm;
+    constexpr int NR = (FH_N + FT_THREADS - 1) / FT_THREADS;   // halo'd positions per thread
+    __shared__ Real sL[3][4][FH_N];   // level input of planes i-1, i, i+1 (slot i % 3)
+    __shared__ Real sX[4][FH_N];      // Gx of plane i
+    __shared__ double sJ[FH_N], sY[FH_N];
+    __shared__ Real sG[4][FT_J][FH_K];   // Gy of plane i, columns k0-1 .. k0+FT_K
+    const int j0 = 2 + blockIdx.y * FT_J, k0 = 2 + blockIdx.x * FT_K;
+    const int iS = lo + blockIdx.z * seg, iE = min(iS + seg - 1, hi);
+    if (iS > hi) return;   // the whole block
+    // this thread's halo'd tile positions t = threadIdx.x + r FT_THREADS:
+    // (j0 - 1 + t / FH_K, k0 - 1 + t % FH_K), where inside the arrays' box
+    long cpos[NR];
+    bool inb[NR];
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const In *u = in.v[v];
-        const Real um = (Real)u[c - sx], u0 = (Real)u[c], up = (Real)u[c + sx];
-        if (!TWOH)
-            out.v[v][c] = Real(0.125) * (um + 6 * u0 + up);
-        else
-            out.v[v][c] = Real(0.25) * (um + 2 * u0 + up);
+    for (int r = 0; r < NR; r++) {
+        const int t = threadIdx.x + r * FT_THREADS;
+        const int j = j0 - 1 + t / FH_K, k = k0 - 1 + t % FH_K;
+        inb[r] = t < FH_N && j <= g.NYY && k <= g.NZZ;
+        cpos[r] = inb[r] ? ix(g, 0, j, k) : 0;
     }
-}
-
-template <bool TWOH, typename Real>
-__global__ void k_filt_y(Geo g, int lo, R4T<Real> gx, L4T<Real> gy, const double *__restrict__ J,
-                         const double *__restrict__ Y)
-{
-    const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = lo + blockIdx.z;
-    if (k > g.NZ || j > g.NY) return;
-    const long c = ix(g, i, j, k), jm = c - g.Zm, jp = c + g.Zm;
-    const Real a = (Real)(Y[jp] - Y[jm]);
-    const Real b = (Real)(Y[jp] - Y[c]);
-    const Real cc = (Real)(Y[c] - Y[jm]);
-    const Real Jm = (Real)J[jm], Jc = (Real)J[c], Jp = (Real)J[jp];
+    const long sx = g.Ym * g.Zm;
+    // the level input of a plane and J, Y_point of another, loaded into
+    // registers one iteration before they go to LDS
+    Real pL[NR][4];
+    double pJ[NR], pY[NR];
+    auto fetch_L = [&](int i) {
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const Real *x = gx.v[v];
-        if (!TWOH)
-            gy.v[v][c] = Real(0.125) * (2 * b / a * x[jm] * Jm + 6 * x[c] * Jc + 2 * cc / a * x[jp] * Jp) / Jc;
-        else
-            gy.v[v][c] = Real(0.5) * (b / a * x[jm] * Jm + x[c] * Jc + cc / a * x[jp] * Jp) / Jc;
+        for (int r = 0; r < NR; r++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) pL[r][v] = inb[r] ? (Real)prev.v[v][cpos[r] + i * sx] : Real(0);
+    };
+    auto fetch_JY = [&](int i) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            pJ[r] = inb[r] ? J[cpos[r] + i * sx] : 0.0;
+            pY[r] = inb[r] ? Y[cpos[r] + i * sx] : 0.0;
+        }
+    };
+    auto put_L = [&](int i) {
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+            if (inb[r])
+#pragma unroll
+                for (int v = 0; v < 4; v++) sL[i % 3][v][threadIdx.x + r * FT_THREADS] = pL[r][v];
+    };
+    auto put_JY = [&]() {
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+            if (inb[r]) {
+                sJ[threadIdx.x + r * FT_THREADS] = pJ[r];
+                sY[threadIdx.x + r * FT_THREADS] = pY[r];
+            }
+    };
+    for (int q = iS - 1; q <= iS + 1; q++) {
+        fetch_L(q);
+        put_L(q);
     }
-}
-
-template <bool TWOH, typename Real>
-__global__ void k_filt_z(Geo g, int lo, R4T<Real> gy, L4T<Real> out)
-{
-    const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = lo + blockIdx.z;
-    if (k > g.NZ || j > g.NY) return;
-    const long c = ix(g, i, j, k);
+    fetch_JY(iS);
+    put_JY();
+    if (iS + 2 <= iE + 1) fetch_L(iS + 2);
+    if (iS + 1 <= iE) fetch_JY(iS + 1);
+    for (int i = iS; i <= iE; i++) {
+        __syncthreads();
+        // Gx on rows j0-1 .. j0+FT_J, columns 2..NZ of the halo'd tile
+        for (int t = threadIdx.x; t < FH_N; t += FT_THREADS) {
+            const int j = j0 - 1 + t / FH_K, k = k0 - 1 + t % FH_K;
+            if (j > g.NYY || k < 2 || k > g.NZ) continue;
+            if (j == 1 || j == g.NYY) {
+                const long c = ix(g, i, j, k);
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const Real *y = gy.v[v];
-        if (!TWOH)
-            out.v[v][c] = Real(0.125) * (y[c - 1] + 6 * y[c] + y[c + 1]);
-        else
-            out.v[v][c] = Real(0.25) * (y[c - 1] + 2 * y[c] + y[c + 1]);
+                for (int v = 0; v < 4; v++) sX[v][t] = (Real)U.v[v][c];
+                continue;
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const Real um = sL[(i + 2) % 3][v][t], u0 = sL[i % 3][v][t], up = sL[(i + 1) % 3][v][t];
+                if (!TWOH)
+                    sX[v][t] = Real(0.125) * (um + 6 * u0 + up);
+                else
+                    sX[v][t] = Real(0.25) * (um + 2 * u0 + up);
+            }
+        }
+        __syncthreads();
+        // plane i-1 is read: its ring slot takes plane i+2
+        if (i + 2 <= iE + 1) put_L(i + 2);
+        // Gy on rows j0 .. j0+FT_J-1 (up to NY), columns k0-1 .. k0+FT_K
+        for (int t = threadIdx.x; t < FT_J * FH_K; t += FT_THREADS) {
+            const int tj = t / FH_K, hk = t % FH_K;
+            const int j = j0 + tj, k = k0 - 1 + hk;
+            if (j > g.NY || k > g.NZZ) continue;
+            if (k == 1 || k == g.NZZ) {
+                const long c = ix(g, i, j, k);
+#pragma unroll
+                for (int v = 0; v < 4; v++) sG[v][tj][hk] = (Real)U.v[v][c];
+                continue;
+            }
+            const int pc = (tj + 1) * FH_K + hk, pm = pc - FH_K, pp = pc + FH_K;
+            const Real a = (Real)(sY[pp] - sY[pm]);
+            const Real b = (Real)(sY[pp] - sY[pc]);
+            const Real cc = (Real)(sY[pc] - sY[pm]);
+            const Real Jm = (Real)sJ[pm], Jc = (Real)sJ[pc], Jp = (Real)sJ[pp];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const Real *x = sX[v];
+                if (!TWOH)
+                    sG[v][tj][hk] = Real(0.125) * (2 * b / a * x[pm] * Jm + 6 * x[pc] * Jc + 2 * cc / a * x[pp] * Jp) / Jc;
+                else
+                    sG[v][tj][hk] = Real(0.5) * (b / a * x[pm] * Jm + x[pc] * Jc + cc / a * x[pp] * Jp) / Jc;
+            }
+        }
+        __syncthreads();
+        // J and Y_point of plane i are read: they take those of plane i+1
+        if (i + 1 <= iE) put_JY();
+        if (i + 3 <= iE + 1) fetch_L(i + 3);
+        if (i + 2 <= iE) fetch_JY(i + 2);
+        {   // the level output: z pass of Gy
+            const int tj = threadIdx.x / FT_K, tk = threadIdx.x % FT_K;
+            const int j = j0 + tj, k = k0 + tk;
+            if (j <= g.NY && k <= g.NZ) {
+                const long c = ix(g, i, j, k);
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const Real *y = sG[v][tj];
+                    if (!TWOH)
+                        out.v[v][c] = Real(0.125) * (y[tk] + 6 * y[tk + 1] + y[tk + 2]);
+                    else
+                        out.v[v][c] = Real(0.25) * (y[tk] + 2 * y[tk + 1] + y[tk + 2]);
+                }
+            }
+        }
     }
 }
 
@@ -240,10 +336,10 @@ __global__ void k_filt_sensor(Geo g, const Real *__restrict__ Eh, const Real *__
     }
 }
 
-long filter_scratch_doubles(const Geo &g) { return 34 * g.N; }
+long filter_scratch_doubles(const Geo &g) { return 26 * g.N; }
 
-// work (34 N doubles; the FP32 path uses the first half as 34 N floats):
-// levels 0..5 x 4 variables, Gx (4), Gy (4), E_h, E_2h
+// work (26 N doubles; the FP32 path uses the first half as 26 N floats):
+// levels 0..5 x 4 variables, E_h, E_2h
 template <typename Real>
 static void filter_run(const Geo &g, double crit, double E_high, double E_low, C5 U_, const double *J,
                        const double *Y, Real *work, double *EpX, double *EpY, double *EpZ, hipStream_t s)
@@ -255,30 +351,33 @@ static void filter_run(const Geo &g, double crit, double E_high, double E_low, C
                           work + (l * 4 + 3) * N}};
     };
     auto cst = [](const L4T<Real> &a) { return R4T<Real>{{a.v[0], a.v[1], a.v[2], a.v[3]}}; };
-    const L4T<Real> Gx = lvl(6), Gy = lvl(7);
-    Real *Eh = work + 32 * N, *E2h = work + 33 * N;
+    Real *Eh = work + 24 * N, *E2h = work + 25 * N;
     dim3 b(64, 4);
     hipLaunchKernelGGL(k_filt_init<Real>, dim3((g.NZZ + 63) / 64, (g.NYY + 3) / 4, g.gend + 4), b, 0, s, g, U, work,
                        N);
+    const int tk = (g.NZ - 1 + FT_K - 1) / FT_K, tj = (g.NY - 1 + FT_J - 1) / FT_J;
     for (int sc = 0; sc < 2; sc++) {
         for (int l = 0; l < 6; l++) {
             int lo, hi;
             level_range(g, l, lo, hi);
-            dim3 gr((g.NZ - 1 + 63) / 64, (g.NY - 1 + 3) / 4, hi - lo + 1);
+            // segments of at least 8 planes, about 2048 blocks
+            const int np = hi - lo + 1;
+            const int seg = std::min(np, std::max(8, (int)((long)np * tk * tj / 2048)));
+            const dim3 gr(tk, tj, (np + seg - 1) / seg), bl(FT_THREADS);
             if (sc == 0) {
                 if (l == 0)
-                    hipLaunchKernelGGL((k_filt_x<false, Real, double>), gr, b, 0, s, g, lo, U, Gx);
+                    hipLaunchKernelGGL((k_filt_level<false, Real, double>), gr, bl, 0, s, g, lo, hi, seg, U, U, J, Y,
+                                       lvl(l));
                 else
-                    hipLaunchKernelGGL((k_filt_x<false, Real, Real>), gr, b, 0, s, g, lo, cst(lvl(l - 1)), Gx);
-                hipLaunchKernelGGL((k_filt_y<false, Real>), gr, b, 0, s, g, lo, cst(Gx), Gy, J, Y);
-                hipLaunchKernelGGL((k_filt_z<false, Real>), gr, b, 0, s, g, lo, cst(Gy), lvl(l));
+                    hipLaunchKernelGGL((k_filt_level<false, Real, Real>), gr, bl, 0, s, g, lo, hi, seg, cst(lvl(l - 1)), U,
+                                       J, Y, lvl(l));
             } else {
                 if (l == 0)
-                    hipLaunchKernelGGL((k_filt_x<true, Real, double>), gr, b, 0, s, g, lo, U, Gx);
+                    hipLaunchKernelGGL((k_filt_level<true, Real, double>), gr, bl, 0, s, g, lo, hi, seg, U, U, J, Y,
+                                       lvl(l));
                 else
-                    hipLaunchKernelGGL((k_filt_x<true, Real, Real>), gr, b, 0, s, g, lo, cst(lvl(l - 1)), Gx);
-                hipLaunchKernelGGL((k_filt_y<true, Real>), gr, b, 0, s, g, lo, cst(Gx), Gy, J, Y);
-                hipLaunchKernelGGL((k_filt_z<true, Real>), gr, b, 0, s, g, lo, cst(Gy), lvl(l));
+                    hipLaunchKernelGGL((k_filt_level<true, Real, Real>), gr, bl, 0, s, g, lo, hi, seg, cst(lvl(l - 1)), U,
+                                       J, Y, lvl(l));
             }
             if (l == 2 && g.xper)
                 hipLaunchKernelGGL(k_filt_exch<Real>, dim3((g.NZZ + 63) / 64, (g.NYY + 3) / 4, 3), b, 0, s, g, lvl(2));
