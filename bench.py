@@ -89,8 +89,10 @@ def parse():
     ap.add_argument("--no-variants", action="store_true", help="skip the initial-guess / precision comparison")
     ap.add_argument("--no-graph", action="store_true", help="skip the eager / graph-replay comparison")
     ap.add_argument("--lusgs-mode", type=int, default=None,
-                    help="LU-SGS schedule (0 plane order, 1 tiled wavefront, 2 with loader waves, "
-                         "3 march-ordered records; default: the library's, 3)")
+                    help="LU-SGS schedule (0 plane order, 1 tiled wavefront, 3 march-ordered records, "
+                         "4 march computing its coefficients; default: the library's, 4)")
+    ap.add_argument("--sweep-waves", type=int, default=0,
+                    help="schedule 4's waves per tile (0: the library's choice by tile count)")
     ap.add_argument("--variant-steps", type=int, default=20, help="timed steps of each variant")
     ap.add_argument("--gen", type=int, default=200, help="recorded steps the network trains on")
     ap.add_argument("--train", type=int, default=1000, help="Adam steps")
@@ -302,6 +304,8 @@ def main():
     ctx = nv.Context(cfg, device=0)
     if a.lusgs_mode is not None:
         ctx.set_lusgs_mode(a.lusgs_mode)
+    if a.sweep_waves:
+        ctx.set_sweep_waves(a.sweep_waves)
     dt = ctx.channel_init(ch)
     print("setup done: dt=%.4e" % dt, file=sys.stderr, flush=True)
     for s in range(a.warmup):

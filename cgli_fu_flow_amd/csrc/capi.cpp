@@ -32,6 +32,7 @@ struct cgli_graph {
     bool norm = false;           // ends with a deferred residual norm
     bool norm_wave = false;
     bool scatters_nr = false;    // holds a schedule-3 scatter: NL is then to be copied from NR where read
+    bool nr_march = false;       // holds a schedule-4 update: NL and NR are then in march order only
 };
 
 struct cgli_ctx {
@@ -52,6 +53,10 @@ struct cgli_ctx {
     bool nl_from_nr = false;     // NL's interior equals NR's and is not written yet (schedule 3)
     long nr_scatters = 0;        // schedule-3 scatters queued (a graph that holds one sets nl_from_nr when launched)
     long nr_scatters_at_capture = 0;
+    bool nr_march = false;       // NL's and NR's interiors are the last schedule-4 increments, held in march
+                                 // order only (written to the fields where read)
+    long nr_marches = 0;         // schedule-4 updates queued
+    long nr_marches_at_capture = 0;
     bool capturing = false;      // between cgli_graph_begin and cgli_graph_end
     long generation = 0;         // bumped whenever a captured graph would go stale
     bool norm_pending = false;   // a deferred residual norm awaits cgli_dual_time_stepping_wait
@@ -171,6 +176,16 @@ static void mark(cgli_ctx *c, int id, int n)
 // to this: its own sweeps never read the field)
 static void nl_sync(cgli_ctx *c)
 {
+    if (c->nr_march) {
+        // schedule 4: the increments go from march order to NL and NR
+        c->nr_march = false;
+        c->nl_from_nr = false;
+        launch_scatter4(c->g, static_cast<char *>(c->x4) + 2 * 16 * march4_field_pairs(c->g),
+                        F5{{c->f[CGLI_NL], c->f[CGLI_NL + 1], c->f[CGLI_NL + 2], c->f[CGLI_NL + 3], c->f[CGLI_NL + 4]}},
+                        F5{{c->f[CGLI_NR], c->f[CGLI_NR + 1], c->f[CGLI_NR + 2], c->f[CGLI_NR + 3], c->f[CGLI_NR + 4]}},
+                        c->stream);
+        return;
+    }
     if (!c->nl_from_nr) return;
     c->nl_from_nr = false;
     launch_copy_interior(c->g, C5{{c->f[CGLI_NR], c->f[CGLI_NR + 1], c->f[CGLI_NR + 2], c->f[CGLI_NR + 3], c->f[CGLI_NR + 4]}},
@@ -415,7 +430,7 @@ int cgli_get_field(cgli_ctx *c, int id, double *host)
     HIPCHK(hipSetDevice(c->device));
     int r = ensure(c, id);
     if (r) return r;
-    if (id >= CGLI_NL && id < CGLI_NL + 5) nl_sync(c);
+    if (id >= CGLI_NL && id < CGLI_NR + 5) nl_sync(c);   // NL, or both, may be held elsewhere
     HIPCHK(hipMemcpyAsync(host, c->f[id], sizeof(double) * c->g.N, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return CGLI_OK;
@@ -888,6 +903,7 @@ int cgli_graph_begin(cgli_ctx *c)
     c->capturing = true;
     c->norm_pending = false;
     c->nr_scatters_at_capture = c->nr_scatters;
+    c->nr_marches_at_capture = c->nr_marches;
     return CGLI_OK;
 }
 
@@ -917,6 +933,7 @@ int cgli_graph_end(cgli_ctx *c, cgli_graph **out)
     G->norm = norm;
     G->norm_wave = wave;
     G->scatters_nr = c->nr_scatters != c->nr_scatters_at_capture;
+    G->nr_march = c->nr_marches != c->nr_marches_at_capture;
     *out = G;
     return CGLI_OK;
 }
@@ -929,6 +946,10 @@ int cgli_graph_launch(cgli_graph *G)
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipGraphLaunch(G->exec, c->stream));
     if (G->scatters_nr) c->nl_from_nr = true;
+    if (G->nr_march) {
+        c->nr_march = true;
+        c->nl_from_nr = false;
+    }
     if (G->norm) {
         HIPCHK(hipEventRecord(c->norm_ev, c->stream));
         c->norm_pending = true;
@@ -975,7 +996,7 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
     if (r) return r;
     const Geo &g = c->g;
     const long ncell = (long)g.nx * g.ny * g.nz;
-    const long nb = (ncell + 255) / 256;
+    const long nb = std::max((ncell + 255) / 256, c->lusgs_mode == 4 ? update4_blocks(g) : 0L);
     r = ensure_scratch(c, 5 * nb + (long)g.NYY * g.NZZ + 64);
     if (r) return r;
     const hipStream_t s = c->stream;
@@ -983,6 +1004,9 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
         r = wave_setup(c);
         if (r) return r;
     }
+    // other schedules write NL and NR in place: bring the last schedule-4
+    // increments to them first (this call may yet fail before writing)
+    if (c->lusgs_mode != 4) nl_sync(c);
     // also zeroes the wavefront's ticket and error flag
     launch_zero_increment_ghosts(g, f5(c, CGLI_NL), f5(c, CGLI_NR), c->lusgs_mode >= 1 ? c->counters : nullptr, s);
     if (c->cfg.sponge || write_residual)
@@ -1054,13 +1078,8 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
                                c->f[CGLI_J], m9(c, CGLI_MET), c5(c, CGLI_NR), c->rec4, c->rec4_bytes,
                                c->morder_dev, c->mtiles, c->counters, c->max_spin, -1, sweep_waves(c), s);
             tend(c, T_BWD);
-            if (sw == sweeps - 1) {
-                tbegin(c, T_SCATTER);
-                launch_scatter4(g, x4 + 2 * f4b, F5{{nullptr, nullptr, nullptr, nullptr, nullptr}}, f5(c, CGLI_NR), s);
-                c->nl_from_nr = true;
-                c->nr_scatters++;
-                tend(c, T_SCATTER);
-            }
+            // the last sweep's increments stay in march order: the update
+            // reads them there (k_update4)
         } else if (c->lusgs_mode == 3) {
             launch_march_prep(g, 1, c->xbuf, c->counters, s);
             tbegin(c, T_FWD);
@@ -1121,8 +1140,18 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
         }
     }
     tbegin(c, T_UPDATE);
-    const int nblk = launch_update_norm(g, c->p, f5(c, CGLI_U_), c->f[CGLI_J], c5(c, CGLI_NR), f5(c, CGLI_RES),
-                                        write_residual, 0, c->scratch, s);
+    int nblk;
+    if (c->lusgs_mode == 4) {
+        // U_ and NR in march order; NL and NR are written where read
+        nblk = launch_update4(g, c->p, f5(c, CGLI_U_), c->f[CGLI_J], c->u4, x4 + 2 * f4b, f5(c, CGLI_RES),
+                              write_residual, c->scratch, s);
+        c->nr_march = true;
+        c->nl_from_nr = false;
+        c->nr_marches++;
+    } else {
+        nblk = launch_update_norm(g, c->p, f5(c, CGLI_U_), c->f[CGLI_J], c5(c, CGLI_NR), f5(c, CGLI_RES),
+                                  write_residual, 0, c->scratch, s);
+    }
     tend(c, T_UPDATE);
     HIPCHK(hipGetLastError());
     mark(c, CGLI_MR, 15);
@@ -1163,6 +1192,7 @@ int cgli_dplusgs(cgli_ctx *c, double deltaT, double er[5], double e, int sweeps)
                            c5(c, CGLI_NL), f5(c, CGLI_NR), s);
         launch_copy_interior(g, c5(c, CGLI_NR), f5(c, CGLI_NL), s);
         c->nl_from_nr = false;
+        c->nr_march = false;   // NL and NR now hold these increments
     }
     for (int v = 0; v < 5; v++) launch_zero(c->f[CGLI_RES + v], g.N, s);
     const int nblk = launch_update_norm(g, c->p, f5(c, CGLI_U_), c->f[CGLI_J], c5(c, CGLI_NR), f5(c, CGLI_RES),
@@ -1203,6 +1233,7 @@ int cgli_dplusgs_noalloc(cgli_ctx *c, double deltaT, double er[5], double e, int
                            c5(c, CGLI_NL), f5(c, CGLI_NR), s);
         launch_copy_interior(g, c5(c, CGLI_NR), f5(c, CGLI_NL), s);
         c->nl_from_nr = false;
+        c->nr_march = false;   // NL and NR now hold these increments
     }
     for (int v = 0; v < 5; v++) launch_zero(c->f[CGLI_RES + v], g.N, s);
     // every partial the final reduction reads is written by this call

@@ -156,6 +156,11 @@ long march4_field_pairs(const Geo &g);
 void launch_gather4(const Geo &g, C5 U_, C5 R, const double *J, M9 met, void *Um, void *Rm, void *Gm, hipStream_t s);
 // march-ordered increments Xm into NR (and NL unless its pointers are null)
 void launch_scatter4(const Geo &g, const void *Xm, F5 NL, F5 NR, hipStream_t s);
+// schedule 4's update and norm from march-ordered U_ (Um) and NR (Xm);
+// returns the number of partial-sum blocks (update4_blocks)
+long update4_blocks(const Geo &g);
+int launch_update4(const Geo &g, const Phys &p, F5 U_, const double *J, const void *Um, const void *Xm, F5 Res,
+                   int write_res, double *partials, hipStream_t s);
 // bytes of one of the two tile-edge exchange record buffers of a sweep (the
 // j products, then the k products); the sweep needs 2 * march4_rec_bytes
 // below 2 GiB

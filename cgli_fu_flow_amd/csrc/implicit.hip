@@ -1719,6 +1719,73 @@ __global__ void __launch_bounds__(256) k_gather4(Geo g, March M, C5 U_, C5 R, co
     }
 }
 
+// Schedule 4's update and norm (k_update_norm's arithmetic) in the march
+// mapping of k_gather4 / k_scatter4: the increments NR and U_ are read in
+// march order, where the sweep left them and the gather put them this
+// iteration, so NR goes to no field (its field is written from march order
+// where read); U_ is written to its field.  Each block writes 5 partial
+// sums of its positions in a fixed order.
+__global__ void __launch_bounds__(256) k_update4(Geo g, March M, Phys p, F5 U_, const double *__restrict__ J,
+                                                 const double2 *__restrict__ Um, const double2 *__restrict__ Xm,
+                                                 F5 Res, int write_res, double *partials)
+{
+    const MGather q = mgather_at(g, M);
+    double x[5] = {0, 0, 0, 0, 0};
+    if (q.ok) {
+        const long c = ix(g, 3 + q.il, 2 + q.jo, 2 + q.ko);
+        const long o = march_row(M, q.T, q.t) * F4P * 64 + q.l;
+        const double2 u01 = Um[o], u23 = Um[o + 64], u4 = Um[o + 128];
+        const double2 n01 = Xm[o], n23 = Xm[o + 64], n4 = Xm[o + 128];
+        const double K = p.K;
+        const double Jc = J[c];
+        const double U1 = u01.x;
+        const double rhoold = U1 * Jc;
+        const double Uold = u01.y / U1;
+        const double Vold = u23.x / U1;
+        const double Wold = u23.y / U1;
+        const double VVold = Uold * Uold + Vold * Vold + Wold * Wold;
+        const double Pold = (u4.x * Jc - 0.5 * rhoold * VVold) * (K - 1);
+        const double Told = Pold / rhoold;
+        const double P = Pold + n01.x * Jc;
+        const double Uu = Uold + n01.y * Jc;
+        const double V = Vold + n23.x * Jc;
+        const double W = Wold + n23.y * Jc;
+        const double T = Told + n4.x * Jc;
+        const double rho = P / T;
+        U_.v[0][c] = rho / Jc;
+        U_.v[1][c] = rho * Uu / Jc;
+        U_.v[2][c] = rho * V / Jc;
+        U_.v[3][c] = rho * W / Jc;
+        U_.v[4][c] = (P / (K - 1) + 0.5 * rho * (Uu * Uu + V * V + W * W)) / Jc;
+        x[0] = (rho - rhoold) * (rho - rhoold);
+        x[1] = (Uu - Uold) * (Uu - Uold);
+        x[2] = (V - Vold) * (V - Vold);
+        x[3] = (W - Wold) * (W - Wold);
+        x[4] = (T - Told) * (T - Told);
+        if (write_res) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) Res.v[v][c] = x[v];
+        }
+    }
+    double o5[5];
+    block_sum<5>(x, o5);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int v = 0; v < 5; v++) partials[5 * (long)blockIdx.x + v] = o5[v];
+}
+
+long update4_blocks(const Geo &g) { return mgather_grid(march_of(g)).x; }
+
+int launch_update4(const Geo &g, const Phys &p, F5 U_, const double *J, const void *Um, const void *Xm, F5 Res,
+                   int write_res, double *partials, hipStream_t s)
+{
+    const March M = march_of(g);
+    const dim3 gr = mgather_grid(M);
+    hipLaunchKernelGGL(k_update4, gr, dim3(256), 0, s, g, M, p, U_, J, static_cast<const double2 *>(Um),
+                       static_cast<const double2 *>(Xm), Res, write_res, partials);
+    return (int)gr.x;
+}
+
 // march-ordered increments back into NR (and NL when NL.v[0] is not null)
 __global__ void __launch_bounds__(256) k_scatter4(Geo g, March M, const double2 *__restrict__ Xm, F5 NL, F5 NR)
 {
