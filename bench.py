@@ -47,7 +47,8 @@ METRIC = "physical steps/s and Mcell-updates/s at 1e-6 inner-residual drop; plai
 HBM_PEAK_GBS = 8000.0
 # algorithmic FP64 bytes per interior cell per launch of each kernel class
 # (each input element read once, each output written once)
-ALGO_BYTES = {"flux_x": 120, "flux_y": 120, "flux_z": 120, "viscous": 320, "rhs_diag": 472,
+# rhs_diag under the default schedule: U_, U, Uq, J, the face fluxes and vF in, MR out
+ALGO_BYTES = {"flux_x": 120, "flux_y": 120, "flux_z": 120, "viscous": 320, "rhs_diag": 320,
               "lusgs_fwd": 240, "lusgs_bwd": 240, "update_norm": 128, "split_coef": 560, "filter": 72, "march_gather": 160,
               "march_scatter": 80}
 

@@ -303,10 +303,13 @@ void launch_sponge(const Geo &g, const Phys &p, double e, int gstart, int gend0,
 }
 
 // --------------------------------------------------------------- RHS+diag
+// INC: also the starting increments NL = NR = D MR (the schedules whose
+// sweeps read them from the fields; schedule 4 keeps its own copies, and
+// then neither the diagonal nor the metrics are touched here)
+template <bool INC>
 __global__ void __launch_bounds__(256) k_rhs_diag(Geo g, Phys p, double deltaT, double e, int with_res, C5 U_,
                                                   C5 U, C5 Uq, const double *__restrict__ J, M9 met, C5 Fx,
-                                                  C5 Fy, C5 Fz, C4 vF, C5 Res, F5 MR, F5 NL, F5 NR,
-                                                  int write_inc)
+                                                  C5 Fy, C5 Fz, C4 vF, C5 Res, F5 MR, F5 NL, F5 NR)
 {
     const int k = 2 + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -354,23 +357,24 @@ __global__ void __launch_bounds__(256) k_rhs_diag(Geo g, Phys p, double deltaT, 
 #pragma unroll
     for (int a = 0; a < 5; a++)
         mr[a] = (d[a][0] * Rk[0] + d[a][1] * Rk[1] + d[a][2] * Rk[2] + d[a][3] * Rk[3] + d[a][4] * Rk[4]);
-    double m[9];
-    load9(met, c, m);
-    const DG D = diag(q, m, p, deltaT);
-    double nl[5];
-    nl[0] = D.d11 * mr[0];
-    nl[1] = D.d22 * mr[1];
-    nl[2] = D.d22 * mr[2];
-    nl[3] = D.d22 * mr[3];
-    nl[4] = D.d51 * mr[0] + D.d22 * mr[4];
 #pragma unroll
     for (int a = 0; a < 5; a++) MR.v[a][c] = mr[a];
-    if (write_inc)   // the starting increments D MR (not needed by a sweep that keeps its own copies)
+    if constexpr (INC) {
+        double m[9];
+        load9(met, c, m);
+        const DG D = diag(q, m, p, deltaT);
+        double nl[5];
+        nl[0] = D.d11 * mr[0];
+        nl[1] = D.d22 * mr[1];
+        nl[2] = D.d22 * mr[2];
+        nl[3] = D.d22 * mr[3];
+        nl[4] = D.d51 * mr[0] + D.d22 * mr[4];
 #pragma unroll
         for (int a = 0; a < 5; a++) {
             NL.v[a][c] = nl[a];
             NR.v[a][c] = nl[a];
         }
+    }
 }
 
 void launch_rhs_diag(const Geo &g, const Phys &p, double deltaT, double e, int with_res, C5 U_, C5 U, C5 Uq,
@@ -379,8 +383,12 @@ void launch_rhs_diag(const Geo &g, const Phys &p, double deltaT, double e, int w
 {
     dim3 b(64, 4);
     dim3 gr((g.NZ - 1 + b.x - 1) / b.x, (g.NY - 1 + b.y - 1) / b.y, g.nx);
-    hipLaunchKernelGGL(k_rhs_diag, gr, b, 0, s, g, p, deltaT, e, with_res, U_, U, Uq, J, met, Fx, Fy, Fz, vF,
-                       Res, MR, NL, NR, write_inc);
+    if (write_inc)
+        hipLaunchKernelGGL(k_rhs_diag<true>, gr, b, 0, s, g, p, deltaT, e, with_res, U_, U, Uq, J, met, Fx, Fy, Fz, vF,
+                           Res, MR, NL, NR);
+    else
+        hipLaunchKernelGGL(k_rhs_diag<false>, gr, b, 0, s, g, p, deltaT, e, with_res, U_, U, Uq, J, met, Fx, Fy, Fz,
+                           vF, Res, MR, NL, NR);
 }
 
 // ------------------------------------------------------------- LU-SGS
