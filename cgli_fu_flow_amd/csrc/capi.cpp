@@ -38,6 +38,7 @@ struct cgli_graph {
 struct cgli_ctx {
     cgli_config cfg;
     int device = 0;
+    int cus = 256;               // compute units of the device
     hipStream_t stream = nullptr;
     Geo g;
     Phys p;
@@ -320,6 +321,11 @@ int cgli_create(const cgli_config *cfg, int device, cgli_ctx **out)
     if (!c) return CGLI_E_NOMEM;
     c->cfg = *cfg;
     c->device = device;
+    {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && n > 0) c->cus = n;
+        else (void)hipGetLastError();
+    }
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return CGLI_E_HIP;
@@ -755,13 +761,15 @@ static int finish_norm(cgli_ctx *c, int nb, double er[5], bool check_wave = fals
     return r ? r : collect_norm(c, er, check_wave);
 }
 
-// schedule 4's waves per tile: by default a chain wave with one helper when
-// there is at most one tile per CU (the march step's latency bounds the
-// sweep: 128^3 forward sweep 0.68 ms against 0.77 with one wave), one wave
-// otherwise (the memory system bounds it: 256x192x256 1.76 ms against 1.93)
+// schedule 4's waves per tile: by default a chain wave with two helpers
+// when there are at most two tiles per CU (the chain wave's instruction
+// stream bounds the sweep, and the helpers find idle SIMDs: 128^3 forward
+// sweep 0.34 ms against 0.47 with one wave, 192x128x192 0.63 against 0.66),
+// one wave otherwise (the helpers then share SIMDs with chain waves:
+// 256x192x256 1.16 ms against 1.06)
 static int sweep_waves(const cgli_ctx *c)
 {
-    return c->sweep_waves ? c->sweep_waves : (c->mtiles <= 256 ? 2 : 1);
+    return c->sweep_waves ? c->sweep_waves : (c->mtiles <= 2 * c->cus ? 3 : 1);
 }
 
 // the wavefront's records (in the layout of the current schedule) and its

@@ -200,9 +200,9 @@ int cgli_set_lusgs_mode(cgli_ctx *ctx, int mode);
 /* Schedule 4's workgroup per tile: 1 = one wave marching and computing each
  * next cell's coefficients, 2-4 = one chain wave marching and 1-3 helper
  * waves computing the coefficients of the steps ahead into LDS slots (the
- * same increments bit for bit), 0 = by tile count (the default: 2 for at
- * most 256 tiles of 8x8 columns, else 1).  No counterpart in the reference
- * (its sweep is one loop per hyperplane). */
+ * same increments bit for bit), 0 = by tile count (the default: 3 for at
+ * most two tiles of 8x8 columns per compute unit, else 1).  No counterpart
+ * in the reference (its sweep is one loop per hyperplane). */
 int cgli_set_sweep_waves(cgli_ctx *ctx, int waves);
 int cgli_set_spin_limit(cgli_ctx *ctx, int max_spin);
 /* DPLUSGS.cpp:18-1238: Jacobi (DP-LUR) variant with `sweeps` sweeps (the
