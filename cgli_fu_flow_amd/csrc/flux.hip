@@ -7,7 +7,10 @@
 // right states of its face from the six cells along the sweep direction
 // (Flux_X.cpp:106-216, Flux_Y.cpp:106-249, Flux_Z.cpp:105-216) and applies
 // the low-Mach preconditioned Roe flux (Flux_X.cpp:229-350).  The
-// reference's ML/MR scratch never goes to HBM.  The expressions are the
+// reference's ML/MR scratch never goes to HBM.  The thread issues all of its
+// face's loads (the six cells' J and conserved variables, the face metrics
+// and Jf) before it waits for the first: a face's work is one round trip to
+// memory, not one per stencil and variable.  The expressions are the
 // reference's, except that a divisor used more than once is inverted once
 // and multiplied (the FP64 division is the costliest instruction sequence
 // of the kernel) and products and sums fuse: results agree with the CPU
@@ -24,81 +27,76 @@
 
 namespace cgli {
 
-// J-weighted reconstruction weights.  The thread forms the reciprocals
-// q[m] = 1/J of the six cells c0 + (m - 2) s along the sweep direction
-// (those its two stencils use) once, and each weight from them; `temp` of
-// the reference (Flux_X.cpp:118-121) is 1 / (sum of coefficient / J), the
-// same sum with the divisions taken once per cell.
-template <typename T>
-struct Rec {
-    int kind;     // 5: 5-point left, -5: 5-point right, 3: 3rd left, -3: 3rd right, 2: average
-    long a, b, c, d, f;
-    T w;
-};
-
+// J-weighted MUSCL reconstruction of one face's left and right states from
+// the six cells c0 + (m - 2) s, m = 0..5, along the sweep direction.  Each
+// stencil shape of the reference (5-point, 3rd order, two-cell average;
+// Flux_X.cpp:106-216) is its coefficient vector over the six positions:
+// the state is sum(c[m] U[m] / ...) weighted by temp = 1 / sum(c[m] / J[m])
+// (Flux_X.cpp:118-121).  A position neither stencil uses reads cell c0
+// instead (its coefficient is zero; the substitute keeps the load in bounds
+// and finite), so every load of the face is issued before the first use.
 template <typename T>
 __device__ __forceinline__ T ld(const double *p, long i)
 {
     return (T)p[i];
 }
 
-// reciprocals of J at c0 + (m - 2) s for m = 0..5; q[0] only where the left
-// stencil is the 5-point one, q[5] only where the right one is (the other
-// entries lie in the array for every face the kernels compute)
 template <typename T>
-__device__ __forceinline__ void recip_j(const double *__restrict__ J, long c0, long s, bool m0, bool m5, T q[6])
+__device__ __forceinline__ void shape(T c[6], T c0, T c1, T c2, T c3, T c4, T c5)
 {
-    q[0] = m0 ? T(1) / ld<T>(J, c0 - 2 * s) : T(0);
+    c[0] = c0; c[1] = c1; c[2] = c2; c[3] = c3; c[4] = c4; c[5] = c5;
+}
+// the shapes by cell position
+template <typename T> __device__ __forceinline__ void sh5L(T c[6]) { shape<T>(c, 2, -13, 47, 27, -3, 0); }
+template <typename T> __device__ __forceinline__ void sh5R(T c[6]) { shape<T>(c, 0, -3, 27, 47, -13, 2); }
+// 3rd-order left state from cells 1, 2, 3 and right state from 2, 3, 4
+template <typename T> __device__ __forceinline__ void sh3L(T c[6]) { shape<T>(c, 0, -1, 5, 2, 0, 0); }
+template <typename T> __device__ __forceinline__ void sh3R(T c[6]) { shape<T>(c, 0, 0, 2, 5, -1, 0); }
+// two-cell averages
+template <typename T> __device__ __forceinline__ void sh12(T c[6]) { shape<T>(c, 0, 1, 1, 0, 0, 0); }
+template <typename T> __device__ __forceinline__ void sh23(T c[6]) { shape<T>(c, 0, 0, 1, 1, 0, 0); }
+template <typename T> __device__ __forceinline__ void sh34(T c[6]) { shape<T>(c, 0, 0, 0, 1, 1, 0); }
+
+template <typename T>
+__device__ __forceinline__ T dot6(const T c[6], const T x[6])
+{
+    T s = c[0] * x[0];
 #pragma unroll
-    for (int m = 1; m < 5; m++) q[m] = T(1) / ld<T>(J, c0 + (m - 2) * s);
-    q[5] = m5 ? T(1) / ld<T>(J, c0 + 3 * s) : T(0);
+    for (int m = 1; m < 6; m++) s += c[m] * x[m];
+    return s;
 }
 
-// the stencils of face c0 + s/2 by stencil-cell position m (cell c0 + (m - 2) s)
+// the six cells' J and conserved variables of one face, loaded together
 template <typename T>
-__device__ __forceinline__ Rec<T> rec5L(const T q[6], long c0, long s)
-{
-    return Rec<T>{5, c0 - 2 * s, c0 - s, c0, c0 + s, c0 + 2 * s,
-                  T(1) / (T(2) * q[0] - T(13) * q[1] + T(47) * q[2] + T(27) * q[3] - T(3) * q[4])};
-}
+struct Sten {
+    T q[6], u[5][6];
+};
+
 template <typename T>
-__device__ __forceinline__ Rec<T> rec5R(const T q[6], long c0, long s)
+__device__ __forceinline__ void load_sten(C5 U, const double *__restrict__ J, long c0, long s, const T cl[6],
+                                          const T cr[6], Sten<T> &S)
 {
-    return Rec<T>{-5, c0 - s, c0, c0 + s, c0 + 2 * s, c0 + 3 * s,
-                  T(1) / (-T(3) * q[1] + T(27) * q[2] + T(47) * q[3] - T(13) * q[4] + T(2) * q[5])};
-}
-// 3rd-order left state from cells (m, c, p) at positions (mi, ci, pi)
-template <typename T>
-__device__ __forceinline__ Rec<T> rec3L(const T q[6], long c0, long s, int mi, int ci, int pi)
-{
-    return Rec<T>{3, c0 + (mi - 2) * s, c0 + (ci - 2) * s, c0 + (pi - 2) * s, 0, 0,
-                  T(1) / (-q[mi] + T(5) * q[ci] + T(2) * q[pi])};
-}
-template <typename T>
-__device__ __forceinline__ Rec<T> rec3R(const T q[6], long c0, long s, int mi, int ci, int pi)
-{
-    return Rec<T>{-3, c0 + (mi - 2) * s, c0 + (ci - 2) * s, c0 + (pi - 2) * s, 0, 0,
-                  T(1) / (T(2) * q[mi] + T(5) * q[ci] - q[pi])};
-}
-template <typename T>
-__device__ __forceinline__ Rec<T> rec2(const T q[6], long c0, long s, int ai, int bi)
-{
-    return Rec<T>{2, c0 + (ai - 2) * s, c0 + (bi - 2) * s, 0, 0, 0, T(1) / (q[ai] + q[bi])};
+    long at[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) at[m] = (cl[m] != T(0) || cr[m] != T(0)) ? c0 + (m - 2) * s : c0;
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+#pragma unroll
+        for (int m = 0; m < 6; m++) S.u[v][m] = ld<T>(U.v[v], at[m]);
+#pragma unroll
+    for (int m = 0; m < 6; m++) S.q[m] = ld<T>(J, at[m]);
 }
 
 template <typename T>
-__device__ __forceinline__ T apply(const Rec<T> &r, const double *__restrict__ U)
+__device__ __forceinline__ void face_states(Sten<T> &S, const T cl[6], const T cr[6], T Lq[5], T Rq[5])
 {
-    switch (r.kind) {
-    case 5:
-        return r.w * (T(2) * ld<T>(U, r.a) - T(13) * ld<T>(U, r.b) + T(47) * ld<T>(U, r.c) + T(27) * ld<T>(U, r.d) -
-                      T(3) * ld<T>(U, r.f));
-    case -5:
-        return r.w * (-T(3) * ld<T>(U, r.a) + T(27) * ld<T>(U, r.b) + T(47) * ld<T>(U, r.c) - T(13) * ld<T>(U, r.d) +
-                      T(2) * ld<T>(U, r.f));
-    case 3: return r.w * (-ld<T>(U, r.a) + T(5) * ld<T>(U, r.b) + T(2) * ld<T>(U, r.c));
-    case -3: return r.w * (T(2) * ld<T>(U, r.a) + T(5) * ld<T>(U, r.b) - ld<T>(U, r.c));
-    default: return r.w * (ld<T>(U, r.a) + ld<T>(U, r.b));
+#pragma unroll
+    for (int m = 0; m < 6; m++) S.q[m] = T(1) / S.q[m];
+    const T wl = T(1) / dot6(cl, S.q), wr = T(1) / dot6(cr, S.q);
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        Lq[v] = wl * dot6(cl, S.u[v]);
+        Rq[v] = wr * dot6(cr, S.u[v]);
     }
 }
 
@@ -202,33 +200,29 @@ __global__ void __launch_bounds__(256) k_flux_x(Geo g, double K, double e, C5 U,
     const long c0 = ix(g, i, cj, ck);
     // left state of face i+1/2; later reference writes win (last-rank patch
     // after first-rank patch after the 5-point loop)
-    const bool L5 = !(g.last && i == g.gend) && !(g.first && (i == 3 || i == 2));
-    const bool R5 = !(g.last && (i == g.gend || i == g.gend - 1)) && !(g.first && i == 2);
-    T q[6];
-    recip_j<T>(J, c0, sx, L5, R5, q);
-    Rec<T> L;
-    if (g.last && i == g.gend) L = rec3L<T>(q, c0, sx, 1, 2, 3);
-    else if (g.first && i == 3) L = rec3L<T>(q, c0, sx, 1, 2, 3);
-    else if (g.first && i == 2) L = rec2<T>(q, c0, sx, 2, 3);
-    else L = rec5L<T>(q, c0, sx);
+    T cl[6], cr[6];
+    if (g.last && i == g.gend) sh3L<T>(cl);
+    else if (g.first && i == 3) sh3L<T>(cl);
+    else if (g.first && i == 2) sh23<T>(cl);
+    else sh5L<T>(cl);
     // right state of face i+1/2 (from cell i+1)
-    Rec<T> R;
-    if (g.last && i == g.gend) R = rec2<T>(q, c0, sx, 2, 3);
-    else if (g.last && i == g.gend - 1) R = rec3R<T>(q, c0, sx, 2, 3, 4);
-    else if (g.first && i == 2) R = rec3R<T>(q, c0, sx, 2, 3, 4);
-    else R = rec5R<T>(q, c0, sx);
-    T Lq[5], Rq[5];
-#pragma unroll
-    for (int v = 0; v < 5; v++) {
-        Lq[v] = apply(L, U.v[v]);
-        Rq[v] = apply(R, U.v[v]);
-    }
+    if (g.last && i == g.gend) sh23<T>(cr);
+    else if (g.last && i == g.gend - 1) sh3R<T>(cr);
+    else if (g.first && i == 2) sh3R<T>(cr);
+    else sh5R<T>(cr);
+    Sten<T> S;
+    load_sten<T>(U, J, c0, sx, cl, cr, S);
     const long f = ix(g, i, j, k);
     const T xix = ld<T>(mf.m[0], f), xiy = ld<T>(mf.m[1], f), xiz = ld<T>(mf.m[2], f);
+    const T jf = ld<T>(Jf, f);
+    const T ep = SCALE ? (EpF ? ld<T>(EpF, f) : T(epS)) : T(1);
+    // every load of the face is in flight before the first waits
+    __builtin_amdgcn_sched_barrier(0);
+    T Lq[5], Rq[5];
+    face_states<T>(S, cl, cr, Lq, Rq);
     const T insqr = T(1.0) / sqrt(xix * xix + xiy * xiy + xiz * xiz);
     T out[5], pl, pr, fav3;
-    const T ep = SCALE ? (EpF ? ld<T>(EpF, f) : T(epS)) : T(1);
-    roe_face<T>(Lq, Rq, xix, xiy, xiz, ld<T>(Jf, f), T(e), T(K), xix * insqr, xiy * insqr, xiz * insqr, ep, out, pl,
+    roe_face<T>(Lq, Rq, xix, xiy, xiz, jf, T(e), T(K), xix * insqr, xiy * insqr, xiz * insqr, ep, out, pl,
                 pr, fav3);
 #pragma unroll
     for (int v = 0; v < 5; v++) F.v[v][f] = (double)out[v];
@@ -249,40 +243,34 @@ __global__ void __launch_bounds__(256) k_flux_y(Geo g, double K, double Ep, doub
     if (k > g.NZ - 1 || j > g.NY) return;
     const long sy = g.Zm;
     const long c0 = ix(g, i + 1, j, k + 1);   // cell j
-    const long c1 = c0 + sy;                  // cell j+1
     const int NY = g.NY;
-    const bool L5 = j != NY && j != NY - 1 && j > 3;
-    const bool R5 = j != NY && j != NY - 1 && j != NY - 2 && j > 2;
-    T q[6];
-    recip_j<T>(J, c0, sy, L5, R5, q);
-    Rec<T> L;
-    if (j == NY) L = rec2<T>(q, c0, sy, 2, 1);
-    else if (j == NY - 1) L = rec3L<T>(q, c0, sy, 1, 2, 3);
-    else if (j == 3) L = rec3L<T>(q, c0, sy, 1, 2, 3);
-    else if (j == 2) L = rec2<T>(q, c0, sy, 2, 3);
-    else if (j == 1) L = rec2<T>(q, c0, sy, 2, 3);
-    else L = rec5L<T>(q, c0, sy);
-    Rec<T> R;
-    if (j == NY) R = rec2<T>(q, c0, sy, 2, 3);
-    else if (j == NY - 1) R = rec2<T>(q, c0, sy, 3, 2);
-    else if (j == NY - 2) R = rec3R<T>(q, c0, sy, 2, 3, 4);
-    else if (j == 2) R = rec3R<T>(q, c0, sy, 2, 3, 4);
-    else if (j == 1) R = rec2<T>(q, c0, sy, 3, 4);
-    else R = rec5R<T>(q, c0, sy);
-    T Lq[5], Rq[5];
-#pragma unroll
-    for (int v = 0; v < 5; v++) {
-        Lq[v] = apply(L, U.v[v]);
-        Rq[v] = apply(R, U.v[v]);
-    }
+    T cl[6], cr[6];
+    if (j == NY) sh12<T>(cl);
+    else if (j == NY - 1) sh3L<T>(cl);
+    else if (j == 3) sh3L<T>(cl);
+    else if (j == 2) sh23<T>(cl);
+    else if (j == 1) sh23<T>(cl);
+    else sh5L<T>(cl);
+    if (j == NY) sh23<T>(cr);
+    else if (j == NY - 1) sh23<T>(cr);
+    else if (j == NY - 2) sh3R<T>(cr);
+    else if (j == 2) sh3R<T>(cr);
+    else if (j == 1) sh34<T>(cr);
+    else sh5R<T>(cr);
+    Sten<T> S;
+    load_sten<T>(U, J, c0, sy, cl, cr, S);
     const long f = ix(g, i, j, k);
     const T etx = ld<T>(mf.m[3], f), ety = ld<T>(mf.m[4], f), etz = ld<T>(mf.m[5], f);
+    const T jf = ld<T>(Jf, f);
+    const T ep = SCALE ? (EpF ? ld<T>(EpF, f) : T(epS)) : T(1);
+    // every load of the face is in flight before the first waits
+    __builtin_amdgcn_sched_barrier(0);
+    T Lq[5], Rq[5];
+    face_states<T>(S, cl, cr, Lq, Rq);
     const T insqr = T(1.0) / sqrt(etx * etx + ety * ety + etz * etz);
     T nx_ = 0, ny_ = 0, nz_ = 0;
     if (xi_mode) { nx_ = etx * insqr; ny_ = ety * insqr; nz_ = etz * insqr; }
     T out[5], pl, pr, fav3;
-    const T jf = ld<T>(Jf, f);
-    const T ep = SCALE ? (EpF ? ld<T>(EpF, f) : T(epS)) : T(1);
     roe_face<T>(Lq, Rq, etx, ety, etz, jf, T(e), T(K), nx_, ny_, nz_, ep, out, pl, pr, fav3);
     if (j == 1 || j == NY) {
         out[0] = 0;
@@ -312,36 +300,31 @@ __global__ void __launch_bounds__(256) k_flux_z(Geo g, double K, double e, int x
     const int j = 1 + t / nk;
     const int i = 2 + blockIdx.z;
     const long c0 = ix(g, i + 1, j + 1, k);   // cell k
-    const long c1 = c0 + 1;                   // cell k+1
     const int NZ = g.NZ;
-    const bool L5 = k != NZ && k > 2;
-    const bool R5 = k != NZ && k != NZ - 1 && k > 1;
-    T q[6];
-    recip_j<T>(J, c0, 1, L5, R5, q);
-    Rec<T> L;
-    if (k == NZ) L = rec3L<T>(q, c0, 1, 1, 2, 3);
-    else if (k == 2) L = rec3L<T>(q, c0, 1, 1, 2, 3);
-    else if (k == 1) L = rec2<T>(q, c0, 1, 2, 3);
-    else L = rec5L<T>(q, c0, 1);
-    Rec<T> R;
-    if (k == NZ) R = rec2<T>(q, c0, 1, 2, 3);
-    else if (k == NZ - 1) R = rec3R<T>(q, c0, 1, 2, 3, 4);
-    else if (k == 1) R = rec3R<T>(q, c0, 1, 2, 3, 4);
-    else R = rec5R<T>(q, c0, 1);
-    T Lq[5], Rq[5];
-#pragma unroll
-    for (int v = 0; v < 5; v++) {
-        Lq[v] = apply(L, U.v[v]);
-        Rq[v] = apply(R, U.v[v]);
-    }
+    T cl[6], cr[6];
+    if (k == NZ) sh3L<T>(cl);
+    else if (k == 2) sh3L<T>(cl);
+    else if (k == 1) sh23<T>(cl);
+    else sh5L<T>(cl);
+    if (k == NZ) sh23<T>(cr);
+    else if (k == NZ - 1) sh3R<T>(cr);
+    else if (k == 1) sh3R<T>(cr);
+    else sh5R<T>(cr);
+    Sten<T> S;
+    load_sten<T>(U, J, c0, 1, cl, cr, S);
     const long f = ix(g, i, j, k);
     const T ztx = ld<T>(mf.m[6], f), zty = ld<T>(mf.m[7], f), ztz = ld<T>(mf.m[8], f);
+    const T jf = ld<T>(Jf, f);
+    const T ep = SCALE ? (EpF ? ld<T>(EpF, f) : T(epS)) : T(1);
+    // every load of the face is in flight before the first waits
+    __builtin_amdgcn_sched_barrier(0);
+    T Lq[5], Rq[5];
+    face_states<T>(S, cl, cr, Lq, Rq);
     const T insqr = T(1.0) / (sqrt(ztx * ztx + zty * zty + ztz * ztz));
     T nx_ = 0, ny_ = 0, nz_ = 0;
     if (xi_mode) { nx_ = ztx * insqr; ny_ = zty * insqr; nz_ = ztz * insqr; }
     T out[5], pl, pr, fav3;
-    const T ep = SCALE ? (EpF ? ld<T>(EpF, f) : T(epS)) : T(1);
-    roe_face<T>(Lq, Rq, ztx, zty, ztz, ld<T>(Jf, f), T(e), T(K), nx_, ny_, nz_, ep, out, pl, pr, fav3);
+    roe_face<T>(Lq, Rq, ztx, zty, ztz, jf, T(e), T(K), nx_, ny_, nz_, ep, out, pl, pr, fav3);
 #pragma unroll
     for (int v = 0; v < 5; v++) F.v[v][f] = (double)out[v];
 }

@@ -14,11 +14,13 @@
 // planes: a ring of three planes of primitives (tile plus a 1-cell halo) and
 // the tile's y and z face fluxes live in LDS, the x face flux of the cell
 // below travels in registers, so that U and each face's J and metrics are
-// read once and vF is the only store.  The arithmetic is the reference's
-// except for divisions: a face's 1/Jf and the constant Cv K / Pr are formed
-// once and multiplied, the Sutherland power (T/Ts)^1.5 is x sqrt(x), and the
-// temperature is formed without the J that cancels from it; results agree
-// with the reference's to rounding.
+// read once and vF is the only store.  Loads run ahead of their use: a
+// plane's conserved variables one plane ahead, each face's metrics one face
+// ahead, so that the march seldom waits on memory.  The arithmetic is the
+// reference's except for divisions: a face's 1/Jf and the constant Cv K / Pr
+// are formed once and multiplied, the Sutherland power (T/Ts)^1.5 is
+// x sqrt(x), and the temperature is formed without the J that cancels from
+// it; results agree with the reference's to rounding.
 #include <algorithm>
 
 #include "cgli_internal.hpp"
@@ -262,22 +264,46 @@ __device__ __forceinline__ void zflux(const Phys &p, const Pt<Real> &O, const Pt
 constexpr int VT_J = 8, VT_K = 32, VT_THREADS = VT_J * VT_K;
 constexpr int VH_J = VT_J + 2, VH_K = VT_K + 2;   // staged primitives: positions plus a 1-cell halo
 
-// primitives (u, v, w, T) of plane i of the tile's halo into ring slot sl.
+// The conserved variables of a plane's halo positions, two per thread,
+// loaded a plane ahead of their conversion.  The loads run for every slot
+// (a slot past the halo, or a position outside the primitive box, reads the
+// nearest position inside it and is not stored): no branch around them, so
+// that the compiler's wait counts stay exact.
+static_assert(VH_J * VH_K <= 2 * VT_THREADS, "two halo positions per thread");
+struct Uplane {
+    double r[2][5];
+};
+
+__device__ __forceinline__ void u_load(const Geo &g, C5 U, int i, int j0, int k0, Uplane &R)
+{
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const int t = min((int)threadIdx.x + s * VT_THREADS, VH_J * VH_K - 1);
+        const int j = min(j0 - 1 + t / VH_K, g.NYY), k = min(k0 - 1 + t % VH_K, g.NZZ);
+        const long c = ix(g, i, j, k);
+#pragma unroll
+        for (int v = 0; v < 5; v++) R.r[s][v] = U.v[v][c];
+    }
+}
+
+// primitives (u, v, w, T) of plane i of the tile's halo, from its
+// conserved variables R, into ring slot i % 3.
 // The reference's T = P / (rho R) / J with P = (U5 - rho VV / 2)(K - 1) J
 // (Viscous_terms.cpp:260-266); J cancels, so T is formed from the stored
 // state alone, with one division per cell.
-__device__ __forceinline__ void visc_plane(const Geo &g, const Phys &p, C5 U, int i, int j0, int k0,
+__device__ __forceinline__ void visc_plane(const Geo &g, const Phys &p, const Uplane &R, int i, int j0, int k0,
                                            double (*sP)[4][VH_J][VH_K])
 {
     const double KR = (p.K - 1) / p.R;
-    for (int t = threadIdx.x; t < VH_J * VH_K; t += VT_THREADS) {
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const int t = threadIdx.x + s * VT_THREADS;
         const int hj = t / VH_K, hk = t % VH_K;
-        const int j = j0 - 1 + hj, k = k0 - 1 + hk;
-        if (j > g.NYY || k > g.NZZ) continue;   // outside the primitive box: never read for a used face
-        const long c = ix(g, i, j, k);
-        const double r = U.v[0][c], ir = 1.0 / r;
-        const double u = U.v[1][c] * ir, v = U.v[2][c] * ir, w = U.v[3][c] * ir;
-        const double T = (U.v[4][c] - 0.5 * r * (u * u + v * v + w * w)) * KR * ir;
+        // outside the primitive box: never read for a used face
+        if (t >= VH_J * VH_K || j0 - 1 + hj > g.NYY || k0 - 1 + hk > g.NZZ) continue;
+        const double r = R.r[s][0], ir = 1.0 / r;
+        const double u = R.r[s][1] * ir, v = R.r[s][2] * ir, w = R.r[s][3] * ir;
+        const double T = (R.r[s][4] - 0.5 * r * (u * u + v * v + w * w)) * KR * ir;
         double(*pl)[VH_J][VH_K] = sP[i % 3];
         pl[0][hj][hk] = u;
         pl[1][hj][hk] = v;
@@ -293,19 +319,29 @@ __device__ __forceinline__ Pt<Real> lp(const double (*sP)[4][VH_J][VH_K], int i,
     return Pt<Real>{(Real)pl[0][hj][hk], (Real)pl[1][hj][hk], (Real)pl[2][hj][hk], (Real)pl[3][hj][hk]};
 }
 
+// a face's metrics and Jacobian
+template <typename Real>
+struct Mf {
+    Real m[9], J;
+};
+
+template <typename Real>
+__device__ __forceinline__ void ldm(Mf<Real> &M, const M9 &mm, const double *__restrict__ Jx, long fid)
+{
+#pragma unroll
+    for (int q = 0; q < 9; q++) M.m[q] = (Real)mm.m[q][fid];
+    M.J = (Real)Jx[fid];
+}
+
 // x face of cell (i, j, k) (between planes i-1 and i) at the thread's position
 template <typename Real>
-__device__ __forceinline__ void xface(const Geo &g, const Phys &p, const double (*sP)[4][VH_J][VH_K], int i, int j,
-                                      int k, int hj, int hk, const double *__restrict__ Ju, const M9 &mu, double F[4])
+__device__ __forceinline__ void xface(const Phys &p, const double (*sP)[4][VH_J][VH_K], int i, int hj, int hk,
+                                      const Mf<Real> &M, double F[4])
 {
-    const long fid = ix(g, i - 1, j - 1, k - 1);
-    Real m[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) m[q] = (Real)mu.m[q][fid];
     xflux<Real>(p, lp<Real>(sP, i, hj, hk), lp<Real>(sP, i - 1, hj, hk), lp<Real>(sP, i, hj + 1, hk),
                 lp<Real>(sP, i - 1, hj + 1, hk), lp<Real>(sP, i, hj - 1, hk), lp<Real>(sP, i - 1, hj - 1, hk),
                 lp<Real>(sP, i, hj, hk + 1), lp<Real>(sP, i - 1, hj, hk + 1), lp<Real>(sP, i, hj, hk - 1),
-                lp<Real>(sP, i - 1, hj, hk - 1), m, (Real)Ju[fid], F);
+                lp<Real>(sP, i - 1, hj, hk - 1), M.m, M.J, F);
 }
 
 // vF of the interior cells.  Grid: (tiles in k, tiles in j, segments of seg
@@ -325,43 +361,56 @@ __global__ void __launch_bounds__(VT_THREADS, 2) k_viscous(Geo g, Phys p, C5 U, 
     const bool out = tj < VT_J - 1 && tk < VT_K - 1 && j <= g.NY && k <= g.NZ;   // an output cell
     const bool yok = j <= g.NYY && k <= g.NZ, zok = j <= g.NY && k <= g.NZZ;   // its low y / z face exists
     const double invXI = 1. / (p.dxi), invET = 1. / (p.deta), invZT = 1. / (p.dzeta);
-    visc_plane(g, p, U, iS - 1, j0, k0, sP);
-    visc_plane(g, p, U, iS, j0, k0, sP);
+    Uplane R;
+    u_load(g, U, iS - 1, j0, k0, R);
+    visc_plane(g, p, R, iS - 1, j0, k0, sP);
+    u_load(g, U, iS, j0, k0, R);
+    visc_plane(g, p, R, iS, j0, k0, sP);
+    u_load(g, U, iS + 1, j0, k0, R);
+    // face metrics are loaded one face ahead of their use: the x face of
+    // the next plane while the z face computes, the y face while the x face
+    // computes, the z face while the y face computes
+    Mf<Real> MX, MY, MZ;
+    // the loads run for every thread (no branch around them, so that the
+    // compiler's wait counts stay exact); a position without the face reads
+    // the nearest one that has it and ignores the values
+    const long cjk = (long)(min(j, g.NYY) - 1) * g.Zm + (min(k, g.NZZ) - 1), sx = (long)g.Ym * g.Zm;
+    ldm<Real>(MX, mu, Ju, (iS - 1) * sx + cjk);
     __syncthreads();
     double Fx[4] = {0.0, 0.0, 0.0, 0.0};   // x face below the current cell
-    if (out) xface<Real>(g, p, sP, iS, j, k, hj, hk, Ju, mu, Fx);
+    if (out) xface<Real>(p, sP, iS, hj, hk, MX, Fx);
+    ldm<Real>(MX, mu, Ju, iS * sx + cjk);
     for (int i = iS; i < iE; i++) {
-        visc_plane(g, p, U, i + 1, j0, k0, sP);
+        const long fid = (i - 1) * sx + cjk;
+        visc_plane(g, p, R, i + 1, j0, k0, sP);
+        ldm<Real>(MY, mv, Jv, fid);
         __syncthreads();
+        // the next plane's conserved variables, converted at the top of the
+        // next iteration (the last iteration reloads plane iE, unused)
+        u_load(g, U, min(i + 2, iE), j0, k0, R);
         // the three faces one after the other: the scheduling barriers keep the
         // compiler from hoisting all their stencil reads at once
         double Fh[4] = {0.0, 0.0, 0.0, 0.0};
-        if (out) xface<Real>(g, p, sP, i + 1, j, k, hj, hk, Ju, mu, Fh);
+        if (out) xface<Real>(p, sP, i + 1, hj, hk, MX, Fh);
+        ldm<Real>(MZ, mw, Jw, fid);
         __builtin_amdgcn_sched_barrier(0);
         if (yok) {
-            const long fid = ix(g, i - 1, j - 1, k - 1);
-            Real m[9];
-#pragma unroll
-            for (int q = 0; q < 9; q++) m[q] = (Real)mv.m[q][fid];
             double F[4];
             yflux<Real>(p, lp<Real>(sP, i, hj, hk), lp<Real>(sP, i, hj - 1, hk), lp<Real>(sP, i + 1, hj, hk),
                         lp<Real>(sP, i + 1, hj - 1, hk), lp<Real>(sP, i - 1, hj, hk), lp<Real>(sP, i - 1, hj - 1, hk),
                         lp<Real>(sP, i, hj, hk + 1), lp<Real>(sP, i, hj - 1, hk + 1), lp<Real>(sP, i, hj, hk - 1),
-                        lp<Real>(sP, i, hj - 1, hk - 1), m, (Real)Jv[fid], F);
+                        lp<Real>(sP, i, hj - 1, hk - 1), MY.m, MY.J, F);
 #pragma unroll
             for (int v = 0; v < 4; v++) sF[0][v][tj][tk] = F[v];
         }
+        ldm<Real>(MX, mu, Ju, (min(i + 1, iE - 1)) * sx + cjk);
         __builtin_amdgcn_sched_barrier(0);
         if (zok) {
-            const long fid = ix(g, i - 1, j - 1, k - 1);
-            Real m[9];
-#pragma unroll
-            for (int q = 0; q < 9; q++) m[q] = (Real)mw.m[q][fid];
             double F[4];
             zflux<Real>(p, lp<Real>(sP, i, hj, hk), lp<Real>(sP, i, hj, hk - 1), lp<Real>(sP, i + 1, hj, hk),
                         lp<Real>(sP, i + 1, hj, hk - 1), lp<Real>(sP, i - 1, hj, hk), lp<Real>(sP, i - 1, hj, hk - 1),
                         lp<Real>(sP, i, hj + 1, hk), lp<Real>(sP, i, hj + 1, hk - 1), lp<Real>(sP, i, hj - 1, hk),
-                        lp<Real>(sP, i, hj - 1, hk - 1), lp<Real>(sP, i - 1, hj + 1, hk), m, (Real)Jw[fid], F);
+                        lp<Real>(sP, i, hj - 1, hk - 1), lp<Real>(sP, i - 1, hj + 1, hk), MZ.m, MZ.J, F);
 #pragma unroll
             for (int v = 0; v < 4; v++) sF[1][v][tj][tk] = F[v];
         }
