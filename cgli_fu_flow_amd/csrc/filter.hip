@@ -15,12 +15,15 @@
 // bit-identical with the reference built by g++ on x86 (no FMA
 // contraction: the library is compiled with -ffp-contract=off).
 // Bytes per level: 4 fields in (plus a 1-cell halo in j and k), J and
-// Y_point, 4 fields out.
+// Y_point, 4 fields out.  Segments are short (about 4096 blocks per level
+// launch): more blocks in flight hide the plane-to-plane latency better than
+// the halo planes each segment re-reads cost.
 // Every kernel is templated on the arithmetic type Real: FP64, or FP32 for
 // the mixed-precision path (work arrays, passes, energies and the sensor
 // ratio in float; U_, J and Y_point read as FP64 and rounded once; the
 // E_high/E_low outputs are FP64 fields either way).
 #include <algorithm>
+#include <type_traits>
 
 #include "cgli_internal.hpp"
 
@@ -70,14 +73,18 @@ __global__ void k_filt_init(Geo g, R4 U, Real *__restrict__ work, long N)
 //   Gy = J/Y_point-weighted y pass of Gx
 //   level output = z pass of Gy
 // on j = 2..NY, k = 2..NZ and the level's planes lo..hi.  A block owns an
-// 8 x 32 tile of (j, k) and marches along i through a segment of planes:
-// the level input of planes i-1, i, i+1 on the tile plus a 1-cell halo
-// lives in an LDS ring (each plane read from HBM once per block), Gx and Gy
-// of plane i stay in LDS, and the level output is the only store.  Where
-// the reference reads Gx on the ghost rows j = 1, NYY or Gy on the ghost
-// columns k = 1, NZZ, which no pass writes and which hold U_ from its
-// initial copy (Filter.cpp:117-134), this kernel reads U_.  Expressions and
-// operation order are the reference's.
+// 8 x 32 tile of (j, k) and marches along i through a segment of planes.
+// Each thread keeps the level input of its halo positions (the tile plus a
+// 1-cell halo) for planes i-1, i, i+1 in registers, with plane i+2 in
+// flight, so each plane is read from HBM once per block and the x pass
+// needs no LDS; Gx and Gy of plane i go through LDS for the y and z passes,
+// and the level output is the only store.  Where the reference reads Gx on
+// the ghost rows j = 1, NYY or Gy on the ghost columns k = 1, NZZ, which no
+// pass writes and which hold U_ from its initial copy (Filter.cpp:117-134),
+// this kernel reads U_: those positions load U_ in place of the level
+// input.  Expressions and operation order are the reference's; the y
+// pass's quotients that share a divisor use its reciprocal and div_r, which
+// gives the division's bits.
 constexpr int FT_J = 8, FT_K = 32, FT_THREADS = FT_J * FT_K;
 constexpr int FH_K = FT_K + 2, FH_N = (FT_J + 2) * FH_K;
 
@@ -87,8 +94,7 @@ __global__ void __launch_bounds__(FT_THREADS) k_filt_level(Geo g, int lo, int hi
                                                           L4T<Real> out)
 {
     constexpr int NR = (FH_N + FT_THREADS - 1) / FT_THREADS;   // halo'd positions per thread
-    __shared__ Real sL[3][4][FH_N];   // level input of planes i-1, i, i+1 (slot i % 3)
-    __shared__ Real sX[4][FH_N];      // Gx of plane i
+    __shared__ Real sX[4][FH_N];      // Gx of plane i (U_ on the ghost rows and columns)
     __shared__ double sJ[FH_N], sY[FH_N];
     __shared__ Real sG[4][FT_J][FH_K];   // Gy of plane i, columns k0-1 .. k0+FT_K
     const int j0 = 2 + blockIdx.y * FT_J, k0 = 2 + blockIdx.x * FT_K;
@@ -97,38 +103,46 @@ __global__ void __launch_bounds__(FT_THREADS) k_filt_level(Geo g, int lo, int hi
     // this thread's halo'd tile positions t = threadIdx.x + r FT_THREADS:
     // (j0 - 1 + t / FH_K, k0 - 1 + t % FH_K), where inside the arrays' box
     long cpos[NR];
-    bool inb[NR];
+    bool inb[NR], ghost[NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         const int t = threadIdx.x + r * FT_THREADS;
         const int j = j0 - 1 + t / FH_K, k = k0 - 1 + t % FH_K;
         inb[r] = t < FH_N && j <= g.NYY && k <= g.NZZ;
+        ghost[r] = j == 1 || j == g.NYY || k == 1 || k == g.NZZ;
         cpos[r] = inb[r] ? ix(g, 0, j, k) : 0;
     }
     const long sx = g.Ym * g.Zm;
-    // the level input of a plane and J, Y_point of another, loaded into
-    // registers one iteration before they go to LDS
-    Real pL[NR][4];
+    // The x pass runs down each position's own column: the level input of
+    // planes i-1, i, i+1 in registers (xm, x0, xp), plane i+2 in flight
+    // (pL), so the x pass needs no LDS.  A ghost row or column holds U_
+    // instead (the passes read U_ there).  J and Y_point of a plane are
+    // loaded into registers one iteration before they go to LDS.  Every
+    // load runs for every slot (a position outside the box reads plane i's
+    // first cell and is not stored): no branch around them.
+    Real xm[NR][4], x0[NR][4], xp[NR][4], pL[NR][4];
     double pJ[NR], pY[NR];
-    auto fetch_L = [&](int i) {
+    auto fetch = [&](Real (*dst)[4], int i) {
 #pragma unroll
         for (int r = 0; r < NR; r++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) pL[r][v] = inb[r] ? (Real)prev.v[v][cpos[r] + i * sx] : Real(0);
+            for (int v = 0; v < 4; v++) {
+                const long c = cpos[r] + i * sx;
+                if constexpr (std::is_same<In, double>::value) {
+                    const double *src = ghost[r] ? U.v[v] : prev.v[v];
+                    dst[r][v] = (Real)src[c];
+                } else {
+                    const Real a = (Real)prev.v[v][c], b = (Real)U.v[v][c];
+                    dst[r][v] = ghost[r] ? b : a;
+                }
+            }
     };
     auto fetch_JY = [&](int i) {
 #pragma unroll
         for (int r = 0; r < NR; r++) {
-            pJ[r] = inb[r] ? J[cpos[r] + i * sx] : 0.0;
-            pY[r] = inb[r] ? Y[cpos[r] + i * sx] : 0.0;
+            pJ[r] = J[cpos[r] + i * sx];
+            pY[r] = Y[cpos[r] + i * sx];
         }
-    };
-    auto put_L = [&](int i) {
-#pragma unroll
-        for (int r = 0; r < NR; r++)
-            if (inb[r])
-#pragma unroll
-                for (int v = 0; v < 4; v++) sL[i % 3][v][threadIdx.x + r * FT_THREADS] = pL[r][v];
     };
     auto put_JY = [&]() {
 #pragma unroll
@@ -138,68 +152,73 @@ __global__ void __launch_bounds__(FT_THREADS) k_filt_level(Geo g, int lo, int hi
                 sY[threadIdx.x + r * FT_THREADS] = pY[r];
             }
     };
-    for (int q = iS - 1; q <= iS + 1; q++) {
-        fetch_L(q);
-        put_L(q);
-    }
+    fetch(xm, iS - 1);
+    fetch(x0, iS);
+    fetch(xp, iS + 1);
     fetch_JY(iS);
     put_JY();
-    if (iS + 2 <= iE + 1) fetch_L(iS + 2);
-    if (iS + 1 <= iE) fetch_JY(iS + 1);
+    fetch(pL, min(iS + 2, iE + 1));
+    fetch_JY(min(iS + 1, iE));
     for (int i = iS; i <= iE; i++) {
         __syncthreads();
-        // Gx on rows j0-1 .. j0+FT_J, columns 2..NZ of the halo'd tile
-        for (int t = threadIdx.x; t < FH_N; t += FT_THREADS) {
-            const int j = j0 - 1 + t / FH_K, k = k0 - 1 + t % FH_K;
-            if (j > g.NYY || k < 2 || k > g.NZ) continue;
-            if (j == 1 || j == g.NYY) {
-                const long c = ix(g, i, j, k);
+        // Gx on rows j0-1 .. j0+FT_J, columns 1..NZZ of the halo'd tile
 #pragma unroll
-                for (int v = 0; v < 4; v++) sX[v][t] = (Real)U.v[v][c];
-                continue;
-            }
+        for (int r = 0; r < NR; r++) {
+            const int t = threadIdx.x + r * FT_THREADS;
+            if (!inb[r]) continue;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const Real um = sL[(i + 2) % 3][v][t], u0 = sL[i % 3][v][t], up = sL[(i + 1) % 3][v][t];
-                if (!TWOH)
-                    sX[v][t] = Real(0.125) * (um + 6 * u0 + up);
+                if (ghost[r])
+                    sX[v][t] = x0[r][v];
+                else if (!TWOH)
+                    sX[v][t] = Real(0.125) * (xm[r][v] + 6 * x0[r][v] + xp[r][v]);
                 else
-                    sX[v][t] = Real(0.25) * (um + 2 * u0 + up);
+                    sX[v][t] = Real(0.25) * (xm[r][v] + 2 * x0[r][v] + xp[r][v]);
             }
         }
         __syncthreads();
-        // plane i-1 is read: its ring slot takes plane i+2
-        if (i + 2 <= iE + 1) put_L(i + 2);
         // Gy on rows j0 .. j0+FT_J-1 (up to NY), columns k0-1 .. k0+FT_K
         for (int t = threadIdx.x; t < FT_J * FH_K; t += FT_THREADS) {
             const int tj = t / FH_K, hk = t % FH_K;
             const int j = j0 + tj, k = k0 - 1 + hk;
             if (j > g.NY || k > g.NZZ) continue;
+            const int pc = (tj + 1) * FH_K + hk, pm = pc - FH_K, pp = pc + FH_K;
             if (k == 1 || k == g.NZZ) {
-                const long c = ix(g, i, j, k);
 #pragma unroll
-                for (int v = 0; v < 4; v++) sG[v][tj][hk] = (Real)U.v[v][c];
+                for (int v = 0; v < 4; v++) sG[v][tj][hk] = sX[v][pc];
                 continue;
             }
-            const int pc = (tj + 1) * FH_K + hk, pm = pc - FH_K, pp = pc + FH_K;
             const Real a = (Real)(sY[pp] - sY[pm]);
             const Real b = (Real)(sY[pp] - sY[pc]);
             const Real cc = (Real)(sY[pc] - sY[pm]);
             const Real Jm = (Real)sJ[pm], Jc = (Real)sJ[pc], Jp = (Real)sJ[pp];
+            // the quotients by a and by Jc through their reciprocals (div_r:
+            // the division's bits)
+            const Real ia = Real(1) / a, iJc = Real(1) / Jc;
+            const Real wm = div_r(TWOH ? b : 2 * b, a, ia), wp = div_r(TWOH ? cc : 2 * cc, a, ia);
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const Real *x = sX[v];
                 if (!TWOH)
-                    sG[v][tj][hk] = Real(0.125) * (2 * b / a * x[pm] * Jm + 6 * x[pc] * Jc + 2 * cc / a * x[pp] * Jp) / Jc;
+                    sG[v][tj][hk] = div_r(Real(0.125) * (wm * x[pm] * Jm + 6 * x[pc] * Jc + wp * x[pp] * Jp), Jc, iJc);
                 else
-                    sG[v][tj][hk] = Real(0.5) * (b / a * x[pm] * Jm + x[pc] * Jc + cc / a * x[pp] * Jp) / Jc;
+                    sG[v][tj][hk] = div_r(Real(0.5) * (wm * x[pm] * Jm + x[pc] * Jc + wp * x[pp] * Jp), Jc, iJc);
             }
         }
         __syncthreads();
-        // J and Y_point of plane i are read: they take those of plane i+1
-        if (i + 1 <= iE) put_JY();
-        if (i + 3 <= iE + 1) fetch_L(i + 3);
-        if (i + 2 <= iE) fetch_JY(i + 2);
+        // J and Y_point of plane i are read: they take those of plane i+1;
+        // the column window moves up one plane
+        put_JY();
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                xm[r][v] = x0[r][v];
+                x0[r][v] = xp[r][v];
+                xp[r][v] = pL[r][v];
+            }
+        fetch(pL, min(i + 3, iE + 1));
+        fetch_JY(min(i + 2, iE));
         {   // the level output: z pass of Gy
             const int tj = threadIdx.x / FT_K, tk = threadIdx.x % FT_K;
             const int j = j0 + tj, k = k0 + tk;
@@ -360,9 +379,9 @@ static void filter_run(const Geo &g, double crit, double E_high, double E_low, C
         for (int l = 0; l < 6; l++) {
             int lo, hi;
             level_range(g, l, lo, hi);
-            // segments of at least 8 planes, about 2048 blocks
+            // segments of at least 8 planes, about 4096 blocks
             const int np = hi - lo + 1;
-            const int seg = std::min(np, std::max(8, (int)((long)np * tk * tj / 2048)));
+            const int seg = std::min(np, std::max(8, (int)((long)np * tk * tj / 4096)));
             const dim3 gr(tk, tj, (np + seg - 1) / seg), bl(FT_THREADS);
             if (sc == 0) {
                 if (l == 0)

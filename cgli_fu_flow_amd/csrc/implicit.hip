@@ -18,23 +18,32 @@ namespace cgli {
 
 struct CP { double rho, u, v, w, VV, P, C, T, H, beta; };
 
-// Dual_time_stepping.cpp:759-769
-__device__ __forceinline__ CP cell_prim(const C5 &U, const double *__restrict__ J, long c, double K, double e)
+// Dual_time_stepping.cpp:759-769, from the cell's conserved variables and
+// J.  Quotients that share a divisor (U1, rho) use its reciprocal and
+// div_r: the division's bits, four divisions instead of seven.
+__device__ __forceinline__ CP cell_prim_v(const double U[5], double Jc, double K, double e)
 {
     CP q;
-    const double U1 = U.v[0][c], Jc = J[c];
+    const double U1 = U[0], iU1 = 1.0 / U1;
     q.rho = U1 * Jc;
-    q.u = U.v[1][c] / U1;
-    q.v = U.v[2][c] / U1;
-    q.w = U.v[3][c] / U1;
+    q.u = div_r(U[1], U1, iU1);
+    q.v = div_r(U[2], U1, iU1);
+    q.w = div_r(U[3], U1, iU1);
     q.VV = q.u * q.u + q.v * q.v + q.w * q.w;
-    q.P = (U.v[4][c] * Jc - 0.5 * q.rho * q.VV) * (K - 1);
-    q.C = K * q.P / q.rho;
-    q.T = q.P / q.rho;
+    q.P = (U[4] * Jc - 0.5 * q.rho * q.VV) * (K - 1);
+    const double irho = 1.0 / q.rho;
+    q.C = div_r(K * q.P, q.rho, irho);
+    q.T = div_r(q.P, q.rho, irho);
     q.H = 0.5 * q.VV + q.C / (K - 1);
     const double r = q.VV / q.C;
     q.beta = (r > e) ? r : e;
     return q;
+}
+
+__device__ __forceinline__ CP cell_prim(const C5 &U, const double *__restrict__ J, long c, double K, double e)
+{
+    const double u[5] = {U.v[0][c], U.v[1][c], U.v[2][c], U.v[3][c], U.v[4][c]};
+    return cell_prim_v(u, J[c], K, e);
 }
 
 // split coefficients of one neighbour along one direction
@@ -303,11 +312,19 @@ void launch_sponge(const Geo &g, const Phys &p, double e, int gstart, int gend0,
 }
 
 // --------------------------------------------------------------- RHS+diag
+// Reciprocals of the constant divisors, formed on the host (correctly
+// rounded) for div_r.
+struct RInv { double dt2, dxi, deta, dzeta; };
+
 // INC: also the starting increments NL = NR = D MR (the schedules whose
 // sweeps read them from the fields; schedule 4 keeps its own copies, and
-// then neither the diagonal nor the metrics are touched here)
-template <bool INC>
-__global__ void __launch_bounds__(256) k_rhs_diag(Geo g, Phys p, double deltaT, double e, int with_res, C5 U_,
+// then neither the diagonal nor the metrics are touched here).  WR: the
+// residual source Res / J is added.  Every load of the cell is issued before
+// the first is used (the kernel is one round trip to memory), and each
+// division of Dual_time_stepping.cpp:773-828 is div_r with its divisor's
+// reciprocal: MR has the bits of the plain divisions.
+template <bool INC, bool WR>
+__global__ void __launch_bounds__(256) k_rhs_diag(Geo g, Phys p, double deltaT, RInv ri, double e, C5 U_,
                                                   C5 U, C5 Uq, const double *__restrict__ J, M9 met, C5 Fx,
                                                   C5 Fy, C5 Fz, C4 vF, C5 Res, F5 MR, F5 NL, F5 NR)
 {
@@ -318,39 +335,57 @@ __global__ void __launch_bounds__(256) k_rhs_diag(Geo g, Phys p, double deltaT, 
     const double K = p.K;
     const long c = ix(g, i, j, k);
     const long sx = g.Ym * g.Zm, sy = g.Zm;
-    const CP q = cell_prim(U_, J, c, K, e);
+    // flux face indices: x faces of cell (i,j,k) at [i][j-1][k-1] and
+    // [i-1][j-1][k-1]; y at [i-1][j][k-1] and [i-1][j-1][k-1]; z at
+    // [i-1][j-1][k] and [i-1][j-1][k-1] (Dual_time_stepping.cpp:810-828)
+    const long f0 = c - sx - sy - 1;
+    double u_[5], un[5], uq[5], fx[5][2], fy[5][2], fz[5][2], vf[4], rs[5];
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+        u_[n] = U_.v[n][c];
+        un[n] = U.v[n][c];
+        uq[n] = Uq.v[n][c];
+        fx[n][0] = Fx.v[n][f0];
+        fx[n][1] = Fx.v[n][f0 + sx];
+        fy[n][0] = Fy.v[n][f0];
+        fy[n][1] = Fy.v[n][f0 + sy];
+        fz[n][0] = Fz.v[n][f0];
+        fz[n][1] = Fz.v[n][f0 + 1];
+        rs[n] = WR ? Res.v[n][c] : 0.0;
+    }
+#pragma unroll
+    for (int n = 0; n < 4; n++) vf[n] = vF.v[n][c];
+    const double Jc = J[c];
+    __builtin_amdgcn_sched_barrier(0);   // the loads above stay ahead of all arithmetic
+    const CP q = cell_prim_v(u_, Jc, K, e);
     const double rho = q.rho, u = q.u, v = q.v, w = q.w, VV = q.VV, T = q.T, H = q.H, beta = q.beta;
-    const double temp = rho * K;
+    const double temp = rho * K, irho = 1.0 / rho, itemp = 1.0 / temp, iJc = 1.0 / Jc;
     double d[5][5];
     d[0][0] = -(H * (K - 1) + VV - K * (T + VV)) * beta;
     d[0][1] = -(K - 1) * u * beta;
     d[0][2] = -(K - 1) * v * beta;
     d[0][3] = -(K - 1) * w * beta;
     d[0][4] = (K - 1) * beta;
-    d[1][0] = -u / rho; d[1][1] = 1 / rho; d[1][2] = 0; d[1][3] = 0; d[1][4] = 0;
-    d[2][0] = -v / rho; d[2][1] = 0; d[2][2] = 1 / rho; d[2][3] = 0; d[2][4] = 0;
-    d[3][0] = -w / rho; d[3][1] = 0; d[3][2] = 0; d[3][3] = 1 / rho; d[3][4] = 0;
-    d[4][0] = (K - 1) * (VV + K * T * beta + (K - 1) * VV * beta + H * (-1 + beta - K * beta)) / temp;
-    d[4][1] = -(K - 1) * u * (1 + (K - 1) * beta) / temp;
-    d[4][2] = -(K - 1) * v * (1 + (K - 1) * beta) / temp;
-    d[4][3] = -(K - 1) * w * (1 + (K - 1) * beta) / temp;
-    d[4][4] = (K - 1) * (1 + (K - 1) * beta) / temp;
-    // flux face indices: x faces of cell (i,j,k) at [i][j-1][k-1] and
-    // [i-1][j-1][k-1]; y at [i-1][j][k-1] and [i-1][j-1][k-1]; z at
-    // [i-1][j-1][k] and [i-1][j-1][k-1] (Dual_time_stepping.cpp:810-828)
-    const long f0 = c - sx - sy - 1;
+    d[1][0] = div_r(-u, rho, irho); d[1][1] = irho; d[1][2] = 0; d[1][3] = 0; d[1][4] = 0;
+    d[2][0] = div_r(-v, rho, irho); d[2][1] = 0; d[2][2] = irho; d[2][3] = 0; d[2][4] = 0;
+    d[3][0] = div_r(-w, rho, irho); d[3][1] = 0; d[3][2] = 0; d[3][3] = irho; d[3][4] = 0;
+    d[4][0] = div_r((K - 1) * (VV + K * T * beta + (K - 1) * VV * beta + H * (-1 + beta - K * beta)), temp, itemp);
+    d[4][1] = div_r(-(K - 1) * u * (1 + (K - 1) * beta), temp, itemp);
+    d[4][2] = div_r(-(K - 1) * v * (1 + (K - 1) * beta), temp, itemp);
+    d[4][3] = div_r(-(K - 1) * w * (1 + (K - 1) * beta), temp, itemp);
+    d[4][4] = div_r((K - 1) * (1 + (K - 1) * beta), temp, itemp);
     const double f = -(rho - p.rho_ref) * p.grav + p.f_const;
-    const double Jc = J[c];
+    const double dt2 = 2 * deltaT;
     double Rk[5];
 #pragma unroll
     for (int n = 0; n < 5; n++) {
-        const double Rp = -(3 * U_.v[n][c] - 4 * U.v[n][c] + Uq.v[n][c]) / (2 * deltaT);
-        const double Rf = -((Fx.v[n][f0 + sx] - Fx.v[n][f0]) / p.dxi + (Fy.v[n][f0 + sy] - Fy.v[n][f0]) / p.deta +
-                            (Fz.v[n][f0 + 1] - Fz.v[n][f0]) / p.dzeta);
-        double r = with_res ? Res.v[n][c] / Jc + Rp + Rf : Rp + Rf;
-        if (n >= 1) r = r + vF.v[n - 1][c];
-        if (n == 1) r = r + f / Jc;
-        if (n == 4) r = r + f * u / Jc;
+        const double Rp = div_r(-(3 * u_[n] - 4 * un[n] + uq[n]), dt2, ri.dt2);
+        const double Rf = -(div_r(fx[n][1] - fx[n][0], p.dxi, ri.dxi) + div_r(fy[n][1] - fy[n][0], p.deta, ri.deta) +
+                            div_r(fz[n][1] - fz[n][0], p.dzeta, ri.dzeta));
+        double r = WR ? div_r(rs[n], Jc, iJc) + Rp + Rf : Rp + Rf;
+        if (n >= 1) r = r + vf[n - 1];
+        if (n == 1) r = r + div_r(f, Jc, iJc);
+        if (n == 4) r = r + div_r(f * u, Jc, iJc);
         Rk[n] = r;
     }
     double mr[5];
@@ -383,12 +418,10 @@ void launch_rhs_diag(const Geo &g, const Phys &p, double deltaT, double e, int w
 {
     dim3 b(64, 4);
     dim3 gr((g.NZ - 1 + b.x - 1) / b.x, (g.NY - 1 + b.y - 1) / b.y, g.nx);
-    if (write_inc)
-        hipLaunchKernelGGL(k_rhs_diag<true>, gr, b, 0, s, g, p, deltaT, e, with_res, U_, U, Uq, J, met, Fx, Fy, Fz, vF,
-                           Res, MR, NL, NR);
-    else
-        hipLaunchKernelGGL(k_rhs_diag<false>, gr, b, 0, s, g, p, deltaT, e, with_res, U_, U, Uq, J, met, Fx, Fy, Fz,
-                           vF, Res, MR, NL, NR);
+    const RInv ri{1.0 / (2 * deltaT), 1.0 / p.dxi, 1.0 / p.deta, 1.0 / p.dzeta};
+    auto kern = write_inc ? (with_res ? k_rhs_diag<true, true> : k_rhs_diag<true, false>)
+                          : (with_res ? k_rhs_diag<false, true> : k_rhs_diag<false, false>);
+    hipLaunchKernelGGL(kern, gr, b, 0, s, g, p, deltaT, ri, e, U_, U, Uq, J, met, Fx, Fy, Fz, vF, Res, MR, NL, NR);
 }
 
 // ------------------------------------------------------------- LU-SGS
