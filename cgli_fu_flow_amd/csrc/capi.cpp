@@ -33,6 +33,7 @@ struct cgli_graph {
     bool norm_wave = false;
     bool scatters_nr = false;    // holds a schedule-3 scatter: NL is then to be copied from NR where read
     bool nr_march = false;       // holds a schedule-4 update: NL and NR are then in march order only
+    bool u4_fresh = false;       // holds one, which leaves U_'s march-ordered copy current
 };
 
 struct cgli_ctx {
@@ -58,6 +59,7 @@ struct cgli_ctx {
                                  // order only (written to the fields where read)
     long nr_marches = 0;         // schedule-4 updates queued
     long nr_marches_at_capture = 0;
+    bool u4_fresh = false;       // u4's cell slots hold U_'s interior as the last schedule-4 update wrote it
     bool capturing = false;      // between cgli_graph_begin and cgli_graph_end
     long generation = 0;         // bumped whenever a captured graph would go stale
     bool norm_pending = false;   // a deferred residual norm awaits cgli_dual_time_stepping_wait
@@ -794,6 +796,7 @@ static int wave_setup(cgli_ctx *c)
         c->geo4 = p;
         if (hipMalloc(&p, fb) != hipSuccess) return CGLI_E_NOMEM;
         c->u4 = p;
+        c->u4_fresh = false;
         if (hipMalloc(&p, 3 * fb) != hipSuccess) return CGLI_E_NOMEM;
         c->x4 = p;
         // slots with no cell are never written: NaN in them
@@ -942,6 +945,7 @@ int cgli_graph_end(cgli_ctx *c, cgli_graph **out)
     G->norm_wave = wave;
     G->scatters_nr = c->nr_scatters != c->nr_scatters_at_capture;
     G->nr_march = c->nr_marches != c->nr_marches_at_capture;
+    G->u4_fresh = G->nr_march;
     *out = G;
     return CGLI_OK;
 }
@@ -958,6 +962,7 @@ int cgli_graph_launch(cgli_graph *G)
         c->nr_march = true;
         c->nl_from_nr = false;
     }
+    if (G->u4_fresh) c->u4_fresh = true;
     if (G->norm) {
         HIPCHK(hipEventRecord(c->norm_ev, c->stream));
         c->norm_pending = true;
@@ -994,7 +999,8 @@ int ctx_dts_wait(cgli_ctx *c, double er[5])
     return collect_norm(c, er, c->norm_wave);
 }
 
-int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_residual, bool rhs_ready)
+int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_residual, bool rhs_ready,
+                  bool u4_current)
 {
     if (!c || sweeps < 1) return CGLI_E_ARG;
     HIPCHK(hipSetDevice(c->device));
@@ -1056,9 +1062,12 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
             else full_geo = true;
             c->geo4_version = c->geo_version;
         }
+        // U_'s march-ordered copy is still the one the last update wrote
+        // when the caller vouches that U_'s interior has not changed since
+        const bool keep_u4 = u4_current && c->u4_fresh;
         tbegin(c, T_GATHER);
-        launch_gather4(g, c5(c, CGLI_U_), c5(c, CGLI_MR), c->f[CGLI_J], m9(c, CGLI_MET), c->u4, x4,
-                       full_geo ? c->geo4 : nullptr, s);
+        launch_gather4(g, c5(c, CGLI_U_), c5(c, CGLI_MR), c->f[CGLI_J], m9(c, CGLI_MET), keep_u4 ? nullptr : c->u4,
+                       x4, full_geo ? c->geo4 : nullptr, s);
         tend(c, T_GATHER);
     } else if (c->lusgs_mode >= 1) {
         tbegin(c, T_COEF);
@@ -1156,7 +1165,9 @@ int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_re
         c->nr_march = true;
         c->nl_from_nr = false;
         c->nr_marches++;
+        c->u4_fresh = true;
     } else {
+        c->u4_fresh = false;
         nblk = launch_update_norm(g, c->p, f5(c, CGLI_U_), c->f[CGLI_J], c5(c, CGLI_NR), f5(c, CGLI_RES),
                                   write_residual, 0, c->scratch, s);
     }
@@ -1201,6 +1212,7 @@ int cgli_dplusgs(cgli_ctx *c, double deltaT, double er[5], double e, int sweeps)
         launch_copy_interior(g, c5(c, CGLI_NR), f5(c, CGLI_NL), s);
         c->nl_from_nr = false;
         c->nr_march = false;   // NL and NR now hold these increments
+        c->u4_fresh = false;   // U_ changes outside the march order
     }
     for (int v = 0; v < 5; v++) launch_zero(c->f[CGLI_RES + v], g.N, s);
     const int nblk = launch_update_norm(g, c->p, f5(c, CGLI_U_), c->f[CGLI_J], c5(c, CGLI_NR), f5(c, CGLI_RES),
@@ -1242,6 +1254,7 @@ int cgli_dplusgs_noalloc(cgli_ctx *c, double deltaT, double er[5], double e, int
         launch_copy_interior(g, c5(c, CGLI_NR), f5(c, CGLI_NL), s);
         c->nl_from_nr = false;
         c->nr_march = false;   // NL and NR now hold these increments
+        c->u4_fresh = false;   // U_ changes outside the march order
     }
     for (int v = 0; v < 5; v++) launch_zero(c->f[CGLI_RES + v], g.N, s);
     // every partial the final reduction reads is written by this call

@@ -180,7 +180,7 @@ void launch_scatter4(const Geo &g, const void *Xm, F5 NL, F5 NR, hipStream_t s);
 // schedule 4's update and norm from march-ordered U_ (Um) and NR (Xm);
 // returns the number of partial-sum blocks (update4_blocks)
 long update4_blocks(const Geo &g);
-int launch_update4(const Geo &g, const Phys &p, F5 U_, const double *J, const void *Um, const void *Xm, F5 Res,
+int launch_update4(const Geo &g, const Phys &p, F5 U_, const double *J, void *Um, const void *Xm, F5 Res,
                    int write_res, double *partials, hipStream_t s);
 // bytes of one of the two tile-edge exchange record buffers of a sweep (the
 // j products, then the k products); the sweep needs 2 * march4_rec_bytes
@@ -251,7 +251,10 @@ int ctx_rhs_sumsq(cgli_ctx *c, double deltaT, double e, double out[5]);
 int ctx_dts(cgli_ctx *c, double deltaT, double e, double er[5], int sweeps, int write_residual, bool rhs_ready);
 // the same in two halves: queue everything up to the norm's copy to the
 // host, then wait for it and check it
-int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_residual, bool rhs_ready);
+// u4_current: U_'s interior is unchanged since the previous call's update
+// (the caller's promise; an inner iteration after the first of one step)
+int ctx_dts_async(cgli_ctx *c, double deltaT, double e, int sweeps, int write_residual, bool rhs_ready,
+                  bool u4_current = false);
 int ctx_dts_wait(cgli_ctx *c, double er[5]);
 bool ctx_timing_on(cgli_ctx *c);
 // per-context state of the channel driver (channel.cpp)
@@ -266,11 +269,13 @@ struct ChannelState {
     int since_check = 0;     // steps since the network's guess was last checked
     bool verify = true;      // check at the next step (first use, after a rejection)
     // inner iterations replayed from captured graphs, one per (residual
-    // precision, deltaT, sweeps); a key runs eagerly the first time
+    // precision, deltaT, sweeps, first or later iteration of a step); a key
+    // runs eagerly the first time
     struct GraphSlot {
         int fp32;
         double dt;
         int sweeps;
+        int later;
         cgli_graph *g;
     };
     bool graph = false;

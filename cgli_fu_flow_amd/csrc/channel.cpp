@@ -272,8 +272,9 @@ int queue_prefix(cgli_ctx *ctx)
 // one inner iteration queued on the context stream: ghosts, fluxes and
 // viscous terms (unless the right-hand side is already in place; prefix:
 // the ghost fill and x fluxes are already queued), then the dual-time
-// update up to the norm's copy to the host
-int queue_iteration(cgli_ctx *ctx, double deltaT, int sweeps, bool reuse, bool prefix = false)
+// update up to the norm's copy to the host.  later: not the step's first
+// iteration, so U_'s interior is the previous iteration's update.
+int queue_iteration(cgli_ctx *ctx, double deltaT, int sweeps, bool reuse, bool later, bool prefix = false)
 {
     const double e = ctx_channel(ctx).e;
     int r = CGLI_OK;
@@ -283,22 +284,23 @@ int queue_iteration(cgli_ctx *ctx, double deltaT, int sweeps, bool reuse, bool p
         if (!r) r = cgli_flux_z(ctx, 1.0, e);
         if (!r) r = cgli_viscous(ctx);
     }
-    if (!r) r = ctx_dts_async(ctx, deltaT, e, sweeps, 0, reuse);
+    if (!r) r = ctx_dts_async(ctx, deltaT, e, sweeps, 0, reuse, later);
     return r;
 }
 
 // queue one inner iteration, from the graph of its key when graphs are on:
 // a key's first occurrence runs eagerly (it allocates what the iteration
 // needs), its second is captured; a graph gone stale is captured again
-int run_iteration(cgli_ctx *ctx, ChannelState &cs, double deltaT, int sweeps, bool reuse, bool fp32, bool prefix)
+int run_iteration(cgli_ctx *ctx, ChannelState &cs, double deltaT, int sweeps, bool reuse, bool later, bool fp32,
+                  bool prefix)
 {
-    if (!cs.graph || reuse || ctx_timing_on(ctx)) return queue_iteration(ctx, deltaT, sweeps, reuse, prefix);
+    if (!cs.graph || reuse || ctx_timing_on(ctx)) return queue_iteration(ctx, deltaT, sweeps, reuse, later, prefix);
     ChannelState::GraphSlot *slot = nullptr;
     for (auto &q : cs.graphs)
-        if (q.fp32 == (int)fp32 && q.dt == deltaT && q.sweeps == sweeps) slot = &q;
+        if (q.fp32 == (int)fp32 && q.dt == deltaT && q.sweeps == sweeps && q.later == (int)later) slot = &q;
     if (!slot) {
-        cs.graphs.push_back({(int)fp32, deltaT, sweeps, nullptr});
-        return queue_iteration(ctx, deltaT, sweeps, reuse);
+        cs.graphs.push_back({(int)fp32, deltaT, sweeps, (int)later, nullptr});
+        return queue_iteration(ctx, deltaT, sweeps, reuse, later);
     }
     if (slot->g) {
         const int r = cgli_graph_launch(slot->g);
@@ -306,11 +308,11 @@ int run_iteration(cgli_ctx *ctx, ChannelState &cs, double deltaT, int sweeps, bo
         // stale (the context changed): eagerly now, captured next time
         cgli_graph_destroy(slot->g);
         slot->g = nullptr;
-        return queue_iteration(ctx, deltaT, sweeps, reuse);
+        return queue_iteration(ctx, deltaT, sweeps, reuse, later);
     }
     int r = cgli_graph_begin(ctx);
     if (r) return r;
-    r = queue_iteration(ctx, deltaT, sweeps, reuse);
+    r = queue_iteration(ctx, deltaT, sweeps, reuse, later);
     cgli_graph *G = nullptr;
     const int re = cgli_graph_end(ctx, &G);
     if (!r) r = re;
@@ -318,7 +320,7 @@ int run_iteration(cgli_ctx *ctx, ChannelState &cs, double deltaT, int sweeps, bo
         if (G) cgli_graph_destroy(G);
         // CGLI_E_STATE: the iteration needed an allocation (a new schedule's
         // buffers); nothing ran, so run it eagerly and capture next time
-        return r == CGLI_E_STATE ? queue_iteration(ctx, deltaT, sweeps, reuse) : r;
+        return r == CGLI_E_STATE ? queue_iteration(ctx, deltaT, sweeps, reuse, later) : r;
     }
     slot->g = G;
     return cgli_graph_launch(G);
@@ -500,7 +502,7 @@ int cgli_channel_step(cgli_ctx *ctx, double deltaT, double tol, int max_inner, i
         // the first iteration reuses the residual evaluation of the guess
         // check when the network's guess was taken
         const bool reuse = ready && it == 1;
-        int r = run_iteration(ctx, cs, deltaT, sweeps, reuse, fp32, prefix);
+        int r = run_iteration(ctx, cs, deltaT, sweeps, reuse, it > 1, fp32, prefix);
         prefix = false;
         if (!r && it < max_inner && !(cs.graph && !ctx_timing_on(ctx))) {
             r = queue_prefix(ctx);

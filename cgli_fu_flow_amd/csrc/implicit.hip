@@ -1731,8 +1731,8 @@ constexpr long G4ROWB = (long)G4P * 64 * 16, F4ROWB = (long)F4P * 64 * 16;
 long march4_geo_pairs(const Geo &g) { return march_rows(g) * G4P * 64; }
 long march4_field_pairs(const Geo &g) { return march_rows(g) * F4P * 64; }
 
-// U_ and a right-hand side R (when Rm) and the geometry (when Gm) into
-// march order; the slots of cells only (the others keep their fill)
+// U_ (when Um), a right-hand side R (when Rm) and the geometry (when Gm)
+// into march order; the slots of cells only (the others keep their fill)
 __global__ void __launch_bounds__(256) k_gather4(Geo g, March M, C5 U_, C5 R, const double *__restrict__ J, M9 met,
                                                  double2 *__restrict__ Um, double2 *__restrict__ Rm,
                                                  double2 *__restrict__ Gm)
@@ -1742,9 +1742,11 @@ __global__ void __launch_bounds__(256) k_gather4(Geo g, March M, C5 U_, C5 R, co
     const long c = ix(g, 3 + q.il, 2 + q.jo, 2 + q.ko);
     const long row = march_row(M, q.T, q.t);
     const long o = row * F4P * 64 + q.l;
-    Um[o] = make_double2(U_.v[0][c], U_.v[1][c]);
-    Um[o + 64] = make_double2(U_.v[2][c], U_.v[3][c]);
-    Um[o + 128] = make_double2(U_.v[4][c], 0.0);
+    if (Um) {
+        Um[o] = make_double2(U_.v[0][c], U_.v[1][c]);
+        Um[o + 64] = make_double2(U_.v[2][c], U_.v[3][c]);
+        Um[o + 128] = make_double2(U_.v[4][c], 0.0);
+    }
     if (Rm) {
         Rm[o] = make_double2(R.v[0][c], R.v[1][c]);
         Rm[o + 64] = make_double2(R.v[2][c], R.v[3][c]);
@@ -1764,10 +1766,12 @@ __global__ void __launch_bounds__(256) k_gather4(Geo g, March M, C5 U_, C5 R, co
 // mapping of k_gather4 / k_scatter4: the increments NR and U_ are read in
 // march order, where the sweep left them and the gather put them this
 // iteration, so NR goes to no field (its field is written from march order
-// where read); U_ is written to its field.  Each block writes 5 partial
-// sums of its positions in a fixed order.
+// where read); the new U_ is written to its field and, in place, to its
+// march-ordered copy, which the next inner iteration's gather then need not
+// rebuild.  Each block writes 5 partial sums of its positions in a fixed
+// order.
 __global__ void __launch_bounds__(256) k_update4(Geo g, March M, Phys p, F5 U_, const double *__restrict__ J,
-                                                 const double2 *__restrict__ Um, const double2 *__restrict__ Xm,
+                                                 double2 *__restrict__ Um, const double2 *__restrict__ Xm,
                                                  F5 Res, int write_res, double *partials)
 {
     const MGather q = mgather_at(g, M);
@@ -1793,11 +1797,16 @@ __global__ void __launch_bounds__(256) k_update4(Geo g, March M, Phys p, F5 U_, 
         const double W = Wold + n23.y * Jc;
         const double T = Told + n4.x * Jc;
         const double rho = P / T;
-        U_.v[0][c] = rho / Jc;
-        U_.v[1][c] = rho * Uu / Jc;
-        U_.v[2][c] = rho * V / Jc;
-        U_.v[3][c] = rho * W / Jc;
-        U_.v[4][c] = (P / (K - 1) + 0.5 * rho * (Uu * Uu + V * V + W * W)) / Jc;
+        const double w0 = rho / Jc, w1 = rho * Uu / Jc, w2 = rho * V / Jc, w3 = rho * W / Jc;
+        const double w4 = (P / (K - 1) + 0.5 * rho * (Uu * Uu + V * V + W * W)) / Jc;
+        U_.v[0][c] = w0;
+        U_.v[1][c] = w1;
+        U_.v[2][c] = w2;
+        U_.v[3][c] = w3;
+        U_.v[4][c] = w4;
+        Um[o] = make_double2(w0, w1);
+        Um[o + 64] = make_double2(w2, w3);
+        Um[o + 128] = make_double2(w4, 0.0);
         x[0] = (rho - rhoold) * (rho - rhoold);
         x[1] = (Uu - Uold) * (Uu - Uold);
         x[2] = (V - Vold) * (V - Vold);
@@ -1817,12 +1826,12 @@ __global__ void __launch_bounds__(256) k_update4(Geo g, March M, Phys p, F5 U_, 
 
 long update4_blocks(const Geo &g) { return mgather_grid(march_of(g)).x; }
 
-int launch_update4(const Geo &g, const Phys &p, F5 U_, const double *J, const void *Um, const void *Xm, F5 Res,
+int launch_update4(const Geo &g, const Phys &p, F5 U_, const double *J, void *Um, const void *Xm, F5 Res,
                    int write_res, double *partials, hipStream_t s)
 {
     const March M = march_of(g);
     const dim3 gr = mgather_grid(M);
-    hipLaunchKernelGGL(k_update4, gr, dim3(256), 0, s, g, M, p, U_, J, static_cast<const double2 *>(Um),
+    hipLaunchKernelGGL(k_update4, gr, dim3(256), 0, s, g, M, p, U_, J, static_cast<double2 *>(Um),
                        static_cast<const double2 *>(Xm), Res, write_res, partials);
     return (int)gr.x;
 }
