@@ -3,22 +3,26 @@
 // Templated on the arithmetic type T: double (the reference path) or float
 // (the FP32 residual path: fields stay FP64 in HBM, each thread converts its
 // stencil on load and the face flux back on store).
-// One thread per face: the thread rebuilds the J-weighted MUSCL-5 left and
-// right states of its face from the six cells along the sweep direction
-// (Flux_X.cpp:106-216, Flux_Y.cpp:106-249, Flux_Z.cpp:105-216) and applies
-// the low-Mach preconditioned Roe flux (Flux_X.cpp:229-350).  The
-// reference's ML/MR scratch never goes to HBM.  The thread issues all of its
-// face's loads (the six cells' J and conserved variables, the face metrics
-// and Jf) before it waits for the first: a face's work is one round trip to
-// memory, not one per stencil and variable.  The expressions are the
-// reference's, except that a divisor used more than once is inverted once
-// and multiplied (the FP64 division is the costliest instruction sequence
-// of the kernel) and products and sums fuse: results agree with the CPU
-// build to rounding (a few ulps), not bit for bit.
+// Each face's J-weighted MUSCL-5 left and right states are rebuilt from the
+// six cells along the sweep direction (Flux_X.cpp:106-216,
+// Flux_Y.cpp:106-249, Flux_Z.cpp:105-216), and the low-Mach preconditioned
+// Roe flux (Flux_X.cpp:229-350) is taken between them.  The reference's
+// ML/MR scratch never goes to HBM.  x and y faces: a thread marches along
+// the sweep direction through a segment of faces, with the six cells of the
+// current face (conserved variables and 1/J) in registers and the next cell
+// and face metrics in flight, so each cell is read once per segment.  z
+// faces (the direction of contiguous memory): one thread per face, which
+// issues all of its face's loads before it waits for the first.  The
+// expressions are the reference's, except that a divisor used more than once
+// is inverted once and multiplied (the FP64 division is the costliest
+// instruction sequence of the kernel) and products and sums fuse: results
+// agree with the CPU build to rounding (a few ulps), not bit for bit.
 // Optional dissipation scaling (the older flux files, Flux_X~~.cpp:552-556):
 // each face's Roe dissipation Fav is multiplied by epS, or by EpF[face] (the
 // filter sensor, face-indexed like the fluxes) when EpF is given.  The live
 // reference has epS = 1 and no EpF; multiplying by 1 is exact.
+#include <algorithm>
+
 #include "cgli_internal.hpp"
 
 // Products and sums in this file contract to fused multiply-adds; the
@@ -185,102 +189,229 @@ __device__ __forceinline__ void roe_face(const T L[5], const T Rq[5], T mx, T my
 
 // ---------------------------------------------------------------- x faces
 // Face i+1/2 between cells (i, j+1, k+1) and (i+1, j+1, k+1), stored at
-// [i][j][k] for i=2..gend, j=1..NY-1, k=1..NZ-1.
+// [i][j][k] for i=2..gend, j=1..NY-1, k=1..NZ-1.  A thread owns a (j, k)
+// column and marches i through a segment of faces: the six cells of the
+// current face (planes i-2 .. i+3: conserved variables and 1/J) stay in
+// registers and move up one plane per face, so each cell is loaded once per
+// segment and each 1/J formed once; the next plane's cell and the next
+// face's metrics are loaded a face ahead.  The same arithmetic as one
+// thread per face (the window holds the values its loads would give).
 template <typename T, bool SCALE>
 __global__ void __launch_bounds__(256) k_flux_x(Geo g, double K, double e, C5 U, const double *__restrict__ J,
                                                 const double *__restrict__ Jf, M9 mf, F5 F, double epS,
-                                                const double *__restrict__ EpF)
+                                                const double *__restrict__ EpF, int seg)
 {
     const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = 2 + blockIdx.z;
+    const int iS = 2 + blockIdx.z * seg, iE = min(iS + seg - 1, g.gend);
     if (k > g.NZ - 1 || j > g.NY - 1) return;
-    const int cj = j + 1, ck = k + 1;
     const long sx = g.Ym * g.Zm;
-    const long c0 = ix(g, i, cj, ck);
-    // left state of face i+1/2; later reference writes win (last-rank patch
-    // after first-rank patch after the 5-point loop)
-    T cl[6], cr[6];
-    if (g.last && i == g.gend) sh3L<T>(cl);
-    else if (g.first && i == 3) sh3L<T>(cl);
-    else if (g.first && i == 2) sh23<T>(cl);
-    else sh5L<T>(cl);
-    // right state of face i+1/2 (from cell i+1)
-    if (g.last && i == g.gend) sh23<T>(cr);
-    else if (g.last && i == g.gend - 1) sh3R<T>(cr);
-    else if (g.first && i == 2) sh3R<T>(cr);
-    else sh5R<T>(cr);
-    Sten<T> S;
-    load_sten<T>(U, J, c0, sx, cl, cr, S);
-    const long f = ix(g, i, j, k);
-    const T xix = ld<T>(mf.m[0], f), xiy = ld<T>(mf.m[1], f), xiz = ld<T>(mf.m[2], f);
-    const T jf = ld<T>(Jf, f);
-    const T ep = SCALE ? (EpF ? ld<T>(EpF, f) : T(epS)) : T(1);
-    // every load of the face is in flight before the first waits
-    __builtin_amdgcn_sched_barrier(0);
-    T Lq[5], Rq[5];
-    face_states<T>(S, cl, cr, Lq, Rq);
-    const T insqr = T(1.0) / sqrt(xix * xix + xiy * xiy + xiz * xiz);
-    T out[5], pl, pr, fav3;
-    roe_face<T>(Lq, Rq, xix, xiy, xiz, jf, T(e), T(K), xix * insqr, xiy * insqr, xiz * insqr, ep, out, pl,
-                pr, fav3);
+    const long cc = ix(g, 0, j + 1, k + 1), fc = ix(g, 0, j, k);
+    T wu[6][5], wq[6];        // planes i-2 .. i+3 of the current face i
+    T nu[5], nJ;              // plane i+4, in flight
+    T mx, my, mz, jf, ep;     // the current face's metrics
+    T nmx, nmy, nmz, njf, nep;   // the next face's, in flight
+    auto load_cell = [&](int pl, T u[5], T &Jv) {
 #pragma unroll
-    for (int v = 0; v < 5; v++) F.v[v][f] = (double)out[v];
+        for (int v = 0; v < 5; v++) u[v] = ld<T>(U.v[v], cc + pl * sx);
+        Jv = ld<T>(J, cc + pl * sx);
+    };
+    auto load_face = [&](int i, T &a, T &b, T &c, T &d, T &f) {
+        const long fi = fc + i * sx;
+        a = ld<T>(mf.m[0], fi);
+        b = ld<T>(mf.m[1], fi);
+        c = ld<T>(mf.m[2], fi);
+        d = ld<T>(Jf, fi);
+        f = SCALE ? (EpF ? ld<T>(EpF, fi) : T(epS)) : T(1);
+    };
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        T Jv;
+        load_cell(iS - 2 + m, wu[m], Jv);
+        wq[m] = Jv;
+    }
+    load_face(iS, mx, my, mz, jf, ep);
+#pragma unroll
+    for (int m = 0; m < 6; m++) wq[m] = T(1) / wq[m];
+    for (int i = iS; i <= iE; i++) {
+        // a face ahead (the last face's loads repeat its own: in range, unused)
+        load_cell(min(i + 4, iE + 3), nu, nJ);
+        load_face(min(i + 1, iE), nmx, nmy, nmz, njf, nep);
+        // left state of face i+1/2; later reference writes win (last-rank
+        // patch after first-rank patch after the 5-point loop)
+        T cl[6], cr[6];
+        if (g.last && i == g.gend) sh3L<T>(cl);
+        else if (g.first && i == 3) sh3L<T>(cl);
+        else if (g.first && i == 2) sh23<T>(cl);
+        else sh5L<T>(cl);
+        // right state of face i+1/2 (from cell i+1)
+        if (g.last && i == g.gend) sh23<T>(cr);
+        else if (g.last && i == g.gend - 1) sh3R<T>(cr);
+        else if (g.first && i == 2) sh3R<T>(cr);
+        else sh5R<T>(cr);
+        Sten<T> S;
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            S.q[m] = wq[m];
+#pragma unroll
+            for (int v = 0; v < 5; v++) S.u[v][m] = wu[m][v];
+        }
+        if ((g.first && i <= 3) || (g.last && i >= g.gend - 1)) {
+            // a position neither stencil uses takes cell i's values (the
+            // loads of one thread per face read cell i there)
+#pragma unroll
+            for (int m = 0; m < 6; m++)
+                if (cl[m] == T(0) && cr[m] == T(0)) {
+                    S.q[m] = wq[2];
+#pragma unroll
+                    for (int v = 0; v < 5; v++) S.u[v][m] = wu[2][v];
+                }
+        }
+        const T wl = T(1) / dot6(cl, S.q), wr = T(1) / dot6(cr, S.q);
+        T Lq[5], Rq[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            Lq[v] = wl * dot6(cl, S.u[v]);
+            Rq[v] = wr * dot6(cr, S.u[v]);
+        }
+        const T insqr = T(1.0) / sqrt(mx * mx + my * my + mz * mz);
+        T out[5], pl, pr, fav3;
+        roe_face<T>(Lq, Rq, mx, my, mz, jf, T(e), T(K), mx * insqr, my * insqr, mz * insqr, ep, out, pl, pr, fav3);
+        const long f = fc + i * sx;
+#pragma unroll
+        for (int v = 0; v < 5; v++) F.v[v][f] = (double)out[v];
+        // move the window up one plane
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            wq[m] = wq[m + 1];
+#pragma unroll
+            for (int v = 0; v < 5; v++) wu[m][v] = wu[m + 1][v];
+        }
+#pragma unroll
+        for (int v = 0; v < 5; v++) wu[5][v] = nu[v];
+        wq[5] = T(1) / nJ;
+        mx = nmx;
+        my = nmy;
+        mz = nmz;
+        jf = njf;
+        ep = nep;
+    }
 }
 
 // ---------------------------------------------------------------- y faces
 // Face j+1/2 between cells (i+1, j, k+1) and (i+1, j+1, k+1), stored at
 // [i][j][k] for i=2..gend-1, j=1..NY, k=1..NZ-1.  Near-wall order drop of
-// Flux_Y.cpp:135-249 and the wall-face override of Flux_Y.cpp:400-408.
+// Flux_Y.cpp:135-249 and the wall-face override of Flux_Y.cpp:400-408.  A
+// thread owns an (i, k) column and marches j through a segment of faces
+// with the six cells of the current face in registers, as the x faces do.
+// Rows outside the array's j range are read from the nearest row (only
+// where the stencil does not use them).
 template <typename T, bool SCALE>
 __global__ void __launch_bounds__(256) k_flux_y(Geo g, double K, double Ep, double e, int xi_mode, C5 U,
                                                 const double *__restrict__ J, const double *__restrict__ Jf,
-                                                M9 mf, F5 F, double epS, const double *__restrict__ EpF)
+                                                M9 mf, F5 F, double epS, const double *__restrict__ EpF, int seg)
 {
     const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = 2 + blockIdx.z;
-    if (k > g.NZ - 1 || j > g.NY) return;
+    const int i = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int jS = 1 + blockIdx.z * seg, jE = min(jS + seg - 1, g.NY);
+    if (k > g.NZ - 1 || i > g.gend - 1) return;
+    const int NY = g.NY, rmax = (int)g.Ym - 1;
+    const long cc = ix(g, i + 1, 0, k + 1), fc = ix(g, i, 0, k);
     const long sy = g.Zm;
-    const long c0 = ix(g, i + 1, j, k + 1);   // cell j
-    const int NY = g.NY;
-    T cl[6], cr[6];
-    if (j == NY) sh12<T>(cl);
-    else if (j == NY - 1) sh3L<T>(cl);
-    else if (j == 3) sh3L<T>(cl);
-    else if (j == 2) sh23<T>(cl);
-    else if (j == 1) sh23<T>(cl);
-    else sh5L<T>(cl);
-    if (j == NY) sh23<T>(cr);
-    else if (j == NY - 1) sh23<T>(cr);
-    else if (j == NY - 2) sh3R<T>(cr);
-    else if (j == 2) sh3R<T>(cr);
-    else if (j == 1) sh34<T>(cr);
-    else sh5R<T>(cr);
-    Sten<T> S;
-    load_sten<T>(U, J, c0, sy, cl, cr, S);
-    const long f = ix(g, i, j, k);
-    const T etx = ld<T>(mf.m[3], f), ety = ld<T>(mf.m[4], f), etz = ld<T>(mf.m[5], f);
-    const T jf = ld<T>(Jf, f);
-    const T ep = SCALE ? (EpF ? ld<T>(EpF, f) : T(epS)) : T(1);
-    // every load of the face is in flight before the first waits
-    __builtin_amdgcn_sched_barrier(0);
-    T Lq[5], Rq[5];
-    face_states<T>(S, cl, cr, Lq, Rq);
-    const T insqr = T(1.0) / sqrt(etx * etx + ety * ety + etz * etz);
-    T nx_ = 0, ny_ = 0, nz_ = 0;
-    if (xi_mode) { nx_ = etx * insqr; ny_ = ety * insqr; nz_ = etz * insqr; }
-    T out[5], pl, pr, fav3;
-    roe_face<T>(Lq, Rq, etx, ety, etz, jf, T(e), T(K), nx_, ny_, nz_, ep, out, pl, pr, fav3);
-    if (j == 1 || j == NY) {
-        out[0] = 0;
-        out[1] = 0;
-        out[2] = T(0.5) * (ety * (pl + pr) - T(Ep) * fav3) / jf;
-        out[3] = 0;
-        out[4] = 0;
-    }
+    T wu[6][5], wq[6];        // rows j-2 .. j+3 of the current face j
+    T nu[5], nJ;              // row j+4, in flight
+    T mx, my, mz, jf, ep, nmx, nmy, nmz, njf, nep;
+    auto load_cell = [&](int r, T u[5], T &Jv) {
+        const long c = cc + (long)min(max(r, 0), rmax) * sy;
 #pragma unroll
-    for (int v = 0; v < 5; v++) F.v[v][f] = (double)out[v];
+        for (int v = 0; v < 5; v++) u[v] = ld<T>(U.v[v], c);
+        Jv = ld<T>(J, c);
+    };
+    auto load_face = [&](int j, T &a, T &b, T &c, T &d, T &f) {
+        const long fj = fc + (long)j * sy;
+        a = ld<T>(mf.m[3], fj);
+        b = ld<T>(mf.m[4], fj);
+        c = ld<T>(mf.m[5], fj);
+        d = ld<T>(Jf, fj);
+        f = SCALE ? (EpF ? ld<T>(EpF, fj) : T(epS)) : T(1);
+    };
+#pragma unroll
+    for (int m = 0; m < 6; m++) load_cell(jS - 2 + m, wu[m], wq[m]);
+    load_face(jS, mx, my, mz, jf, ep);
+    // 1/J where the row exists (a row outside it is never used)
+#pragma unroll
+    for (int m = 0; m < 6; m++) wq[m] = T(1) / wq[m];
+    for (int j = jS; j <= jE; j++) {
+        load_cell(j + 4, nu, nJ);
+        load_face(min(j + 1, jE), nmx, nmy, nmz, njf, nep);
+        T cl[6], cr[6];
+        if (j == NY) sh12<T>(cl);
+        else if (j == NY - 1) sh3L<T>(cl);
+        else if (j == 3) sh3L<T>(cl);
+        else if (j == 2) sh23<T>(cl);
+        else if (j == 1) sh23<T>(cl);
+        else sh5L<T>(cl);
+        if (j == NY) sh23<T>(cr);
+        else if (j == NY - 1) sh23<T>(cr);
+        else if (j == NY - 2) sh3R<T>(cr);
+        else if (j == 2) sh3R<T>(cr);
+        else if (j == 1) sh34<T>(cr);
+        else sh5R<T>(cr);
+        Sten<T> S;
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            S.q[m] = wq[m];
+#pragma unroll
+            for (int v = 0; v < 5; v++) S.u[v][m] = wu[m][v];
+        }
+        if (j <= 3 || j >= NY - 2) {
+            // a position neither stencil uses takes cell j's values (the
+            // loads of one thread per face read cell j there)
+#pragma unroll
+            for (int m = 0; m < 6; m++)
+                if (cl[m] == T(0) && cr[m] == T(0)) {
+                    S.q[m] = wq[2];
+#pragma unroll
+                    for (int v = 0; v < 5; v++) S.u[v][m] = wu[2][v];
+                }
+        }
+        const T wl = T(1) / dot6(cl, S.q), wr = T(1) / dot6(cr, S.q);
+        T Lq[5], Rq[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            Lq[v] = wl * dot6(cl, S.u[v]);
+            Rq[v] = wr * dot6(cr, S.u[v]);
+        }
+        const T insqr = T(1.0) / sqrt(mx * mx + my * my + mz * mz);
+        T nx_ = 0, ny_ = 0, nz_ = 0;
+        if (xi_mode) { nx_ = mx * insqr; ny_ = my * insqr; nz_ = mz * insqr; }
+        T out[5], pl, pr, fav3;
+        roe_face<T>(Lq, Rq, mx, my, mz, jf, T(e), T(K), nx_, ny_, nz_, ep, out, pl, pr, fav3);
+        if (j == 1 || j == NY) {
+            out[0] = 0;
+            out[1] = 0;
+            out[2] = T(0.5) * (my * (pl + pr) - T(Ep) * fav3) / jf;
+            out[3] = 0;
+            out[4] = 0;
+        }
+        const long f = fc + (long)j * sy;
+#pragma unroll
+        for (int v = 0; v < 5; v++) F.v[v][f] = (double)out[v];
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            wq[m] = wq[m + 1];
+#pragma unroll
+            for (int v = 0; v < 5; v++) wu[m][v] = wu[m + 1][v];
+        }
+#pragma unroll
+        for (int v = 0; v < 5; v++) wu[5][v] = nu[v];
+        wq[5] = T(1) / nJ;
+        mx = nmx;
+        my = nmy;
+        mz = nmz;
+        jf = njf;
+        ep = nep;
+    }
 }
 
 // ---------------------------------------------------------------- z faces
@@ -342,24 +473,34 @@ void launch_flux_x(const Geo &g, const Phys &p, double e, C5 U, const double *J,
                    F5 F, double epS, const double *EpF, hipStream_t s)
 {
     dim3 b(64, 4);
+    // faces i = 2..gend in segments: about three waves per SIMD in one round
+    const int nf = g.gend - 1;
+    const long cols = (long)((g.NZ - 1 + 63) / 64) * ((g.NY - 1 + 3) / 4) * 4;   // waves per segment row
+    const int nseg = (int)std::max(1L, std::min((long)nf, 3072 / std::max(1L, cols)));
+    const int seg = (nf + nseg - 1) / nseg;
+    const dim3 gr = grid_for(g.NZ - 1, g.NY - 1, (nf + seg - 1) / seg, b);
     if (p.fp32)
-        hipLaunchKernelGGL(SCALED(k_flux_x, float), grid_for(g.NZ - 1, g.NY - 1, g.gend - 1, b), b, 0, s, g, p.K, e, U, J,
-                           Jf, mf, F, epS, EpF);
+        hipLaunchKernelGGL(SCALED(k_flux_x, float), gr, b, 0, s, g, p.K, e, U, J, Jf, mf, F, epS, EpF, seg);
     else
-        hipLaunchKernelGGL(SCALED(k_flux_x, double), grid_for(g.NZ - 1, g.NY - 1, g.gend - 1, b), b, 0, s, g, p.K, e, U, J,
-                           Jf, mf, F, epS, EpF);
+        hipLaunchKernelGGL(SCALED(k_flux_x, double), gr, b, 0, s, g, p.K, e, U, J, Jf, mf, F, epS, EpF, seg);
 }
 
 void launch_flux_y(const Geo &g, const Phys &p, double Ep, double e, int xi_mode, C5 U, const double *J,
                    const double *Jf, M9 mf, F5 F, double epS, const double *EpF, hipStream_t s)
 {
     dim3 b(64, 4);
+    // faces j = 1..NY in segments: about three waves per SIMD in one round
+    const int nf = g.NY;
+    const long cols = (long)((g.NZ - 1 + 63) / 64) * ((g.gend - 2 + 3) / 4) * 4;
+    const int nseg = (int)std::max(1L, std::min((long)nf, 3072 / std::max(1L, cols)));
+    const int seg = (nf + nseg - 1) / nseg;
+    const dim3 gr = grid_for(g.NZ - 1, g.gend - 2, (nf + seg - 1) / seg, b);
     if (p.fp32)
-        hipLaunchKernelGGL(SCALED(k_flux_y, float), grid_for(g.NZ - 1, g.NY, g.gend - 2, b), b, 0, s, g, p.K, Ep, e, xi_mode,
-                           U, J, Jf, mf, F, epS, EpF);
+        hipLaunchKernelGGL(SCALED(k_flux_y, float), gr, b, 0, s, g, p.K, Ep, e, xi_mode, U, J, Jf, mf, F, epS, EpF,
+                           seg);
     else
-        hipLaunchKernelGGL(SCALED(k_flux_y, double), grid_for(g.NZ - 1, g.NY, g.gend - 2, b), b, 0, s, g, p.K, Ep, e, xi_mode,
-                           U, J, Jf, mf, F, epS, EpF);
+        hipLaunchKernelGGL(SCALED(k_flux_y, double), gr, b, 0, s, g, p.K, Ep, e, xi_mode, U, J, Jf, mf, F, epS, EpF,
+                           seg);
 }
 
 void launch_flux_z(const Geo &g, const Phys &p, double e, int xi_mode, C5 U, const double *J, const double *Jf,

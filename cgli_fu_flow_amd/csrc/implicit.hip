@@ -1783,11 +1783,13 @@ __global__ void __launch_bounds__(256) k_update4(Geo g, March M, Phys p, F5 U_, 
         const double2 n01 = Xm[o], n23 = Xm[o + 64], n4 = Xm[o + 128];
         const double K = p.K;
         const double Jc = J[c];
-        const double U1 = u01.x;
+        // quotients that share a divisor (U1, Jc) through its reciprocal
+        // and div_r: the division's bits
+        const double U1 = u01.x, iU1 = 1.0 / U1, iJc = 1.0 / Jc;
         const double rhoold = U1 * Jc;
-        const double Uold = u01.y / U1;
-        const double Vold = u23.x / U1;
-        const double Wold = u23.y / U1;
+        const double Uold = div_r(u01.y, U1, iU1);
+        const double Vold = div_r(u23.x, U1, iU1);
+        const double Wold = div_r(u23.y, U1, iU1);
         const double VVold = Uold * Uold + Vold * Vold + Wold * Wold;
         const double Pold = (u4.x * Jc - 0.5 * rhoold * VVold) * (K - 1);
         const double Told = Pold / rhoold;
@@ -1797,8 +1799,9 @@ __global__ void __launch_bounds__(256) k_update4(Geo g, March M, Phys p, F5 U_, 
         const double W = Wold + n23.y * Jc;
         const double T = Told + n4.x * Jc;
         const double rho = P / T;
-        const double w0 = rho / Jc, w1 = rho * Uu / Jc, w2 = rho * V / Jc, w3 = rho * W / Jc;
-        const double w4 = (P / (K - 1) + 0.5 * rho * (Uu * Uu + V * V + W * W)) / Jc;
+        const double w0 = div_r(rho, Jc, iJc), w1 = div_r(rho * Uu, Jc, iJc), w2 = div_r(rho * V, Jc, iJc);
+        const double w3 = div_r(rho * W, Jc, iJc);
+        const double w4 = div_r(P / (K - 1) + 0.5 * rho * (Uu * Uu + V * V + W * W), Jc, iJc);
         U_.v[0][c] = w0;
         U_.v[1][c] = w1;
         U_.v[2][c] = w2;
