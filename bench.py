@@ -46,10 +46,14 @@ sys.path.insert(0, ROOT)
 METRIC = "physical steps/s and Mcell-updates/s at 1e-6 inner-residual drop; plain vs mixed+learned"
 HBM_PEAK_GBS = 8000.0
 # algorithmic FP64 bytes per interior cell per launch of each kernel class
-# (each input element read once, each output written once)
-# rhs_diag under the default schedule: U_, U, Uq, J, the face fluxes and vF in, MR out
+# (each input element read once, each output written once), in the forms the
+# default schedule runs: rhs_diag reads U_, U, Uq, J, the face fluxes and vF
+# and writes MR; update_norm (k_update4) reads U_'s march-ordered copy, the
+# increments and J and writes U_ to its field and its march-ordered copy;
+# march_gather in an inner iteration after a step's first moves MR only
+# (the first also U_: 160)
 ALGO_BYTES = {"flux_x": 120, "flux_y": 120, "flux_z": 120, "viscous": 320, "rhs_diag": 320,
-              "lusgs_fwd": 240, "lusgs_bwd": 240, "update_norm": 128, "split_coef": 560, "filter": 72, "march_gather": 160,
+              "lusgs_fwd": 240, "lusgs_bwd": 240, "update_norm": 168, "split_coef": 560, "filter": 72, "march_gather": 80,
               "march_scatter": 80}
 
 
