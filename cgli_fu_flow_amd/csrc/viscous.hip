@@ -11,10 +11,12 @@
 //      Sutherland viscosity;
 //   3. vF = dF/dxi + dG/deta + dH/dzeta on interior cells.
 // A block owns a (j, k) tile and marches along i through a segment of
-// planes: a ring of three planes of primitives (tile plus a 1-cell halo) and
-// the tile's y and z face fluxes live in LDS, the x face flux of the cell
-// below travels in registers, so that U and each face's J and metrics are
-// read once and vF is the only store.  Loads run ahead of their use: a
+// planes: a ring of four planes of primitives (tile plus a 1-cell halo) and
+// the tile's y and z face fluxes of two planes live in LDS, the x face flux
+// of the cell below travels in registers, so that U and each face's J and
+// metrics are read once and vF is the only store.  A plane's divergence is
+// taken after the next plane's barrier, so one barrier per plane orders
+// every LDS access.  Loads run ahead of their use: a
 // plane's conserved variables one plane ahead, each face's metrics one face
 // ahead, so that the march seldom waits on memory.  The arithmetic is the
 // reference's except for divisions: a face's 1/Jf and the constant Cv K / Pr
@@ -287,7 +289,7 @@ __device__ __forceinline__ void u_load(const Geo &g, C5 U, int i, int j0, int k0
 }
 
 // primitives (u, v, w, T) of plane i of the tile's halo, from its
-// conserved variables R, into ring slot i % 3.
+// conserved variables R, into ring slot i & 3.
 // The reference's T = P / (rho R) / J with P = (U5 - rho VV / 2)(K - 1) J
 // (Viscous_terms.cpp:260-266); J cancels, so T is formed from the stored
 // state alone, with one division per cell.
@@ -304,7 +306,7 @@ __device__ __forceinline__ void visc_plane(const Geo &g, const Phys &p, const Up
         const double r = R.r[s][0], ir = 1.0 / r;
         const double u = R.r[s][1] * ir, v = R.r[s][2] * ir, w = R.r[s][3] * ir;
         const double T = (R.r[s][4] - 0.5 * r * (u * u + v * v + w * w)) * KR * ir;
-        double(*pl)[VH_J][VH_K] = sP[i % 3];
+        double(*pl)[VH_J][VH_K] = sP[i & 3];
         pl[0][hj][hk] = u;
         pl[1][hj][hk] = v;
         pl[2][hj][hk] = w;
@@ -315,7 +317,7 @@ __device__ __forceinline__ void visc_plane(const Geo &g, const Phys &p, const Up
 template <typename Real>
 __device__ __forceinline__ Pt<Real> lp(const double (*sP)[4][VH_J][VH_K], int i, int hj, int hk)
 {
-    const double(*pl)[VH_J][VH_K] = sP[i % 3];
+    const double(*pl)[VH_J][VH_K] = sP[i & 3];
     return Pt<Real>{(Real)pl[0][hj][hk], (Real)pl[1][hj][hk], (Real)pl[2][hj][hk], (Real)pl[3][hj][hk]};
 }
 
@@ -352,8 +354,11 @@ __global__ void __launch_bounds__(VT_THREADS, 2) k_viscous(Geo g, Phys p, C5 U, 
                                                         const double *__restrict__ Jv, M9 mv,
                                                         const double *__restrict__ Jw, M9 mw, F4 vF, int seg)
 {
-    __shared__ double sP[3][4][VH_J][VH_K];    // primitives of planes i-1, i, i+1 (slot i % 3)
-    __shared__ double sF[2][4][VT_J][VT_K];    // y and z face fluxes of the current plane
+    // primitives of planes i-1 .. i+2 (slot i & 3): the ring's fourth slot
+    // and the two parities of the face fluxes let one barrier per plane
+    // order every LDS access
+    __shared__ double sP[4][4][VH_J][VH_K];
+    __shared__ double sF[2][2][4][VT_J][VT_K];   // y and z face fluxes of planes of either parity
     const int tk = threadIdx.x % VT_K, tj = threadIdx.x / VT_K;
     const int j0 = 2 + blockIdx.y * (VT_J - 1), k0 = 2 + blockIdx.x * (VT_K - 1);
     const int j = j0 + tj, k = k0 + tk, hj = tj + 1, hk = tk + 1;
@@ -378,13 +383,25 @@ __global__ void __launch_bounds__(VT_THREADS, 2) k_viscous(Geo g, Phys p, C5 U, 
     ldm<Real>(MX, mu, Ju, (iS - 1) * sx + cjk);
     __syncthreads();
     double Fx[4] = {0.0, 0.0, 0.0, 0.0};   // x face below the current cell
+    double dX[4] = {0.0, 0.0, 0.0, 0.0};   // the x part of the previous plane's divergence
     if (out) xface<Real>(p, sP, iS, hj, hk, MX, Fx);
     ldm<Real>(MX, mu, Ju, iS * sx + cjk);
+    // vF of plane q from its faces (the y and z fluxes in sF[q & 1])
+    auto divergence = [&](int q) {
+        if (!out) return;
+        const long c = ix(g, q, j, k);
+        const double(*fy)[VT_J][VT_K] = sF[q & 1][0];
+        const double(*fz)[VT_J][VT_K] = sF[q & 1][1];
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            vF.v[v][c] = dX[v] + (fy[v][tj + 1][tk] - fy[v][tj][tk]) * invET + (fz[v][tj][tk + 1] - fz[v][tj][tk]) * invZT;
+    };
     for (int i = iS; i < iE; i++) {
         const long fid = (i - 1) * sx + cjk;
         visc_plane(g, p, R, i + 1, j0, k0, sP);
         ldm<Real>(MY, mv, Jv, fid);
         __syncthreads();
+        if (i > iS) divergence(i - 1);
         // the next plane's conserved variables, converted at the top of the
         // next iteration (the last iteration reloads plane iE, unused)
         u_load(g, U, min(i + 2, iE), j0, k0, R);
@@ -401,7 +418,7 @@ __global__ void __launch_bounds__(VT_THREADS, 2) k_viscous(Geo g, Phys p, C5 U, 
                         lp<Real>(sP, i, hj, hk + 1), lp<Real>(sP, i, hj - 1, hk + 1), lp<Real>(sP, i, hj, hk - 1),
                         lp<Real>(sP, i, hj - 1, hk - 1), MY.m, MY.J, F);
 #pragma unroll
-            for (int v = 0; v < 4; v++) sF[0][v][tj][tk] = F[v];
+            for (int v = 0; v < 4; v++) sF[i & 1][0][v][tj][tk] = F[v];
         }
         ldm<Real>(MX, mu, Ju, (min(i + 1, iE - 1)) * sx + cjk);
         __builtin_amdgcn_sched_barrier(0);
@@ -412,19 +429,16 @@ __global__ void __launch_bounds__(VT_THREADS, 2) k_viscous(Geo g, Phys p, C5 U, 
                         lp<Real>(sP, i, hj + 1, hk), lp<Real>(sP, i, hj + 1, hk - 1), lp<Real>(sP, i, hj - 1, hk),
                         lp<Real>(sP, i, hj - 1, hk - 1), lp<Real>(sP, i - 1, hj + 1, hk), MZ.m, MZ.J, F);
 #pragma unroll
-            for (int v = 0; v < 4; v++) sF[1][v][tj][tk] = F[v];
+            for (int v = 0; v < 4; v++) sF[i & 1][1][v][tj][tk] = F[v];
         }
-        __syncthreads();
-        if (out) {
-            const long c = ix(g, i, j, k);
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                vF.v[v][c] = (Fh[v] - Fx[v]) * invXI + (sF[0][v][tj + 1][tk] - sF[0][v][tj][tk]) * invET +
-                             (sF[1][v][tj][tk + 1] - sF[1][v][tj][tk]) * invZT;
-                Fx[v] = Fh[v];
-            }
+        for (int v = 0; v < 4; v++) {
+            dX[v] = (Fh[v] - Fx[v]) * invXI;
+            Fx[v] = Fh[v];
         }
     }
+    __syncthreads();
+    divergence(iE - 1);
 }
 
 void launch_viscous(const Geo &g, const Phys &p, C5 U, const double *J, const double *Ju, M9 mu,
