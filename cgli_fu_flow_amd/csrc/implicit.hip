@@ -1241,35 +1241,40 @@ __device__ __forceinline__ MPos mpos_of_lane(int q)
     return MPos{0, 0};
 }
 
-// Work units: (tile column TKi, group of 4 tile rows, block of MR_ROWS
-// steps, tile row TJi); a block of 4 waves takes one.  The grid is
-// XCD-aware: workgroup b runs on XCD b % 8, and XCD x takes the units of a
-// contiguous group of tile columns, k-neighbours first, then step blocks.
-// A cache line of a field-layout row of k shared by two tiles, and the
-// planes two step blocks share, are then read through one XCD's L2.  (With
-// another placement of workgroups the result is the same, only slower.)
+// Work units: (MG_KP adjacent tile columns, 4 / MG_KP tile rows, block of
+// MR_ROWS steps, tile row TJi); a block of 4 waves takes one, a wave per
+// (column, row): the waves of a row cover 8 MG_KP consecutive k, whole cache
+// lines of a field-layout row (4 columns: 32 k, 2.6 % faster gather and
+// update than 2).  The grid is XCD-aware: workgroup b runs on XCD b % 8,
+// and XCD x takes the units of a contiguous group of tile columns,
+// k-neighbours first, then step blocks.  The planes two step blocks share
+// are then read through one XCD's L2.  (With another placement of workgroups
+// the result is the same, only slower.)
 constexpr int MG_XCD = 8;
+constexpr int MG_KP = 4;   // tile columns per block
 struct MGather {
     int T, t, l, jl, kl, jo, ko, il;
     bool ok;
 };
 __device__ __forceinline__ MGather mgather_at(const Geo &g, const March &M)
 {
-    constexpr int TJ = MARCH_TJ, TK = MARCH_TK, WPB = 4;
-    constexpr int JB = (TJ + WPB - 1) / WPB;   // blocks per tile row set
+    constexpr int TJ = MARCH_TJ, TK = MARCH_TK, WPB = 4, JW = WPB / MG_KP;
+    constexpr int JB = (TJ + JW - 1) / JW;   // blocks per tile row set
     MGather q;
     const int wv = threadIdx.x >> 6;
     const int x = blockIdx.x % MG_XCD, u = blockIdx.x / MG_XCD;
     const int k0 = x * M.ntk / MG_XCD, nk = (x + 1) * M.ntk / MG_XCD - k0;
+    const int nkp = (nk + MG_KP - 1) / MG_KP;
     const int ntb = (M.S + MR_ROWS - 1) / MR_ROWS;
     q.ok = false;
-    if (nk == 0 || u >= nk * JB * ntb * M.ntj) return q;
-    const int TKi = k0 + u % nk;
-    int r = u / nk;
+    if (nk == 0 || u >= nkp * JB * ntb * M.ntj) return q;
+    const int kt = (u % nkp) * MG_KP + wv % MG_KP;   // the wave's column in the XCD's group
+    const int TKi = k0 + kt;
+    int r = u / nkp;
     const int jb = r % JB;
     r /= JB;
     const int tb = r % ntb, TJi = r / ntb;
-    q.jl = jb * WPB + wv;
+    q.jl = jb * JW + wv / MG_KP;
     const MPos P = mpos_of_lane(threadIdx.x & 63);
     q.T = TJi * M.ntk + TKi;
     q.t = tb * MR_ROWS + P.r;
@@ -1278,13 +1283,14 @@ __device__ __forceinline__ MGather mgather_at(const Geo &g, const March &M)
     q.jo = TJi * TJ + q.jl;
     q.ko = TKi * TK + q.kl;
     q.il = q.t - q.jl - q.kl;
-    q.ok = q.jl < TJ && q.t < M.S && q.il >= 0 && q.il < g.nx && q.jo < g.ny && q.ko < g.nz;
+    q.ok = kt < nk && q.jl < TJ && q.t < M.S && q.il >= 0 && q.il < g.nx && q.jo < g.ny && q.ko < g.nz;
     return q;
 }
 static dim3 mgather_grid(const March &M)
 {
-    constexpr int JB = (MARCH_TJ + 3) / 4;
-    const long per_xcd = (long)((M.ntk + MG_XCD - 1) / MG_XCD) * JB * ((M.S + MR_ROWS - 1) / MR_ROWS) * M.ntj;
+    constexpr int JB = (MARCH_TJ + 4 / MG_KP - 1) / (4 / MG_KP);
+    const long cols = (M.ntk + MG_XCD - 1) / MG_XCD;
+    const long per_xcd = (cols + MG_KP - 1) / MG_KP * JB * ((M.S + MR_ROWS - 1) / MR_ROWS) * M.ntj;
     return dim3((unsigned)(MG_XCD * per_xcd));
 }
 
