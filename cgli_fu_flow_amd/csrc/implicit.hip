@@ -416,7 +416,10 @@ void launch_rhs_diag(const Geo &g, const Phys &p, double deltaT, double e, int w
                      const double *J, M9 met, C5 Fx, C5 Fy, C5 Fz, C4 vF, C5 Res, F5 MR, F5 NL, F5 NR,
                      hipStream_t s, int write_inc)
 {
-    dim3 b(64, 4);
+    // a block spans the whole k row where it fits (255 cells at NZ = 256: one
+    // block per row, every wave reading whole lines of the field layout)
+    const int bx = g.NZ - 1 > 128 ? 256 : g.NZ - 1 > 64 ? 128 : 64;
+    dim3 b(bx, 256 / bx);
     dim3 gr((g.NZ - 1 + b.x - 1) / b.x, (g.NY - 1 + b.y - 1) / b.y, g.nx);
     const RInv ri{1.0 / (2 * deltaT), 1.0 / p.dxi, 1.0 / p.deta, 1.0 / p.dzeta};
     auto kern = write_inc ? (with_res ? k_rhs_diag<true, true> : k_rhs_diag<true, false>)
