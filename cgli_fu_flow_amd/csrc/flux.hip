@@ -472,10 +472,12 @@ static dim3 grid_for(int nk, int nj, int ni, dim3 b)
 void launch_flux_x(const Geo &g, const Phys &p, double e, C5 U, const double *J, const double *Jf, M9 mf,
                    F5 F, double epS, const double *EpF, hipStream_t s)
 {
-    dim3 b(64, 4);
+    // a block spans the whole k row where it fits (255 faces at NZ = 256)
+    const int bx = g.NZ - 1 > 128 ? 256 : g.NZ - 1 > 64 ? 128 : 64;
+    dim3 b(bx, 256 / bx);
     // faces i = 2..gend in segments: about three waves per SIMD in one round
     const int nf = g.gend - 1;
-    const long cols = (long)((g.NZ - 1 + 63) / 64) * ((g.NY - 1 + 3) / 4) * 4;   // waves per segment row
+    const long cols = (long)((g.NZ - 1 + bx - 1) / bx) * ((g.NY - 1 + b.y - 1) / b.y) * 4;   // waves per segment row
     const int nseg = (int)std::max(1L, std::min((long)nf, 3072 / std::max(1L, cols)));
     const int seg = (nf + nseg - 1) / nseg;
     const dim3 gr = grid_for(g.NZ - 1, g.NY - 1, (nf + seg - 1) / seg, b);
@@ -488,10 +490,11 @@ void launch_flux_x(const Geo &g, const Phys &p, double e, C5 U, const double *J,
 void launch_flux_y(const Geo &g, const Phys &p, double Ep, double e, int xi_mode, C5 U, const double *J,
                    const double *Jf, M9 mf, F5 F, double epS, const double *EpF, hipStream_t s)
 {
-    dim3 b(64, 4);
+    const int bx = g.NZ - 1 > 128 ? 256 : g.NZ - 1 > 64 ? 128 : 64;
+    dim3 b(bx, 256 / bx);
     // faces j = 1..NY in segments: about three waves per SIMD in one round
     const int nf = g.NY;
-    const long cols = (long)((g.NZ - 1 + 63) / 64) * ((g.gend - 2 + 3) / 4) * 4;
+    const long cols = (long)((g.NZ - 1 + bx - 1) / bx) * ((g.gend - 2 + b.y - 1) / b.y) * 4;
     const int nseg = (int)std::max(1L, std::min((long)nf, 3072 / std::max(1L, cols)));
     const int seg = (nf + nseg - 1) / nseg;
     const dim3 gr = grid_for(g.NZ - 1, g.gend - 2, (nf + seg - 1) / seg, b);
