@@ -73,7 +73,7 @@ __global__ void k_filt_init(Geo g, R4 U, Real *__restrict__ work, long N)
 //   Gy = J/Y_point-weighted y pass of Gx
 //   level output = z pass of Gy
 // on j = 2..NY, k = 2..NZ and the level's planes lo..hi.  A block owns an
-// 8 x 32 tile of (j, k) and marches along i through a segment of planes.
+// 4 x 64 tile of (j, k) and marches along i through a segment of planes.
 // Each thread keeps the level input of its halo positions (the tile plus a
 // 1-cell halo) for planes i-1, i, i+1 in registers, with plane i+2 in
 // flight, so each plane is read from HBM once per block and the x pass
@@ -85,7 +85,7 @@ __global__ void k_filt_init(Geo g, R4 U, Real *__restrict__ work, long N)
 // input.  Expressions and operation order are the reference's; the y
 // pass's quotients that share a divisor use its reciprocal and div_r, which
 // gives the division's bits.
-constexpr int FT_J = 8, FT_K = 32, FT_THREADS = FT_J * FT_K;
+constexpr int FT_J = 4, FT_K = 64, FT_THREADS = FT_J * FT_K;
 constexpr int FH_K = FT_K + 2, FH_N = (FT_J + 2) * FH_K;
 
 template <bool TWOH, typename Real, typename In>
